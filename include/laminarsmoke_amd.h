@@ -1,0 +1,178 @@
+/*
+ * laminarsmoke_amd.h - C-ABI of the MI355X chemistry-step integrator.
+ *
+ * Drop-in boundary for laminarSMOKE's operator-splitting reaction step: the
+ * per-cell constant-pressure batch-reactor integration that
+ * solvers/laminarSMOKE/unsteady/chemistry_DI.H:76-331 performs serially with
+ * OdeSMOKE::MultiValueSolver<MethodGear<KernelDense<...>>> becomes one batched
+ * call.  All functions return 0 (LSM_OK) or a negative LSM_E_* code; the text of
+ * the last error is available from lsm_last_error().  No function aborts or
+ * exits the process (the reference's FatalErrorMessage does, MultiValueSolver.hpp:352-359).
+ *
+ * Threading: a handle may be used from one host thread at a time; different
+ * handles are independent.  Device entry points enqueue work on the caller's
+ * HIP stream and return without synchronising.
+ */
+#ifndef LAMINARSMOKE_AMD_H
+#define LAMINARSMOKE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "lsm_mechanism.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * ODE solver selection and parameters.
+ * Mirrors OpenSMOKE::ODE_Parameters (math/external-ode-solvers/ODE_Parameters.h:52-160)
+ * with one new integrator value, LSM_ODE_OPENSMOKE_GPU.
+ * ------------------------------------------------------------------------- */
+enum lsm_ode_integrator {              /* ODE_Parameters::ODE_INTEGRATOR (ODE_Parameters.h:52-55) */
+    LSM_ODE_OPENSMOKE = 0,
+    LSM_ODE_BZZODE = 1,
+    LSM_ODE_CVODE = 2,
+    LSM_ODE_DVODE = 3,
+    LSM_ODE_DASPK = 4,
+    LSM_ODE_DLSODE = 5,
+    LSM_ODE_DLSODA = 6,
+    LSM_ODE_RADAU5 = 7,
+    LSM_ODE_MEBDF = 8,
+    LSM_ODE_CHEMEQ2 = 9,
+    LSM_ODE_OPENSMOKE_GPU = 100        /* this library: OpenSMOKE Gear/BDF on the GPU */
+};
+
+typedef struct lsm_ode_parameters {
+    int32_t type;                      /* lsm_ode_integrator                                   */
+    double relative_tolerance;         /* default 100*FLT_EPSILON (ODE_Parameters.hpp:248)     */
+    double absolute_tolerance;         /* default 1e-10                                        */
+    double minimum_step;               /* -1: solver default                                   */
+    double maximum_step;               /* -1: solver default                                   */
+    double initial_step;               /* -1: solver default                                   */
+    int32_t maximum_order;             /* -1: solver default (5)                               */
+    int32_t maximum_number_of_steps;   /* -1: solver default (500000)                          */
+    int32_t full_pivoting;             /* 0/1                                                  */
+    char linear_algebra[32];           /* "Eigen" (the only value KernelDense accepts)         */
+} lsm_ode_parameters;
+
+/* Defaults of ODE_Parameters::ODE_Parameters() (ODE_Parameters.hpp:239-256), with
+ * type = LSM_ODE_OPENSMOKE_GPU. */
+int lsm_ode_parameters_default(lsm_ode_parameters *p);
+
+/* Parse a parameter dictionary into *p (starting from the values already in *p).
+ * Accepts both the laminarSMOKE OpenFOAM sub-dictionary form read at
+ * solvers/laminarSMOKE/readOptions.H:474-530
+ *     OdeHomogeneous { odeSolver "OpenSMOKE_GPU"; relTolerance 1e-7;
+ *                      absTolerance 1e-12; maximumOrder 5; fullPivoting false; }
+ * and the OpenSMOKE @-keyword form of ODE_Parameters::SetupFromDictionary
+ * (ODE_Parameters.hpp:39-236, grammar ODE_Parameters.h:163-239)
+ *     @OdeSolver OpenSMOKE_GPU; @RelativeTolerance 1e-7; @FullPivoting true; ...
+ * Unknown solver names, unknown keys and malformed values are rejected with
+ * LSM_E_PARSE (readOptions.H:495-504 aborts instead). */
+int lsm_ode_parameters_parse(const char *text, lsm_ode_parameters *p);
+
+/* ---------------------------------------------------------------------------
+ * Reactor / cell-loop options (the variables chemistry_DI.H reads).
+ * ------------------------------------------------------------------------- */
+typedef struct lsm_reactor_options {
+    double pressure;            /* thermodynamicPressure [Pa] (chemistry_DI.H:134)            */
+    double min_temperature;     /* minTemperature4Chemistry: cells with T <= it are not
+                                   integrated (chemistry_DI.H:97)                              */
+    int32_t energy_equation;    /* energyEquation (chemistry_DI.H:135, 231-233)               */
+    double mass_fractions_tol;  /* massFractionsTol: |sum Y - 1| warning threshold
+                                   (utilities.H:159)                                          */
+    double t0;                  /* start time of the chemistry step, t0 = t - dt of the CFD
+                                   step (Policy_MomentumTransportReaction.H:62)               */
+} lsm_reactor_options;
+
+int lsm_reactor_options_default(lsm_reactor_options *r);
+
+/* ---------------------------------------------------------------------------
+ * Integrator selection inside the GPU library.
+ * ------------------------------------------------------------------------- */
+enum lsm_method {
+    LSM_METHOD_AUTO = 0,        /* team kernel when the mechanism fits, else plain            */
+    LSM_METHOD_PLAIN = 1,       /* one cell per thread, FP64, state in HBM                    */
+    LSM_METHOD_TEAM = 2         /* one cell per wavefront, persistent work queue, FP64        */
+};
+
+/* OdeStatus values (math/native-ode-solvers/OdeSolverUtilities.h:83-99); a cell
+ * below min_temperature reports LSM_STATUS_SKIPPED. */
+enum {
+    LSM_STATUS_SKIPPED = 0,
+    LSM_STATUS_CONTINUATION = 2,
+    LSM_STATUS_MAX_NUMBER_OF_STEPS_REACHED = -1,
+    LSM_STATUS_TOO_STRICT_TOLERANCES = -2,
+    LSM_STATUS_MAX_NUMBER_ERRORTEST_FAILURES = -4,
+    LSM_STATUS_MAX_NUMBER_CONVERGENCETEST_FAILURES = -5,
+    LSM_STATUS_TOO_SMALL_STEP_SIZE = -6,
+    LSM_STATUS_ILLEGAL_CONSTRAINTS = -9
+};
+
+/* Per-cell counters (int32, LSM_NCOUNTERS per cell, cell-major), mirroring
+ * MethodGear.h:85-120 / MultiValueSolver.h:199-234 / KernelDense.h. */
+enum {
+    LSM_CNT_STEPS = 0,                  /* numberOfSteps()                      */
+    LSM_CNT_FUNCTION_CALLS = 1,         /* numberOfFunctionCalls()              */
+    LSM_CNT_FUNCTION_CALLS_JACOBIAN = 2,/* numberOfFunctionCallsForJacobian     */
+    LSM_CNT_JACOBIANS = 3,              /* numberOfJacobianEvaluations()        */
+    LSM_CNT_FACTORIZATIONS = 4,         /* numberOfMatrixFactorizations()       */
+    LSM_CNT_LINEAR_SOLUTIONS = 5,       /* numberOfLinearSystemSolutions()      */
+    LSM_CNT_CONVERGENCE_FAILURES = 6,   /* numberOfConvergenceFailure()         */
+    LSM_CNT_ERROR_TEST_FAILURES = 7,    /* numberOfErrorCheckFailure()          */
+    LSM_CNT_MAX_ORDER_USED = 8,         /* maxOrderUsed()                       */
+    LSM_CNT_RHS_TOTAL = 9,              /* every RHS evaluation (incl. Jacobian) */
+    LSM_CNT_CLAMPS = 10,                /* yMin/yMax constraint activations     */
+    LSM_CNT_NEWTON_ITERATIONS = 11,     /* sum of Newton iterations             */
+    LSM_NCOUNTERS = 12
+};
+
+/* Error codes. */
+enum {
+    LSM_OK = 0,
+    LSM_E_INVALID = -1,         /* bad argument                                 */
+    LSM_E_PARSE = -2,           /* dictionary parse error                       */
+    LSM_E_UNSUPPORTED = -3,     /* option / mechanism not supported             */
+    LSM_E_HIP = -4,             /* HIP runtime error                            */
+    LSM_E_NOMEM = -5            /* allocation failed                            */
+};
+
+typedef struct lsm_handle lsm_handle;
+
+/* Upload the mechanism tables to the current HIP device and bind the ODE
+ * parameters (replaces the construction of the maps at readOptions.H:76-78, the
+ * reactor at memoryAllocation.H:51 and the solver at memoryAllocation.H:58-62). */
+int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode,
+               const lsm_reactor_options *reactor, lsm_handle **out);
+
+int lsm_set_method(lsm_handle *h, int32_t method);
+int lsm_destroy(lsm_handle *h);
+
+/* Chemistry step over n_cells cells, host pointers (the whole forAll loop of
+ * chemistry_DI.H:92-328).  y_in/y_out: SoA [ns+1][n_cells] (species rows, then T).
+ * dt: per-cell integration time (DeltaTCells).  status (may be NULL): OdeStatus per
+ * cell.  q (may be NULL): heat release QR [W/m3] of integrated cells; entries of
+ * skipped cells are left untouched (QCells keeps its value).  counters (may be
+ * NULL): [n_cells][LSM_NCOUNTERS].  Synchronous. */
+int lsm_integrate_batch(lsm_handle *h, int64_t n_cells, const double *dt, const double *y_in,
+                        double *y_out, int32_t *status, double *q, int32_t *counters);
+
+/* Same contract on device pointers, enqueued on `stream` (a hipStream_t; NULL =
+ * default stream).  y_in and y_out may alias.  Not synchronising. */
+int lsm_integrate_batch_device(lsm_handle *h, int64_t n_cells, const double *d_dt,
+                               const double *d_y_in, double *d_y_out, int32_t *d_status,
+                               double *d_q, int32_t *d_counters, void *stream);
+
+/* Text of the last error of this thread ("" if none). */
+const char *lsm_last_error(void);
+
+/* Library version string. */
+const char *lsm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LAMINARSMOKE_AMD_H */

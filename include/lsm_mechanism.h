@@ -1,0 +1,99 @@
+/*
+ * lsm_mechanism.h - flat, read-only description of a CHEMKIN-style gas-phase
+ * mechanism, shared by the HIP integrator (laminarsmoke_amd) and the CPU oracle.
+ *
+ * The tables carry exactly what OpenSMOKE++'s maps hold after reading
+ * kinetics.xml, in SI units (kmol, m3, s, J, K):
+ *
+ *   nasa[18*k + i]   per-species NASA-7 block in the order consumed by
+ *                    ThermodynamicsMap_CHEMKIN::SetCoefficients
+ *                    (maps/ThermodynamicsMap_CHEMKIN.hpp:263-343):
+ *                    [0..4] high-T a1..a5, [5] high a6, [6] high a7,
+ *                    [7..11] low-T a1..a5, [12] low a6, [13] low a7,
+ *                    [14] T_low, [15] T_high, [16] T_mid, [17] MW [kg/kmol].
+ *                    The high-T set is used iff T > T_mid (strict), with no
+ *                    clamping outside [T_low, T_high] (hpp:436-496).
+ *   lnA/beta/E_over_R  forward Arrhenius k = exp(lnA + beta ln T - E_over_R/T)
+ *                    (KineticsMap_CHEMKIN.hpp:1198-1212).  For fall-off
+ *                    reactions these are the LOW-pressure (k0) parameters, as
+ *                    after the preprocessor swap
+ *                    (kernel/kinetics/ReactionPolicy_CHEMKIN.hpp:618-631).
+ *   rflags           LSM_RXN_REVERSIBLE: reverse rate from the equilibrium
+ *                    constant (hpp:1215-1222, 1737-1742);
+ *                    LSM_RXN_NEGATIVE_A: k is multiplied by -1 (hpp:1208-1210).
+ *   reac_* / prod_*  reaction-major CSR of (species, coefficient).  The
+ *                    coefficient is both the stoichiometric coefficient and the
+ *                    reaction order (elementary reactions), as in
+ *                    StoichiometricMap (maps/StoichiometricMap.hpp:512-598,
+ *                    656-730, 916-1001).  A species may appear at most once per
+ *                    side.
+ *   tb_*             third-body reactions: M = cTot + sum_k (eff_k - 1) c_k
+ *                    (KineticsMap_CHEMKIN.hpp:1388-1398); efficiencies are
+ *                    stored as eff - 1, like the map does (hpp:655-661).
+ *   fo_*             fall-off reactions (Lindemann or Troe), k_inf Arrhenius
+ *                    parameters, Troe (a, T***, T*, T**) and either a single
+ *                    collider species or an M efficiency list (hpp:1496-1550).
+ *
+ * All indices are 0-based.  The arrays are owned by the caller and are only
+ * read during lsm_create()/oracle calls.
+ */
+#ifndef LSM_MECHANISM_H
+#define LSM_MECHANISM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSM_NASA_STRIDE 18
+
+enum {
+    LSM_RXN_REVERSIBLE = 1,  /* thermodynamically reversible (1/Kc from NASA-7) */
+    LSM_RXN_NEGATIVE_A = 2   /* negative frequency factor: k -> -k            */
+};
+
+enum {
+    LSM_FALLOFF_LINDEMANN = 0,
+    LSM_FALLOFF_TROE = 1
+};
+
+typedef struct lsm_mechanism {
+    int32_t ns;                /* number of species                          */
+    int32_t nr;                /* number of reactions                        */
+    const double *nasa;        /* [ns*18]                                    */
+    const double *lnA;         /* [nr] ln|A| (SI)                            */
+    const double *beta;        /* [nr]                                       */
+    const double *E_over_R;    /* [nr] K                                     */
+    const int32_t *rflags;     /* [nr] LSM_RXN_* bits                        */
+    const int32_t *reac_ptr;   /* [nr+1]                                     */
+    const int32_t *reac_sp;    /* [reac_ptr[nr]]                             */
+    const double *reac_nu;     /* [reac_ptr[nr]]                             */
+    const int32_t *prod_ptr;   /* [nr+1]                                     */
+    const int32_t *prod_sp;    /* [prod_ptr[nr]]                             */
+    const double *prod_nu;     /* [prod_ptr[nr]]                             */
+    int32_t ntb;               /* number of third-body reactions             */
+    const int32_t *tb_rxn;     /* [ntb] reaction index                       */
+    const int32_t *tb_ptr;     /* [ntb+1]                                    */
+    const int32_t *tb_sp;      /* [tb_ptr[ntb]]                              */
+    const double *tb_eff_m1;   /* [tb_ptr[ntb]] efficiency - 1               */
+    int32_t nfo;               /* number of fall-off reactions               */
+    const int32_t *fo_rxn;     /* [nfo] reaction index                       */
+    const int32_t *fo_type;    /* [nfo] LSM_FALLOFF_*                        */
+    const double *fo_inf;      /* [nfo*3] lnA_inf, beta_inf, E_over_R_inf     */
+    const double *fo_troe;     /* [nfo*4] a, T*** , T*, T** (T** = 0: absent) */
+    const int32_t *fo_single;  /* [nfo] collider species, or -1 for M        */
+    const int32_t *fo_ptr;     /* [nfo+1]                                    */
+    const int32_t *fo_sp;      /* [fo_ptr[nfo]]                              */
+    const double *fo_eff_m1;   /* [fo_ptr[nfo]] efficiency - 1               */
+} lsm_mechanism;
+
+/* Physical constants of the reference (math/PhysicalConstants.h:24). */
+#define LSM_R_J_KMOL 8314.4621
+#define LSM_P_ATM 101325.0
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSM_MECHANISM_H */

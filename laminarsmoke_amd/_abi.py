@@ -1,0 +1,53 @@
+"""ctypes mirrors of the C-ABI structs declared in include/*.h."""
+import ctypes
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int32)
+
+
+class LsmMechanism(ctypes.Structure):
+    """include/lsm_mechanism.h: struct lsm_mechanism."""
+    _fields_ = [
+        ("ns", ctypes.c_int32), ("nr", ctypes.c_int32),
+        ("nasa", _d), ("lnA", _d), ("beta", _d), ("E_over_R", _d), ("rflags", _i),
+        ("reac_ptr", _i), ("reac_sp", _i), ("reac_nu", _d),
+        ("prod_ptr", _i), ("prod_sp", _i), ("prod_nu", _d),
+        ("ntb", ctypes.c_int32), ("tb_rxn", _i), ("tb_ptr", _i), ("tb_sp", _i), ("tb_eff_m1", _d),
+        ("nfo", ctypes.c_int32), ("fo_rxn", _i), ("fo_type", _i), ("fo_inf", _d), ("fo_troe", _d),
+        ("fo_single", _i), ("fo_ptr", _i), ("fo_sp", _i), ("fo_eff_m1", _d),
+    ]
+
+
+class LsmOdeParameters(ctypes.Structure):
+    """include/laminarsmoke_amd.h: struct lsm_ode_parameters."""
+    _fields_ = [
+        ("type", ctypes.c_int32),
+        ("relative_tolerance", ctypes.c_double),
+        ("absolute_tolerance", ctypes.c_double),
+        ("minimum_step", ctypes.c_double),
+        ("maximum_step", ctypes.c_double),
+        ("initial_step", ctypes.c_double),
+        ("maximum_order", ctypes.c_int32),
+        ("maximum_number_of_steps", ctypes.c_int32),
+        ("full_pivoting", ctypes.c_int32),
+        ("linear_algebra", ctypes.c_char * 32),
+    ]
+
+
+class LsmReactorOptions(ctypes.Structure):
+    """include/laminarsmoke_amd.h: struct lsm_reactor_options."""
+    _fields_ = [
+        ("pressure", ctypes.c_double),
+        ("min_temperature", ctypes.c_double),
+        ("energy_equation", ctypes.c_int32),
+        ("mass_fractions_tol", ctypes.c_double),
+        ("t0", ctypes.c_double),
+    ]
+
+
+N_COUNTERS = 12
+COUNTER_NAMES = (
+    "steps", "function_calls", "function_calls_jacobian", "jacobians", "factorizations",
+    "linear_solutions", "convergence_failures", "error_test_failures", "max_order_used",
+    "rhs_total", "clamp_events", "newton_iterations",
+)

@@ -1,0 +1,168 @@
+"""Python view of the C-ABI (include/laminarsmoke_amd.h), for tests and benchmarks.
+
+The names mirror the reference interface of the chemistry step:
+
+* :class:`ODE_Parameters` - OpenSMOKE::ODE_Parameters
+  (math/external-ode-solvers/ODE_Parameters.h:52-160): SetType,
+  SetRelativeTolerance, SetAbsoluteTolerance, SetFullPivoting, SetMaximumOrder,
+  SetMaximumNumberOfSteps, SetupFromDictionary;
+* :class:`ChemistryStep` - the OpenSMOKE branch of the cell loop
+  (LS/unsteady/chemistry_DI.H:76-331) on the GPU: one call integrates every cell.
+
+Everything runs in liblsm_hip.so; this module only moves pointers.  Loading fails
+loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from ._abi import LsmOdeParameters, LsmReactorOptions, N_COUNTERS, COUNTER_NAMES
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblsm_hip.so")
+
+ODE_INTEGRATOR = {"OpenSMOKE": 0, "BzzOde": 1, "CVODE": 2, "DVODE": 3, "DASPK": 4, "DLSODE": 5, "DLSODA": 6,
+                  "RADAU5": 7, "MEBDF": 8, "CHEMEQ2": 9, "OpenSMOKE_GPU": 100}
+METHODS = {"auto": 0, "plain": 1, "team": 2}
+STATUS_NAMES = {0: "skipped", 2: "ok", -1: "max steps", -2: "too strict tolerances", -4: "error-test failures",
+                -5: "convergence failures", -6: "step too small", -9: "illegal constraints"}
+
+_lib = None
+
+
+class LsmError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LsmError(f"{LIB_PATH} is missing: build it with `python -m laminarsmoke_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64 = ctypes.c_void_p, ctypes.c_int64
+        L.lsm_last_error.restype = ctypes.c_char_p
+        L.lsm_version.restype = ctypes.c_char_p
+        L.lsm_ode_parameters_default.argtypes = [vp]
+        L.lsm_ode_parameters_parse.argtypes = [ctypes.c_char_p, vp]
+        L.lsm_reactor_options_default.argtypes = [vp]
+        L.lsm_create.argtypes = [vp, vp, vp, ctypes.POINTER(vp)]
+        L.lsm_set_method.argtypes = [vp, ctypes.c_int32]
+        L.lsm_destroy.argtypes = [vp]
+        L.lsm_integrate_batch.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
+        L.lsm_integrate_batch_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise LsmError(f"lsm error {rc}: {lib().lsm_last_error().decode()}")
+
+
+class ODE_Parameters:
+    """OpenSMOKE::ODE_Parameters (ODE_Parameters.h:52-160) on top of lsm_ode_parameters."""
+
+    def __init__(self):
+        self.c = LsmOdeParameters()
+        _check(lib().lsm_ode_parameters_default(ctypes.byref(self.c)))
+
+    def SetType(self, name_or_value):
+        self.c.type = ODE_INTEGRATOR[name_or_value] if isinstance(name_or_value, str) else int(name_or_value)
+
+    def SetRelativeTolerance(self, v):
+        self.c.relative_tolerance = float(v)
+
+    def SetAbsoluteTolerance(self, v):
+        self.c.absolute_tolerance = float(v)
+
+    def SetFullPivoting(self, flag):
+        self.c.full_pivoting = int(bool(flag))
+
+    def SetMaximumOrder(self, v):
+        self.c.maximum_order = int(v)
+
+    def SetMaximumNumberOfSteps(self, v):
+        self.c.maximum_number_of_steps = int(v)
+
+    def SetupFromDictionary(self, text: str):
+        _check(lib().lsm_ode_parameters_parse(text.encode(), ctypes.byref(self.c)))
+
+    def type(self):
+        return self.c.type
+
+    def relative_tolerance(self):
+        return self.c.relative_tolerance
+
+    def absolute_tolerance(self):
+        return self.c.absolute_tolerance
+
+    def full_pivoting(self):
+        return bool(self.c.full_pivoting)
+
+    def maximum_order(self):
+        return self.c.maximum_order
+
+
+def reactor_options(pressure=101325.0, min_temperature=0.0, energy=True, mass_fractions_tol=1e-4, t0=0.0):
+    r = LsmReactorOptions()
+    _check(lib().lsm_reactor_options_default(ctypes.byref(r)))
+    r.pressure, r.min_temperature, r.energy_equation = pressure, min_temperature, int(bool(energy))
+    r.mass_fractions_tol, r.t0 = mass_fractions_tol, t0
+    return r
+
+
+class ChemistryStep:
+    """Batched chemistry step of one mechanism on the current HIP device."""
+
+    def __init__(self, mech, ode: ODE_Parameters | None = None, method="auto", **reactor_kw):
+        self.mech = mech
+        self.ode = ode or ODE_Parameters()
+        self.reactor = reactor_options(**reactor_kw)
+        h = ctypes.c_void_p()
+        _check(lib().lsm_create(ctypes.byref(mech.struct()), ctypes.byref(self.ode.c),
+                                ctypes.byref(self.reactor), ctypes.byref(h)))
+        self.h = h
+        self.set_method(method)
+
+    def set_method(self, method):
+        _check(lib().lsm_set_method(self.h, METHODS[method] if isinstance(method, str) else int(method)))
+
+    def close(self):
+        if self.h:
+            lib().lsm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def integrate(self, y, dt, q=None, with_counters=True):
+        """Host arrays: y[ns+1, n] -> (y_out, status, q, counters[n, 12])."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        neq, n = y.shape
+        if neq != self.mech.neq:
+            raise ValueError(f"y has {neq} rows, mechanism needs {self.mech.neq}")
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (n,)))
+        yout = np.empty_like(y)
+        st = np.zeros(n, dtype=np.int32)
+        qa = np.zeros(n) if q is None else np.ascontiguousarray(q, dtype=np.float64).copy()
+        cnt = np.zeros((n, N_COUNTERS), dtype=np.int32) if with_counters else None
+        p = lambda a: a.ctypes.data if a is not None else None
+        _check(lib().lsm_integrate_batch(self.h, n, p(dt), p(y), p(yout), p(st), p(qa), p(cnt)))
+        return yout, st, qa, cnt
+
+    def integrate_device(self, n, dt_ptr, y_in_ptr, y_out_ptr, status_ptr=None, q_ptr=None, counters_ptr=None,
+                         stream=None):
+        """Device pointers (ints), enqueued on `stream` (int handle or None)."""
+        _check(lib().lsm_integrate_batch_device(self.h, int(n), dt_ptr, y_in_ptr, y_out_ptr, status_ptr, q_ptr,
+                                                counters_ptr, stream))
+
+
+def counters_dict(cnt: np.ndarray) -> dict:
+    return {name: cnt[:, i] for i, name in enumerate(COUNTER_NAMES)}

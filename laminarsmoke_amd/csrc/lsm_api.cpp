@@ -1,0 +1,611 @@
+// lsm_api.cpp - host side of the C-ABI (include/laminarsmoke_amd.h).
+//
+// Replaces, for the chemistry step, what laminarSMOKE does on the host:
+//   * solver selection / parameter dictionary: LS/readOptions.H:474-530 and
+//     OS/math/external-ode-solvers/ODE_Parameters.hpp:39-256;
+//   * map construction from kinetics.xml: LS/readOptions.H:68-78 (here: flat
+//     tables already in memory, include/lsm_mechanism.h);
+//   * the per-cell loop of LS/unsteady/chemistry_DI.H:92-328, which becomes one
+//     batched launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/laminarsmoke_amd.h"
+#include "lsm_internal.hpp"
+
+using namespace lsm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_OK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail(LSM_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// dictionary parsing
+// ---------------------------------------------------------------------------
+struct Tok {
+    std::string s;
+    bool quoted;
+};
+
+std::vector<Tok> tokenize(const char *text, std::string &err) {
+    std::vector<Tok> out;
+    const char *p = text;
+    while (*p) {
+        if (isspace((unsigned char)*p)) { p++; continue; }
+        if (p[0] == '/' && p[1] == '/') { while (*p && *p != '\n') p++; continue; }
+        if (p[0] == '/' && p[1] == '*') {
+            const char *e = strstr(p + 2, "*/");
+            if (!e) { err = "unterminated comment"; return {}; }
+            p = e + 2;
+            continue;
+        }
+        if (*p == '{' || *p == '}' || *p == ';') { out.push_back({std::string(1, *p), false}); p++; continue; }
+        if (*p == '"') {
+            const char *e = strchr(p + 1, '"');
+            if (!e) { err = "unterminated string"; return {}; }
+            out.push_back({std::string(p + 1, e), true});
+            p = e + 1;
+            continue;
+        }
+        const char *b = p;
+        while (*p && !isspace((unsigned char)*p) && *p != ';' && *p != '{' && *p != '}' && *p != '"') p++;
+        out.push_back({std::string(b, p), false});
+    }
+    return out;
+}
+
+bool parse_double(const std::string &s, double &v) {
+    if (s.empty()) return false;
+    char *end = nullptr;
+    v = strtod(s.c_str(), &end);
+    return end && *end == '\0' && std::isfinite(v);
+}
+
+bool parse_int(const std::string &s, int &v) {
+    if (s.empty()) return false;
+    char *end = nullptr;
+    long x = strtol(s.c_str(), &end, 10);
+    if (!end || *end != '\0' || x < -2147483647L || x > 2147483647L) return false;
+    v = (int)x;
+    return true;
+}
+
+// OpenFOAM Switch words and OpenSMOKE booleans
+bool parse_bool(std::string s, int &v) {
+    for (auto &c : s) c = (char)tolower((unsigned char)c);
+    if (s == "true" || s == "on" || s == "yes" || s == "y" || s == "t" || s == "1") { v = 1; return true; }
+    if (s == "false" || s == "off" || s == "no" || s == "n" || s == "f" || s == "none" || s == "0") { v = 0; return true; }
+    return false;
+}
+
+// names accepted by ODE_Parameters::SetupFromDictionary (ODE_Parameters.hpp:43-120)
+// and readOptions.H:495-513, plus the GPU integrator
+bool solver_type(const std::string &name, int &type) {
+    static const struct { const char *n; int t; } tab[] = {
+        {"OpenSMOKE", LSM_ODE_OPENSMOKE}, {"BzzOde", LSM_ODE_BZZODE}, {"CVODE", LSM_ODE_CVODE},
+        {"DVODE", LSM_ODE_DVODE},         {"DASPK", LSM_ODE_DASPK},   {"DLSODE", LSM_ODE_DLSODE},
+        {"DLSODA", LSM_ODE_DLSODA},       {"RADAU5", LSM_ODE_RADAU5}, {"MEBDF", LSM_ODE_MEBDF},
+        {"CHEMEQ2", LSM_ODE_CHEMEQ2},     {"OpenSMOKE_GPU", LSM_ODE_OPENSMOKE_GPU},
+    };
+    for (auto &e : tab)
+        if (name == e.n) { type = e.t; return true; }
+    return false;
+}
+
+int apply_key(const std::string &key, const std::string &val, lsm_ode_parameters *p) {
+    double d;
+    int i;
+    auto bad = [&](void) { return fail(LSM_E_PARSE, "invalid value '" + val + "' for key '" + key + "'"); };
+    if (key == "odeSolver" || key == "@OdeSolver") {
+        if (!solver_type(val, i)) return fail(LSM_E_PARSE, "unknown ODE solver '" + val + "'");
+        p->type = i;
+    } else if (key == "relTolerance" || key == "@RelativeTolerance") {
+        if (!parse_double(val, d)) return bad();
+        p->relative_tolerance = d;
+    } else if (key == "absTolerance" || key == "@AbsoluteTolerance") {
+        if (!parse_double(val, d)) return bad();
+        p->absolute_tolerance = d;
+    } else if (key == "maximumOrder" || key == "@MaximumOrder") {
+        if (!parse_int(val, i)) return bad();
+        p->maximum_order = i;
+    } else if (key == "fullPivoting" || key == "@FullPivoting") {
+        if (!parse_bool(val, i)) return bad();
+        p->full_pivoting = i;
+    } else if (key == "@MaximumNumberOfSteps") {
+        if (!parse_int(val, i)) return bad();
+        p->maximum_number_of_steps = i;
+    } else if (key == "@MaximumStep") {
+        if (!parse_double(val, d)) return bad();
+        p->maximum_step = d;
+    } else if (key == "@MinimumStep") {
+        if (!parse_double(val, d)) return bad();
+        p->minimum_step = d;
+    } else if (key == "@InitialStep") {
+        if (!parse_double(val, d)) return bad();
+        p->initial_step = d;
+    } else if (key == "@LinearAlgebra") {
+        if (val.size() >= sizeof(p->linear_algebra)) return bad();
+        snprintf(p->linear_algebra, sizeof(p->linear_algebra), "%s", val.c_str());
+    } else if (key == "@SparseSolver" || key == "@SparsePreconditioner") {
+        // sparse options of the grammar (ODE_Parameters.h:182-192): accepted, no effect on the dense kernel
+    } else if (key == "@SparsePreconditionerDropTolerance") {
+        if (!parse_double(val, d)) return bad();
+    } else if (key == "@SparsePreconditionerFillFactor") {
+        if (!parse_int(val, i)) return bad();
+    } else {
+        return fail(LSM_E_PARSE, "unknown key '" + key + "'");
+    }
+    return LSM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Gear l-vectors, MethodGear::MemoryAllocationMethod (MethodGear.hpp:155-178)
+// ---------------------------------------------------------------------------
+double factorial(unsigned n) {
+    double f = 1.;
+    for (unsigned i = 2; i <= n; i++) f *= (double)i;
+    return f;
+}
+
+void gear_r(double r[MAX_ORDER + 1][MAX_ORDER]) {
+    for (int i = 0; i <= MAX_ORDER; i++)
+        for (int j = 0; j < MAX_ORDER; j++) r[i][j] = 0.;
+    r[0][0] = 1.;
+    r[1][0] = 1.;
+    for (int j = 2; j <= MAX_ORDER; j++) {
+        r[0][j - 1] = factorial(j);
+        for (int i = 2; i <= j; i++) r[i - 1][j - 1] = (double)j * r[i - 1][j - 2] + r[i - 2][j - 2];
+        r[j][j - 1] = 1.;
+    }
+    double sum = 1.;
+    for (int j = 2; j <= MAX_ORDER; j++) {
+        sum += 1. / (double)j;
+        for (int i = 1; i <= j + 1; i++) r[i - 1][j - 1] /= (factorial(j) * sum);
+    }
+}
+
+int nu_class(double nu) {
+    if (nu == 1.) return 1;
+    if (nu == 2.) return 2;
+    if (nu == 3.) return 3;
+    if (nu == 0.5) return 4;
+    return 5;
+}
+
+// host image of the device tables: one aligned blob, offsets patched afterwards
+struct Blob {
+    std::vector<char> bytes;
+    template <class T>
+    size_t add(const std::vector<T> &v) {
+        size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + std::max<size_t>(v.size(), 1) * sizeof(T), 0);
+        if (!v.empty()) memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+};
+
+}  // namespace
+
+struct lsm_handle {
+    int device = 0;
+    int num_cu = 0;
+    lsm_ode_parameters ode{};
+    lsm_reactor_options reactor{};
+    int method = LSM_METHOD_AUTO;
+    DevMech dm{};
+    KParams kp{};
+    void *d_tables = nullptr;
+    unsigned *d_queue = nullptr;
+    double *d_work = nullptr;
+    int64_t work_slots = 0;
+    // staging for host-pointer calls
+    double *d_y = nullptr, *d_dt = nullptr, *d_q = nullptr;
+    int *d_status = nullptr, *d_cnt = nullptr;
+    int64_t cap = 0;
+    int team_grid = 0;
+};
+
+extern "C" {
+
+const char *lsm_last_error(void) { return g_err.c_str(); }
+
+const char *lsm_version(void) { return "laminarsmoke_amd 0.1 (gfx950)"; }
+
+int lsm_ode_parameters_default(lsm_ode_parameters *p) {
+    if (!p) return fail(LSM_E_INVALID, "null parameters");
+    memset(p, 0, sizeof(*p));
+    p->type = LSM_ODE_OPENSMOKE_GPU;
+    p->relative_tolerance = 100. * (double)1.192092896e-07F;   // 100*MachEpsFloat()
+    p->absolute_tolerance = 1.e-10;
+    p->minimum_step = -1.;
+    p->maximum_step = -1.;
+    p->initial_step = -1.;
+    p->maximum_order = -1;
+    p->maximum_number_of_steps = -1;
+    p->full_pivoting = 0;
+    snprintf(p->linear_algebra, sizeof(p->linear_algebra), "Eigen");
+    return LSM_OK;
+}
+
+int lsm_ode_parameters_parse(const char *text, lsm_ode_parameters *p) {
+    if (!text || !p) return fail(LSM_E_INVALID, "null argument");
+    std::string err;
+    std::vector<Tok> t = tokenize(text, err);
+    if (!err.empty()) return fail(LSM_E_PARSE, err);
+    lsm_ode_parameters q = *p;
+    size_t i = 0;
+    int depth = 0;
+    while (i < t.size()) {
+        if (!t[i].quoted && t[i].s == "}") {
+            if (--depth < 0) return fail(LSM_E_PARSE, "unbalanced '}'");
+            i++;
+            continue;
+        }
+        if (!t[i].quoted && t[i].s == ";") { i++; continue; }
+        if (i + 1 < t.size() && !t[i + 1].quoted && t[i + 1].s == "{") {   // sub-dictionary name
+            depth++;
+            i += 2;
+            continue;
+        }
+        const std::string key = t[i].s;
+        if (i + 1 >= t.size() || (!t[i + 1].quoted && (t[i + 1].s == ";" || t[i + 1].s == "}")))
+            return fail(LSM_E_PARSE, "missing value for key '" + key + "'");
+        const std::string val = t[i + 1].s;
+        int rc = apply_key(key, val, &q);
+        if (rc != LSM_OK) return rc;
+        i += 2;
+        if (i < t.size() && !t[i].quoted && t[i].s == ";") i++;
+        else if (key[0] != '@' && !(i < t.size() && t[i].s == "}"))
+            return fail(LSM_E_PARSE, "missing ';' after key '" + key + "'");
+    }
+    if (depth != 0) return fail(LSM_E_PARSE, "unbalanced '{'");
+    *p = q;
+    return LSM_OK;
+}
+
+int lsm_reactor_options_default(lsm_reactor_options *r) {
+    if (!r) return fail(LSM_E_INVALID, "null options");
+    r->pressure = 101325.;
+    r->min_temperature = 0.;
+    r->energy_equation = 1;
+    r->mass_fractions_tol = 1.e-4;
+    r->t0 = 0.;
+    return LSM_OK;
+}
+
+static int check_mechanism(const lsm_mechanism *m) {
+    if (!m) return fail(LSM_E_INVALID, "null mechanism");
+    if (m->ns < 1 || m->nr < 0) return fail(LSM_E_INVALID, "mechanism needs ns >= 1 and nr >= 0");
+    if (!m->nasa || (m->nr > 0 && (!m->lnA || !m->beta || !m->E_over_R || !m->rflags || !m->reac_ptr ||
+                                   !m->reac_sp || !m->reac_nu || !m->prod_ptr || !m->prod_sp || !m->prod_nu)))
+        return fail(LSM_E_INVALID, "mechanism table pointer is null");
+    for (int j = 0; j < m->nr; j++) {
+        for (int side = 0; side < 2; side++) {
+            const int32_t *ptr = side ? m->prod_ptr : m->reac_ptr;
+            const int32_t *sp = side ? m->prod_sp : m->reac_sp;
+            if (ptr[j] > ptr[j + 1]) return fail(LSM_E_INVALID, "reaction CSR pointers decrease");
+            for (int q = ptr[j]; q < ptr[j + 1]; q++)
+                if (sp[q] < 0 || sp[q] >= m->ns) return fail(LSM_E_INVALID, "species index out of range");
+        }
+    }
+    for (int s = 0; s < m->ntb; s++) {
+        if (m->tb_rxn[s] < 0 || m->tb_rxn[s] >= m->nr) return fail(LSM_E_INVALID, "third-body reaction index");
+        for (int q = m->tb_ptr[s]; q < m->tb_ptr[s + 1]; q++)
+            if (m->tb_sp[q] < 0 || m->tb_sp[q] >= m->ns) return fail(LSM_E_INVALID, "third-body species index");
+    }
+    for (int s = 0; s < m->nfo; s++) {
+        if (m->fo_rxn[s] < 0 || m->fo_rxn[s] >= m->nr) return fail(LSM_E_INVALID, "fall-off reaction index");
+        if (m->fo_type[s] != LSM_FALLOFF_LINDEMANN && m->fo_type[s] != LSM_FALLOFF_TROE)
+            return fail(LSM_E_UNSUPPORTED, "fall-off type (only Lindemann and Troe)");
+        if (m->fo_single[s] >= m->ns) return fail(LSM_E_INVALID, "fall-off collider species index");
+        for (int q = m->fo_ptr[s]; q < m->fo_ptr[s + 1]; q++)
+            if (m->fo_sp[q] < 0 || m->fo_sp[q] >= m->ns) return fail(LSM_E_INVALID, "fall-off species index");
+    }
+    return LSM_OK;
+}
+
+static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
+    const int ns = m->ns, nr = m->nr;
+    DevMech &d = h->dm;
+    d.ns = ns; d.nr = nr; d.ne = ns + 1; d.ntb = m->ntb; d.nfo = m->nfo;
+    d.ns_pad = (ns + 63) / 64 * 64;
+    // species coefficient rows (SetCoefficients, ThermodynamicsMap_CHEMKIN.hpp:263-343)
+    std::vector<double> sp((size_t)NSP_ROWS * d.ns_pad, 0.);
+    const double one_third = 1. / 3.;
+    for (int k = 0; k < ns; k++) {
+        const double *c = m->nasa + LSM_NASA_STRIDE * k;
+        auto S = [&](int row) -> double & { return sp[(size_t)row * d.ns_pad + k]; };
+        for (int hi = 0; hi < 2; hi++) {
+            const double *a = hi ? c : c + 7;
+            const double a6 = hi ? c[5] : c[12], a7 = hi ? c[6] : c[13];
+            const int rc = hi ? SP_CP_HT : SP_CP_LT, rh = hi ? SP_DH_HT : SP_DH_LT, rs = hi ? SP_DS_HT : SP_DS_LT;
+            for (int i = 0; i < 5; i++) S(rc + i) = a[i];
+            S(rh + 0) = a[0]; S(rh + 1) = 0.50 * a[1]; S(rh + 2) = one_third * a[2]; S(rh + 3) = 0.25 * a[3];
+            S(rh + 4) = 0.20 * a[4]; S(rh + 5) = a6;
+            S(rs + 0) = a[0]; S(rs + 1) = a[1]; S(rs + 2) = 0.50 * a[2]; S(rs + 3) = one_third * a[3];
+            S(rs + 4) = 0.25 * a[4]; S(rs + 5) = a7;
+        }
+        S(SP_TMID) = c[16];
+        S(SP_MW) = c[17];
+    }
+    // reaction renumbering: elementary, third-body, fall-off
+    std::vector<int> tb_of(nr, -1), fo_of(nr, -1);
+    for (int s = 0; s < m->ntb; s++) tb_of[m->tb_rxn[s]] = s;
+    for (int s = 0; s < m->nfo; s++) fo_of[m->fo_rxn[s]] = s;
+    std::vector<int> perm, inv(nr);
+    for (int pass = 0; pass < 3; pass++)
+        for (int j = 0; j < nr; j++) {
+            const int cls = fo_of[j] >= 0 ? 2 : (tb_of[j] >= 0 ? 1 : 0);
+            if (cls == pass) perm.push_back(j);
+        }
+    for (int jn = 0; jn < nr; jn++) inv[perm[jn]] = jn;
+    std::vector<double> rA(nr), rB(nr), rE(nr), rdn(nr), fw_nu, bw_nu, rx_nu;
+    std::vector<int> rinfo(nr), rtb(nr), rfo(nr), fw_ptr(nr + 1), bw_ptr(nr + 1), rx_ptr(nr + 1), fw_sp, bw_sp, rx_sp;
+    for (int jn = 0; jn < nr; jn++) {
+        const int j = perm[jn];
+        rA[jn] = m->lnA[j]; rB[jn] = m->beta[j]; rE[jn] = m->E_over_R[j];
+        rinfo[jn] = m->rflags[j]; rtb[jn] = tb_of[j]; rfo[jn] = fo_of[j];
+        std::vector<std::pair<int, double>> re, pr;
+        double dn = 0.;
+        for (int q = m->reac_ptr[j]; q < m->reac_ptr[j + 1]; q++) { re.push_back({m->reac_sp[q], m->reac_nu[q]}); }
+        for (int q = m->prod_ptr[j]; q < m->prod_ptr[j + 1]; q++) { pr.push_back({m->prod_sp[q], m->prod_nu[q]}); }
+        for (auto &e : pr) dn += e.second;
+        for (auto &e : re) dn -= e.second;
+        rdn[jn] = dn;
+        auto bysp = [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; };
+        std::stable_sort(re.begin(), re.end(), bysp);
+        std::stable_sort(pr.begin(), pr.end(), bysp);
+        fw_ptr[jn] = (int)fw_sp.size();
+        for (auto &e : re) { fw_sp.push_back(e.first); fw_nu.push_back(e.second); }
+        bw_ptr[jn] = (int)bw_sp.size();
+        for (auto &e : pr) { bw_sp.push_back(e.first); bw_nu.push_back(e.second); }
+        std::vector<std::pair<int, double>> mx;
+        for (auto &e : re) mx.push_back({e.first, -e.second});
+        for (auto &e : pr) mx.push_back(e);
+        std::stable_sort(mx.begin(), mx.end(), bysp);
+        rx_ptr[jn] = (int)rx_sp.size();
+        for (auto &e : mx) { rx_sp.push_back(e.first); rx_nu.push_back(e.second); }
+    }
+    fw_ptr[nr] = (int)fw_sp.size(); bw_ptr[nr] = (int)bw_sp.size(); rx_ptr[nr] = (int)rx_sp.size();
+    // species-major formation list (StoichiometricMap.hpp:656-730 class order)
+    std::vector<int> f_ptr(ns + 1), f_rx;
+    std::vector<double> f_nu;
+    {
+        std::vector<std::vector<std::pair<int, double>>> lists(ns);
+        for (int side = 0; side < 2; side++)
+            for (int cls = 1; cls <= 5; cls++)
+                for (int j = 0; j < nr; j++) {
+                    const int32_t *ptr = side ? m->prod_ptr : m->reac_ptr;
+                    const int32_t *spp = side ? m->prod_sp : m->reac_sp;
+                    const double *nup = side ? m->prod_nu : m->reac_nu;
+                    for (int q = ptr[j]; q < ptr[j + 1]; q++)
+                        if (nu_class(nup[q]) == cls)
+                            lists[spp[q]].push_back({inv[j], side ? nup[q] : -nup[q]});
+                }
+        // per species: reactant classes then product classes, as built above per side
+        std::vector<std::vector<std::pair<int, double>>> ordered(ns);
+        for (int k = 0; k < ns; k++) {
+            for (auto &e : lists[k]) if (e.second < 0) ordered[k].push_back(e);
+            for (auto &e : lists[k]) if (e.second > 0) ordered[k].push_back(e);
+        }
+        for (int k = 0; k < ns; k++) {
+            f_ptr[k] = (int)f_rx.size();
+            for (auto &e : ordered[k]) { f_rx.push_back(e.first); f_nu.push_back(e.second); }
+        }
+        f_ptr[ns] = (int)f_rx.size();
+    }
+    std::vector<int> tb_ptr(m->ntb + 1), tb_sp;
+    std::vector<double> tb_eff;
+    for (int s = 0; s < m->ntb; s++) {
+        tb_ptr[s] = (int)tb_sp.size();
+        for (int q = m->tb_ptr[s]; q < m->tb_ptr[s + 1]; q++) { tb_sp.push_back(m->tb_sp[q]); tb_eff.push_back(m->tb_eff_m1[q]); }
+    }
+    tb_ptr[m->ntb] = (int)tb_sp.size();
+    std::vector<int> fo_type(m->nfo), fo_single(m->nfo), fo_ptr(m->nfo + 1), fo_sp;
+    std::vector<double> fo_par((size_t)8 * m->nfo, 0.), fo_eff;
+    for (int s = 0; s < m->nfo; s++) {
+        fo_type[s] = m->fo_type[s];
+        fo_single[s] = m->fo_single[s];
+        for (int i = 0; i < 3; i++) fo_par[8 * s + i] = m->fo_inf[3 * s + i];
+        for (int i = 0; i < 4; i++) fo_par[8 * s + 3 + i] = m->fo_troe[4 * s + i];
+        fo_ptr[s] = (int)fo_sp.size();
+        for (int q = m->fo_ptr[s]; q < m->fo_ptr[s + 1]; q++) { fo_sp.push_back(m->fo_sp[q]); fo_eff.push_back(m->fo_eff_m1[q]); }
+    }
+    fo_ptr[m->nfo] = (int)fo_sp.size();
+
+    Blob b;
+    size_t o_sp = b.add(sp), o_rA = b.add(rA), o_rB = b.add(rB), o_rE = b.add(rE), o_rdn = b.add(rdn);
+    size_t o_rinfo = b.add(rinfo), o_rtb = b.add(rtb), o_rfo = b.add(rfo);
+    size_t o_fwp = b.add(fw_ptr), o_fws = b.add(fw_sp), o_fwn = b.add(fw_nu);
+    size_t o_bwp = b.add(bw_ptr), o_bws = b.add(bw_sp), o_bwn = b.add(bw_nu);
+    size_t o_rxp = b.add(rx_ptr), o_rxs = b.add(rx_sp), o_rxn = b.add(rx_nu);
+    size_t o_tbp = b.add(tb_ptr), o_tbs = b.add(tb_sp), o_tbe = b.add(tb_eff);
+    size_t o_fot = b.add(fo_type), o_fos1 = b.add(fo_single), o_fopar = b.add(fo_par);
+    size_t o_fop = b.add(fo_ptr), o_fos = b.add(fo_sp), o_foe = b.add(fo_eff);
+    size_t o_fp = b.add(f_ptr), o_frx = b.add(f_rx), o_fnu = b.add(f_nu);
+    HIP_OK(hipMalloc(&h->d_tables, b.bytes.size()));
+    HIP_OK(hipMemcpy(h->d_tables, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice));
+    char *base = (char *)h->d_tables;
+    auto D = [&](size_t o) { return (const double *)(base + o); };
+    auto I = [&](size_t o) { return (const int *)(base + o); };
+    d.sp = D(o_sp); d.rA = D(o_rA); d.rB = D(o_rB); d.rE = D(o_rE); d.rdn = D(o_rdn);
+    d.rinfo = I(o_rinfo); d.rtb = I(o_rtb); d.rfo = I(o_rfo);
+    d.fw_ptr = I(o_fwp); d.fw_sp = I(o_fws); d.fw_nu = D(o_fwn);
+    d.bw_ptr = I(o_bwp); d.bw_sp = I(o_bws); d.bw_nu = D(o_bwn);
+    d.rx_ptr = I(o_rxp); d.rx_sp = I(o_rxs); d.rx_nu = D(o_rxn);
+    d.tb_ptr = I(o_tbp); d.tb_sp = I(o_tbs); d.tb_eff = D(o_tbe);
+    d.fo_type = I(o_fot); d.fo_single = I(o_fos1); d.fo_par = D(o_fopar);
+    d.fo_ptr = I(o_fop); d.fo_sp = I(o_fos); d.fo_eff = D(o_foe);
+    d.f_ptr = I(o_fp); d.f_rx = I(o_frx); d.f_nu = D(o_fnu);
+    return LSM_OK;
+}
+
+int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const lsm_reactor_options *reactor,
+               lsm_handle **out) {
+    if (!out) return fail(LSM_E_INVALID, "null output handle");
+    *out = nullptr;
+    int rc = check_mechanism(mech);
+    if (rc) return rc;
+    lsm_ode_parameters p;
+    if (ode) p = *ode; else lsm_ode_parameters_default(&p);
+    lsm_reactor_options r;
+    if (reactor) r = *reactor; else lsm_reactor_options_default(&r);
+    if (p.type != LSM_ODE_OPENSMOKE_GPU && p.type != LSM_ODE_OPENSMOKE)
+        return fail(LSM_E_UNSUPPORTED, "this library integrates with the OpenSMOKE Gear/BDF method only "
+                                       "(odeSolver OpenSMOKE_GPU or OpenSMOKE)");
+    // MultiValueSolver::SetAbsoluteTolerances / SetRelativeTolerances (:196-214)
+    if (!(p.absolute_tolerance >= 1.e-32)) return fail(LSM_E_INVALID, "The user-defined absolute tolerance is too small");
+    if (!(p.relative_tolerance >= 1.e-32)) return fail(LSM_E_INVALID, "The user-defined relative tolerance is too small");
+    if (strcmp(p.linear_algebra, "Eigen") != 0 && p.linear_algebra[0] != '\0')
+        return fail(LSM_E_UNSUPPORTED, "Requested linear algebra is not supported!");   // KernelDense.hpp:73-81
+    if (!(r.pressure > 0.)) return fail(LSM_E_INVALID, "pressure must be positive");
+    lsm_handle *h = new lsm_handle();
+    h->ode = p;
+    h->reactor = r;
+    if (hipGetDevice(&h->device) != hipSuccess) { delete h; return fail(LSM_E_HIP, "no HIP device"); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) { delete h; return fail(LSM_E_HIP, "hipGetDeviceProperties failed"); }
+    h->num_cu = prop.multiProcessorCount;
+    rc = upload_tables(h, mech);
+    if (rc) { lsm_destroy(h); return rc; }
+    if (hipMalloc(&h->d_queue, 256) != hipSuccess) { lsm_destroy(h); return fail(LSM_E_HIP, "hipMalloc queue"); }
+    KParams &k = h->kp;
+    gear_r(k.bk.r);
+    k.bk.abs_tol = p.absolute_tolerance;
+    k.bk.rel_tol = p.relative_tolerance;
+    k.bk.t0 = r.t0;
+    k.bk.max_steps = p.maximum_number_of_steps > 0 ? (unsigned)p.maximum_number_of_steps : 500000u;
+    k.pressure = r.pressure;
+    k.min_temperature = r.min_temperature;
+    k.energy = r.energy_equation ? 1 : 0;
+    k.full_pivoting = p.full_pivoting ? 1 : 0;
+    // persistent team grid: as many one-wave blocks as fit
+    int occ = 0;
+    const size_t lds = team_lds_bytes(h->dm);
+    if (h->dm.ne <= team_max_neq()) {
+        occ = (int)(prop.sharedMemPerMultiprocessor / std::max<size_t>(lds, 1));
+        occ = std::max(1, std::min(occ, 8));
+    }
+    h->team_grid = std::max(1, occ) * h->num_cu;
+    *out = h;
+    return LSM_OK;
+}
+
+int lsm_set_method(lsm_handle *h, int32_t method) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (method < LSM_METHOD_AUTO || method > LSM_METHOD_TEAM) return fail(LSM_E_INVALID, "unknown method");
+    if (method == LSM_METHOD_TEAM && (h->dm.ne > team_max_neq() || h->kp.full_pivoting))
+        return fail(LSM_E_UNSUPPORTED, "team kernel needs ns+1 <= 64 and partial pivoting");
+    h->method = method;
+    return LSM_OK;
+}
+
+int lsm_destroy(lsm_handle *h) {
+    if (!h) return LSM_OK;
+    if (h->d_tables) (void)hipFree(h->d_tables);
+    if (h->d_queue) (void)hipFree(h->d_queue);
+    if (h->d_work) (void)hipFree(h->d_work);
+    if (h->d_y) (void)hipFree(h->d_y);
+    if (h->d_dt) (void)hipFree(h->d_dt);
+    if (h->d_q) (void)hipFree(h->d_q);
+    if (h->d_status) (void)hipFree(h->d_status);
+    if (h->d_cnt) (void)hipFree(h->d_cnt);
+    delete h;
+    return LSM_OK;
+}
+
+static int pick_method(const lsm_handle *h) {
+    if (h->method != LSM_METHOD_AUTO) return h->method;
+    return (h->dm.ne <= team_max_neq() && !h->kp.full_pivoting) ? LSM_METHOD_TEAM : LSM_METHOD_PLAIN;
+}
+
+int lsm_integrate_batch_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in, double *d_y_out,
+                               int32_t *d_status, double *d_q, int32_t *d_counters, void *stream) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
+    if (n == 0) return LSM_OK;
+    if (!d_dt || !d_y_in || !d_y_out) return fail(LSM_E_INVALID, "null dt / y pointer");
+    if (n > (int64_t)0x7fffffff) return fail(LSM_E_INVALID, "more than 2^31-1 cells in one call");
+    const int method = pick_method(h);
+    if (method == LSM_METHOD_TEAM) {
+        if (launch_reset_queue(h->d_queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
+        int rc = launch_team(h->dm, h->kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue,
+                             h->team_grid, stream);
+        if (rc) return fail(LSM_E_HIP, std::string("team kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
+        return LSM_OK;
+    }
+    // plain: one thread per cell slot, workspace in HBM
+    size_t free_b = 0, total_b = 0;
+    HIP_OK(hipMemGetInfo(&free_b, &total_b));
+    const size_t per = plain_work_doubles(h->dm, 1) * sizeof(double);
+    int64_t slots = std::min<int64_t>(n, (int64_t)h->num_cu * 1024);
+    const size_t budget = std::min<size_t>((size_t)32 << 30, free_b / 4 + (h->d_work ? (size_t)h->work_slots * per : 0));
+    slots = std::max<int64_t>(1, std::min<int64_t>(slots, (int64_t)(budget / per)));
+    if (slots > h->work_slots) {
+        if (h->d_work) { HIP_OK(hipStreamSynchronize((hipStream_t)stream)); (void)hipFree(h->d_work); h->d_work = nullptr; }
+        HIP_OK(hipMalloc(&h->d_work, (size_t)slots * per));
+        h->work_slots = slots;
+    }
+    int rc = launch_plain(h->dm, h->kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_work, h->work_slots,
+                          stream);
+    if (rc) return fail(LSM_E_HIP, std::string("plain kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
+    return LSM_OK;
+}
+
+int lsm_integrate_batch(lsm_handle *h, int64_t n, const double *dt, const double *y_in, double *y_out,
+                        int32_t *status, double *q, int32_t *counters) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
+    if (n == 0) return LSM_OK;
+    if (!dt || !y_in || !y_out) return fail(LSM_E_INVALID, "null dt / y pointer");
+    const int ne = h->dm.ne;
+    if (n > h->cap) {
+        if (h->d_y) (void)hipFree(h->d_y);
+        if (h->d_dt) (void)hipFree(h->d_dt);
+        if (h->d_q) (void)hipFree(h->d_q);
+        if (h->d_status) (void)hipFree(h->d_status);
+        if (h->d_cnt) (void)hipFree(h->d_cnt);
+        h->d_y = h->d_dt = h->d_q = nullptr;
+        h->d_status = h->d_cnt = nullptr;
+        h->cap = 0;
+        HIP_OK(hipMalloc(&h->d_y, sizeof(double) * ne * n));
+        HIP_OK(hipMalloc(&h->d_dt, sizeof(double) * n));
+        HIP_OK(hipMalloc(&h->d_q, sizeof(double) * n));
+        HIP_OK(hipMalloc(&h->d_status, sizeof(int) * n));
+        HIP_OK(hipMalloc(&h->d_cnt, sizeof(int) * LSM_NCOUNTERS * n));
+        h->cap = n;
+    }
+    HIP_OK(hipMemcpy(h->d_y, y_in, sizeof(double) * ne * n, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->d_dt, dt, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (q) HIP_OK(hipMemcpy(h->d_q, q, sizeof(double) * n, hipMemcpyHostToDevice));
+    int rc = lsm_integrate_batch_device(h, n, h->d_dt, h->d_y, h->d_y, status ? h->d_status : nullptr,
+                                        q ? h->d_q : nullptr, counters ? h->d_cnt : nullptr, nullptr);
+    if (rc) return rc;
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(y_out, h->d_y, sizeof(double) * ne * n, hipMemcpyDeviceToHost));
+    if (status) HIP_OK(hipMemcpy(status, h->d_status, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (q) HIP_OK(hipMemcpy(q, h->d_q, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (counters) HIP_OK(hipMemcpy(counters, h->d_cnt, sizeof(int) * LSM_NCOUNTERS * n, hipMemcpyDeviceToHost));
+    return LSM_OK;
+}
+
+}  // extern "C"

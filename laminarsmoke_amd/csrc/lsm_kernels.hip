@@ -1,0 +1,637 @@
+// lsm_kernels.hip - CDNA4 (gfx950) kernels of the chemistry step.
+//
+//  * team kernel  : one cell per 64-lane wavefront, persistent work queue.  Lane
+//                   k owns species k (element k of every vector), the reaction
+//                   loop is strided over lanes, the Newton matrix is factorised
+//                   in registers (lane i holds row i) with the pivot row
+//                   broadcast by v_readlane, the Jacobian lives in LDS.
+//  * plain kernel : one cell per thread, all per-cell state in HBM with the
+//                   cell index on the fast axis (coalesced across the wave).
+// Both run the same Gear/BDF driver (lsm_bdf.hpp) and the same reaction-rate
+// body, which restates KineticsMap_CHEMKIN::KineticConstants/ReactionRates
+// (maps/KineticsMap_CHEMKIN.hpp:1192-1386, 1683-1768) for one reaction.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "../../include/lsm_mechanism.h"
+#include "lsm_internal.hpp"
+
+namespace lsm {
+
+constexpr int LSM_NCOUNTERS_DEV = 12;
+constexpr double RGAS = LSM_R_J_KMOL;
+constexpr double NEG_INF = -__builtin_huge_val();
+constexpr double POS_INF = __builtin_huge_val();
+
+// ---------------------------------------------------------------------------
+// wavefront helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double uniform_d(double x) {
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double readlane_d(double x, int l) {
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    return uniform_d(x);
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long y = __shfl_xor(x, o, 64);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// c^nu by stoichiometric class (StoichiometricMap.hpp:537-552)
+__device__ __forceinline__ double conc_pow(double c, double nu) {
+    if (nu == 1.) return c;
+    if (nu == 2.) return c * c;
+    if (nu == 3.) return (c * c) * c;
+    if (nu == 0.5) return sqrt(c);
+    return pow(c, nu);
+}
+
+// Net rate of reaction j [kmol/m3/s].  C, H, S return c_k, h_k/RT, s_k/R.
+template <class FC, class FH, class FS>
+__device__ __forceinline__ double reaction_rate(const DevMech &M, int j, double T, double logT, double uT,
+                                                double logPRT, double cTot, FC C, FH H, FS S) {
+    // Arrhenius (hpp:1198-1212)
+    double kf = exp(M.rA[j] + M.rB[j] * logT - M.rE[j] * uT);
+    const int info = M.rinfo[j];
+    if (info & LSM_RXN_NEGATIVE_A) kf = -kf;
+    double kmod = kf;
+    // third body (hpp:1388-1398, 1703-1708)
+    const int tb = M.rtb[j];
+    if (tb >= 0) {
+        double Me = cTot;
+        for (int q = M.tb_ptr[tb]; q < M.tb_ptr[tb + 1]; q++) Me += C(M.tb_sp[q]) * M.tb_eff[q];
+        kmod *= Me;
+    }
+    // fall-off (hpp:1256-1290, 1496-1550, 1710-1716)
+    const int fo = M.rfo[j];
+    if (fo >= 0) {
+        const double *fp = M.fo_par + 8 * fo;
+        const double kinf = exp(fp[0] + fp[1] * logT - fp[2] * uT);
+        double Mc;
+        if (M.fo_single[fo] < 0) {
+            Mc = cTot;
+            for (int q = M.fo_ptr[fo]; q < M.fo_ptr[fo + 1]; q++) Mc += C(M.fo_sp[q]) * M.fo_eff[q];
+        } else {
+            Mc = C(M.fo_single[fo]) + 1.e-16;
+        }
+        const double Pr = kf * Mc / kinf;
+        double wF = 1.;
+        if (M.fo_type[fo] == LSM_FALLOFF_TROE) {
+            const double a = fp[3];
+            double F = (1. - a) * exp(-T / fp[4]) + a * exp(-T / fp[5]);
+            if (fp[6] != 0.) F += exp(-fp[6] / T);
+            const double lf = (F < 1.e-300) ? -300. : log10(F);
+            double q;
+            if (Pr > 1.e-32) {
+                const double nT = 0.75 - 1.27 * lf;
+                const double cT = -0.4 - 0.67 * lf;
+                const double sT = log10(Pr) + cT;
+                q = sT / (nT - 0.14 * sT);
+            } else {
+                q = 1. / 0.14;
+            }
+            wF = pow(10., lf / (1. + q * q));
+        }
+        kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
+    }
+    // products of concentrations (StoichiometricMap.hpp:512-598)
+    double pf = 1.;
+    for (int q = M.fw_ptr[j]; q < M.fw_ptr[j + 1]; q++) pf *= conc_pow(C(M.fw_sp[q]), M.fw_nu[q]);
+    double net = pf;
+    if (info & LSM_RXN_REVERSIBLE) {
+        // reaction dH/RT, dS/R (StoichiometricMap.hpp:916-1001) and 1/Kc (hpp:1215-1222)
+        double dh = 0., ds = 0.;
+        for (int q = M.rx_ptr[j]; q < M.rx_ptr[j + 1]; q++) {
+            const int k = M.rx_sp[q];
+            const double nu = M.rx_nu[q];
+            dh += nu * H(k);
+            ds += nu * S(k);
+        }
+        const double uKeq = exp(-ds + dh - logPRT * M.rdn[j]);
+        double pb = 1.;
+        for (int q = M.bw_ptr[j]; q < M.bw_ptr[j + 1]; q++) pb *= conc_pow(C(M.bw_sp[q]), M.bw_nu[q]);
+        net -= pb * uKeq;
+    }
+    return net * kmod;
+}
+
+// NASA-7 cp/R, h/RT, s/R of species k (ThermodynamicsMap_CHEMKIN.hpp:436-496)
+__device__ __forceinline__ void nasa7(const DevMech &M, int k, double T, double T2, double T3, double T4,
+                                      double uT, double logT, double &cp, double &h, double &s) {
+    const double *sp = M.sp;
+    const int P = M.ns_pad;
+    const bool hi = T > sp[SP_TMID * P + k];
+    const int rc = hi ? SP_CP_HT : SP_CP_LT, rh = hi ? SP_DH_HT : SP_DH_LT, rs = hi ? SP_DS_HT : SP_DS_LT;
+    cp = sp[rc * P + k] + T * sp[(rc + 1) * P + k] + T2 * sp[(rc + 2) * P + k] + T3 * sp[(rc + 3) * P + k] +
+         T4 * sp[(rc + 4) * P + k];
+    h = sp[rh * P + k] + T * sp[(rh + 1) * P + k] + T2 * sp[(rh + 2) * P + k] + T3 * sp[(rh + 3) * P + k] +
+        T4 * sp[(rh + 4) * P + k] + uT * sp[(rh + 5) * P + k];
+    s = sp[rs * P + k] * logT + T * sp[(rs + 1) * P + k] + T2 * sp[(rs + 2) * P + k] +
+        T3 * sp[(rs + 3) * P + k] + T4 * sp[(rs + 4) * P + k] + sp[(rs + 5) * P + k];
+}
+
+// ---------------------------------------------------------------------------
+// Team context: one cell per wavefront (requires ne <= NMAX <= 64)
+// ---------------------------------------------------------------------------
+template <int NMAX>
+struct TeamCtx {
+    double V[NVEC][1];
+    double G[NMAX];
+    int ps;          // elimination step at which this lane's row became the pivot
+    double dinv;     // 1 / pivot of this lane's row
+    double *Jl, *sc, *sh, *ss, *sr;
+    const DevMech &M;
+    const KParams &K;
+    int lane, ne, ns, nr;
+    double qr;
+
+    __device__ TeamCtx(const DevMech &m, const KParams &k, double *lds)
+        : M(m), K(k), lane(threadIdx.x & 63), ne(m.ne), ns(m.ns), nr(m.nr), qr(0.) {
+        Jl = lds;
+        sc = Jl + 64 * m.ne;
+        sh = sc + 64;
+        ss = sh + 64;
+        sr = ss + 64;
+    }
+
+    template <class F>
+    __device__ __forceinline__ void each(F f) { f(0); }
+    __device__ __forceinline__ double &x(int vid, int e) { return V[vid][e]; }
+    __device__ __forceinline__ double ymin(int) const {
+        return lane < ns ? 0. : (lane == ns ? 200. : NEG_INF);
+    }
+    __device__ __forceinline__ double ymax(int) const {
+        return lane < ns ? 1. : (lane == ns ? 6000. : POS_INF);
+    }
+    __device__ __forceinline__ double reduce(double v) { return wave_sum(v); }
+    __device__ __forceinline__ int count(bool b) { return __popcll(__ballot(b)); }
+
+    // BatchReactorHomogeneousConstantPressure::Equations (:175-247)
+    __device__ __forceinline__ void rhs(int src, int dst) {
+        const double T = readlane_d(V[src][0], ns);
+        const int k = lane;
+        const bool isp = k < ns;
+        const double Yk = isp ? V[src][0] : 0.;
+        const double mwk = isp ? M.sp[SP_MW * M.ns_pad + k] : 1.;
+        const double MW = 1. / wave_sum(Yk / mwk);
+        const double xk = (MW * Yk) / mwk;
+        const double cTotP = K.pressure / RGAS / T;
+        const double ck = cTotP * xk;
+        const double rho = cTotP * MW;
+        const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, uT = 1. / T, logT = log(T);
+        double cp = 0., h = 0., s = 0.;
+        if (isp) {
+            nasa7(M, k, T, T2, T3, T4, uT, logT, cp, h, s);
+            sc[k] = ck;
+            sh[k] = h;
+            ss[k] = s;
+        }
+        const double cTot = wave_sum(ck);
+        const double cpx = wave_sum(isp ? cp * xk : 0.);
+        wave_sync();
+        const double logPRT = log(101325. / RGAS / T);
+        for (int j = lane; j < nr; j += 64) {
+            sr[j] = reaction_rate(
+                M, j, T, logT, uT, logPRT, cTot, [&](int i) { return sc[i]; }, [&](int i) { return sh[i]; },
+                [&](int i) { return ss[i]; });
+        }
+        wave_sync();
+        double Rk = 0.;
+        if (isp)
+            for (int e = M.f_ptr[k]; e < M.f_ptr[k + 1]; e++) Rk += M.f_nu[e] * sr[M.f_rx[e]];
+        double dy = isp ? (mwk * Rk) / rho : 0.;
+        if (K.energy) {
+            const double hs = wave_sum(isp ? Rk * h : 0.);
+            qr = -hs * RGAS * T;
+            const double CpMixMass = (cpx * RGAS) / MW;
+            if (k == ns) dy = qr / (rho * CpMixMass);
+        }
+        wave_sync();
+        V[dst][0] = dy;
+    }
+
+    // KernelDense::NumericalJacobian, max-constraint branch (KernelDense.hpp:160-183)
+    __device__ __forceinline__ void jacobian(double h) {
+        const double ZERO_DER = sqrt(TINY_F), ETA2 = sqrt(EPS_D), BETA = 1.e+3 * EPS_D;
+        double fw = V[F_][0] * V[OOE_][0];
+        double hf = BETA * fabs(h) * sqrt(wave_sum(fw * fw) / (double)ne) * (double)ne;
+        if (hf < 1.e-10) hf = 1.;
+        for (int j = 0; j < ne; j++) {
+            const double yh = readlane_d(V[VB_][0], j);
+            const double ej = readlane_d(V[OOE_][0], j);
+            const double a = fabs(yh), b = fabs(1. / ej);
+            double hJ = ETA2 * (a > b ? a : b);
+            hJ = fmax(hJ, hf / ej);
+            hJ = fmax(hJ, ZERO_DER);
+            const double ymx = j < ns ? 1. : 6000.;
+            if (yh + hJ > ymx) hJ = -hJ;
+            if (lane == j) V[VB_][0] = yh + hJ;
+            rhs(VB_, AUX_);
+            if (lane == j) V[VB_][0] = yh;
+            const double hInv = 1. / hJ;
+            Jl[j * 64 + lane] = hInv * (V[AUX_][0] - V[F_][0]);
+        }
+    }
+
+    // G = I - hr0 J and row-pivoted LU in registers (KernelDense.hpp:196-221).
+    // Rows are not moved: lane i keeps row i and records the step ps at which it
+    // was chosen as pivot; the largest |G(i,k)| among rows not yet chosen wins
+    // (ties and the 6 lowest mantissa bits resolved toward the lower lane).
+    __device__ __forceinline__ void factor(double hr0) {
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < NMAX; j++) {
+            const double jij = (j < ne) ? Jl[j * 64 + lane] : 0.;
+            G[j] = jij * -hr0 + ((lane == j) ? 1. : 0.);
+        }
+        ps = (lane < ne) ? -1 : 1 << 20;
+        dinv = 1.;
+#pragma unroll
+        for (int k = 0; k < NMAX; k++) {
+            if (k < ne) {
+                const bool elig = ps < 0;
+                const unsigned long long ab = (unsigned long long)__double_as_longlong(fabs(G[k]));
+                unsigned long long key = elig ? ((ab & ~63ull) | (unsigned long long)(63 - lane)) : 0ull;
+                key = wave_max_u64(key);
+                int p = 63 - (int)(key & 63ull);
+                if (key == 0ull) p = __ffsll((unsigned long long)__ballot(elig)) - 1;
+                p = __builtin_amdgcn_readfirstlane(p);
+                const double piv = readlane_d(G[k], p);
+                const double inv = 1. / piv;
+                if (lane == p) { ps = k; dinv = inv; }
+                double l = 0.;
+                if (ps < 0 && piv != 0.) {
+                    l = G[k] * inv;
+                    G[k] = l;
+                }
+#pragma unroll
+                for (int j = k + 1; j < NMAX; j++) {
+                    const double u = readlane_d(G[j], p);
+                    G[j] = fma(-l, u, G[j]);
+                }
+            }
+        }
+    }
+
+    // Forward / backward substitution in pivot order.
+    __device__ __forceinline__ void solve(int vid) {
+        double b = V[vid][0];
+#pragma unroll
+        for (int k = 0; k < NMAX; k++) {
+            if (k < ne) {
+                const int p = __ffsll((unsigned long long)__ballot(ps == k)) - 1;
+                const double yk = readlane_d(b, p);
+                if (ps > k) b = fma(-G[k], yk, b);
+            }
+        }
+        double xr = 0.;
+#pragma unroll
+        for (int k = NMAX - 1; k >= 0; k--) {
+            if (k < ne) {
+                const int p = __ffsll((unsigned long long)__ballot(ps == k)) - 1;
+                const double xk = readlane_d(b * dinv, p);
+                if (ps < k) b = fma(-G[k], xk, b);
+                if (lane == k) xr = xk;
+            }
+        }
+        V[vid][0] = xr;
+    }
+
+    __device__ __forceinline__ void jac_times(int src, int dst) {
+        const double v = V[src][0];
+        double acc = 0.;
+        for (int j = 0; j < ne; j++) acc += Jl[j * 64 + lane] * readlane_d(v, j);
+        V[dst][0] = acc;
+    }
+};
+
+template <int NMAX>
+__global__ void __launch_bounds__(64) team_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
+                                                  const double *yin, double *yout, int *status, double *q,
+                                                  int *counters, unsigned *queue) {
+    extern __shared__ double lds[];
+    TeamCtx<NMAX> c(M, K, lds);
+    const int lane = c.lane, ns = M.ns, ne = M.ne;
+    for (;;) {
+        unsigned cell_u = 0;
+        if (lane == 0) cell_u = atomicAdd(queue, 1u);
+        cell_u = __shfl(cell_u, 0, 64);
+        const int64_t cell = (int64_t)__builtin_amdgcn_readfirstlane((int)cell_u);
+        if (cell >= n) break;
+        const double yi = lane < ne ? yin[(int64_t)lane * n + cell] : 0.;
+        const double T0 = readlane_d(yi, ns);
+        double y;
+        int st = 0;
+        BdfCounters cnt = BdfCounters{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (T0 > K.min_temperature) {                                  // chemistry_DI.H:97
+            double y0 = yi;                                            // :101-128
+            if (lane < ns && y0 < 0.) y0 = 0.;
+            const double sum = wave_sum(lane < ns ? y0 : 0.);
+            if (lane < ns) y0 /= sum;
+            c.V[Y0_][0] = lane < ne ? y0 : 0.;
+            st = bdf_integrate(c, K.bk, K.bk.t0 + dt[cell], cnt);       // :138-157
+            y = st > 0 ? c.V[Y_][0] : c.V[Y0_][0];
+            if (st > 0 && q != nullptr && lane == 0) q[cell] = c.qr;    // :168
+        } else {
+            y = yi;
+        }
+        const double sum = wave_sum(lane < ns ? y : 0.);               // :219 normalizeMassFractions
+        if (lane < ns) y /= sum;
+        if (!K.energy && lane == ns) y = yi;
+        if (lane < ne) yout[(int64_t)lane * n + cell] = y;
+        if (lane == 0) {
+            if (status) status[cell] = st;
+        }
+        if (counters != nullptr && lane < LSM_NCOUNTERS_DEV) {
+            const unsigned vals[12] = {cnt.steps, cnt.fcalls, cnt.fcalls_jac, cnt.jacobians, cnt.factorizations,
+                                       cnt.solves, cnt.conv_failures, cnt.err_failures, cnt.max_order,
+                                       cnt.rhs_total, cnt.clamps, cnt.newton};
+            unsigned v = 0;
+#pragma unroll
+            for (int i = 0; i < 12; i++)
+                if (lane == i) v = vals[i];
+            counters[cell * LSM_NCOUNTERS_DEV + lane] = (int)v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain context: one cell per thread, all state in HBM
+// ---------------------------------------------------------------------------
+struct PlainCtx {
+    double *W;        // vectors  [NVEC][ne][slots]
+    double *Jm, *Gm;  // matrices [ne][ne][slots]
+    int *piv;         // [2][ne][slots] row / column permutation
+    double *sc, *sh, *ss, *sr;
+    const DevMech &M;
+    const KParams &K;
+    int64_t S;        // stride (slots)
+    int64_t slot;
+    int ne, ns, nr;
+    double qr;
+
+    __device__ PlainCtx(const DevMech &m, const KParams &k, double *work, int64_t slots, int64_t sl)
+        : M(m), K(k), S(slots), slot(sl), ne(m.ne), ns(m.ns), nr(m.nr), qr(0.) {
+        W = work;
+        Jm = W + (int64_t)NVEC * ne * S;
+        Gm = Jm + (int64_t)ne * ne * S;
+        sc = Gm + (int64_t)ne * ne * S;
+        sh = sc + (int64_t)ns * S;
+        ss = sh + (int64_t)ns * S;
+        sr = ss + (int64_t)ns * S;
+        piv = (int *)(sr + (int64_t)nr * S);
+    }
+
+    template <class F>
+    __device__ __forceinline__ void each(F f) {
+        for (int e = 0; e < ne; e++) f(e);
+    }
+    __device__ __forceinline__ double &x(int vid, int e) { return W[((int64_t)vid * ne + e) * S + slot]; }
+    __device__ __forceinline__ double ymin(int e) const { return e < ns ? 0. : 200.; }
+    __device__ __forceinline__ double ymax(int e) const { return e < ns ? 1. : 6000.; }
+    __device__ __forceinline__ double reduce(double v) { return v; }
+    __device__ __forceinline__ int count(bool b) { return b ? 1 : 0; }
+    __device__ __forceinline__ double &J(int i, int j) { return Jm[((int64_t)i * ne + j) * S + slot]; }
+    __device__ __forceinline__ double &Gv(int i, int j) { return Gm[((int64_t)i * ne + j) * S + slot]; }
+    __device__ __forceinline__ int &P(int w, int i) { return piv[((int64_t)w * ne + i) * S + slot]; }
+
+    __device__ void rhs(int src, int dst) {
+        const double T = x(src, ns);
+        double u = 0.;
+        for (int k = 0; k < ns; k++) u += x(src, k) / M.sp[SP_MW * M.ns_pad + k];
+        const double MW = 1. / u;
+        const double cTotP = K.pressure / RGAS / T;
+        const double rho = cTotP * MW;
+        const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, uT = 1. / T, logT = log(T);
+        double cTot = 0., cpx = 0.;
+        for (int k = 0; k < ns; k++) {
+            const double xk = (MW * x(src, k)) / M.sp[SP_MW * M.ns_pad + k];
+            const double ck = cTotP * xk;
+            double cp, h, s;
+            nasa7(M, k, T, T2, T3, T4, uT, logT, cp, h, s);
+            sc[k * S + slot] = ck;
+            sh[k * S + slot] = h;
+            ss[k * S + slot] = s;
+            cTot += ck;
+            cpx += cp * xk;
+        }
+        const double logPRT = log(101325. / RGAS / T);
+        for (int j = 0; j < nr; j++)
+            sr[(int64_t)j * S + slot] = reaction_rate(
+                M, j, T, logT, uT, logPRT, cTot, [&](int i) { return sc[(int64_t)i * S + slot]; },
+                [&](int i) { return sh[(int64_t)i * S + slot]; }, [&](int i) { return ss[(int64_t)i * S + slot]; });
+        double hs = 0.;
+        for (int k = 0; k < ns; k++) {
+            double Rk = 0.;
+            for (int e = M.f_ptr[k]; e < M.f_ptr[k + 1]; e++) Rk += M.f_nu[e] * sr[(int64_t)M.f_rx[e] * S + slot];
+            x(dst, k) = (M.sp[SP_MW * M.ns_pad + k] * Rk) / rho;
+            hs += Rk * sh[k * S + slot];
+        }
+        double dT = 0.;
+        if (K.energy) {
+            qr = -hs * RGAS * T;
+            dT = qr / (rho * ((cpx * RGAS) / MW));
+        }
+        x(dst, ns) = dT;
+    }
+
+    __device__ void jacobian(double h) {
+        const double ZERO_DER = sqrt(TINY_F), ETA2 = sqrt(EPS_D), BETA = 1.e+3 * EPS_D;
+        double acc = 0.;
+        for (int e = 0; e < ne; e++) { const double t = x(F_, e) * x(OOE_, e); acc += t * t; }
+        double hf = BETA * fabs(h) * sqrt(acc / (double)ne) * (double)ne;
+        if (hf < 1.e-10) hf = 1.;
+        for (int j = 0; j < ne; j++) {
+            const double yh = x(VB_, j);
+            const double ej = x(OOE_, j);
+            const double a = fabs(yh), b = fabs(1. / ej);
+            double hJ = ETA2 * (a > b ? a : b);
+            hJ = fmax(hJ, hf / ej);
+            hJ = fmax(hJ, ZERO_DER);
+            if (yh + hJ > ymax(j)) hJ = -hJ;
+            x(VB_, j) = yh + hJ;
+            rhs(VB_, AUX_);
+            x(VB_, j) = yh;
+            const double hInv = 1. / hJ;
+            for (int i = 0; i < ne; i++) J(i, j) = hInv * (x(AUX_, i) - x(F_, i));
+        }
+    }
+
+    // Eigen PartialPivLU / FullPivLU restated (KernelDense.hpp:196-221).
+    __device__ void factor(double hr0) {
+        for (int i = 0; i < ne; i++)
+            for (int j = 0; j < ne; j++) Gv(i, j) = J(i, j) * -hr0 + (i == j ? 1. : 0.);
+        for (int i = 0; i < ne; i++) { P(0, i) = i; P(1, i) = i; }
+        const bool full = K.full_pivoting;
+        for (int k = 0; k < ne; k++) {
+            int p = k, qc = k;
+            double best = -1.;
+            if (!full) {
+                for (int i = k; i < ne; i++) { const double a = fabs(Gv(i, k)); if (a > best) { best = a; p = i; } }
+            } else {
+                for (int j = k; j < ne; j++)
+                    for (int i = k; i < ne; i++) { const double a = fabs(Gv(i, j)); if (a > best) { best = a; p = i; qc = j; } }
+            }
+            if (p != k) {
+                for (int j = 0; j < ne; j++) { const double t = Gv(k, j); Gv(k, j) = Gv(p, j); Gv(p, j) = t; }
+                const int t = P(0, k); P(0, k) = P(0, p); P(0, p) = t;
+            }
+            if (full && qc != k) {
+                for (int i = 0; i < ne; i++) { const double t = Gv(i, k); Gv(i, k) = Gv(i, qc); Gv(i, qc) = t; }
+                const int t = P(1, k); P(1, k) = P(1, qc); P(1, qc) = t;
+            }
+            const double pv = Gv(k, k);
+            if (pv == 0.) continue;
+            for (int i = k + 1; i < ne; i++) {
+                const double l = Gv(i, k) / pv;
+                Gv(i, k) = l;
+                for (int j = k + 1; j < ne; j++) Gv(i, j) -= l * Gv(k, j);
+            }
+        }
+    }
+
+    __device__ void solve(int vid) {
+        // t = P b (AUX as scratch), L t = t, U t = t, b = Q t
+        for (int i = 0; i < ne; i++) x(AUX_, i) = x(vid, P(0, i));
+        for (int i = 0; i < ne; i++) {
+            double t = x(AUX_, i);
+            for (int k = 0; k < i; k++) t -= Gv(i, k) * x(AUX_, k);
+            x(AUX_, i) = t;
+        }
+        for (int i = ne - 1; i >= 0; i--) {
+            double t = x(AUX_, i);
+            for (int k = i + 1; k < ne; k++) t -= Gv(i, k) * x(AUX_, k);
+            x(AUX_, i) = t / Gv(i, i);
+        }
+        if (K.full_pivoting) for (int i = 0; i < ne; i++) x(vid, P(1, i)) = x(AUX_, i);
+        else for (int i = 0; i < ne; i++) x(vid, i) = x(AUX_, i);
+    }
+
+    __device__ void jac_times(int src, int dst) {
+        for (int i = 0; i < ne; i++) {
+            double acc = 0.;
+            for (int j = 0; j < ne; j++) acc += J(i, j) * x(src, j);
+            x(dst, i) = acc;
+        }
+    }
+};
+
+__global__ void __launch_bounds__(256) plain_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
+                                                    const double *yin, double *yout, int *status, double *q,
+                                                    int *counters, double *work, int64_t slots) {
+    const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= slots) return;
+    PlainCtx c(M, K, work, slots, slot);
+    const int ns = M.ns, ne = M.ne;
+    for (int64_t cell = slot; cell < n; cell += slots) {
+        const double T0 = yin[(int64_t)ns * n + cell];
+        int st = 0;
+        BdfCounters cnt = BdfCounters{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (T0 > K.min_temperature) {
+            double sum = 0.;
+            for (int i = 0; i < ns; i++) {
+                const double v = yin[(int64_t)i * n + cell];
+                c.x(Y0_, i) = v < 0. ? 0. : v;
+                sum += c.x(Y0_, i);
+            }
+            for (int i = 0; i < ns; i++) c.x(Y0_, i) /= sum;
+            c.x(Y0_, ns) = T0;
+            st = bdf_integrate(c, K.bk, K.bk.t0 + dt[cell], cnt);
+            const int src = st > 0 ? Y_ : Y0_;
+            for (int i = 0; i < ne; i++) c.x(AUX_, i) = c.x(src, i);
+            if (st > 0 && q != nullptr) q[cell] = c.qr;
+        } else {
+            for (int i = 0; i < ne; i++) c.x(AUX_, i) = yin[(int64_t)i * n + cell];
+        }
+        double sum = 0.;
+        for (int i = 0; i < ns; i++) sum += c.x(AUX_, i);
+        for (int i = 0; i < ns; i++) yout[(int64_t)i * n + cell] = c.x(AUX_, i) / sum;
+        yout[(int64_t)ns * n + cell] = K.energy ? c.x(AUX_, ns) : T0;
+        if (status) status[cell] = st;
+        if (counters) {
+            int *o = counters + cell * LSM_NCOUNTERS_DEV;
+            o[0] = cnt.steps; o[1] = cnt.fcalls; o[2] = cnt.fcalls_jac; o[3] = cnt.jacobians;
+            o[4] = cnt.factorizations; o[5] = cnt.solves; o[6] = cnt.conv_failures; o[7] = cnt.err_failures;
+            o[8] = cnt.max_order; o[9] = cnt.rhs_total; o[10] = cnt.clamps; o[11] = cnt.newton;
+        }
+    }
+}
+
+__global__ void reset_queue_kernel(unsigned *queue) { *queue = 0u; }
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+int team_max_neq() { return 64; }
+
+size_t team_lds_bytes(const DevMech &m) {
+    return sizeof(double) * ((size_t)64 * m.ne + 3 * 64 + (size_t)m.nr);
+}
+
+template <int NMAX>
+static int launch_team_n(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
+                         double *yout, int *status, double *q, int *counters, unsigned *queue, int grid,
+                         hipStream_t s) {
+    const size_t lds = team_lds_bytes(m);
+    hipLaunchKernelGGL(team_kernel<NMAX>, dim3(grid), dim3(64), lds, s, m, k, n, dt, yin, yout, status, q,
+                       counters, queue);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
+                int *status, double *q, int *counters, unsigned *queue, int grid, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (m.ne <= 16) return launch_team_n<16>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, s);
+    if (m.ne <= 32) return launch_team_n<32>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, s);
+    if (m.ne <= 48) return launch_team_n<48>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, s);
+    if (m.ne <= 56) return launch_team_n<56>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, s);
+    if (m.ne <= 64) return launch_team_n<64>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, s);
+    return -2;
+}
+
+size_t plain_work_doubles(const DevMech &m, int64_t slots) {
+    const int64_t ne = m.ne;
+    const int64_t per = (int64_t)NVEC * ne + 2 * ne * ne + 3 * (int64_t)m.ns + m.nr + ne;  // piv: 2 ints = 1 double
+    return (size_t)(per * slots);
+}
+
+int launch_plain(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
+                 int *status, double *q, int *counters, double *work, int64_t slots, void *stream) {
+    const int bs = 256;
+    const int grid = (int)((slots + bs - 1) / bs);
+    hipLaunchKernelGGL(plain_kernel, dim3(grid), dim3(bs), 0, (hipStream_t)stream, m, k, n, dt, yin, yout, status,
+                       q, counters, work, slots);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_reset_queue(unsigned *queue, void *stream) {
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, queue);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace lsm
