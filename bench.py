@@ -1,0 +1,140 @@
+#!/usr/bin/env python3
+"""Chemistry-step benchmark on one MI355X.
+
+Workload: the seeded 53-species / 325-reaction mechanism on the 512x512 coflow-like
+field (262144 cells, 60% cold 300-600 K, 40% flame 1200-2200 K), constant-pressure
+adiabatic integration over dt = 1e-5 s at rtol 1e-7, atol 1e-12.  A "step" is one
+chemistry step over all cells (the batched forAll loop of chemistry_DI.H); every
+step integrates the same input field, which stays resident in HBM.
+
+Prints one JSON line: cell-integrations/s, p50/p99 batch latency, the roofline of
+the integrator kernel (algorithmic FP64 operations from the per-cell counters /
+kernel time from HIP events on the launch stream) and the CPU baseline (the
+oracle on a bounded, stratified subsample of the same cells, on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-ODE integrations/s per chemistry step at stated rtol; p50/p99 batch latency"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_baseline(mech, y, n_sample, threads, dt, rtol, atol):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.build()
+    idx = np.linspace(0, y.shape[1] - 1, n_sample).astype(np.int64)
+    ys = np.ascontiguousarray(y[:, idx])
+    t0 = time.perf_counter()
+    _, st, _, cnt = pyoracle.chemistry_step(mech, ys, dt, rtol=rtol, atol=atol, threads=threads)
+    el = time.perf_counter() - t0
+    return dict(value=n_sample / el, unit="cell-integrations/s", cores=threads, kind="port",
+                sample=f"{n_sample} cells (every {y.shape[1] // n_sample}-th cell of the same field), "
+                       f"{el:.1f} s, oracle C restatement, {threads} thread(s), host {platform.processor() or platform.machine()} "
+                       f"with {os.cpu_count()} logical CPUs")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=None, help="subsample of the field (default: all 262144)")
+    ap.add_argument("--method", default="team", choices=["team", "plain", "auto"])
+    ap.add_argument("--cpu-sample", type=int, default=96)
+    ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(cells shard with no exchange; see the design notes on sharding)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from laminarsmoke_amd import api, workmodel
+    from laminarsmoke_amd.fields import workload_cells, WORKLOADS
+
+    w = WORKLOADS[2]
+    rtol, atol, dt = w["rtol"], w["atol"], w["dt"]
+    mech, y = workload_cells(2, n=args.cells)
+    n = y.shape[1]
+    dev = torch.device("cuda:0")
+    y_in = torch.from_numpy(y).to(dev)
+    dts = torch.full((n,), dt, dtype=torch.float64, device=dev)
+    y_out = torch.empty_like(y_in)
+    st = torch.zeros(n, dtype=torch.int32, device=dev)
+    q = torch.zeros(n, dtype=torch.float64, device=dev)
+    cnt = torch.zeros((n, 12), dtype=torch.int32, device=dev)
+
+    ode = api.ODE_Parameters()
+    ode.SetType("OpenSMOKE_GPU")
+    ode.SetRelativeTolerance(rtol)
+    ode.SetAbsoluteTolerance(atol)
+    step = api.ChemistryStep(mech, ode, method=args.method)
+    stream = torch.cuda.current_stream(dev)
+
+    def run():
+        step.integrate_device(n, dts.data_ptr(), y_in.data_ptr(), y_out.data_ptr(), st.data_ptr(), q.data_ptr(),
+                              cnt.data_ptr(), stream.cuda_stream)
+
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        run()
+        b.record(stream)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    lat_ms = np.array([a.elapsed_time(b) for a, b in evs])
+
+    status = st.cpu().numpy()
+    counters = cnt.cpu().numpy()
+    work = workmodel.flops_from_counters(mech, counters)
+    kernel_s = float(lat_ms.mean()) / 1e3
+    achieved_tflops = work["flops"] / kernel_s / 1e12
+    value = n * args.steps / total
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-integrations/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded mechanism and cell field)",
+        "config": {"workload": w["name"] if args.cells is None else f"{w['name']}-sub{n}",
+                   "mechanism": f"{mech.ns} species / {mech.nr} reactions", "cells": n, "dt_s": dt,
+                   "rtol": rtol, "atol": atol, "method": args.method},
+        "p50_batch_ms": float(np.percentile(lat_ms, 50)), "p99_batch_ms": float(np.percentile(lat_ms, 99)),
+        "roofline": {"bound": "mfma", "pipe": "fp64", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": None,
+                     "algorithmic_flops_per_launch": work["flops"],
+                     "special_ops_per_launch": work["special_ops"],
+                     "kernel_ms": kernel_s * 1e3},
+        "extra": {"status_counts": {str(k): int(v) for k, v in zip(*np.unique(status, return_counts=True))},
+                  "steps_per_cell_p50": float(np.median(counters[:, 0])),
+                  "steps_per_cell_max": int(counters[:, 0].max()),
+                  "rhs_per_cell_mean": float(counters[:, 9].mean()),
+                  "min_hbm_GBps": workmodel.hbm_bytes_min(mech, n) / kernel_s / 1e9},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(mech, y, min(args.cpu_sample, n), args.cpu_threads, dt, rtol, atol)
+    step.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
