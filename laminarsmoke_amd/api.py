@@ -21,7 +21,7 @@ import numpy as np
 
 from ._abi import LsmOdeParameters, LsmReactorOptions, N_COUNTERS, COUNTER_NAMES
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblsm_hip.so")
+LIB_PATH = os.environ.get("LSM_LIB_PATH") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "liblsm_hip.so")
 
 ODE_INTEGRATOR = {"OpenSMOKE": 0, "BzzOde": 1, "CVODE": 2, "DVODE": 3, "DASPK": 4, "DLSODE": 5, "DLSODA": 6,
                   "RADAU5": 7, "MEBDF": 8, "CHEMEQ2": 9, "OpenSMOKE_GPU": 100}

@@ -222,6 +222,7 @@ struct lsm_handle {
     int *d_status = nullptr, *d_cnt = nullptr;
     int64_t cap = 0;
     int team_grid = 0;
+    int team_wpb = 0;
 };
 
 extern "C" {
@@ -321,6 +322,92 @@ static int check_mechanism(const lsm_mechanism *m) {
             if (m->fo_sp[q] < 0 || m->fo_sp[q] >= m->ns) return fail(LSM_E_INVALID, "fall-off species index");
     }
     return LSM_OK;
+}
+
+}  // extern "C"
+
+// Compact LDS image of the mechanism for the team kernel (layout: lsm_internal.hpp).
+struct Image {
+    std::vector<char> b;
+    template <class T>
+    int put(const std::vector<T> &v) {
+        const size_t off = (b.size() + 15) & ~size_t(15);
+        b.resize(off + v.size() * sizeof(T), 0);
+        if (!v.empty()) memcpy(b.data() + off, v.data(), v.size() * sizeof(T));
+        return (int)off;
+    }
+};
+
+static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, const std::vector<int> &tb_of,
+                        const std::vector<int> &fo_of, int ns_pad, LdsImage &L, std::vector<char> &out) {
+    const int ns = m->ns, nr = m->nr;
+    L = LdsImage{};
+    if (ns > 63 || m->ntb > 254 || m->nfo > 254) return false;
+    Image im;
+    std::vector<double> sp((size_t)LSP_ROWS * ns_pad, 0.);
+    for (int k = 0; k < ns; k++) {
+        const double *c = m->nasa + LSM_NASA_STRIDE * k;
+        for (int i = 0; i < 7; i++) { sp[(size_t)i * ns_pad + k] = c[i]; sp[(size_t)(7 + i) * ns_pad + k] = c[7 + i]; }
+        sp[(size_t)14 * ns_pad + k] = c[16];
+        sp[(size_t)15 * ns_pad + k] = c[17];
+    }
+    std::vector<double> A(nr), B(nr), E(nr), dn(nr), nuv;
+    std::vector<unsigned> info(nr), list(nr), ent;
+    auto code_of = [&](double nu) -> unsigned {
+        const double t = 2. * nu;
+        if (t == std::floor(t) && t >= 1. && t <= 15.) return (unsigned)t;
+        for (size_t i = 0; i < nuv.size(); i++) if (nuv[i] == nu) return 16u + (unsigned)i;
+        nuv.push_back(nu);
+        return 16u + (unsigned)(nuv.size() - 1);
+    };
+    for (int jn = 0; jn < nr; jn++) {
+        const int j = perm[jn];
+        A[jn] = m->lnA[j]; B[jn] = m->beta[j]; E[jn] = m->E_over_R[j];
+        double d = 0.;
+        std::vector<std::pair<int, double>> re, pr;
+        for (int q = m->reac_ptr[j]; q < m->reac_ptr[j + 1]; q++) { re.push_back({m->reac_sp[q], m->reac_nu[q]}); d -= m->reac_nu[q]; }
+        for (int q = m->prod_ptr[j]; q < m->prod_ptr[j + 1]; q++) { pr.push_back({m->prod_sp[q], m->prod_nu[q]}); d += m->prod_nu[q]; }
+        dn[jn] = d;
+        auto bysp = [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; };
+        std::stable_sort(re.begin(), re.end(), bysp);
+        std::stable_sort(pr.begin(), pr.end(), bysp);
+        if (re.size() > 255 || pr.size() > 255 || ent.size() + re.size() + pr.size() > 65535) return false;
+        list[jn] = (unsigned)ent.size() | ((unsigned)re.size() << 16) | ((unsigned)pr.size() << 24);
+        for (auto &e : re) ent.push_back((unsigned)e.first | (code_of(e.second) << 12));
+        for (auto &e : pr) ent.push_back((unsigned)e.first | (code_of(e.second) << 12));
+        info[jn] = (unsigned)(m->rflags[j] & 3) | ((unsigned)(tb_of[j] + 1) << 8) | ((unsigned)(fo_of[j] + 1) << 16);
+    }
+    if (nuv.size() > (1u << 19)) return false;
+    std::vector<unsigned> tbp(m->ntb + 1), tbs;
+    std::vector<double> tbe;
+    for (int s = 0; s < m->ntb; s++) {
+        tbp[s] = (unsigned)tbs.size();
+        for (int q = m->tb_ptr[s]; q < m->tb_ptr[s + 1]; q++) { tbs.push_back((unsigned)m->tb_sp[q]); tbe.push_back(m->tb_eff_m1[q]); }
+    }
+    tbp[m->ntb] = (unsigned)tbs.size();
+    std::vector<double> fop((size_t)8 * m->nfo);
+    std::vector<int> fos(m->nfo);
+    std::vector<unsigned> fol(m->nfo + 1), fos2;
+    std::vector<double> foe;
+    for (int s = 0; s < m->nfo; s++) {
+        for (int i = 0; i < 3; i++) fop[8 * s + i] = m->fo_inf[3 * s + i];
+        for (int i = 0; i < 4; i++) fop[8 * s + 3 + i] = m->fo_troe[4 * s + i];
+        fop[8 * s + 7] = (double)m->fo_type[s];
+        fos[s] = m->fo_single[s];
+        fol[s] = (unsigned)fos2.size();
+        for (int q = m->fo_ptr[s]; q < m->fo_ptr[s + 1]; q++) { fos2.push_back((unsigned)m->fo_sp[q]); foe.push_back(m->fo_eff_m1[q]); }
+    }
+    fol[m->nfo] = (unsigned)fos2.size();
+    L.o_sp = im.put(sp);
+    L.o_A = im.put(A); L.o_B = im.put(B); L.o_E = im.put(E); L.o_dn = im.put(dn);
+    L.o_info = im.put(info); L.o_list = im.put(list); L.o_ent = im.put(ent); L.o_nuv = im.put(nuv);
+    L.o_tbp = im.put(tbp); L.o_tbs = im.put(tbs); L.o_tbe = im.put(tbe);
+    L.o_fop = im.put(fop); L.o_fos = im.put(fos); L.o_fol = im.put(fol); L.o_fos2 = im.put(fos2); L.o_foe = im.put(foe);
+    im.b.resize((im.b.size() + 15) & ~size_t(15), 0);
+    L.bytes = (int)im.b.size();
+    L.team_ok = 1;
+    out.swap(im.b);
+    return true;
 }
 
 static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
@@ -442,6 +529,9 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     size_t o_fot = b.add(fo_type), o_fos1 = b.add(fo_single), o_fopar = b.add(fo_par);
     size_t o_fop = b.add(fo_ptr), o_fos = b.add(fo_sp), o_foe = b.add(fo_eff);
     size_t o_fp = b.add(f_ptr), o_frx = b.add(f_rx), o_fnu = b.add(f_nu);
+    std::vector<char> image;
+    const bool img_ok = build_image(m, perm, tb_of, fo_of, d.ns_pad, d.img, image);
+    size_t o_img = img_ok ? b.add(image) : 0;
     HIP_OK(hipMalloc(&h->d_tables, b.bytes.size()));
     HIP_OK(hipMemcpy(h->d_tables, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice));
     char *base = (char *)h->d_tables;
@@ -456,8 +546,11 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     d.fo_type = I(o_fot); d.fo_single = I(o_fos1); d.fo_par = D(o_fopar);
     d.fo_ptr = I(o_fop); d.fo_sp = I(o_fos); d.fo_eff = D(o_foe);
     d.f_ptr = I(o_fp); d.f_rx = I(o_frx); d.f_nu = D(o_fnu);
+    d.img.src = img_ok ? (const void *)(base + o_img) : nullptr;
     return LSM_OK;
 }
+
+extern "C" {
 
 int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const lsm_reactor_options *reactor,
                lsm_handle **out) {
@@ -498,14 +591,18 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
     k.min_temperature = r.min_temperature;
     k.energy = r.energy_equation ? 1 : 0;
     k.full_pivoting = p.full_pivoting ? 1 : 0;
-    // persistent team grid: as many one-wave blocks as fit
-    int occ = 0;
-    const size_t lds = team_lds_bytes(h->dm);
-    if (h->dm.ne <= team_max_neq()) {
-        occ = (int)(prop.sharedMemPerMultiprocessor / std::max<size_t>(lds, 1));
-        occ = std::max(1, std::min(occ, 8));
+    // persistent team grid: one cell per wave, up to 4 waves per block sharing the
+    // LDS copy of the mechanism; enough blocks to fill every CU (one wave per SIMD
+    // at this kernel's register footprint), capped by LDS capacity.
+    const size_t lds_block = std::max<size_t>(prop.sharedMemPerBlock, 65536);
+    const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? prop.maxSharedMemoryPerMultiProcessor : lds_block;
+    h->team_wpb = team_waves_per_block(h->dm, std::min(lds_block, lds_cu));
+    if (h->team_wpb > 0) {
+        const size_t lds = (size_t)h->dm.img.bytes + (size_t)h->team_wpb * team_wave_bytes(h->dm);
+        int blocks = (int)std::max<size_t>(1, lds_cu / lds);
+        blocks = std::min(blocks, std::max(1, 4 / h->team_wpb));
+        h->team_grid = blocks * h->num_cu;
     }
-    h->team_grid = std::max(1, occ) * h->num_cu;
     *out = h;
     return LSM_OK;
 }
@@ -513,8 +610,9 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
 int lsm_set_method(lsm_handle *h, int32_t method) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (method < LSM_METHOD_AUTO || method > LSM_METHOD_TEAM) return fail(LSM_E_INVALID, "unknown method");
-    if (method == LSM_METHOD_TEAM && (h->dm.ne > team_max_neq() || h->kp.full_pivoting))
-        return fail(LSM_E_UNSUPPORTED, "team kernel needs ns+1 <= 64 and partial pivoting");
+    if (method == LSM_METHOD_TEAM && (h->team_wpb < 1 || h->kp.full_pivoting))
+        return fail(LSM_E_UNSUPPORTED, "team kernel needs ns+1 <= 64, a mechanism image that fits in LDS "
+                                       "and partial pivoting");
     h->method = method;
     return LSM_OK;
 }
@@ -535,7 +633,7 @@ int lsm_destroy(lsm_handle *h) {
 
 static int pick_method(const lsm_handle *h) {
     if (h->method != LSM_METHOD_AUTO) return h->method;
-    return (h->dm.ne <= team_max_neq() && !h->kp.full_pivoting) ? LSM_METHOD_TEAM : LSM_METHOD_PLAIN;
+    return (h->team_wpb > 0 && !h->kp.full_pivoting) ? LSM_METHOD_TEAM : LSM_METHOD_PLAIN;
 }
 
 int lsm_integrate_batch_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in, double *d_y_out,
@@ -549,7 +647,7 @@ int lsm_integrate_batch_device(lsm_handle *h, int64_t n, const double *d_dt, con
     if (method == LSM_METHOD_TEAM) {
         if (launch_reset_queue(h->d_queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
         int rc = launch_team(h->dm, h->kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue,
-                             h->team_grid, stream);
+                             h->team_grid, h->team_wpb, stream);
         if (rc) return fail(LSM_E_HIP, std::string("team kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
         return LSM_OK;
     }

@@ -17,6 +17,28 @@ enum SpRow {
     SP_TMID = 34, SP_MW = 35, NSP_ROWS = 36
 };
 
+// Compact mechanism image that the team kernel copies into LDS once per
+// workgroup (byte offsets from the image start; every block 16-byte aligned).
+//   sp   double [LSP_ROWS][ns_pad]  rows 0-6 high-T a1..a7, 7-13 low-T a1..a7, 14 T_mid, 15 MW
+//   A,B,E,dn double [nr]            lnA, beta, E/R, change of moles (device reaction order)
+//   info uint [nr]                  bit0 reversible, bit1 negative A, bits 8-15 third-body slot+1,
+//                                   bits 16-23 fall-off slot+1
+//   list uint [nr]                  first entry (bits 0-15), reactants (16-23), products (24-31)
+//   ent  uint [nent]                species (bits 0-11) | coefficient code << 12; code c < 16 means
+//                                   nu = c/2, otherwise nu = nuv[c - 16]
+//   nuv  double []                  coefficients that are not multiples of 1/2
+//   tbp  uint [ntb+1], tbs uint [], tbe double []   third-body lists (eff - 1)
+//   fop  double [nfo][8]            lnA_inf, beta_inf, E/R_inf, a, T***, T*, T**, type
+//   fos  int [nfo] single collider or -1; fol uint [nfo+1]; fos2 uint []; foe double []
+enum { LSP_ROWS = 16 };
+struct LdsImage {
+    const void *src;
+    int bytes;
+    int o_sp, o_A, o_B, o_E, o_dn, o_info, o_list, o_ent, o_nuv, o_tbp, o_tbs, o_tbe, o_fop, o_fos, o_fol,
+        o_fos2, o_foe;
+    int team_ok;   // the mechanism fits the image limits (ns <= 63, < 65536 entries, <= 254 TB/FO)
+};
+
 // Reactions are renumbered on upload: elementary first, then third-body, then
 // fall-off, so that the wavefront lanes that take the extra third-body and
 // fall-off work are contiguous.  rperm maps device order -> mechanism order.
@@ -40,6 +62,7 @@ struct DevMech {
     const double *fo_eff;
     const int *f_ptr, *f_rx;                 // species-major formation list
     const double *f_nu;                      // signed coefficient
+    LdsImage img;
 };
 
 struct KParams {
@@ -52,10 +75,13 @@ struct KParams {
 
 // Host-side launchers (lsm_kernels.hip).
 int team_max_neq();
-size_t team_lds_bytes(const DevMech &m);
+// LDS layout of a team workgroup: the mechanism image, then per wave the Jacobian
+// (ne x 64 doubles) and the species scratch (4 x 64 doubles).
+size_t team_wave_bytes(const DevMech &m);
+int team_waves_per_block(const DevMech &m, size_t lds_per_block);
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                 double *yout, int *status, double *q, int *counters, unsigned *queue, int grid,
-                void *stream);
+                int waves_per_block, void *stream);
 int launch_plain(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                  double *yout, int *status, double *q, int *counters, double *work, int64_t slots,
                  void *stream);

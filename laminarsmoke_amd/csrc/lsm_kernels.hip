@@ -41,19 +41,57 @@ __device__ __forceinline__ double readlane_d(double x, int l) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// 64-bit DPP move of both halves; lanes without a source (or outside row_mask) get 0.
+template <int CTRL, int ROWM>
+__device__ __forceinline__ long long dpp0_i64(long long v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(v & 0xffffffffll), CTRL, ROWM, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, ROWM, 0xf, false);
+    return ((long long)hi << 32) | (unsigned)lo;
+}
+template <int CTRL, int ROWM>
+__device__ __forceinline__ double dpp0_d(double v) {
+    return __longlong_as_double(dpp0_i64<CTRL, ROWM>(__double_as_longlong(v)));
+}
+
+// Inclusive wavefront scan with row_shr:1,2,4,8 then row_bcast:15,31 (gfx9 DPP);
+// lane 63 ends with the total, which is broadcast.  Fixed summation order.
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
-    return uniform_d(x);
+    x += dpp0_d<0x111, 0xf>(x);
+    x += dpp0_d<0x112, 0xf>(x);
+    x += dpp0_d<0x114, 0xf>(x);
+    x += dpp0_d<0x118, 0xf>(x);
+    x += dpp0_d<0x142, 0xa>(x);
+    x += dpp0_d<0x143, 0xc>(x);
+    return readlane_d(x, 63);
+}
+
+// Two independent sums in one pass (interleaved for latency).
+__device__ __forceinline__ void wave_sum2(double &a, double &b) {
+    a += dpp0_d<0x111, 0xf>(a); b += dpp0_d<0x111, 0xf>(b);
+    a += dpp0_d<0x112, 0xf>(a); b += dpp0_d<0x112, 0xf>(b);
+    a += dpp0_d<0x114, 0xf>(a); b += dpp0_d<0x114, 0xf>(b);
+    a += dpp0_d<0x118, 0xf>(a); b += dpp0_d<0x118, 0xf>(b);
+    a += dpp0_d<0x142, 0xa>(a); b += dpp0_d<0x142, 0xa>(b);
+    a += dpp0_d<0x143, 0xc>(a); b += dpp0_d<0x143, 0xc>(b);
+    a = readlane_d(a, 63);
+    b = readlane_d(b, 63);
+}
+
+__device__ __forceinline__ unsigned long long umax64(unsigned long long a, unsigned long long b) {
+    return a > b ? a : b;
 }
 
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        const unsigned long long y = __shfl_xor(x, o, 64);
-        x = y > x ? y : x;
-    }
-    return x;
+    x = umax64(x, (unsigned long long)dpp0_i64<0x111, 0xf>((long long)x));
+    x = umax64(x, (unsigned long long)dpp0_i64<0x112, 0xf>((long long)x));
+    x = umax64(x, (unsigned long long)dpp0_i64<0x114, 0xf>((long long)x));
+    x = umax64(x, (unsigned long long)dpp0_i64<0x118, 0xf>((long long)x));
+    x = umax64(x, (unsigned long long)dpp0_i64<0x142, 0xa>((long long)x));
+    x = umax64(x, (unsigned long long)dpp0_i64<0x143, 0xc>((long long)x));
+    const long long v = (long long)x;
+    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), 63);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -158,25 +196,45 @@ __device__ __forceinline__ void nasa7(const DevMech &M, int k, double T, double 
 // ---------------------------------------------------------------------------
 // Team context: one cell per wavefront (requires ne <= NMAX <= 64)
 // ---------------------------------------------------------------------------
+// Typed views of the mechanism image once it sits in LDS (layout: lsm_internal.hpp).
+struct LdsMech {
+    const double *sp, *A, *B, *E, *dn, *nuv, *tbe, *fop, *foe;
+    const unsigned *info, *list, *ent, *tbp, *tbs, *fol, *fos2;
+    const int *fos;
+    int P;   // species stride (ns_pad)
+    __device__ LdsMech(const unsigned char *b, const LdsImage &L, int ns_pad)
+        : sp((const double *)(b + L.o_sp)), A((const double *)(b + L.o_A)), B((const double *)(b + L.o_B)),
+          E((const double *)(b + L.o_E)), dn((const double *)(b + L.o_dn)), nuv((const double *)(b + L.o_nuv)),
+          tbe((const double *)(b + L.o_tbe)), fop((const double *)(b + L.o_fop)), foe((const double *)(b + L.o_foe)),
+          info((const unsigned *)(b + L.o_info)), list((const unsigned *)(b + L.o_list)),
+          ent((const unsigned *)(b + L.o_ent)), tbp((const unsigned *)(b + L.o_tbp)),
+          tbs((const unsigned *)(b + L.o_tbs)), fol((const unsigned *)(b + L.o_fol)),
+          fos2((const unsigned *)(b + L.o_fos2)), fos((const int *)(b + L.o_fos)), P(ns_pad) {}
+    __device__ __forceinline__ double nu(unsigned e) const {
+        const unsigned c = e >> 12;
+        return c < 16u ? 0.5 * (double)c : nuv[c - 16u];
+    }
+};
+
 template <int NMAX>
 struct TeamCtx {
     double V[NVEC][1];
     double G[NMAX];
     int ps;          // elimination step at which this lane's row became the pivot
     double dinv;     // 1 / pivot of this lane's row
-    double *Jl, *sc, *sh, *ss, *sr;
-    const DevMech &M;
+    double *Jl, *sc, *sh, *ss, *sR;   // per-wave LDS: Jacobian [ne][64], c, h/RT, s/R, R accumulators
+    LdsMech L;
     const KParams &K;
     int lane, ne, ns, nr;
     double qr;
 
-    __device__ TeamCtx(const DevMech &m, const KParams &k, double *lds)
-        : M(m), K(k), lane(threadIdx.x & 63), ne(m.ne), ns(m.ns), nr(m.nr), qr(0.) {
-        Jl = lds;
+    __device__ TeamCtx(const DevMech &m, const KParams &k, const unsigned char *img, double *wave_lds)
+        : L(img, m.img, m.ns_pad), K(k), lane(threadIdx.x & 63), ne(m.ne), ns(m.ns), nr(m.nr), qr(0.) {
+        Jl = wave_lds;
         sc = Jl + 64 * m.ne;
         sh = sc + 64;
         ss = sh + 64;
-        sr = ss + 64;
+        sR = ss + 64;
     }
 
     template <class F>
@@ -191,39 +249,120 @@ struct TeamCtx {
     __device__ __forceinline__ double reduce(double v) { return wave_sum(v); }
     __device__ __forceinline__ int count(bool b) { return __popcll(__ballot(b)); }
 
+    // One reaction (KineticsMap_CHEMKIN.hpp:1192-1386, 1683-1768), with the formation
+    // rates (StoichiometricMap.hpp:656-730) accumulated reaction-major into LDS.
+    __device__ __forceinline__ void reaction(int j, double T, double logT, double uT, double logPRT, double cTot) {
+        double kf = exp(L.A[j] + L.B[j] * logT - L.E[j] * uT);
+        const unsigned info = L.info[j];
+        if (info & 2u) kf = -kf;
+        double kmod = kf;
+        const int tb = (int)((info >> 8) & 0xffu) - 1;
+        if (tb >= 0) {
+            double Me = cTot;
+            for (unsigned q = L.tbp[tb]; q < L.tbp[tb + 1]; q++) Me += sc[L.tbs[q]] * L.tbe[q];
+            kmod *= Me;
+        }
+        const int fo = (int)((info >> 16) & 0xffu) - 1;
+        if (fo >= 0) {
+            const double *fp = L.fop + 8 * fo;
+            const double kinf = exp(fp[0] + fp[1] * logT - fp[2] * uT);
+            double Mc;
+            if (L.fos[fo] < 0) {
+                Mc = cTot;
+                for (unsigned q = L.fol[fo]; q < L.fol[fo + 1]; q++) Mc += sc[L.fos2[q]] * L.foe[q];
+            } else {
+                Mc = sc[L.fos[fo]] + 1.e-16;
+            }
+            const double Pr = kf * Mc / kinf;
+            double wF = 1.;
+            if (fp[7] != 0.) {   // Troe
+                const double a = fp[3];
+                double F = (1. - a) * exp(-T / fp[4]) + a * exp(-T / fp[5]);
+                if (fp[6] != 0.) F += exp(-fp[6] / T);
+                const double lf = (F < 1.e-300) ? -300. : log10(F);
+                double q;
+                if (Pr > 1.e-32) {
+                    const double nT = 0.75 - 1.27 * lf;
+                    const double cT = -0.4 - 0.67 * lf;
+                    const double sT = log10(Pr) + cT;
+                    q = sT / (nT - 0.14 * sT);
+                } else {
+                    q = 1. / 0.14;
+                }
+                wF = pow(10., lf / (1. + q * q));
+            }
+            kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
+        }
+        const unsigned lst = L.list[j];
+        const unsigned e0 = lst & 0xffffu, nre = (lst >> 16) & 0xffu, npr = lst >> 24;
+        double pf = 1.;
+        for (unsigned q = 0; q < nre; q++) {
+            const unsigned e = L.ent[e0 + q];
+            pf *= conc_pow(sc[e & 0xfffu], L.nu(e));
+        }
+        double net = pf;
+        if (info & 1u) {
+            double dh = 0., ds = 0., pb = 1.;
+            for (unsigned q = 0; q < nre; q++) {
+                const unsigned e = L.ent[e0 + q];
+                const unsigned k = e & 0xfffu;
+                const double nu = L.nu(e);
+                dh -= nu * sh[k];
+                ds -= nu * ss[k];
+            }
+            for (unsigned q = 0; q < npr; q++) {
+                const unsigned e = L.ent[e0 + nre + q];
+                const unsigned k = e & 0xfffu;
+                const double nu = L.nu(e);
+                dh += nu * sh[k];
+                ds += nu * ss[k];
+                pb *= conc_pow(sc[k], nu);
+            }
+            net -= pb * exp(-ds + dh - logPRT * L.dn[j]);
+        }
+        const double r = net * kmod;
+        for (unsigned q = 0; q < nre + npr; q++) {
+            const unsigned e = L.ent[e0 + q];
+            const double nu = L.nu(e);
+            atomicAdd(&sR[e & 0xfffu], q < nre ? -nu * r : nu * r);
+        }
+    }
+
     // BatchReactorHomogeneousConstantPressure::Equations (:175-247)
     __device__ __forceinline__ void rhs(int src, int dst) {
         const double T = readlane_d(V[src][0], ns);
         const int k = lane;
         const bool isp = k < ns;
         const double Yk = isp ? V[src][0] : 0.;
-        const double mwk = isp ? M.sp[SP_MW * M.ns_pad + k] : 1.;
+        const int P = L.P;
+        const double mwk = isp ? L.sp[15 * P + k] : 1.;
         const double MW = 1. / wave_sum(Yk / mwk);
         const double xk = (MW * Yk) / mwk;
         const double cTotP = K.pressure / RGAS / T;
         const double ck = cTotP * xk;
         const double rho = cTotP * MW;
         const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, uT = 1. / T, logT = log(T);
-        double cp = 0., h = 0., s = 0.;
+        double cp = 0., h = 0.;
         if (isp) {
-            nasa7(M, k, T, T2, T3, T4, uT, logT, cp, h, s);
+            // NASA-7 (ThermodynamicsMap_CHEMKIN.hpp:263-343, 436-496), high set iff T > T_mid
+            const double *a = L.sp + ((T > L.sp[14 * P + k]) ? 0 : 7 * P) + k;
+            const double a0 = a[0], a1 = a[P], a2 = a[2 * P], a3 = a[3 * P], a4 = a[4 * P];
+            cp = a0 + T * a1 + T2 * a2 + T3 * a3 + T4 * a4;
+            h = a0 + T * (0.50 * a1) + T2 * ((1. / 3.) * a2) + T3 * (0.25 * a3) + T4 * (0.20 * a4) + uT * a[5 * P];
+            const double s = a0 * logT + T * a1 + T2 * (0.50 * a2) + T3 * ((1. / 3.) * a3) + T4 * (0.25 * a4) +
+                             a[6 * P];
             sc[k] = ck;
             sh[k] = h;
             ss[k] = s;
         }
-        const double cTot = wave_sum(ck);
-        const double cpx = wave_sum(isp ? cp * xk : 0.);
+        sR[k] = 0.;
+        double cTot = ck, cpx = isp ? cp * xk : 0.;
+        wave_sum2(cTot, cpx);
         wave_sync();
         const double logPRT = log(101325. / RGAS / T);
-        for (int j = lane; j < nr; j += 64) {
-            sr[j] = reaction_rate(
-                M, j, T, logT, uT, logPRT, cTot, [&](int i) { return sc[i]; }, [&](int i) { return sh[i]; },
-                [&](int i) { return ss[i]; });
-        }
+        for (int j = lane; j < nr; j += 64) reaction(j, T, logT, uT, logPRT, cTot);
         wave_sync();
-        double Rk = 0.;
-        if (isp)
-            for (int e = M.f_ptr[k]; e < M.f_ptr[k + 1]; e++) Rk += M.f_nu[e] * sr[M.f_rx[e]];
+        const double Rk = isp ? sR[k] : 0.;
         double dy = isp ? (mwk * Rk) / rho : 0.;
         if (K.energy) {
             const double hs = wave_sum(isp ? Rk * h : 0.);
@@ -331,11 +470,20 @@ struct TeamCtx {
 };
 
 template <int NMAX>
-__global__ void __launch_bounds__(64) team_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
-                                                  const double *yin, double *yout, int *status, double *q,
-                                                  int *counters, unsigned *queue) {
-    extern __shared__ double lds[];
-    TeamCtx<NMAX> c(M, K, lds);
+__global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
+                                                   const double *yin, double *yout, int *status, double *q,
+                                                   int *counters, unsigned *queue) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    {   // stage the mechanism image (once per workgroup)
+        const int4 *src = (const int4 *)M.img.src;
+        int4 *dst = (int4 *)lds_raw;
+        const int nv = M.img.bytes / 16;
+        for (int i = threadIdx.x; i < nv; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+    const int wave = threadIdx.x >> 6;
+    double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * (8 * (64 * M.ne + 256)));
+    TeamCtx<NMAX> c(M, K, lds_raw, wave_lds);
     const int lane = c.lane, ns = M.ns, ne = M.ne;
     for (;;) {
         unsigned cell_u = 0;
@@ -589,28 +737,39 @@ __global__ void reset_queue_kernel(unsigned *queue) { *queue = 0u; }
 // ---------------------------------------------------------------------------
 int team_max_neq() { return 64; }
 
-size_t team_lds_bytes(const DevMech &m) {
-    return sizeof(double) * ((size_t)64 * m.ne + 3 * 64 + (size_t)m.nr);
+size_t team_wave_bytes(const DevMech &m) { return (size_t)8 * (64 * (size_t)m.ne + 256); }
+
+int team_waves_per_block(const DevMech &m, size_t lds_per_block) {
+    if (!m.img.team_ok || m.ne > 64) return 0;
+    const size_t per = team_wave_bytes(m);
+    if ((size_t)m.img.bytes + per > lds_per_block) return 0;
+    const size_t w = (lds_per_block - (size_t)m.img.bytes) / per;
+    return (int)(w < 4 ? w : 4);
 }
 
 template <int NMAX>
 static int launch_team_n(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
-                         double *yout, int *status, double *q, int *counters, unsigned *queue, int grid,
+                         double *yout, int *status, double *q, int *counters, unsigned *queue, int grid, int wpb,
                          hipStream_t s) {
-    const size_t lds = team_lds_bytes(m);
-    hipLaunchKernelGGL(team_kernel<NMAX>, dim3(grid), dim3(64), lds, s, m, k, n, dt, yin, yout, status, q,
+    const size_t lds = (size_t)m.img.bytes + (size_t)wpb * team_wave_bytes(m);
+    if (lds > 65536 &&
+        hipFuncSetAttribute((const void *)team_kernel<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+            hipSuccess)
+        return -1;
+    hipLaunchKernelGGL(team_kernel<NMAX>, dim3(grid), dim3(64 * wpb), lds, s, m, k, n, dt, yin, yout, status, q,
                        counters, queue);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
-                int *status, double *q, int *counters, unsigned *queue, int grid, void *stream) {
+                int *status, double *q, int *counters, unsigned *queue, int grid, int wpb, void *stream) {
     hipStream_t s = (hipStream_t)stream;
-    if (m.ne <= 16) return launch_team_n<16>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, s);
-    if (m.ne <= 32) return launch_team_n<32>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, s);
-    if (m.ne <= 48) return launch_team_n<48>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, s);
-    if (m.ne <= 56) return launch_team_n<56>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, s);
-    if (m.ne <= 64) return launch_team_n<64>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, s);
+    if (wpb < 1 || wpb > 4) return -2;
+    if (m.ne <= 16) return launch_team_n<16>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, s);
+    if (m.ne <= 32) return launch_team_n<32>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, s);
+    if (m.ne <= 48) return launch_team_n<48>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, s);
+    if (m.ne <= 56) return launch_team_n<56>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, s);
+    if (m.ne <= 64) return launch_team_n<64>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, s);
     return -2;
 }
 
