@@ -34,26 +34,26 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False, extra=()) -> str:
-    """Compile every source (always; the build check must compile)."""
+def build(verbose: bool = False, extra=(), out: str = LIB) -> str:
+    """Compile every source (always; the build check must compile) into `out`."""
     objs = []
     for src in SOURCES:
-        obj = os.path.join(CSRC, src + ".o")
+        obj = os.path.join(os.path.dirname(out), os.path.basename(out) + "." + src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
         cmd = [hipcc(), *HIPCC_FLAGS, *extra, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     for o in objs:
         os.remove(o)
-    return LIB
+    return out
 
 
 if __name__ == "__main__":

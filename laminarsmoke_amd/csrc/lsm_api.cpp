@@ -223,6 +223,7 @@ struct lsm_handle {
     int64_t cap = 0;
     int team_grid = 0;
     int team_wpb = 0;
+    double *d_jws = nullptr;
 };
 
 extern "C" {
@@ -324,6 +325,7 @@ static int check_mechanism(const lsm_mechanism *m) {
     return LSM_OK;
 }
 
+
 }  // extern "C"
 
 // Compact LDS image of the mechanism for the team kernel (layout: lsm_internal.hpp).
@@ -351,8 +353,8 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
         sp[(size_t)14 * ns_pad + k] = c[16];
         sp[(size_t)15 * ns_pad + k] = c[17];
     }
-    std::vector<double> A(nr), B(nr), E(nr), dn(nr), nuv;
-    std::vector<unsigned> info(nr), list(nr), ent;
+    std::vector<double> P((size_t)4 * nr), nuv;
+    std::vector<unsigned> W((size_t)4 * nr), ent;
     auto code_of = [&](double nu) -> unsigned {
         const double t = 2. * nu;
         if (t == std::floor(t) && t >= 1. && t <= 15.) return (unsigned)t;
@@ -362,20 +364,37 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
     };
     for (int jn = 0; jn < nr; jn++) {
         const int j = perm[jn];
-        A[jn] = m->lnA[j]; B[jn] = m->beta[j]; E[jn] = m->E_over_R[j];
+        P[4 * jn + 0] = m->lnA[j]; P[4 * jn + 1] = m->beta[j]; P[4 * jn + 2] = m->E_over_R[j];
         double d = 0.;
         std::vector<std::pair<int, double>> re, pr;
         for (int q = m->reac_ptr[j]; q < m->reac_ptr[j + 1]; q++) { re.push_back({m->reac_sp[q], m->reac_nu[q]}); d -= m->reac_nu[q]; }
         for (int q = m->prod_ptr[j]; q < m->prod_ptr[j + 1]; q++) { pr.push_back({m->prod_sp[q], m->prod_nu[q]}); d += m->prod_nu[q]; }
-        dn[jn] = d;
+        P[4 * jn + 3] = d;
         auto bysp = [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; };
         std::stable_sort(re.begin(), re.end(), bysp);
         std::stable_sort(pr.begin(), pr.end(), bysp);
         if (re.size() > 255 || pr.size() > 255 || ent.size() + re.size() + pr.size() > 65535) return false;
-        list[jn] = (unsigned)ent.size() | ((unsigned)re.size() << 16) | ((unsigned)pr.size() << 24);
+        W[4 * jn + 3] = (unsigned)ent.size() | ((unsigned)re.size() << 16) | ((unsigned)pr.size() << 24);
         for (auto &e : re) ent.push_back((unsigned)e.first | (code_of(e.second) << 12));
         for (auto &e : pr) ent.push_back((unsigned)e.first | (code_of(e.second) << 12));
-        info[jn] = (unsigned)(m->rflags[j] & 3) | ((unsigned)(tb_of[j] + 1) << 8) | ((unsigned)(fo_of[j] + 1) << 16);
+        // fixed-format record: integer orders, at most 3 species per side (multiplicity counted)
+        auto pack = [&](const std::vector<std::pair<int, double>> &side, unsigned &word) {
+            std::vector<unsigned> sl;
+            for (auto &e : side) {
+                if (e.second != std::floor(e.second) || e.second < 1. || e.second > 3.) return false;
+                for (int r = 0; r < (int)e.second; r++) sl.push_back((unsigned)e.first);
+            }
+            if (sl.size() > 3) return false;
+            while (sl.size() < 3) sl.push_back((unsigned)LSM_DUMMY_SP);
+            word = sl[0] | (sl[1] << 8) | (sl[2] << 16);
+            return true;
+        };
+        unsigned wr = 0, wp = 0;
+        const bool fast = pack(re, wr) && pack(pr, wp);
+        W[4 * jn + 1] = wr;
+        W[4 * jn + 2] = wp;
+        W[4 * jn + 0] = (unsigned)(m->rflags[j] & 3) | (fast ? 4u : 0u) | ((unsigned)(tb_of[j] + 1) << 8) |
+                        ((unsigned)(fo_of[j] + 1) << 16);
     }
     if (nuv.size() > (1u << 19)) return false;
     std::vector<unsigned> tbp(m->ntb + 1), tbs;
@@ -399,8 +418,8 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
     }
     fol[m->nfo] = (unsigned)fos2.size();
     L.o_sp = im.put(sp);
-    L.o_A = im.put(A); L.o_B = im.put(B); L.o_E = im.put(E); L.o_dn = im.put(dn);
-    L.o_info = im.put(info); L.o_list = im.put(list); L.o_ent = im.put(ent); L.o_nuv = im.put(nuv);
+    L.o_P = im.put(P); L.o_W = im.put(W);
+    L.o_ent = im.put(ent); L.o_nuv = im.put(nuv);
     L.o_tbp = im.put(tbp); L.o_tbs = im.put(tbs); L.o_tbe = im.put(tbe);
     L.o_fop = im.put(fop); L.o_fos = im.put(fos); L.o_fol = im.put(fol); L.o_fos2 = im.put(fos2); L.o_foe = im.put(foe);
     im.b.resize((im.b.size() + 15) & ~size_t(15), 0);
@@ -591,6 +610,14 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
     k.min_temperature = r.min_temperature;
     k.energy = r.energy_equation ? 1 : 0;
     k.full_pivoting = p.full_pivoting ? 1 : 0;
+    k.phase_cycles = nullptr;
+#ifdef LSM_PHASE_TIMERS
+    if (hipMalloc(&k.phase_cycles, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(k.phase_cycles, 0, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        lsm_destroy(h);
+        return fail(LSM_E_HIP, "phase timer buffer");
+    }
+#endif
     // persistent team grid: one cell per wave, up to 4 waves per block sharing the
     // LDS copy of the mechanism; enough blocks to fill every CU (one wave per SIMD
     // at this kernel's register footprint), capped by LDS capacity.
@@ -600,8 +627,10 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
     if (h->team_wpb > 0) {
         const size_t lds = (size_t)h->dm.img.bytes + (size_t)h->team_wpb * team_wave_bytes(h->dm);
         int blocks = (int)std::max<size_t>(1, lds_cu / lds);
-        blocks = std::min(blocks, std::max(1, 4 / h->team_wpb));
+        blocks = std::min(blocks, std::max(1, 8 / h->team_wpb));    // <= 2 waves per SIMD
         h->team_grid = blocks * h->num_cu;
+        const size_t jbytes = sizeof(double) * team_jac_doubles(h->dm) * (size_t)h->team_grid * h->team_wpb;
+        if (hipMalloc(&h->d_jws, jbytes) != hipSuccess) { lsm_destroy(h); return fail(LSM_E_NOMEM, "Jacobian workspace"); }
     }
     *out = h;
     return LSM_OK;
@@ -622,6 +651,8 @@ int lsm_destroy(lsm_handle *h) {
     if (h->d_tables) (void)hipFree(h->d_tables);
     if (h->d_queue) (void)hipFree(h->d_queue);
     if (h->d_work) (void)hipFree(h->d_work);
+    if (h->d_jws) (void)hipFree(h->d_jws);
+    if (h->kp.phase_cycles) (void)hipFree(h->kp.phase_cycles);
     if (h->d_y) (void)hipFree(h->d_y);
     if (h->d_dt) (void)hipFree(h->d_dt);
     if (h->d_q) (void)hipFree(h->d_q);
@@ -647,7 +678,7 @@ int lsm_integrate_batch_device(lsm_handle *h, int64_t n, const double *d_dt, con
     if (method == LSM_METHOD_TEAM) {
         if (launch_reset_queue(h->d_queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
         int rc = launch_team(h->dm, h->kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue,
-                             h->team_grid, h->team_wpb, stream);
+                             h->team_grid, h->team_wpb, h->d_jws, stream);
         if (rc) return fail(LSM_E_HIP, std::string("team kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
         return LSM_OK;
     }
@@ -705,5 +736,16 @@ int lsm_integrate_batch(lsm_handle *h, int64_t n, const double *dt, const double
     if (counters) HIP_OK(hipMemcpy(counters, h->d_cnt, sizeof(int) * LSM_NCOUNTERS * n, hipMemcpyDeviceToHost));
     return LSM_OK;
 }
+
+#ifdef LSM_PHASE_TIMERS
+// Instrumented builds only: cumulative per-phase cycles (rhs, jacobian, factor,
+// solve, J*v, cells, cell count) since lsm_create.
+int lsm_debug_phase_cycles(lsm_handle *h, unsigned long long *out) {
+    if (!h || !out || !h->kp.phase_cycles) return fail(LSM_E_INVALID, "no phase timers");
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(out, h->kp.phase_cycles, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    return LSM_OK;
+}
+#endif
 
 }  // extern "C"

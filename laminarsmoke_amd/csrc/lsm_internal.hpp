@@ -20,22 +20,25 @@ enum SpRow {
 // Compact mechanism image that the team kernel copies into LDS once per
 // workgroup (byte offsets from the image start; every block 16-byte aligned).
 //   sp   double [LSP_ROWS][ns_pad]  rows 0-6 high-T a1..a7, 7-13 low-T a1..a7, 14 T_mid, 15 MW
-//   A,B,E,dn double [nr]            lnA, beta, E/R, change of moles (device reaction order)
-//   info uint [nr]                  bit0 reversible, bit1 negative A, bits 8-15 third-body slot+1,
-//                                   bits 16-23 fall-off slot+1
-//   list uint [nr]                  first entry (bits 0-15), reactants (16-23), products (24-31)
+//   P    double [nr][4]             lnA, beta, E/R, change of moles (device reaction order)
+//   W    uint   [nr][4]             info, reactant bytes, product bytes, general list:
+//        info: bit0 reversible, bit1 negative A, bit2 fixed-format record, bits 8-15
+//              third-body slot+1, bits 16-23 fall-off slot+1;
+//        fixed-format record: up to 3 reactant and 3 product species, integer orders as
+//              repeated species, unused slots = LSM_DUMMY_SP (c = 1, g = 0);
+//        general list: first entry (bits 0-15), reactants (16-23), products (24-31)
 //   ent  uint [nent]                species (bits 0-11) | coefficient code << 12; code c < 16 means
 //                                   nu = c/2, otherwise nu = nuv[c - 16]
 //   nuv  double []                  coefficients that are not multiples of 1/2
 //   tbp  uint [ntb+1], tbs uint [], tbe double []   third-body lists (eff - 1)
 //   fop  double [nfo][8]            lnA_inf, beta_inf, E/R_inf, a, T***, T*, T**, type
 //   fos  int [nfo] single collider or -1; fol uint [nfo+1]; fos2 uint []; foe double []
+enum { LSM_DUMMY_SP = 63 };
 enum { LSP_ROWS = 16 };
 struct LdsImage {
     const void *src;
     int bytes;
-    int o_sp, o_A, o_B, o_E, o_dn, o_info, o_list, o_ent, o_nuv, o_tbp, o_tbs, o_tbe, o_fop, o_fos, o_fol,
-        o_fos2, o_foe;
+    int o_sp, o_P, o_W, o_ent, o_nuv, o_tbp, o_tbs, o_tbe, o_fop, o_fos, o_fol, o_fos2, o_foe;
     int team_ok;   // the mechanism fits the image limits (ns <= 63, < 65536 entries, <= 254 TB/FO)
 };
 
@@ -67,6 +70,7 @@ struct DevMech {
 
 struct KParams {
     BdfConst bk;
+    unsigned long long *phase_cycles;   // [8] per-phase cycle totals (builds with LSM_PHASE_TIMERS only)
     double pressure;
     double min_temperature;
     int energy;
@@ -75,13 +79,15 @@ struct KParams {
 
 // Host-side launchers (lsm_kernels.hip).
 int team_max_neq();
-// LDS layout of a team workgroup: the mechanism image, then per wave the Jacobian
-// (ne x 64 doubles) and the species scratch (4 x 64 doubles).
+// LDS layout of a team workgroup: the mechanism image, then per wave the species
+// scratch (4 x 64 doubles).  Each wave's Jacobian (ne x 64 doubles) lives in an
+// HBM workspace indexed by the global wave number.
 size_t team_wave_bytes(const DevMech &m);
+size_t team_jac_doubles(const DevMech &m);
 int team_waves_per_block(const DevMech &m, size_t lds_per_block);
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                 double *yout, int *status, double *q, int *counters, unsigned *queue, int grid,
-                int waves_per_block, void *stream);
+                int waves_per_block, double *jws, void *stream);
 int launch_plain(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                  double *yout, int *status, double *q, int *counters, double *work, int64_t slots,
                  void *stream);

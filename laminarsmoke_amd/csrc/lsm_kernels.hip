@@ -24,6 +24,17 @@ constexpr double RGAS = LSM_R_J_KMOL;
 constexpr double NEG_INF = -__builtin_huge_val();
 constexpr double POS_INF = __builtin_huge_val();
 
+// Optional phase timers (-DLSM_PHASE_TIMERS): cycles spent in the RHS, the
+// Jacobian (including its RHS calls), the factorisation, the solves and J*v,
+// summed over waves into KParams::phase_cycles[0..4]; [5] whole cells, [6] cells.
+#ifdef LSM_PHASE_TIMERS
+#define LSM_TIC(v) const unsigned long long v = clock64()
+#define LSM_TOC(i, v) (tp[i] += clock64() - (v))
+#else
+#define LSM_TIC(v)
+#define LSM_TOC(i, v)
+#endif
+
 // ---------------------------------------------------------------------------
 // wavefront helpers
 // ---------------------------------------------------------------------------
@@ -198,18 +209,17 @@ __device__ __forceinline__ void nasa7(const DevMech &M, int k, double T, double 
 // ---------------------------------------------------------------------------
 // Typed views of the mechanism image once it sits in LDS (layout: lsm_internal.hpp).
 struct LdsMech {
-    const double *sp, *A, *B, *E, *dn, *nuv, *tbe, *fop, *foe;
-    const unsigned *info, *list, *ent, *tbp, *tbs, *fol, *fos2;
+    const double *sp, *P, *nuv, *tbe, *fop, *foe;
+    const uint4 *W;
+    const unsigned *ent, *tbp, *tbs, *fol, *fos2;
     const int *fos;
-    int P;   // species stride (ns_pad)
+    int SP;   // species stride (ns_pad)
     __device__ LdsMech(const unsigned char *b, const LdsImage &L, int ns_pad)
-        : sp((const double *)(b + L.o_sp)), A((const double *)(b + L.o_A)), B((const double *)(b + L.o_B)),
-          E((const double *)(b + L.o_E)), dn((const double *)(b + L.o_dn)), nuv((const double *)(b + L.o_nuv)),
+        : sp((const double *)(b + L.o_sp)), P((const double *)(b + L.o_P)), nuv((const double *)(b + L.o_nuv)),
           tbe((const double *)(b + L.o_tbe)), fop((const double *)(b + L.o_fop)), foe((const double *)(b + L.o_foe)),
-          info((const unsigned *)(b + L.o_info)), list((const unsigned *)(b + L.o_list)),
-          ent((const unsigned *)(b + L.o_ent)), tbp((const unsigned *)(b + L.o_tbp)),
+          W((const uint4 *)(b + L.o_W)), ent((const unsigned *)(b + L.o_ent)), tbp((const unsigned *)(b + L.o_tbp)),
           tbs((const unsigned *)(b + L.o_tbs)), fol((const unsigned *)(b + L.o_fol)),
-          fos2((const unsigned *)(b + L.o_fos2)), fos((const int *)(b + L.o_fos)), P(ns_pad) {}
+          fos2((const unsigned *)(b + L.o_fos2)), fos((const int *)(b + L.o_fos)), SP(ns_pad) {}
     __device__ __forceinline__ double nu(unsigned e) const {
         const unsigned c = e >> 12;
         return c < 16u ? 0.5 * (double)c : nuv[c - 16u];
@@ -219,22 +229,27 @@ struct LdsMech {
 template <int NMAX>
 struct TeamCtx {
     double V[NVEC][1];
-    double G[NMAX];
     int ps;          // elimination step at which this lane's row became the pivot
     double dinv;     // 1 / pivot of this lane's row
-    double *Jl, *sc, *sh, *ss, *sR;   // per-wave LDS: Jacobian [ne][64], c, h/RT, s/R, R accumulators
+    double *Jl;                       // Jacobian [ne][64] of this wave (HBM/L2 workspace)
+    double *LU;                       // per-wave LDS: LU factors [ne][64], lane i holds row i
+    double2 *scg;                     // per-wave LDS: (c_k, g_k = h_k/RT - s_k/R) per species slot
+    double *sR;                       // per-wave LDS: formation-rate accumulators
     LdsMech L;
     const KParams &K;
     int lane, ne, ns, nr;
     double qr;
+#ifdef LSM_PHASE_TIMERS
+    unsigned long long tp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int in_jac = 0;
+#endif
 
-    __device__ TeamCtx(const DevMech &m, const KParams &k, const unsigned char *img, double *wave_lds)
+    __device__ TeamCtx(const DevMech &m, const KParams &k, const unsigned char *img, double *wave_lds, double *jac)
         : L(img, m.img, m.ns_pad), K(k), lane(threadIdx.x & 63), ne(m.ne), ns(m.ns), nr(m.nr), qr(0.) {
-        Jl = wave_lds;
-        sc = Jl + 64 * m.ne;
-        sh = sc + 64;
-        ss = sh + 64;
-        sR = ss + 64;
+        Jl = jac;
+        scg = (double2 *)wave_lds;
+        sR = wave_lds + 128;
+        LU = wave_lds + 192;
     }
 
     template <class F>
@@ -251,15 +266,20 @@ struct TeamCtx {
 
     // One reaction (KineticsMap_CHEMKIN.hpp:1192-1386, 1683-1768), with the formation
     // rates (StoichiometricMap.hpp:656-730) accumulated reaction-major into LDS.
+    // 1/Kc = exp(sum_k nu_k g_k - dn ln(Patm/RT)) with g = h/RT - s/R, which equals
+    // the reference's exp(-dS/R + dH/RT - dn ln(Patm/RT)).
     __device__ __forceinline__ void reaction(int j, double T, double logT, double uT, double logPRT, double cTot) {
-        double kf = exp(L.A[j] + L.B[j] * logT - L.E[j] * uT);
-        const unsigned info = L.info[j];
+        const uint4 w = L.W[j];
+        const double2 ab = ((const double2 *)L.P)[2 * j];
+        const double2 ed = ((const double2 *)L.P)[2 * j + 1];
+        const unsigned info = w.x;
+        double kf = exp(ab.x + ab.y * logT - ed.x * uT);
         if (info & 2u) kf = -kf;
         double kmod = kf;
         const int tb = (int)((info >> 8) & 0xffu) - 1;
         if (tb >= 0) {
             double Me = cTot;
-            for (unsigned q = L.tbp[tb]; q < L.tbp[tb + 1]; q++) Me += sc[L.tbs[q]] * L.tbe[q];
+            for (unsigned q = L.tbp[tb]; q < L.tbp[tb + 1]; q++) Me += scg[L.tbs[q]].x * L.tbe[q];
             kmod *= Me;
         }
         const int fo = (int)((info >> 16) & 0xffu) - 1;
@@ -269,9 +289,9 @@ struct TeamCtx {
             double Mc;
             if (L.fos[fo] < 0) {
                 Mc = cTot;
-                for (unsigned q = L.fol[fo]; q < L.fol[fo + 1]; q++) Mc += sc[L.fos2[q]] * L.foe[q];
+                for (unsigned q = L.fol[fo]; q < L.fol[fo + 1]; q++) Mc += scg[L.fos2[q]].x * L.foe[q];
             } else {
-                Mc = sc[L.fos[fo]] + 1.e-16;
+                Mc = scg[L.fos[fo]].x + 1.e-16;
             }
             const double Pr = kf * Mc / kinf;
             double wF = 1.;
@@ -293,32 +313,49 @@ struct TeamCtx {
             }
             kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
         }
-        const unsigned lst = L.list[j];
+        if (info & 4u) {
+            // fixed-format record: three reactant and three product slots
+            const unsigned r0 = w.y & 0xffu, r1 = (w.y >> 8) & 0xffu, r2 = (w.y >> 16) & 0xffu;
+            const unsigned p0 = w.z & 0xffu, p1 = (w.z >> 8) & 0xffu, p2 = (w.z >> 16) & 0xffu;
+            const double2 a0 = scg[r0], a1 = scg[r1], a2 = scg[r2];
+            const double2 b0 = scg[p0], b1 = scg[p1], b2 = scg[p2];
+            double net = (a0.x * a1.x) * a2.x;
+            if (info & 1u) {
+                const double dg = (b0.y + b1.y + b2.y) - (a0.y + a1.y + a2.y);
+                net -= ((b0.x * b1.x) * b2.x) * exp(dg - logPRT * ed.y);
+            }
+            const double r = net * kmod;
+            atomicAdd(&sR[r0], -r);
+            atomicAdd(&sR[r1], -r);
+            atomicAdd(&sR[r2], -r);
+            atomicAdd(&sR[p0], r);
+            atomicAdd(&sR[p1], r);
+            atomicAdd(&sR[p2], r);
+            return;
+        }
+        // general stoichiometry (any orders, any number of species)
+        const unsigned lst = w.w;
         const unsigned e0 = lst & 0xffffu, nre = (lst >> 16) & 0xffu, npr = lst >> 24;
         double pf = 1.;
         for (unsigned q = 0; q < nre; q++) {
             const unsigned e = L.ent[e0 + q];
-            pf *= conc_pow(sc[e & 0xfffu], L.nu(e));
+            pf *= conc_pow(scg[e & 0xfffu].x, L.nu(e));
         }
         double net = pf;
         if (info & 1u) {
-            double dh = 0., ds = 0., pb = 1.;
+            double dg = 0., pb = 1.;
             for (unsigned q = 0; q < nre; q++) {
                 const unsigned e = L.ent[e0 + q];
-                const unsigned k = e & 0xfffu;
-                const double nu = L.nu(e);
-                dh -= nu * sh[k];
-                ds -= nu * ss[k];
+                dg -= L.nu(e) * scg[e & 0xfffu].y;
             }
             for (unsigned q = 0; q < npr; q++) {
                 const unsigned e = L.ent[e0 + nre + q];
-                const unsigned k = e & 0xfffu;
+                const double2 cg = scg[e & 0xfffu];
                 const double nu = L.nu(e);
-                dh += nu * sh[k];
-                ds += nu * ss[k];
-                pb *= conc_pow(sc[k], nu);
+                dg += nu * cg.y;
+                pb *= conc_pow(cg.x, nu);
             }
-            net -= pb * exp(-ds + dh - logPRT * L.dn[j]);
+            net -= pb * exp(dg - logPRT * ed.y);
         }
         const double r = net * kmod;
         for (unsigned q = 0; q < nre + npr; q++) {
@@ -328,13 +365,21 @@ struct TeamCtx {
         }
     }
 
-    // BatchReactorHomogeneousConstantPressure::Equations (:175-247)
     __device__ __forceinline__ void rhs(int src, int dst) {
+        LSM_TIC(t0);
+        rhs_impl(src, dst);
+#ifdef LSM_PHASE_TIMERS
+        if (!in_jac) LSM_TOC(0, t0);
+#endif
+    }
+
+    // BatchReactorHomogeneousConstantPressure::Equations (:175-247)
+    __device__ __forceinline__ void rhs_impl(int src, int dst) {
         const double T = readlane_d(V[src][0], ns);
         const int k = lane;
         const bool isp = k < ns;
         const double Yk = isp ? V[src][0] : 0.;
-        const int P = L.P;
+        const int P = L.SP;
         const double mwk = isp ? L.sp[15 * P + k] : 1.;
         const double MW = 1. / wave_sum(Yk / mwk);
         const double xk = (MW * Yk) / mwk;
@@ -351,10 +396,9 @@ struct TeamCtx {
             h = a0 + T * (0.50 * a1) + T2 * ((1. / 3.) * a2) + T3 * (0.25 * a3) + T4 * (0.20 * a4) + uT * a[5 * P];
             const double s = a0 * logT + T * a1 + T2 * (0.50 * a2) + T3 * ((1. / 3.) * a3) + T4 * (0.25 * a4) +
                              a[6 * P];
-            sc[k] = ck;
-            sh[k] = h;
-            ss[k] = s;
+            scg[k] = make_double2(ck, h - s);
         }
+        if (k == LSM_DUMMY_SP) scg[k] = make_double2(1., 0.);
         sR[k] = 0.;
         double cTot = ck, cpx = isp ? cp * xk : 0.;
         wave_sum2(cTot, cpx);
@@ -375,7 +419,7 @@ struct TeamCtx {
     }
 
     // KernelDense::NumericalJacobian, max-constraint branch (KernelDense.hpp:160-183)
-    __device__ __forceinline__ void jacobian(double h) {
+    __device__ __forceinline__ void jacobian_impl(double h) {
         const double ZERO_DER = sqrt(TINY_F), ETA2 = sqrt(EPS_D), BETA = 1.e+3 * EPS_D;
         double fw = V[F_][0] * V[OOE_][0];
         double hf = BETA * fabs(h) * sqrt(wave_sum(fw * fw) / (double)ne) * (double)ne;
@@ -397,16 +441,24 @@ struct TeamCtx {
         }
     }
 
-    // G = I - hr0 J and row-pivoted LU in registers (KernelDense.hpp:196-221).
-    // Rows are not moved: lane i keeps row i and records the step ps at which it
-    // was chosen as pivot; the largest |G(i,k)| among rows not yet chosen wins
-    // (ties and the 6 lowest mantissa bits resolved toward the lower lane).
-    __device__ __forceinline__ void factor(double hr0) {
-        wave_sync();
+    // G = I - hr0 J and row-pivoted LU (KernelDense.hpp:196-221), factorised in
+    // registers (lane i holds row i, the pivot row is broadcast with v_readlane)
+    // and stored to LDS for the solves.  Rows are not moved: lane i records the
+    // step ps at which it was chosen; the largest |G(i,k)| among rows not yet
+    // chosen wins (ties and the 6 lowest mantissa bits go to the lower lane).
+    __device__ __forceinline__ void factor_impl(double hr0) {
+        // Opaque copies of lane and ne: without them the compiler hoists the NMAX
+        // identity entries and the NMAX "j < ne" masks out of the persistent cell
+        // loop and keeps them live (in AGPRs / spilled SGPRs) for the whole kernel.
+        int lane = this->lane, ne = this->ne;
+        asm volatile("" : "+v"(lane));
+        asm volatile("" : "+s"(ne));
+        double G[NMAX];
 #pragma unroll
         for (int j = 0; j < NMAX; j++) {
             const double jij = (j < ne) ? Jl[j * 64 + lane] : 0.;
             G[j] = jij * -hr0 + ((lane == j) ? 1. : 0.);
+            if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
         }
         ps = (lane < ne) ? -1 : 1 << 20;
         dinv = 1.;
@@ -432,47 +484,61 @@ struct TeamCtx {
                 for (int j = k + 1; j < NMAX; j++) {
                     const double u = readlane_d(G[j], p);
                     G[j] = fma(-l, u, G[j]);
+                    if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
                 }
+                LU[k * 64 + lane] = G[k];
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
 
-    // Forward / backward substitution in pivot order.
-    __device__ __forceinline__ void solve(int vid) {
+    // Forward / backward substitution in pivot order, factors read from LDS.
+    __device__ __forceinline__ void solve_impl(int vid) {
         double b = V[vid][0];
-#pragma unroll
-        for (int k = 0; k < NMAX; k++) {
-            if (k < ne) {
-                const int p = __ffsll((unsigned long long)__ballot(ps == k)) - 1;
-                const double yk = readlane_d(b, p);
-                if (ps > k) b = fma(-G[k], yk, b);
-            }
+        for (int k = 0; k < ne; k++) {
+            const int p = __ffsll((unsigned long long)__ballot(ps == k)) - 1;
+            const double lk = LU[k * 64 + lane];
+            const double yk = readlane_d(b, p);
+            if (ps > k) b = fma(-lk, yk, b);
         }
         double xr = 0.;
-#pragma unroll
-        for (int k = NMAX - 1; k >= 0; k--) {
-            if (k < ne) {
-                const int p = __ffsll((unsigned long long)__ballot(ps == k)) - 1;
-                const double xk = readlane_d(b * dinv, p);
-                if (ps < k) b = fma(-G[k], xk, b);
-                if (lane == k) xr = xk;
-            }
+        for (int k = ne - 1; k >= 0; k--) {
+            const int p = __ffsll((unsigned long long)__ballot(ps == k)) - 1;
+            const double uk = LU[k * 64 + lane];
+            const double xk = readlane_d(b * dinv, p);
+            if (ps < k) b = fma(-uk, xk, b);
+            if (lane == k) xr = xk;
         }
         V[vid][0] = xr;
     }
 
-    __device__ __forceinline__ void jac_times(int src, int dst) {
+    __device__ __forceinline__ void jac_times_impl(int src, int dst) {
         const double v = V[src][0];
         double acc = 0.;
         for (int j = 0; j < ne; j++) acc += Jl[j * 64 + lane] * readlane_d(v, j);
         V[dst][0] = acc;
     }
+    __device__ __forceinline__ void jacobian(double h) {
+        LSM_TIC(t0);
+#ifdef LSM_PHASE_TIMERS
+        in_jac = 1;
+#endif
+        jacobian_impl(h);
+#ifdef LSM_PHASE_TIMERS
+        in_jac = 0;
+#endif
+        LSM_TOC(1, t0);
+    }
+    __device__ __forceinline__ void factor(double hr0) { LSM_TIC(t0); factor_impl(hr0); LSM_TOC(2, t0); }
+    __device__ __forceinline__ void solve(int vid) { LSM_TIC(t0); solve_impl(vid); LSM_TOC(3, t0); }
+    __device__ __forceinline__ void jac_times(int src, int dst) { LSM_TIC(t0); jac_times_impl(src, dst); LSM_TOC(4, t0); }
 };
+
 
 template <int NMAX>
 __global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
                                                    const double *yin, double *yout, int *status, double *q,
-                                                   int *counters, unsigned *queue) {
+                                                   int *counters, unsigned *queue, double *jws) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     {   // stage the mechanism image (once per workgroup)
         const int4 *src = (const int4 *)M.img.src;
@@ -482,8 +548,9 @@ __global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t
         __syncthreads();
     }
     const int wave = threadIdx.x >> 6;
-    double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * (8 * (64 * M.ne + 256)));
-    TeamCtx<NMAX> c(M, K, lds_raw, wave_lds);
+    double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * (8 * (192 + 64 * (size_t)M.ne)));
+    const size_t gw = (size_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    TeamCtx<NMAX> c(M, K, lds_raw, wave_lds, jws + gw * 64 * (size_t)M.ne);
     const int lane = c.lane, ns = M.ns, ne = M.ne;
     for (;;) {
         unsigned cell_u = 0;
@@ -491,6 +558,14 @@ __global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t
         cell_u = __shfl(cell_u, 0, 64);
         const int64_t cell = (int64_t)__builtin_amdgcn_readfirstlane((int)cell_u);
         if (cell >= n) break;
+        {   // opaque per-cell copies: keep lane/size-derived constants from being hoisted
+            // out of the cell loop (they would stay live in registers for the whole kernel)
+            int l = threadIdx.x & 63, a = M.ne, b = M.ns, r = M.nr;
+            asm volatile("" : "+v"(l));
+            asm volatile("" : "+s"(a), "+s"(b), "+s"(r));
+            c.lane = l; c.ne = a; c.ns = b; c.nr = r;
+        }
+        LSM_TIC(tcell);
         const double yi = lane < ne ? yin[(int64_t)lane * n + cell] : 0.;
         const double T0 = readlane_d(yi, ns);
         double y;
@@ -515,6 +590,10 @@ __global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t
         if (lane == 0) {
             if (status) status[cell] = st;
         }
+#ifdef LSM_PHASE_TIMERS
+        c.tp[5] += clock64() - tcell;
+        c.tp[6] += 1;
+#endif
         if (counters != nullptr && lane < LSM_NCOUNTERS_DEV) {
             const unsigned vals[12] = {cnt.steps, cnt.fcalls, cnt.fcalls_jac, cnt.jacobians, cnt.factorizations,
                                        cnt.solves, cnt.conv_failures, cnt.err_failures, cnt.max_order,
@@ -526,6 +605,10 @@ __global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t
             counters[cell * LSM_NCOUNTERS_DEV + lane] = (int)v;
         }
     }
+#ifdef LSM_PHASE_TIMERS
+    if (lane == 0 && K.phase_cycles)
+        for (int i = 0; i < 7; i++) atomicAdd(&K.phase_cycles[i], c.tp[i]);
+#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -737,7 +820,9 @@ __global__ void reset_queue_kernel(unsigned *queue) { *queue = 0u; }
 // ---------------------------------------------------------------------------
 int team_max_neq() { return 64; }
 
-size_t team_wave_bytes(const DevMech &m) { return (size_t)8 * (64 * (size_t)m.ne + 256); }
+size_t team_wave_bytes(const DevMech &m) { return (size_t)8 * (192 + 64 * (size_t)m.ne); }
+
+size_t team_jac_doubles(const DevMech &m) { return (size_t)64 * m.ne; }
 
 int team_waves_per_block(const DevMech &m, size_t lds_per_block) {
     if (!m.img.team_ok || m.ne > 64) return 0;
@@ -750,26 +835,26 @@ int team_waves_per_block(const DevMech &m, size_t lds_per_block) {
 template <int NMAX>
 static int launch_team_n(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                          double *yout, int *status, double *q, int *counters, unsigned *queue, int grid, int wpb,
-                         hipStream_t s) {
+                         double *jws, hipStream_t s) {
     const size_t lds = (size_t)m.img.bytes + (size_t)wpb * team_wave_bytes(m);
     if (lds > 65536 &&
         hipFuncSetAttribute((const void *)team_kernel<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
             hipSuccess)
         return -1;
     hipLaunchKernelGGL(team_kernel<NMAX>, dim3(grid), dim3(64 * wpb), lds, s, m, k, n, dt, yin, yout, status, q,
-                       counters, queue);
+                       counters, queue, jws);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
-                int *status, double *q, int *counters, unsigned *queue, int grid, int wpb, void *stream) {
+                int *status, double *q, int *counters, unsigned *queue, int grid, int wpb, double *jws, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     if (wpb < 1 || wpb > 4) return -2;
-    if (m.ne <= 16) return launch_team_n<16>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, s);
-    if (m.ne <= 32) return launch_team_n<32>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, s);
-    if (m.ne <= 48) return launch_team_n<48>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, s);
-    if (m.ne <= 56) return launch_team_n<56>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, s);
-    if (m.ne <= 64) return launch_team_n<64>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, s);
+    if (m.ne <= 16) return launch_team_n<16>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
+    if (m.ne <= 32) return launch_team_n<32>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
+    if (m.ne <= 48) return launch_team_n<48>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
+    if (m.ne <= 56) return launch_team_n<56>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
+    if (m.ne <= 64) return launch_team_n<64>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
     return -2;
 }
 
