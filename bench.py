@@ -53,6 +53,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=None, help="subsample of the field (default: all 262144)")
     ap.add_argument("--method", default="team", choices=["team", "plain", "auto"])
+    ap.add_argument("--jacobian", default="numerical", choices=["numerical", "analytic"])
     ap.add_argument("--cpu-sample", type=int, default=96)
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -82,7 +83,7 @@ def main():
     ode.SetType("OpenSMOKE_GPU")
     ode.SetRelativeTolerance(rtol)
     ode.SetAbsoluteTolerance(atol)
-    step = api.ChemistryStep(mech, ode, method=args.method)
+    step = api.ChemistryStep(mech, ode, method=args.method, jacobian=args.jacobian)
     stream = torch.cuda.current_stream(dev)
 
     def run():
@@ -117,7 +118,8 @@ def main():
         "data": "synthetic (seeded mechanism and cell field)",
         "config": {"workload": w["name"] if args.cells is None else f"{w['name']}-sub{n}",
                    "mechanism": f"{mech.ns} species / {mech.nr} reactions", "cells": n, "dt_s": dt,
-                   "rtol": rtol, "atol": atol, "method": args.method},
+                   "rtol": rtol, "atol": atol, "method": args.method,
+                   "jacobian": args.jacobian},
         "p50_batch_ms": float(np.percentile(lat_ms, 50)), "p99_batch_ms": float(np.percentile(lat_ms, 99)),
         "roofline": {"bound": "mfma", "pipe": "fp64", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": None,

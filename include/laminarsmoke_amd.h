@@ -150,6 +150,20 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode,
 int lsm_set_method(lsm_handle *h, int32_t method);
 int lsm_destroy(lsm_handle *h);
 
+/* Jacobian used by the Newton iteration of the team kernel.  NUMERICAL (default)
+ * is the forward-difference Jacobian of KernelDense::NumericalJacobian
+ * (math/multivalue-ode-solvers/KernelDense.hpp:160-183), ne extra RHS evaluations
+ * per Jacobian, as the reference does for every OpenSMOKE integration.  ANALYTIC
+ * builds dR/dc and dR/dT of the CHEMKIN kinetics in closed form (no extra RHS
+ * evaluations); it changes the Newton matrix only, not the error control, so the
+ * result stays within the integration tolerance of the reference's.  The plain
+ * kernel always uses NUMERICAL. */
+typedef enum lsm_jacobian_type {
+    LSM_JACOBIAN_NUMERICAL = 0,
+    LSM_JACOBIAN_ANALYTIC = 1
+} lsm_jacobian_type;
+int lsm_set_jacobian(lsm_handle *h, int32_t type);
+
 /* Chemistry step over n_cells cells, host pointers (the whole forAll loop of
  * chemistry_DI.H:92-328).  y_in/y_out: SoA [ns+1][n_cells] (species rows, then T).
  * dt: per-cell integration time (DeltaTCells).  status (may be NULL): OdeStatus per
@@ -164,6 +178,18 @@ int lsm_integrate_batch(lsm_handle *h, int64_t n_cells, const double *dt, const 
 int lsm_integrate_batch_device(lsm_handle *h, int64_t n_cells, const double *d_dt,
                                const double *d_y_in, double *d_y_out, int32_t *d_status,
                                double *d_q, int32_t *d_counters, void *stream);
+
+/* Right-hand side and analytic Jacobian of the constant-pressure batch reactor at
+ * n_cells states: BatchReactorHomogeneousConstantPressure_ODE_OpenSMOKE::GetEquations
+ * (solvers/laminarSMOKE/unsteady/BatchReactorHomogeneousConstantPressure_ODE_Interface.H:96)
+ * on a batch, plus the Jacobian that GetAnalyticalJacobian (same file, :137) leaves
+ * empty.  y, f: SoA [ns+1][n_cells]; jac: [n_cells][ns+1][ns+1] row-major,
+ * jac[c][i][j] = d f_i / d y_j.  f or jac may be NULL.  Host pointers, synchronous;
+ * the _device variant takes device pointers and a hipStream_t and does not
+ * synchronise.  Needs ns+1 <= 64. */
+int lsm_equations(lsm_handle *h, int64_t n_cells, const double *y, double *f, double *jac);
+int lsm_equations_device(lsm_handle *h, int64_t n_cells, const double *d_y, double *d_f, double *d_jac,
+                         void *stream);
 
 /* Text of the last error of this thread ("" if none). */
 const char *lsm_last_error(void);

@@ -26,6 +26,7 @@ LIB_PATH = os.environ.get("LSM_LIB_PATH") or os.path.join(os.path.dirname(os.pat
 ODE_INTEGRATOR = {"OpenSMOKE": 0, "BzzOde": 1, "CVODE": 2, "DVODE": 3, "DASPK": 4, "DLSODE": 5, "DLSODA": 6,
                   "RADAU5": 7, "MEBDF": 8, "CHEMEQ2": 9, "OpenSMOKE_GPU": 100}
 METHODS = {"auto": 0, "plain": 1, "team": 2}
+JACOBIANS = {"numerical": 0, "analytic": 1}
 STATUS_NAMES = {0: "skipped", 2: "ok", -1: "max steps", -2: "too strict tolerances", -4: "error-test failures",
                 -5: "convergence failures", -6: "step too small", -9: "illegal constraints"}
 
@@ -51,6 +52,9 @@ def lib():
         L.lsm_reactor_options_default.argtypes = [vp]
         L.lsm_create.argtypes = [vp, vp, vp, ctypes.POINTER(vp)]
         L.lsm_set_method.argtypes = [vp, ctypes.c_int32]
+        L.lsm_set_jacobian.argtypes = [vp, ctypes.c_int32]
+        L.lsm_equations.argtypes = [vp, i64, vp, vp, vp]
+        L.lsm_equations_device.argtypes = [vp, i64, vp, vp, vp, vp]
         L.lsm_destroy.argtypes = [vp]
         L.lsm_integrate_batch.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.lsm_integrate_batch_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
@@ -118,7 +122,7 @@ def reactor_options(pressure=101325.0, min_temperature=0.0, energy=True, mass_fr
 class ChemistryStep:
     """Batched chemistry step of one mechanism on the current HIP device."""
 
-    def __init__(self, mech, ode: ODE_Parameters | None = None, method="auto", **reactor_kw):
+    def __init__(self, mech, ode: ODE_Parameters | None = None, method="auto", jacobian="numerical", **reactor_kw):
         self.mech = mech
         self.ode = ode or ODE_Parameters()
         self.reactor = reactor_options(**reactor_kw)
@@ -127,9 +131,24 @@ class ChemistryStep:
                                 ctypes.byref(self.reactor), ctypes.byref(h)))
         self.h = h
         self.set_method(method)
+        self.set_jacobian(jacobian)
 
     def set_method(self, method):
         _check(lib().lsm_set_method(self.h, METHODS[method] if isinstance(method, str) else int(method)))
+
+    def set_jacobian(self, kind):
+        _check(lib().lsm_set_jacobian(self.h, JACOBIANS[kind] if isinstance(kind, str) else int(kind)))
+
+    def equations(self, y, jacobian=True):
+        """Reactor RHS f[ns+1, n] and (optionally) analytic Jacobian J[n, ns+1, ns+1] at states y[ns+1, n]."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        neq, n = y.shape
+        if neq != self.mech.neq:
+            raise ValueError(f"y has {neq} rows, mechanism needs {self.mech.neq}")
+        f = np.empty_like(y)
+        J = np.empty((n, neq, neq)) if jacobian else None
+        _check(lib().lsm_equations(self.h, n, y.ctypes.data, f.ctypes.data, J.ctypes.data if J is not None else None))
+        return f, J
 
     def close(self):
         if self.h:

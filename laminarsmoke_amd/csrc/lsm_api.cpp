@@ -322,6 +322,13 @@ static int check_mechanism(const lsm_mechanism *m) {
         for (int q = m->fo_ptr[s]; q < m->fo_ptr[s + 1]; q++)
             if (m->fo_sp[q] < 0 || m->fo_sp[q] >= m->ns) return fail(LSM_E_INVALID, "fall-off species index");
     }
+    // a CHEMKIN reaction is either "+M" (third body) or "(+M)" (fall-off), once
+    std::vector<char> seen(m->nr, 0);
+    for (int s = 0; s < m->ntb + m->nfo; s++) {
+        const int j = s < m->ntb ? m->tb_rxn[s] : m->fo_rxn[s - m->ntb];
+        if (seen[j]) return fail(LSM_E_INVALID, "reaction listed twice among third-body / fall-off reactions");
+        seen[j] = 1;
+    }
     return LSM_OK;
 }
 
@@ -610,6 +617,7 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
     k.min_temperature = r.min_temperature;
     k.energy = r.energy_equation ? 1 : 0;
     k.full_pivoting = p.full_pivoting ? 1 : 0;
+    k.jac_mode = LSM_JACOBIAN_NUMERICAL;
     k.phase_cycles = nullptr;
 #ifdef LSM_PHASE_TIMERS
     if (hipMalloc(&k.phase_cycles, 8 * sizeof(unsigned long long)) != hipSuccess ||
@@ -643,6 +651,13 @@ int lsm_set_method(lsm_handle *h, int32_t method) {
         return fail(LSM_E_UNSUPPORTED, "team kernel needs ns+1 <= 64, a mechanism image that fits in LDS "
                                        "and partial pivoting");
     h->method = method;
+    return LSM_OK;
+}
+
+int lsm_set_jacobian(lsm_handle *h, int32_t type) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (type != LSM_JACOBIAN_NUMERICAL && type != LSM_JACOBIAN_ANALYTIC) return fail(LSM_E_INVALID, "unknown Jacobian type");
+    h->kp.jac_mode = type;
     return LSM_OK;
 }
 
@@ -735,6 +750,45 @@ int lsm_integrate_batch(lsm_handle *h, int64_t n, const double *dt, const double
     if (q) HIP_OK(hipMemcpy(q, h->d_q, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (counters) HIP_OK(hipMemcpy(counters, h->d_cnt, sizeof(int) * LSM_NCOUNTERS * n, hipMemcpyDeviceToHost));
     return LSM_OK;
+}
+
+int lsm_equations_device(lsm_handle *h, int64_t n, const double *d_y, double *d_f, double *d_jac, void *stream) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
+    if (n == 0 || (!d_f && !d_jac)) return LSM_OK;
+    if (!d_y) return fail(LSM_E_INVALID, "null y pointer");
+    if (h->team_wpb < 1) return fail(LSM_E_UNSUPPORTED, "equations need ns+1 <= 64 and a mechanism image that fits in LDS");
+    if (launch_equations(h->dm, h->kp, n, d_y, d_f, d_jac, h->team_grid, h->team_wpb, h->d_jws, stream))
+        return fail(LSM_E_HIP, std::string("equations kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
+    return LSM_OK;
+}
+
+int lsm_equations(lsm_handle *h, int64_t n, const double *y, double *f, double *jac) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
+    if (n == 0 || (!f && !jac)) return LSM_OK;
+    if (!y) return fail(LSM_E_INVALID, "null y pointer");
+    const size_t ne = (size_t)h->dm.ne;
+    double *dy = nullptr, *df = nullptr, *dj = nullptr;
+    auto release = [&]() {
+        if (dy) (void)hipFree(dy);
+        if (df) (void)hipFree(df);
+        if (dj) (void)hipFree(dj);
+    };
+    int rc = LSM_OK;
+    if (hipMalloc(&dy, sizeof(double) * ne * n) != hipSuccess || (f && hipMalloc(&df, sizeof(double) * ne * n) != hipSuccess) ||
+        (jac && hipMalloc(&dj, sizeof(double) * ne * ne * n) != hipSuccess)) {
+        release();
+        return fail(LSM_E_NOMEM, "equations buffers");
+    }
+    if (hipMemcpy(dy, y, sizeof(double) * ne * n, hipMemcpyHostToDevice) != hipSuccess) rc = fail(LSM_E_HIP, "copy y");
+    if (!rc) rc = lsm_equations_device(h, n, dy, df, dj, nullptr);
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = fail(LSM_E_HIP, "equations kernel failed");
+    if (!rc && f && hipMemcpy(f, df, sizeof(double) * ne * n, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(LSM_E_HIP, "copy f");
+    if (!rc && jac && hipMemcpy(jac, dj, sizeof(double) * ne * ne * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(LSM_E_HIP, "copy J");
+    release();
+    return rc;
 }
 
 #ifdef LSM_PHASE_TIMERS

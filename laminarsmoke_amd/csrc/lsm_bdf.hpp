@@ -21,7 +21,8 @@
 //   double reduce(double partial)   sum over the elements of a cell (uniform result)
 //   int count(bool)                 number of elements for which the flag holds
 //   void rhs(int src, int dst)      dst = f(src); updates c.qr
-//   void jacobian(double h)         FD Jacobian at VB with f at F, weights OOE
+//   int jacobian(double h)          Jacobian at VB (FD with f at F, weights OOE, or
+//                                   analytic); returns the RHS evaluations it used
 //   void factor(double hr0)         G = I - hr0 J, LU
 //   void solve(int vid)             in place G^-1
 //   void jac_times(int src, int dst)
@@ -131,9 +132,9 @@ __device__ __forceinline__ unsigned find_correction(C &c, const BdfConst &K, Bdf
     if (s.jacobianStatus == J_CHANGE) {
         s.stepOfLastJacobian = n.steps;
         s.jacobianStatus = J_MODIFIED;
-        c.jacobian(s.h);
-        n.fcalls_jac += c.ne;
-        n.rhs_total += c.ne;
+        const int nj = c.jacobian(s.h);   // FD: ne RHS evaluations; analytic: none
+        n.fcalls_jac += nj;
+        n.rhs_total += nj;
         n.jacobians++;
         s.factorizationStatus = M_TO_FACTOR;
     }

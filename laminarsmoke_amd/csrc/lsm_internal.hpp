@@ -75,6 +75,7 @@ struct KParams {
     double min_temperature;
     int energy;
     int full_pivoting;
+    int jac_mode;                       // 0: forward differences (KernelDense), 1: analytic
 };
 
 // Host-side launchers (lsm_kernels.hip).
@@ -82,7 +83,7 @@ int team_max_neq();
 // LDS layout of a team workgroup: the mechanism image, then per wave the species
 // scratch (4 x 64 doubles).  Each wave's Jacobian (ne x 64 doubles) lives in an
 // HBM workspace indexed by the global wave number.
-size_t team_wave_bytes(const DevMech &m);
+__host__ __device__ size_t team_wave_bytes(const DevMech &m);
 size_t team_jac_doubles(const DevMech &m);
 int team_waves_per_block(const DevMech &m, size_t lds_per_block);
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
@@ -92,6 +93,10 @@ int launch_plain(const DevMech &m, const KParams &k, int64_t n, const double *dt
                  double *yout, int *status, double *q, int *counters, double *work, int64_t slots,
                  void *stream);
 size_t plain_work_doubles(const DevMech &m, int64_t slots);
+// Reactor right-hand side f (SoA [ne][n]) and analytic Jacobian J ([n][ne][ne]) at
+// the states yin; either output may be null.  Uses the team workspace jws.
+int launch_equations(const DevMech &m, const KParams &k, int64_t n, const double *yin, double *f, double *J,
+                     int grid, int waves_per_block, double *jws, void *stream);
 int launch_reset_queue(unsigned *queue, void *stream);
 
 }  // namespace lsm

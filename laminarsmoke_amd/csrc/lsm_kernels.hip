@@ -20,6 +20,7 @@
 namespace lsm {
 
 constexpr int LSM_NCOUNTERS_DEV = 12;
+constexpr double LN10 = 2.302585092994045684;
 constexpr double RGAS = LSM_R_J_KMOL;
 constexpr double NEG_INF = -__builtin_huge_val();
 constexpr double POS_INF = __builtin_huge_val();
@@ -230,11 +231,14 @@ template <int NMAX>
 struct TeamCtx {
     double V[NVEC][1];
     int ps;          // elimination step at which this lane's row became the pivot
+    int plv;         // lane k holds the row (lane) chosen as pivot at step k
     double dinv;     // 1 / pivot of this lane's row
     double *Jl;                       // Jacobian [ne][64] of this wave (HBM/L2 workspace)
     double *LU;                       // per-wave LDS: LU factors [ne][64], lane i holds row i
     double2 *scg;                     // per-wave LDS: (c_k, g_k = h_k/RT - s_k/R) per species slot
     double *sR;                       // per-wave LDS: formation-rate accumulators
+    double *sDT;                      // per-wave LDS: dR_k/dT accumulators (analytic Jacobian)
+    double *sH;                       // per-wave LDS: h_k = H_k/RT (analytic Jacobian)
     LdsMech L;
     const KParams &K;
     int lane, ne, ns, nr;
@@ -249,7 +253,9 @@ struct TeamCtx {
         Jl = jac;
         scg = (double2 *)wave_lds;
         sR = wave_lds + 128;
-        LU = wave_lds + 192;
+        sDT = wave_lds + 192;
+        sH = wave_lds + 256;
+        LU = wave_lds + 320;
     }
 
     template <class F>
@@ -418,6 +424,292 @@ struct TeamCtx {
         V[dst][0] = dy;
     }
 
+    // ---- analytic Jacobian ------------------------------------------------------
+    // One reaction's contributions to R_k (sR), dR_k/dT at fixed c (sDT) and
+    // dR_k/dc_i (LU region, [i*65 + k]).  kmod is the effective rate constant
+    // (Arrhenius x M for third-body reactions, k_inf Pr/(1+Pr) F for fall-off),
+    // dkdM its derivative with respect to the collider concentration M.
+    __device__ __forceinline__ void reaction_jac(int j, double T, double logT, double uT, double logPRT, double cTot) {
+        double *D = LU;
+        const uint4 w = L.W[j];
+        const double2 ab = ((const double2 *)L.P)[2 * j];
+        const double2 ed = ((const double2 *)L.P)[2 * j + 1];
+        const unsigned info = w.x;
+        double kf = exp(ab.x + ab.y * logT - ed.x * uT);
+        if (info & 2u) kf = -kf;
+        const double dlnkf = (ab.y + ed.x * uT) * uT;
+        double kmod = kf, dlnkmod = dlnkf, dkdM = 0.;
+        int dense = 0, single = -1;
+        const int tb = (int)((info >> 8) & 0xffu) - 1;
+        const int fo = (int)((info >> 16) & 0xffu) - 1;
+        if (tb >= 0) {
+            double Me = cTot;
+            for (unsigned q = L.tbp[tb]; q < L.tbp[tb + 1]; q++) Me += scg[L.tbs[q]].x * L.tbe[q];
+            kmod = kf * Me;
+            dkdM = kf;
+            dense = 1;
+        }
+        if (fo >= 0) {
+            const double *fp = L.fop + 8 * fo;
+            const double kinf = exp(fp[0] + fp[1] * logT - fp[2] * uT);
+            const double dlnkinf = (fp[1] + fp[2] * uT) * uT;
+            double Mc;
+            single = L.fos[fo];
+            if (single < 0) {
+                Mc = cTot;
+                for (unsigned q = L.fol[fo]; q < L.fol[fo + 1]; q++) Mc += scg[L.fos2[q]].x * L.foe[q];
+                dense = 2;
+            } else {
+                Mc = scg[single].x + 1.e-16;
+            }
+            const double Pr = kf * Mc / kinf;
+            double F = 1., dlgF_dlgPr = 0., dlnF_dT = 0.;
+            if (fp[7] != 0.) {   // Troe: log10 F = L / (1 + x^2), x = s / (n - 0.14 s), s = log10 Pr + c
+                const double a = fp[3];
+                const double e3 = exp(-T / fp[4]), e1 = exp(-T / fp[5]);
+                double Fc = (1. - a) * e3 + a * e1;
+                double dFc = -(1. - a) / fp[4] * e3 - a / fp[5] * e1;
+                if (fp[6] != 0.) {
+                    const double e2 = exp(-fp[6] / T);
+                    Fc += e2;
+                    dFc += fp[6] * uT * uT * e2;
+                }
+                const double Lf = (Fc < 1.e-300) ? -300. : log10(Fc);
+                const double dL = (Fc < 1.e-300) ? 0. : dFc / (Fc * LN10);
+                double lgF, dlgF_dL;
+                if (Pr > 1.e-32) {
+                    const double n = 0.75 - 1.27 * Lf, cc = -0.4 - 0.67 * Lf;
+                    const double sT = log10(Pr) + cc;
+                    const double den = n - 0.14 * sT;
+                    const double x = sT / den;
+                    const double q = 1. + x * x;
+                    lgF = Lf / q;
+                    const double dlgF_dx = -Lf * 2. * x / (q * q);
+                    dlgF_dlgPr = dlgF_dx * n / (den * den);
+                    dlgF_dL = 1. / q + dlgF_dx * (1.27 * sT - 0.67 * n) / (den * den);
+                } else {
+                    const double q = 1. + (1. / 0.14) * (1. / 0.14);
+                    lgF = Lf / q;
+                    dlgF_dL = 1. / q;
+                }
+                F = pow(10., lgF);
+                dlnF_dT = dlgF_dlgPr * (dlnkf - dlnkinf) + LN10 * dlgF_dL * dL;
+            }
+            kmod = kinf * (Pr / (1. + Pr)) * F;
+            dkdM = (kf * F / (1. + Pr)) * (1. / (1. + Pr) + dlgF_dlgPr);   // d kmod / d M
+            dlnkmod = dlnkinf + (dlnkf - dlnkinf) / (1. + Pr) + dlnF_dT;
+        }
+        if (info & 4u) {
+            const unsigned sl[6] = {w.y & 0xffu, (w.y >> 8) & 0xffu, (w.y >> 16) & 0xffu,
+                                    w.z & 0xffu, (w.z >> 8) & 0xffu, (w.z >> 16) & 0xffu};
+            double cv[6];
+#pragma unroll
+            for (int e = 0; e < 6; e++) cv[e] = scg[sl[e]].x;
+            const double pf = (cv[0] * cv[1]) * cv[2];
+            double g[6];
+            g[0] = kmod * (cv[1] * cv[2]);
+            g[1] = kmod * (cv[0] * cv[2]);
+            g[2] = kmod * (cv[0] * cv[1]);
+            double uK = 0., pb = 0., dh = 0.;
+            g[3] = g[4] = g[5] = 0.;
+            if (info & 1u) {
+                const double dg = (scg[sl[3]].y + scg[sl[4]].y + scg[sl[5]].y) - (scg[sl[0]].y + scg[sl[1]].y + scg[sl[2]].y);
+                uK = exp(dg - logPRT * ed.y);
+                pb = (cv[3] * cv[4]) * cv[5];
+                dh = (sH[sl[3]] + sH[sl[4]] + sH[sl[5]]) - (sH[sl[0]] + sH[sl[1]] + sH[sl[2]]);
+                const double kb = -kmod * uK;
+                g[3] = kb * (cv[4] * cv[5]);
+                g[4] = kb * (cv[3] * cv[5]);
+                g[5] = kb * (cv[3] * cv[4]);
+            }
+            const double net = pf - pb * uK;
+            const double r = kmod * net;
+            const double drdT = kmod * (dlnkmod * net - pb * uK * (ed.y - dh) * uT);
+#pragma unroll
+            for (int o = 0; o < 6; o++) {
+                atomicAdd(&sR[sl[o]], o < 3 ? -r : r);
+                atomicAdd(&sDT[sl[o]], o < 3 ? -drdT : drdT);
+            }
+#pragma unroll
+            for (int d = 0; d < 6; d++) {
+                if (sl[d] == LSM_DUMMY_SP) continue;
+#pragma unroll
+                for (int o = 0; o < 6; o++)
+                    if (sl[o] != LSM_DUMMY_SP) atomicAdd(&D[sl[d] * 65 + sl[o]], o < 3 ? -g[d] : g[d]);
+            }
+            if (dense) {
+                const double qv = dkdM * net;
+                for (int i = 0; i < ns; i++)
+#pragma unroll
+                    for (int o = 0; o < 6; o++)
+                        if (sl[o] != LSM_DUMMY_SP) atomicAdd(&D[i * 65 + sl[o]], o < 3 ? -qv : qv);
+                const unsigned q0 = dense == 1 ? L.tbp[tb] : L.fol[fo], q1 = dense == 1 ? L.tbp[tb + 1] : L.fol[fo + 1];
+                for (unsigned q = q0; q < q1; q++) {
+                    const unsigned i = dense == 1 ? L.tbs[q] : L.fos2[q];
+                    const double e1 = qv * (dense == 1 ? L.tbe[q] : L.foe[q]);
+#pragma unroll
+                    for (int o = 0; o < 6; o++)
+                        if (sl[o] != LSM_DUMMY_SP) atomicAdd(&D[i * 65 + sl[o]], o < 3 ? -e1 : e1);
+                }
+            } else if (single >= 0) {
+                const double qv = dkdM * net;
+#pragma unroll
+                for (int o = 0; o < 6; o++)
+                    if (sl[o] != LSM_DUMMY_SP) atomicAdd(&D[single * 65 + sl[o]], o < 3 ? -qv : qv);
+            }
+            return;
+        }
+        // general stoichiometry: d(prod c^nu)/dc_i = nu_i c_i^(nu_i - 1) prod_{others}
+        const unsigned lst = w.w;
+        const unsigned e0 = lst & 0xffffu, nre = (lst >> 16) & 0xffu, npr = lst >> 24;
+        double pf = 1., pb = 1., dg = 0., dh = 0., uK = 0.;
+        for (unsigned q = 0; q < nre; q++) {
+            const unsigned e = L.ent[e0 + q];
+            pf *= conc_pow(scg[e & 0xfffu].x, L.nu(e));
+        }
+        if (info & 1u) {
+            for (unsigned q = 0; q < nre + npr; q++) {
+                const unsigned e = L.ent[e0 + q];
+                const double nu = q < nre ? -L.nu(e) : L.nu(e);
+                dg += nu * scg[e & 0xfffu].y;
+                dh += nu * sH[e & 0xfffu];
+                if (q >= nre) pb *= conc_pow(scg[e & 0xfffu].x, nu);
+            }
+            uK = exp(dg - logPRT * ed.y);
+        }
+        const double net = pf - pb * uK;
+        const double r = kmod * net;
+        const double drdT = kmod * (dlnkmod * net - pb * uK * (ed.y - dh) * uT);
+        for (unsigned q = 0; q < nre + npr; q++) {
+            const unsigned e = L.ent[e0 + q];
+            const double nu = L.nu(e);
+            atomicAdd(&sR[e & 0xfffu], q < nre ? -nu * r : nu * r);
+            atomicAdd(&sDT[e & 0xfffu], q < nre ? -nu * drdT : nu * drdT);
+        }
+        for (unsigned d = 0; d < nre + npr; d++) {
+            const bool fwd = d < nre;
+            if (!fwd && !(info & 1u)) break;
+            const unsigned ed_ = L.ent[e0 + d];
+            const unsigned id = ed_ & 0xfffu;
+            const double nud = L.nu(ed_), cd = scg[id].x;
+            // d(c^nu)/dc; at c = 0 a fractional order has no finite slope: 0 is used
+            double der = nud == 1. ? 1. : ((cd > 0. || nud > 1.) ? nud * conc_pow(cd, nud - 1.) : 0.);
+            const unsigned qa = fwd ? 0 : nre, qb = fwd ? nre : nre + npr;
+            for (unsigned q = qa; q < qb; q++)
+                if (q != d) der *= conc_pow(scg[L.ent[e0 + q] & 0xfffu].x, L.nu(L.ent[e0 + q]));
+            const double gv = fwd ? kmod * der : -kmod * uK * der;
+            for (unsigned o = 0; o < nre + npr; o++) {
+                const unsigned eo = L.ent[e0 + o];
+                const double nu = L.nu(eo);
+                atomicAdd(&D[id * 65 + (eo & 0xfffu)], o < nre ? -nu * gv : nu * gv);
+            }
+        }
+        if (dense || single >= 0) {
+            const double qv = dkdM * net;
+            for (unsigned o = 0; o < nre + npr; o++) {
+                const unsigned eo = L.ent[e0 + o];
+                const double sg = o < nre ? -L.nu(eo) : L.nu(eo);
+                const unsigned ko = eo & 0xfffu;
+                if (single >= 0) {
+                    atomicAdd(&D[single * 65 + ko], sg * qv);
+                    continue;
+                }
+                for (int i = 0; i < ns; i++) atomicAdd(&D[i * 65 + ko], sg * qv);
+                const unsigned q0 = dense == 1 ? L.tbp[tb] : L.fol[fo], q1 = dense == 1 ? L.tbp[tb + 1] : L.fol[fo + 1];
+                for (unsigned q = q0; q < q1; q++) {
+                    const unsigned i = dense == 1 ? L.tbs[q] : L.fos2[q];
+                    atomicAdd(&D[i * 65 + ko], sg * qv * (dense == 1 ? L.tbe[q] : L.foe[q]));
+                }
+            }
+        }
+    }
+
+    // Analytic Jacobian of the constant-pressure reactor at the state in VB:
+    //   f_k = MW_k R_k / rho,  f_T = -sum_k H_k R_k / (rho cp)
+    // with D = dR/dc (T fixed), S_k = sum_i D_ki c_i:
+    //   df_k/dY_j = MW_k / (rho MW_j) (rho D_kj - MW S_k) + f_k MW / MW_j
+    //   df_k/dT   = MW_k / rho (dR_k/dT - S_k / T) + f_k / T
+    //   df_T/dY_j = [-(RT / MW_j)(rho sum_k h_k D_kj - MW sum_k h_k S_k) - f_T (rho / MW_j)(Cp_j - MW cp)] / (rho cp)
+    //   df_T/dT   = [-sum_k (Cp_k R_k + H_k (dR_k/dT - S_k / T)) - f_T (-rho cp / T + rho sum_k Y_k Cp'_k / MW_k)] / (rho cp)
+    // (MW mixture molecular weight, cp mass heat capacity, h_k = H_k / RT).  Written to Jl.
+    __device__ __forceinline__ void jacobian_analytic_impl() {
+        double *D = LU;
+        const double T = readlane_d(V[VB_][0], ns);
+        const int k = lane;
+        const bool isp = k < ns;
+        const double Yk = isp ? V[VB_][0] : 0.;
+        const int P = L.SP;
+        const double mwk = isp ? L.sp[15 * P + k] : 1.;
+        const double MW = 1. / wave_sum(Yk / mwk);
+        const double xk = (MW * Yk) / mwk;
+        const double cTotP = K.pressure / RGAS / T;
+        const double ck = cTotP * xk;
+        const double rho = cTotP * MW;
+        const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, uT = 1. / T, logT = log(T);
+        double cp = 0., h = 0., dcp = 0.;
+        if (isp) {
+            const double *a = L.sp + ((T > L.sp[14 * P + k]) ? 0 : 7 * P) + k;
+            const double a0 = a[0], a1 = a[P], a2 = a[2 * P], a3 = a[3 * P], a4 = a[4 * P];
+            cp = a0 + T * a1 + T2 * a2 + T3 * a3 + T4 * a4;
+            dcp = a1 + T * (2. * a2) + T2 * (3. * a3) + T3 * (4. * a4);
+            h = a0 + T * (0.50 * a1) + T2 * ((1. / 3.) * a2) + T3 * (0.25 * a3) + T4 * (0.20 * a4) + uT * a[5 * P];
+            const double s = a0 * logT + T * a1 + T2 * (0.50 * a2) + T3 * ((1. / 3.) * a3) + T4 * (0.25 * a4) +
+                             a[6 * P];
+            scg[k] = make_double2(ck, h - s);
+            sH[k] = h;
+        }
+        if (k == LSM_DUMMY_SP) { scg[k] = make_double2(1., 0.); sH[k] = 0.; }
+        sR[k] = 0.;
+        sDT[k] = 0.;
+        for (int i = 0; i < ns; i++) D[i * 65 + k] = 0.;
+        double cTot = ck, cpx = isp ? cp * xk : 0.;
+        wave_sum2(cTot, cpx);
+        wave_sync();
+        const double logPRT = log(101325. / RGAS / T);
+        for (int j = lane; j < nr; j += 64) reaction_jac(j, T, logT, uT, logPRT, cTot);
+        wave_sync();
+        const double Rk = isp ? sR[k] : 0.;
+        const double dRdT = isp ? sDT[k] : 0.;
+        double S = 0.;
+        if (isp)
+            for (int i = 0; i < ns; i++) S += D[i * 65 + k] * scg[i].x;
+        const double fk = isp ? (mwk * Rk) / rho : 0.;
+        if (isp) {
+            for (int j = 0; j < ns; j++) {
+                const double mwj = L.sp[15 * P + j];
+                Jl[j * 64 + k] = (mwk / (rho * mwj)) * (rho * D[j * 65 + k] - MW * S) + fk * MW / mwj;
+            }
+            Jl[ns * 64 + k] = (mwk / rho) * (dRdT - S * uT) + fk * uT;
+        } else if (k > ns) {
+            for (int j = 0; j < ne; j++) Jl[j * 64 + k] = 0.;
+        }
+        // temperature row (zero without the energy equation)
+        const double cpm = (cpx * RGAS) / MW;            // mass cp
+        const double Phi = rho * cpm;
+        double hs = isp ? h * Rk : 0., hS = isp ? h * S : 0.;
+        wave_sum2(hs, hS);
+        const double QR = -hs * RGAS * T;
+        const double fT = K.energy ? QR / Phi : 0.;
+        double A = 0.;
+        if (isp)
+            for (int i = 0; i < ns; i++) A += sH[i] * D[k * 65 + i];
+        double dQ = isp ? -(cp * RGAS * Rk + h * RGAS * T * (dRdT - S * uT)) : 0.;
+        double dP = isp ? Yk * dcp * RGAS / mwk : 0.;
+        wave_sum2(dQ, dP);
+        if (K.energy) {
+            if (isp) {
+                const double dQdY = -(RGAS * T / mwk) * (rho * A - MW * hS);
+                const double dPdY = (rho / mwk) * (cp * RGAS - MW * cpm);
+                Jl[k * 64 + ns] = (dQdY - fT * dPdY) / Phi;
+            }
+            if (k == ns) Jl[ns * 64 + ns] = (dQ - fT * (-Phi * uT + rho * dP)) / Phi;
+        } else {
+            if (isp) Jl[k * 64 + ns] = 0.;
+            if (k == ns) Jl[ns * 64 + ns] = 0.;
+        }
+        wave_sync();
+    }
+
     // KernelDense::NumericalJacobian, max-constraint branch (KernelDense.hpp:160-183)
     __device__ __forceinline__ void jacobian_impl(double h) {
         const double ZERO_DER = sqrt(TINY_F), ETA2 = sqrt(EPS_D), BETA = 1.e+3 * EPS_D;
@@ -490,23 +782,58 @@ struct TeamCtx {
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
+        // pivot order: lane k learns which row was chosen at step k (sR is free here)
+        int *pl = (int *)sR;
+        if (lane < ne) pl[ps] = lane;
+        wave_sync();
+        plv = (lane < ne) ? pl[lane] : 0;
+        wave_sync();
     }
 
-    // Forward / backward substitution in pivot order, factors read from LDS.
+    // Forward / backward substitution in pivot order, factors read from LDS.  The
+    // factors and pivot rows of 8 steps are fetched ahead of the dependent chain
+    // b -> readlane -> fma, and rows that take no update get a zero coefficient.
     __device__ __forceinline__ void solve_impl(int vid) {
         double b = V[vid][0];
-        for (int k = 0; k < ne; k++) {
-            const int p = __ffsll((unsigned long long)__ballot(ps == k)) - 1;
-            const double lk = LU[k * 64 + lane];
-            const double yk = readlane_d(b, p);
-            if (ps > k) b = fma(-lk, yk, b);
+        int k = 0;
+        for (; k + 8 <= ne; k += 8) {
+            double lk[8];
+            int pk[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                lk[u] = (ps > k + u) ? LU[(k + u) * 64 + lane] : 0.;
+                pk[u] = __builtin_amdgcn_readlane(plv, k + u);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) b = fma(-lk[u], readlane_d(b, pk[u]), b);
+        }
+        for (; k < ne; k++) {
+            const double lk = (ps > k) ? LU[k * 64 + lane] : 0.;
+            b = fma(-lk, readlane_d(b, __builtin_amdgcn_readlane(plv, k)), b);
         }
         double xr = 0.;
-        for (int k = ne - 1; k >= 0; k--) {
-            const int p = __ffsll((unsigned long long)__ballot(ps == k)) - 1;
-            const double uk = LU[k * 64 + lane];
-            const double xk = readlane_d(b * dinv, p);
-            if (ps < k) b = fma(-uk, xk, b);
+        k = ne - 1;
+        for (; k >= 7; k -= 8) {
+            double uk[8], dk[8];
+            int pk[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                uk[u] = (ps < k - u) ? LU[(k - u) * 64 + lane] : 0.;
+                pk[u] = __builtin_amdgcn_readlane(plv, k - u);
+                dk[u] = readlane_d(dinv, pk[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const double xk = readlane_d(b, pk[u]) * dk[u];
+                b = fma(-uk[u], xk, b);
+                if (lane == k - u) xr = xk;
+            }
+        }
+        for (; k >= 0; k--) {
+            const double uk = (ps < k) ? LU[k * 64 + lane] : 0.;
+            const int p = __builtin_amdgcn_readlane(plv, k);
+            const double xk = readlane_d(b, p) * readlane_d(dinv, p);
+            b = fma(-uk, xk, b);
             if (lane == k) xr = xk;
         }
         V[vid][0] = xr;
@@ -518,16 +845,24 @@ struct TeamCtx {
         for (int j = 0; j < ne; j++) acc += Jl[j * 64 + lane] * readlane_d(v, j);
         V[dst][0] = acc;
     }
-    __device__ __forceinline__ void jacobian(double h) {
+    // Returns the number of RHS evaluations the Jacobian took.
+    __device__ __forceinline__ int jacobian(double h) {
         LSM_TIC(t0);
 #ifdef LSM_PHASE_TIMERS
         in_jac = 1;
 #endif
-        jacobian_impl(h);
+        int nrhs = 0;
+        if (K.jac_mode == 1) {
+            jacobian_analytic_impl();
+        } else {
+            jacobian_impl(h);
+            nrhs = ne;
+        }
 #ifdef LSM_PHASE_TIMERS
         in_jac = 0;
 #endif
         LSM_TOC(1, t0);
+        return nrhs;
     }
     __device__ __forceinline__ void factor(double hr0) { LSM_TIC(t0); factor_impl(hr0); LSM_TOC(2, t0); }
     __device__ __forceinline__ void solve(int vid) { LSM_TIC(t0); solve_impl(vid); LSM_TOC(3, t0); }
@@ -548,7 +883,7 @@ __global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t
         __syncthreads();
     }
     const int wave = threadIdx.x >> 6;
-    double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * (8 * (192 + 64 * (size_t)M.ne)));
+    double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * team_wave_bytes(M));
     const size_t gw = (size_t)blockIdx.x * (blockDim.x >> 6) + wave;
     TeamCtx<NMAX> c(M, K, lds_raw, wave_lds, jws + gw * 64 * (size_t)M.ne);
     const int lane = c.lane, ns = M.ns, ne = M.ne;
@@ -609,6 +944,39 @@ __global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t
     if (lane == 0 && K.phase_cycles)
         for (int i = 0; i < 7; i++) atomicAdd(&K.phase_cycles[i], c.tp[i]);
 #endif
+}
+
+// Right-hand side and analytic Jacobian of the constant-pressure reactor at given
+// states (GetEquations of the batch-reactor ODE system), one wave per cell, in
+// grid-stride order over the cells so that the per-wave workspace of the
+// integrator is reused.  y: SoA [ne][n]; f: SoA [ne][n]; J: [n][ne][ne] row-major.
+__global__ void __launch_bounds__(256) equations_kernel(DevMech M, KParams K, int64_t n, const double *yin,
+                                                        double *f, double *J, double *jws) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    {
+        const int4 *src = (const int4 *)M.img.src;
+        int4 *dst = (int4 *)lds_raw;
+        const int nv = M.img.bytes / 16;
+        for (int i = threadIdx.x; i < nv; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+    const int wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * team_wave_bytes(M));
+    const size_t gw = (size_t)blockIdx.x * wpb + wave;
+    TeamCtx<64> c(M, K, lds_raw, wave_lds, jws + gw * 64 * (size_t)M.ne);
+    const int lane = c.lane, ne = M.ne;
+    for (int64_t cell = (int64_t)gw; cell < n; cell += (int64_t)gridDim.x * wpb) {
+        c.V[VB_][0] = lane < ne ? yin[(int64_t)lane * n + cell] : 0.;
+        c.rhs(VB_, F_);
+        if (f != nullptr && lane < ne) f[(int64_t)lane * n + cell] = c.V[F_][0];
+        if (J != nullptr) {
+            c.jacobian_analytic_impl();
+            double *Jc = J + cell * (int64_t)ne * ne;
+            if (lane < ne)
+                for (int j = 0; j < ne; j++) Jc[(int64_t)lane * ne + j] = c.Jl[j * 64 + lane];
+        }
+        wave_sync();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -691,7 +1059,7 @@ struct PlainCtx {
         x(dst, ns) = dT;
     }
 
-    __device__ void jacobian(double h) {
+    __device__ int jacobian(double h) {
         const double ZERO_DER = sqrt(TINY_F), ETA2 = sqrt(EPS_D), BETA = 1.e+3 * EPS_D;
         double acc = 0.;
         for (int e = 0; e < ne; e++) { const double t = x(F_, e) * x(OOE_, e); acc += t * t; }
@@ -711,6 +1079,7 @@ struct PlainCtx {
             const double hInv = 1. / hJ;
             for (int i = 0; i < ne; i++) J(i, j) = hInv * (x(AUX_, i) - x(F_, i));
         }
+        return ne;
     }
 
     // Eigen PartialPivLU / FullPivLU restated (KernelDense.hpp:196-221).
@@ -820,7 +1189,7 @@ __global__ void reset_queue_kernel(unsigned *queue) { *queue = 0u; }
 // ---------------------------------------------------------------------------
 int team_max_neq() { return 64; }
 
-size_t team_wave_bytes(const DevMech &m) { return (size_t)8 * (192 + 64 * (size_t)m.ne); }
+__host__ __device__ size_t team_wave_bytes(const DevMech &m) { return (size_t)8 * (320 + 65 * (size_t)m.ne); }
 
 size_t team_jac_doubles(const DevMech &m) { return (size_t)64 * m.ne; }
 
@@ -856,6 +1225,21 @@ int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt,
     if (m.ne <= 56) return launch_team_n<56>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
     if (m.ne <= 64) return launch_team_n<64>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
     return -2;
+}
+
+int launch_equations(const DevMech &m, const KParams &k, int64_t n, const double *yin, double *f, double *J,
+                     int grid, int wpb, double *jws, void *stream) {
+    if (wpb < 1 || wpb > 4 || m.ne > 64) return -2;
+    const int64_t need = (n + wpb - 1) / wpb;
+    const int g = (int)(need < grid ? need : grid);
+    if (g < 1) return 0;
+    const size_t lds = (size_t)m.img.bytes + (size_t)wpb * team_wave_bytes(m);
+    if (lds > 65536 &&
+        hipFuncSetAttribute((const void *)equations_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+            hipSuccess)
+        return -1;
+    hipLaunchKernelGGL(equations_kernel, dim3(g), dim3(64 * wpb), lds, (hipStream_t)stream, m, k, n, yin, f, J, jws);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 size_t plain_work_doubles(const DevMech &m, int64_t slots) {
