@@ -106,7 +106,7 @@ def main():
 
     status = st.cpu().numpy()
     counters = cnt.cpu().numpy()
-    work = workmodel.flops_from_counters(mech, counters)
+    work = workmodel.flops_from_counters(mech, counters, args.jacobian)
     kernel_s = float(lat_ms.mean()) / 1e3
     achieved_tflops = work["flops"] / kernel_s / 1e12
     value = n * args.steps / total
