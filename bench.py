@@ -1,16 +1,33 @@
 #!/usr/bin/env python3
 """Chemistry-step benchmark on one MI355X.
 
-Workload: the seeded 53-species / 325-reaction mechanism on the 512x512 coflow-like
-field (262144 cells, 60% cold 300-600 K, 40% flame 1200-2200 K), constant-pressure
-adiabatic integration over dt = 1e-5 s at rtol 1e-7, atol 1e-12.  A "step" is one
-chemistry step over all cells (the batched forAll loop of chemistry_DI.H); every
-step integrates the same input field, which stays resident in HBM.
+Default workload: the seeded 53-species / 325-reaction mechanism on the 512x512
+coflow-like field (262144 cells; 60% cold 300-600 K, 40% flame 1200-2200 K).
+Each cell is integrated as a constant-pressure adiabatic reactor over dt = 1e-5 s
+at rtol 1e-7, atol 1e-12.  A "step" is one chemistry step over all cells, i.e. the
+batched forAll loop of chemistry_DI.H.  Every step integrates the same input
+field, which stays resident in HBM.  Other workloads:
+  --workload 3: 1048576 cells with an engineered stiffness spread (T 300-2500 K,
+                radicals log-uniform 1e-12 - 1e-3);
+  --workload 4: the 160-species / 1000-reaction mechanism (wide kernel, FP32
+                Newton LU with FP64 refinement) on a subsample of its
+                131072-cell field, at rtol 1e-6.
 
-Prints one JSON line: cell-integrations/s, p50/p99 batch latency, the roofline of
-the integrator kernel (algorithmic FP64 operations from the per-cell counters /
-kernel time from HIP events on the launch stream) and the CPU baseline (the
-oracle on a bounded, stratified subsample of the same cells, on this host).
+Before the timed steps:
+  * the launch shape is autotuned on a 65536-cell sample;
+  * one warm-up step runs, and its per-cell step counts give the visiting order
+    of the timed steps (most expensive cells first), as the previous CFD step's
+    counters would.
+With --graph, the step (queue reset + integrator) is captured once in a HIP
+graph and replayed.
+
+Prints one JSON line with:
+  * cell-integrations/s and the p50/p99 batch latency;
+  * the roofline of the integrator kernel: algorithmic FP64 operations from the
+    per-cell counters, divided by the kernel time from HIP events on the launch
+    stream;
+  * the CPU baseline: the oracle on a bounded random sample of the same cells,
+    on this host.
 """
 from __future__ import annotations
 
@@ -51,10 +68,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cells", type=int, default=None, help="subsample of the field (default: all 262144)")
-    ap.add_argument("--method", default="team", choices=["team", "plain", "auto"])
-    ap.add_argument("--jacobian", default="numerical", choices=["numerical", "analytic"])
-    ap.add_argument("--cpu-sample", type=int, default=96)
+    ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4])
+    ap.add_argument("--cells", type=int, default=None, help="subsample of the field (default: all of it; "
+                                                               "workload 4: 4096)")
+    ap.add_argument("--method", default="auto", choices=["team", "plain", "wide", "auto"])
+    ap.add_argument("--jacobian", default="analytic", choices=["numerical", "analytic"])
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-order", action="store_true")
+    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -67,9 +89,10 @@ def main():
     from laminarsmoke_amd import api, workmodel
     from laminarsmoke_amd.fields import workload_cells, WORKLOADS
 
-    w = WORKLOADS[2]
+    w = WORKLOADS[args.workload]
     rtol, atol, dt = w["rtol"], w["atol"], w["dt"]
-    mech, y = workload_cells(2, n=args.cells)
+    cells = args.cells if args.cells is not None else (4096 if args.workload == 4 else None)
+    mech, y = workload_cells(args.workload, n=cells)
     n = y.shape[1]
     dev = torch.device("cuda:0")
     y_in = torch.from_numpy(y).to(dev)
@@ -78,6 +101,7 @@ def main():
     st = torch.zeros(n, dtype=torch.int32, device=dev)
     q = torch.zeros(n, dtype=torch.float64, device=dev)
     cnt = torch.zeros((n, 12), dtype=torch.int32, device=dev)
+    order = torch.arange(n, dtype=torch.int32, device=dev)
 
     ode = api.ODE_Parameters()
     ode.SetType("OpenSMOKE_GPU")
@@ -86,19 +110,48 @@ def main():
     step = api.ChemistryStep(mech, ode, method=args.method, jacobian=args.jacobian)
     stream = torch.cuda.current_stream(dev)
 
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = step.autotune(n, dts.data_ptr(), y_in.data_ptr(), 65536, stream.cuda_stream)
+        tune["seconds"] = time.perf_counter() - t0
+
+    use_order = False
+
     def run():
         step.integrate_device(n, dts.data_ptr(), y_in.data_ptr(), y_out.data_ptr(), st.data_ptr(), q.data_ptr(),
-                              cnt.data_ptr(), stream.cuda_stream)
+                              cnt.data_ptr(), stream.cuda_stream, order.data_ptr() if use_order else None)
 
     for _ in range(args.warmup):
         run()
     torch.cuda.synchronize()
+    if not args.no_order and args.warmup > 0:
+        step.cost_order(n, cnt.data_ptr(), order.data_ptr(), stream.cuda_stream)
+        torch.cuda.synchronize()
+        use_order = True
+
+    graph = None
+    if args.graph:
+        side = torch.cuda.Stream(dev)
+        side.wait_stream(stream)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(graph, stream=side):
+                step.integrate_device(n, dts.data_ptr(), y_in.data_ptr(), y_out.data_ptr(), st.data_ptr(),
+                                      q.data_ptr(), cnt.data_ptr(), side.cuda_stream,
+                                      order.data_ptr() if use_order else None)
+        stream.wait_stream(side)
+        torch.cuda.synchronize()
+
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for a, b in evs:
         a.record(stream)
-        run()
+        if graph is not None:
+            graph.replay()
+        else:
+            run()
         b.record(stream)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
@@ -110,16 +163,20 @@ def main():
     kernel_s = float(lat_ms.mean()) / 1e3
     achieved_tflops = work["flops"] / kernel_s / 1e12
     value = n * args.steps / total
+    launch = step.launch()
 
+    name = w["name"] if cells is None else f"{w['name']}-sub{n}"
     out = {
         "metric": METRIC, "value": value, "unit": "cell-integrations/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": total / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded mechanism and cell field)",
-        "config": {"workload": w["name"] if args.cells is None else f"{w['name']}-sub{n}",
-                   "mechanism": f"{mech.ns} species / {mech.nr} reactions", "cells": n, "dt_s": dt,
-                   "rtol": rtol, "atol": atol, "method": args.method,
-                   "jacobian": args.jacobian},
+        "config": {"workload": name, "mechanism": f"{mech.ns} species / {mech.nr} reactions", "cells": n,
+                   "dt_s": dt, "rtol": rtol, "atol": atol, "method": launch["method"], "jacobian": args.jacobian,
+                   "newton_lu": "fp32+refinement" if launch["method"] == "wide" else "fp64",
+                   "waves_per_workgroup": launch["waves_per_block"], "grid": launch["grid"],
+                   "visit_order": "cost (previous step's counters)" if use_order else "cell index",
+                   "hip_graph": graph is not None},
         "p50_batch_ms": float(np.percentile(lat_ms, 50)), "p99_batch_ms": float(np.percentile(lat_ms, 99)),
         "roofline": {"bound": "mfma", "pipe": "fp64", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": None,
@@ -130,10 +187,12 @@ def main():
                   "steps_per_cell_p50": float(np.median(counters[:, 0])),
                   "steps_per_cell_max": int(counters[:, 0].max()),
                   "rhs_per_cell_mean": float(counters[:, 9].mean()),
-                  "min_hbm_GBps": workmodel.hbm_bytes_min(mech, n) / kernel_s / 1e9},
+                  "min_hbm_GBps": workmodel.hbm_bytes_min(mech, n) / kernel_s / 1e9,
+                  "autotune": tune},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(mech, y, min(args.cpu_sample, n), args.cpu_threads, dt, rtol, atol)
+        ns = args.cpu_sample if args.cpu_sample is not None else (8 if args.workload == 4 else 96)
+        out["cpu_baseline"] = cpu_baseline(mech, y, min(ns, n), args.cpu_threads, dt, rtol, atol)
     step.close()
     print(json.dumps(out), flush=True)
 

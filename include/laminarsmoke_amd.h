@@ -93,9 +93,13 @@ int lsm_reactor_options_default(lsm_reactor_options *r);
  * Integrator selection inside the GPU library.
  * ------------------------------------------------------------------------- */
 enum lsm_method {
-    LSM_METHOD_AUTO = 0,        /* team kernel when the mechanism fits, else plain            */
+    LSM_METHOD_AUTO = 0,        /* team when ns+1 <= 64, else wide when the LU fits LDS, else plain */
     LSM_METHOD_PLAIN = 1,       /* one cell per thread, FP64, state in HBM                    */
-    LSM_METHOD_TEAM = 2         /* one cell per wavefront, persistent work queue, FP64        */
+    LSM_METHOD_TEAM = 2,        /* one cell per wavefront, persistent work queue, FP64        */
+    LSM_METHOD_WIDE = 3         /* one cell per 256-thread workgroup, persistent work queue;
+                                   FP64 rates, residuals and error control, Newton-matrix LU
+                                   in FP32 (registers, then LDS for the solves) with iterative
+                                   refinement (ns+1 <= 176)                                    */
 };
 
 /* OdeStatus values (math/native-ode-solvers/OdeSolverUtilities.h:83-99); a cell
@@ -164,6 +168,11 @@ typedef enum lsm_jacobian_type {
 } lsm_jacobian_type;
 int lsm_set_jacobian(lsm_handle *h, int32_t type);
 
+/* Iterative-refinement sweeps after each FP32 Newton solve of the wide kernel
+ * (0-4, default 1): x += LU^-1 (b - G x) with the residual in FP64.  The team and
+ * plain kernels factorise in FP64 and ignore it. */
+int lsm_set_refinement(lsm_handle *h, int32_t sweeps);
+
 /* Chemistry step over n_cells cells, host pointers (the whole forAll loop of
  * chemistry_DI.H:92-328).  y_in/y_out: SoA [ns+1][n_cells] (species rows, then T).
  * dt: per-cell integration time (DeltaTCells).  status (may be NULL): OdeStatus per
@@ -178,6 +187,36 @@ int lsm_integrate_batch(lsm_handle *h, int64_t n_cells, const double *dt, const 
 int lsm_integrate_batch_device(lsm_handle *h, int64_t n_cells, const double *d_dt,
                                const double *d_y_in, double *d_y_out, int32_t *d_status,
                                double *d_q, int32_t *d_counters, void *stream);
+
+/* lsm_integrate_batch_device with a visiting order: the persistent kernels (team,
+ * wide) hand out cells in the order d_order[0..n_cells) (a permutation of the cell
+ * indices; entries outside [0, n_cells) are skipped).  Results do not depend on the
+ * order; the makespan does: expensive cells first shortens the tail.  The plain
+ * kernel ignores the order.  No counterpart in the reference, whose cell loop is
+ * serial (chemistry_DI.H:92). */
+int lsm_integrate_batch_device_ordered(lsm_handle *h, int64_t n_cells, const double *d_dt,
+                                       const double *d_y_in, double *d_y_out, int32_t *d_status,
+                                       double *d_q, int32_t *d_counters, const int32_t *d_order,
+                                       void *stream);
+
+/* Visiting order from the per-cell counters of a previous call on the same cells
+ * (the cpuChemistry field of chemistry_DI.H:170 plays this role in the reference):
+ * cells sorted by BDF steps, most first (device radix sort).  Asynchronous on
+ * `stream`; buffers are kept in the handle. */
+int lsm_cost_order(lsm_handle *h, int64_t n_cells, const int32_t *d_counters, int32_t *d_order,
+                   void *stream);
+
+/* Launch-shape autotuner for the team kernel: times waves per workgroup (the
+ * workgroup shares one LDS copy of the mechanism) x workgroups per CU on a strided
+ * sample of `sample` cells (0: 65536) of the given device batch, keeps the fastest
+ * in the handle.  result (may be NULL): {waves per workgroup, grid, best time in
+ * microseconds}.  Synchronous; allocates temporaries.  Other methods: no-op. */
+int lsm_autotune(lsm_handle *h, int64_t n_cells, const double *d_dt, const double *d_y,
+                 int64_t sample, void *stream, int32_t *result);
+
+/* Launch configuration in use: {method, waves per workgroup, grid, Jacobian type,
+ * refinement sweeps}. */
+int lsm_get_launch(lsm_handle *h, int32_t *out5);
 
 /* Right-hand side and analytic Jacobian of the constant-pressure batch reactor at
  * n_cells states: BatchReactorHomogeneousConstantPressure_ODE_OpenSMOKE::GetEquations

@@ -25,7 +25,7 @@ LIB_PATH = os.environ.get("LSM_LIB_PATH") or os.path.join(os.path.dirname(os.pat
 
 ODE_INTEGRATOR = {"OpenSMOKE": 0, "BzzOde": 1, "CVODE": 2, "DVODE": 3, "DASPK": 4, "DLSODE": 5, "DLSODA": 6,
                   "RADAU5": 7, "MEBDF": 8, "CHEMEQ2": 9, "OpenSMOKE_GPU": 100}
-METHODS = {"auto": 0, "plain": 1, "team": 2}
+METHODS = {"auto": 0, "plain": 1, "team": 2, "wide": 3}
 JACOBIANS = {"numerical": 0, "analytic": 1}
 STATUS_NAMES = {0: "skipped", 2: "ok", -1: "max steps", -2: "too strict tolerances", -4: "error-test failures",
                 -5: "convergence failures", -6: "step too small", -9: "illegal constraints"}
@@ -53,11 +53,16 @@ def lib():
         L.lsm_create.argtypes = [vp, vp, vp, ctypes.POINTER(vp)]
         L.lsm_set_method.argtypes = [vp, ctypes.c_int32]
         L.lsm_set_jacobian.argtypes = [vp, ctypes.c_int32]
+        L.lsm_set_refinement.argtypes = [vp, ctypes.c_int32]
         L.lsm_equations.argtypes = [vp, i64, vp, vp, vp]
         L.lsm_equations_device.argtypes = [vp, i64, vp, vp, vp, vp]
         L.lsm_destroy.argtypes = [vp]
         L.lsm_integrate_batch.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.lsm_integrate_batch_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.lsm_integrate_batch_device_ordered.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lsm_cost_order.argtypes = [vp, i64, vp, vp, vp]
+        L.lsm_autotune.argtypes = [vp, i64, vp, vp, i64, vp, vp]
+        L.lsm_get_launch.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -136,6 +141,9 @@ class ChemistryStep:
     def set_method(self, method):
         _check(lib().lsm_set_method(self.h, METHODS[method] if isinstance(method, str) else int(method)))
 
+    def set_refinement(self, sweeps):
+        _check(lib().lsm_set_refinement(self.h, int(sweeps)))
+
     def set_jacobian(self, kind):
         _check(lib().lsm_set_jacobian(self.h, JACOBIANS[kind] if isinstance(kind, str) else int(kind)))
 
@@ -177,10 +185,29 @@ class ChemistryStep:
         return yout, st, qa, cnt
 
     def integrate_device(self, n, dt_ptr, y_in_ptr, y_out_ptr, status_ptr=None, q_ptr=None, counters_ptr=None,
-                         stream=None):
-        """Device pointers (ints), enqueued on `stream` (int handle or None)."""
-        _check(lib().lsm_integrate_batch_device(self.h, int(n), dt_ptr, y_in_ptr, y_out_ptr, status_ptr, q_ptr,
-                                                counters_ptr, stream))
+                         stream=None, order_ptr=None):
+        """Device pointers (ints), enqueued on `stream` (int handle or None); `order_ptr`: visiting order."""
+        if order_ptr:
+            _check(lib().lsm_integrate_batch_device_ordered(self.h, int(n), dt_ptr, y_in_ptr, y_out_ptr, status_ptr,
+                                                            q_ptr, counters_ptr, order_ptr, stream))
+        else:
+            _check(lib().lsm_integrate_batch_device(self.h, int(n), dt_ptr, y_in_ptr, y_out_ptr, status_ptr, q_ptr,
+                                                    counters_ptr, stream))
+
+    def cost_order(self, n, counters_ptr, order_ptr, stream=None):
+        _check(lib().lsm_cost_order(self.h, int(n), counters_ptr, order_ptr, stream))
+
+    def autotune(self, n, dt_ptr, y_ptr, sample=0, stream=None):
+        out = (ctypes.c_int32 * 3)()
+        _check(lib().lsm_autotune(self.h, int(n), dt_ptr, y_ptr, int(sample), stream, out))
+        return dict(waves_per_block=out[0], grid=out[1], best_us=out[2])
+
+    def launch(self):
+        out = (ctypes.c_int32 * 5)()
+        _check(lib().lsm_get_launch(self.h, out))
+        names = {v: k for k, v in METHODS.items()}
+        return dict(method=names.get(out[0], out[0]), waves_per_block=out[1], grid=out[2],
+                    jacobian="analytic" if out[3] == 1 else "numerical", refine=out[4])
 
 
 def counters_dict(cnt: np.ndarray) -> dict:

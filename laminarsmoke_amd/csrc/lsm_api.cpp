@@ -18,6 +18,8 @@
 #include <string>
 #include <vector>
 
+#include <hipcub/hipcub.hpp>
+
 #include "../../include/laminarsmoke_amd.h"
 #include "lsm_internal.hpp"
 
@@ -224,6 +226,15 @@ struct lsm_handle {
     int team_grid = 0;
     int team_wpb = 0;
     double *d_jws = nullptr;
+    int wide_grid = 0;
+    double *d_wjws = nullptr;   // wide kernel Jacobians, [wide_grid][ne][ne]
+    int team_max_waves = 0;     // waves the team Jacobian workspace is sized for
+    // cost-order sort buffers
+    unsigned *d_keys = nullptr, *d_keys_out = nullptr;
+    int *d_vals = nullptr;
+    void *d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int64_t sort_cap = 0;
 };
 
 extern "C" {
@@ -351,7 +362,8 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
                         const std::vector<int> &fo_of, int ns_pad, LdsImage &L, std::vector<char> &out) {
     const int ns = m->ns, nr = m->nr;
     L = LdsImage{};
-    if (ns > 63 || m->ntb > 254 || m->nfo > 254) return false;
+    if (ns > 254 || m->ntb > 254 || m->nfo > 254) return false;
+    const unsigned dummy = ns <= 63 ? (unsigned)LSM_DUMMY_SP : 255u;
     Image im;
     std::vector<double> sp((size_t)LSP_ROWS * ns_pad, 0.);
     for (int k = 0; k < ns; k++) {
@@ -392,7 +404,7 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
                 for (int r = 0; r < (int)e.second; r++) sl.push_back((unsigned)e.first);
             }
             if (sl.size() > 3) return false;
-            while (sl.size() < 3) sl.push_back((unsigned)LSM_DUMMY_SP);
+            while (sl.size() < 3) sl.push_back(dummy);
             word = sl[0] | (sl[1] << 8) | (sl[2] << 16);
             return true;
         };
@@ -431,7 +443,9 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
     L.o_fop = im.put(fop); L.o_fos = im.put(fos); L.o_fol = im.put(fol); L.o_fos2 = im.put(fos2); L.o_foe = im.put(foe);
     im.b.resize((im.b.size() + 15) & ~size_t(15), 0);
     L.bytes = (int)im.b.size();
-    L.team_ok = 1;
+    L.team_ok = ns <= 63 ? 1 : 0;
+    L.wide_ok = 1;
+    L.dummy = (int)dummy;
     out.swap(im.b);
     return true;
 }
@@ -618,6 +632,7 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
     k.energy = r.energy_equation ? 1 : 0;
     k.full_pivoting = p.full_pivoting ? 1 : 0;
     k.jac_mode = LSM_JACOBIAN_NUMERICAL;
+    k.refine = 1;
     k.phase_cycles = nullptr;
 #ifdef LSM_PHASE_TIMERS
     if (hipMalloc(&k.phase_cycles, 8 * sizeof(unsigned long long)) != hipSuccess ||
@@ -637,8 +652,17 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
         int blocks = (int)std::max<size_t>(1, lds_cu / lds);
         blocks = std::min(blocks, std::max(1, 8 / h->team_wpb));    // <= 2 waves per SIMD
         h->team_grid = blocks * h->num_cu;
-        const size_t jbytes = sizeof(double) * team_jac_doubles(h->dm) * (size_t)h->team_grid * h->team_wpb;
+        h->team_max_waves = 8 * h->num_cu;                            // 2 waves per SIMD (autotuner's range)
+        const size_t jbytes = sizeof(double) * team_jac_doubles(h->dm) * (size_t)h->team_max_waves;
         if (hipMalloc(&h->d_jws, jbytes) != hipSuccess) { lsm_destroy(h); return fail(LSM_E_NOMEM, "Jacobian workspace"); }
+    }
+    // wide grid: one 256-thread workgroup per CU (its FP32 LU takes most of the LDS)
+    if (h->dm.img.wide_ok && h->dm.ne <= WIDE_MAX_NEQ && wide_lds_bytes(h->dm) <= std::min(lds_block, lds_cu)) {
+        h->wide_grid = h->num_cu;
+        if (hipMalloc(&h->d_wjws, sizeof(double) * wide_jac_doubles(h->dm) * (size_t)h->wide_grid) != hipSuccess) {
+            lsm_destroy(h);
+            return fail(LSM_E_NOMEM, "wide Jacobian workspace");
+        }
     }
     *out = h;
     return LSM_OK;
@@ -646,9 +670,12 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
 
 int lsm_set_method(lsm_handle *h, int32_t method) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
-    if (method < LSM_METHOD_AUTO || method > LSM_METHOD_TEAM) return fail(LSM_E_INVALID, "unknown method");
+    if (method < LSM_METHOD_AUTO || method > LSM_METHOD_WIDE) return fail(LSM_E_INVALID, "unknown method");
     if (method == LSM_METHOD_TEAM && (h->team_wpb < 1 || h->kp.full_pivoting))
         return fail(LSM_E_UNSUPPORTED, "team kernel needs ns+1 <= 64, a mechanism image that fits in LDS "
+                                       "and partial pivoting");
+    if (method == LSM_METHOD_WIDE && (h->wide_grid < 1 || h->kp.full_pivoting))
+        return fail(LSM_E_UNSUPPORTED, "wide kernel needs ns+1 <= 176 "
                                        "and partial pivoting");
     h->method = method;
     return LSM_OK;
@@ -661,8 +688,20 @@ int lsm_set_jacobian(lsm_handle *h, int32_t type) {
     return LSM_OK;
 }
 
+int lsm_set_refinement(lsm_handle *h, int32_t sweeps) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (sweeps < 0 || sweeps > 4) return fail(LSM_E_INVALID, "refinement sweeps must be 0-4");
+    h->kp.refine = sweeps;
+    return LSM_OK;
+}
+
 int lsm_destroy(lsm_handle *h) {
     if (!h) return LSM_OK;
+    if (h->d_keys) (void)hipFree(h->d_keys);
+    if (h->d_keys_out) (void)hipFree(h->d_keys_out);
+    if (h->d_vals) (void)hipFree(h->d_vals);
+    if (h->d_sort_tmp) (void)hipFree(h->d_sort_tmp);
+    if (h->d_wjws) (void)hipFree(h->d_wjws);
     if (h->d_tables) (void)hipFree(h->d_tables);
     if (h->d_queue) (void)hipFree(h->d_queue);
     if (h->d_work) (void)hipFree(h->d_work);
@@ -679,11 +718,30 @@ int lsm_destroy(lsm_handle *h) {
 
 static int pick_method(const lsm_handle *h) {
     if (h->method != LSM_METHOD_AUTO) return h->method;
-    return (h->team_wpb > 0 && !h->kp.full_pivoting) ? LSM_METHOD_TEAM : LSM_METHOD_PLAIN;
+    if (h->kp.full_pivoting) return LSM_METHOD_PLAIN;
+    if (h->team_wpb > 0) return LSM_METHOD_TEAM;
+    return h->wide_grid > 0 ? LSM_METHOD_WIDE : LSM_METHOD_PLAIN;
 }
+
+static int integrate_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in, double *d_y_out,
+                            int32_t *d_status, double *d_q, int32_t *d_counters, const int32_t *d_order,
+                            void *stream);
 
 int lsm_integrate_batch_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in, double *d_y_out,
                                int32_t *d_status, double *d_q, int32_t *d_counters, void *stream) {
+    return integrate_device(h, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, nullptr, stream);
+}
+
+int lsm_integrate_batch_device_ordered(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in,
+                                       double *d_y_out, int32_t *d_status, double *d_q, int32_t *d_counters,
+                                       const int32_t *d_order, void *stream) {
+    if (!d_order) return fail(LSM_E_INVALID, "null order");
+    return integrate_device(h, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, d_order, stream);
+}
+
+static int integrate_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in, double *d_y_out,
+                            int32_t *d_status, double *d_q, int32_t *d_counters, const int32_t *d_order,
+                            void *stream) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
     if (n == 0) return LSM_OK;
@@ -692,9 +750,16 @@ int lsm_integrate_batch_device(lsm_handle *h, int64_t n, const double *d_dt, con
     const int method = pick_method(h);
     if (method == LSM_METHOD_TEAM) {
         if (launch_reset_queue(h->d_queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
-        int rc = launch_team(h->dm, h->kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue,
+        int rc = launch_team(h->dm, h->kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue, d_order,
                              h->team_grid, h->team_wpb, h->d_jws, stream);
         if (rc) return fail(LSM_E_HIP, std::string("team kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
+        return LSM_OK;
+    }
+    if (method == LSM_METHOD_WIDE) {
+        if (launch_reset_queue(h->d_queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
+        int rc = launch_wide(h->dm, h->kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue, d_order,
+                             h->wide_grid, h->d_wjws, stream);
+        if (rc) return fail(LSM_E_HIP, std::string("wide kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
         return LSM_OK;
     }
     // plain: one thread per cell slot, workspace in HBM
@@ -789,6 +854,111 @@ int lsm_equations(lsm_handle *h, int64_t n, const double *y, double *f, double *
         rc = fail(LSM_E_HIP, "copy J");
     release();
     return rc;
+}
+
+int lsm_cost_order(lsm_handle *h, int64_t n, const int32_t *d_counters, int32_t *d_order, void *stream) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (n < 0 || n > (int64_t)0x7fffffff) return fail(LSM_E_INVALID, "cell count out of range");
+    if (n == 0) return LSM_OK;
+    if (!d_counters || !d_order) return fail(LSM_E_INVALID, "null counters / order");
+    hipStream_t s = (hipStream_t)stream;
+    if (n > h->sort_cap) {
+        if (h->d_keys) (void)hipFree(h->d_keys);
+        if (h->d_keys_out) (void)hipFree(h->d_keys_out);
+        if (h->d_vals) (void)hipFree(h->d_vals);
+        if (h->d_sort_tmp) (void)hipFree(h->d_sort_tmp);
+        h->d_keys = h->d_keys_out = nullptr;
+        h->d_vals = nullptr;
+        h->d_sort_tmp = nullptr;
+        h->sort_cap = 0;
+        HIP_OK(hipMalloc(&h->d_keys, sizeof(unsigned) * n));
+        HIP_OK(hipMalloc(&h->d_keys_out, sizeof(unsigned) * n));
+        HIP_OK(hipMalloc(&h->d_vals, sizeof(int) * n));
+        size_t bytes = 0;
+        HIP_OK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, bytes, h->d_keys, h->d_keys_out, h->d_vals,
+                                                             (int *)d_order, (int)n, 0, 32, s));
+        HIP_OK(hipMalloc(&h->d_sort_tmp, bytes));
+        h->sort_tmp_bytes = bytes;
+        h->sort_cap = n;
+    }
+    if (launch_order_keys(d_counters, n, h->d_keys, h->d_vals, stream)) return fail(LSM_E_HIP, "order keys launch failed");
+    size_t bytes = h->sort_tmp_bytes;
+    HIP_OK(hipcub::DeviceRadixSort::SortPairsDescending(h->d_sort_tmp, bytes, h->d_keys, h->d_keys_out, h->d_vals,
+                                                         (int *)d_order, (int)n, 0, 32, s));
+    return LSM_OK;
+}
+
+int lsm_autotune(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y, int64_t sample, void *stream,
+                 int32_t *result) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (n <= 0 || !d_dt || !d_y) return fail(LSM_E_INVALID, "autotune needs cells");
+    if (pick_method(h) != LSM_METHOD_TEAM) {
+        if (result) { result[0] = h->team_wpb; result[1] = h->team_grid; result[2] = -1; }
+        return LSM_OK;                                     // only the team kernel has launch shapes to choose
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t m = std::max<int64_t>(1, std::min<int64_t>(n, sample > 0 ? sample : 65536));
+    const int64_t stride = n / m;
+    const int ne = h->dm.ne;
+    double *ys = nullptr, *dts = nullptr, *yo = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (ys) (void)hipFree(ys);
+        if (dts) (void)hipFree(dts);
+        if (yo) (void)hipFree(yo);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    if (hipMalloc(&ys, sizeof(double) * ne * m) != hipSuccess || hipMalloc(&dts, sizeof(double) * m) != hipSuccess ||
+        hipMalloc(&yo, sizeof(double) * ne * m) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        cleanup();
+        return fail(LSM_E_NOMEM, "autotune buffers");
+    }
+    int rc = launch_gather_cells(d_y, d_dt, n, ne, m, stride, ys, dts, stream);
+    hipDeviceProp_t prop;
+    if (!rc && hipGetDeviceProperties(&prop, h->device) != hipSuccess) rc = -1;
+    const size_t lds_block = std::max<size_t>(prop.sharedMemPerBlock, 65536);
+    const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? prop.maxSharedMemoryPerMultiProcessor : lds_block;
+    const int max_wpb = team_waves_per_block(h->dm, std::min(lds_block, lds_cu));
+    int best_wpb = h->team_wpb, best_grid = h->team_grid;
+    float best_ms = 3.4e38f;
+    bool warmed = false;
+    for (int wpb = 1; wpb <= max_wpb && !rc; wpb++) {
+        const size_t lds = (size_t)h->dm.img.bytes + (size_t)wpb * team_wave_bytes(h->dm);
+        for (int bpc = 1; wpb * bpc <= 8 && bpc * lds <= lds_cu; bpc++) {
+            const int grid = bpc * h->num_cu;
+            if (grid * wpb > h->team_max_waves) break;
+            for (int rep = warmed ? 0 : -1; rep < 1 && !rc; rep++) {   // the first candidate runs twice (warm-up)
+                if (launch_reset_queue(h->d_queue, stream)) { rc = -1; break; }
+                if (hipEventRecord(e0, s) != hipSuccess) { rc = -1; break; }
+                if (launch_team(h->dm, h->kp, m, dts, ys, yo, nullptr, nullptr, nullptr, h->d_queue, nullptr, grid, wpb,
+                                h->d_jws, stream)) { rc = -1; break; }
+                if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = -1; break; }
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = -1; break; }
+                warmed = true;
+                if (rep == 0 && ms < best_ms) { best_ms = ms; best_wpb = wpb; best_grid = grid; }
+            }
+        }
+    }
+    cleanup();
+    if (rc) return fail(LSM_E_HIP, "autotune launch failed");
+    h->team_wpb = best_wpb;
+    h->team_grid = best_grid;
+    if (result) { result[0] = best_wpb; result[1] = best_grid; result[2] = (int32_t)(best_ms * 1000.f); }
+    return LSM_OK;
+}
+
+int lsm_get_launch(lsm_handle *h, int32_t *out) {
+    if (!h || !out) return fail(LSM_E_INVALID, "null argument");
+    const int method = pick_method(h);
+    out[0] = method;
+    out[1] = method == LSM_METHOD_TEAM ? h->team_wpb : (method == LSM_METHOD_WIDE ? 4 : 4);
+    out[2] = method == LSM_METHOD_TEAM ? h->team_grid : (method == LSM_METHOD_WIDE ? h->wide_grid : 0);
+    out[3] = h->kp.jac_mode;
+    out[4] = h->kp.refine;
+    return LSM_OK;
 }
 
 #ifdef LSM_PHASE_TIMERS

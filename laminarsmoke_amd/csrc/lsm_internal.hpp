@@ -25,7 +25,8 @@ enum SpRow {
 //        info: bit0 reversible, bit1 negative A, bit2 fixed-format record, bits 8-15
 //              third-body slot+1, bits 16-23 fall-off slot+1;
 //        fixed-format record: up to 3 reactant and 3 product species, integer orders as
-//              repeated species, unused slots = LSM_DUMMY_SP (c = 1, g = 0);
+//              repeated species, unused slots = LdsImage::dummy (c = 1, g = 0): 63
+//              (LSM_DUMMY_SP) when ns <= 63, else 255;
 //        general list: first entry (bits 0-15), reactants (16-23), products (24-31)
 //   ent  uint [nent]                species (bits 0-11) | coefficient code << 12; code c < 16 means
 //                                   nu = c/2, otherwise nu = nuv[c - 16]
@@ -39,7 +40,9 @@ struct LdsImage {
     const void *src;
     int bytes;
     int o_sp, o_P, o_W, o_ent, o_nuv, o_tbp, o_tbs, o_tbe, o_fop, o_fos, o_fol, o_fos2, o_foe;
-    int team_ok;   // the mechanism fits the image limits (ns <= 63, < 65536 entries, <= 254 TB/FO)
+    int team_ok;   // the team kernel can use the image (ns <= 63)
+    int wide_ok;   // the wide kernel can use the image (ns <= 254)
+    int dummy;     // unused slot of fixed-format records: 63 if ns <= 63, else 255
 };
 
 // Reactions are renumbered on upload: elementary first, then third-body, then
@@ -76,6 +79,7 @@ struct KParams {
     int energy;
     int full_pivoting;
     int jac_mode;                       // 0: forward differences (KernelDense), 1: analytic
+    int refine;                         // wide kernel: iterative-refinement sweeps after each FP32 solve
 };
 
 // Host-side launchers (lsm_kernels.hip).
@@ -86,9 +90,10 @@ int team_max_neq();
 __host__ __device__ size_t team_wave_bytes(const DevMech &m);
 size_t team_jac_doubles(const DevMech &m);
 int team_waves_per_block(const DevMech &m, size_t lds_per_block);
+// order (may be null): the cells in the order the queue hands them out.
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
-                double *yout, int *status, double *q, int *counters, unsigned *queue, int grid,
-                int waves_per_block, double *jws, void *stream);
+                double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order,
+                int grid, int waves_per_block, double *jws, void *stream);
 int launch_plain(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                  double *yout, int *status, double *q, int *counters, double *work, int64_t slots,
                  void *stream);
@@ -98,5 +103,18 @@ size_t plain_work_doubles(const DevMech &m, int64_t slots);
 int launch_equations(const DevMech &m, const KParams &k, int64_t n, const double *yin, double *f, double *J,
                      int grid, int waves_per_block, double *jws, void *stream);
 int launch_reset_queue(unsigned *queue, void *stream);
+int launch_order_keys(const int *counters, int64_t n, unsigned *keys, int *vals, void *stream);
+int launch_gather_cells(const double *y, const double *dt, int64_t n, int ne, int64_t m, int64_t stride, double *ys,
+                        double *dts, void *stream);
+
+// Wide kernel: one cell per 256-thread workgroup (ns + 1 <= WIDE_MAX_NEQ), thread e owns
+// element e, Newton matrix factorised in FP32 in LDS, Jacobian in an HBM workspace
+// ([ne][ne] doubles per workgroup), mechanism image read from global memory.
+enum { WIDE_THREADS = 256, WIDE_MAX_NEQ = 176 };
+size_t wide_lds_bytes(const DevMech &m);
+size_t wide_jac_doubles(const DevMech &m);
+int launch_wide(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
+                int *status, double *q, int *counters, unsigned *queue, const int *order, int grid, double *jws,
+                void *stream);
 
 }  // namespace lsm

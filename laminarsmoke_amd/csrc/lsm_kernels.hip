@@ -227,6 +227,312 @@ struct LdsMech {
     }
 };
 
+// One reaction (KineticsMap_CHEMKIN.hpp:1192-1386, 1683-1768), with the formation
+// rates (StoichiometricMap.hpp:656-730) accumulated reaction-major into LDS.
+// 1/Kc = exp(sum_k nu_k g_k - dn ln(Patm/RT)) with g = h/RT - s/R, which equals
+// the reference's exp(-dS/R + dH/RT - dn ln(Patm/RT)).
+// acc(k, v) adds v to the formation rate of species slot k.
+template <class Acc>
+__device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, int j, double T, double logT, double uT,
+                                         double logPRT, double cTot, Acc &&acc) {
+    const uint4 w = L.W[j];
+    const double2 ab = ((const double2 *)L.P)[2 * j];
+    const double2 ed = ((const double2 *)L.P)[2 * j + 1];
+    const unsigned info = w.x;
+    double kf = exp(ab.x + ab.y * logT - ed.x * uT);
+    if (info & 2u) kf = -kf;
+    double kmod = kf;
+    const int tb = (int)((info >> 8) & 0xffu) - 1;
+    if (tb >= 0) {
+        double Me = cTot;
+        for (unsigned q = L.tbp[tb]; q < L.tbp[tb + 1]; q++) Me += scg[L.tbs[q]].x * L.tbe[q];
+        kmod *= Me;
+    }
+    const int fo = (int)((info >> 16) & 0xffu) - 1;
+    if (fo >= 0) {
+        const double *fp = L.fop + 8 * fo;
+        const double kinf = exp(fp[0] + fp[1] * logT - fp[2] * uT);
+        double Mc;
+        if (L.fos[fo] < 0) {
+            Mc = cTot;
+            for (unsigned q = L.fol[fo]; q < L.fol[fo + 1]; q++) Mc += scg[L.fos2[q]].x * L.foe[q];
+        } else {
+            Mc = scg[L.fos[fo]].x + 1.e-16;
+        }
+        const double Pr = kf * Mc / kinf;
+        double wF = 1.;
+        if (fp[7] != 0.) {   // Troe
+            const double a = fp[3];
+            double F = (1. - a) * exp(-T / fp[4]) + a * exp(-T / fp[5]);
+            if (fp[6] != 0.) F += exp(-fp[6] / T);
+            const double lf = (F < 1.e-300) ? -300. : log10(F);
+            double q;
+            if (Pr > 1.e-32) {
+                const double nT = 0.75 - 1.27 * lf;
+                const double cT = -0.4 - 0.67 * lf;
+                const double sT = log10(Pr) + cT;
+                q = sT / (nT - 0.14 * sT);
+            } else {
+                q = 1. / 0.14;
+            }
+            wF = pow(10., lf / (1. + q * q));
+        }
+        kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
+    }
+    if (info & 4u) {
+        // fixed-format record: three reactant and three product slots
+        const unsigned r0 = w.y & 0xffu, r1 = (w.y >> 8) & 0xffu, r2 = (w.y >> 16) & 0xffu;
+        const unsigned p0 = w.z & 0xffu, p1 = (w.z >> 8) & 0xffu, p2 = (w.z >> 16) & 0xffu;
+        const double2 a0 = scg[r0], a1 = scg[r1], a2 = scg[r2];
+        const double2 b0 = scg[p0], b1 = scg[p1], b2 = scg[p2];
+        double net = (a0.x * a1.x) * a2.x;
+        if (info & 1u) {
+            const double dg = (b0.y + b1.y + b2.y) - (a0.y + a1.y + a2.y);
+            net -= ((b0.x * b1.x) * b2.x) * exp(dg - logPRT * ed.y);
+        }
+        const double r = net * kmod;
+        acc(r0, -r);
+        acc(r1, -r);
+        acc(r2, -r);
+        acc(p0, r);
+        acc(p1, r);
+        acc(p2, r);
+        return;
+    }
+    // general stoichiometry (any orders, any number of species)
+    const unsigned lst = w.w;
+    const unsigned e0 = lst & 0xffffu, nre = (lst >> 16) & 0xffu, npr = lst >> 24;
+    double pf = 1.;
+    for (unsigned q = 0; q < nre; q++) {
+        const unsigned e = L.ent[e0 + q];
+        pf *= conc_pow(scg[e & 0xfffu].x, L.nu(e));
+    }
+    double net = pf;
+    if (info & 1u) {
+        double dg = 0., pb = 1.;
+        for (unsigned q = 0; q < nre; q++) {
+            const unsigned e = L.ent[e0 + q];
+            dg -= L.nu(e) * scg[e & 0xfffu].y;
+        }
+        for (unsigned q = 0; q < npr; q++) {
+            const unsigned e = L.ent[e0 + nre + q];
+            const double2 cg = scg[e & 0xfffu];
+            const double nu = L.nu(e);
+            dg += nu * cg.y;
+            pb *= conc_pow(cg.x, nu);
+        }
+        net -= pb * exp(dg - logPRT * ed.y);
+    }
+    const double r = net * kmod;
+    for (unsigned q = 0; q < nre + npr; q++) {
+        const unsigned e = L.ent[e0 + q];
+        const double nu = L.nu(e);
+        acc(e & 0xfffu, q < nre ? -nu * r : nu * r);
+    }
+}
+
+// ---- analytic Jacobian ------------------------------------------------------
+// One reaction's contributions to R_k (sR), dR_k/dT at fixed c (sDT) and
+// dR_k/dc_i (LU region, [i*65 + k]).  kmod is the effective rate constant
+// (Arrhenius x M for third-body reactions, k_inf Pr/(1+Pr) F for fall-off),
+// dkdM its derivative with respect to the collider concentration M.
+// acc.R(k, r, drdT) adds to R_k and dR_k/dT, acc.D(i, k, v) to dR_k/dc_i; ns species,
+// `dummy` the unused slot of fixed-format records (c = 1, g = 0), sH[k] = H_k/RT.
+template <class Acc>
+__device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, const double *sH, int ns, unsigned dummy,
+                                        int j, double T, double logT, double uT, double logPRT, double cTot,
+                                        Acc &&acc) {
+        const uint4 w = L.W[j];
+    const double2 ab = ((const double2 *)L.P)[2 * j];
+    const double2 ed = ((const double2 *)L.P)[2 * j + 1];
+    const unsigned info = w.x;
+    double kf = exp(ab.x + ab.y * logT - ed.x * uT);
+    if (info & 2u) kf = -kf;
+    const double dlnkf = (ab.y + ed.x * uT) * uT;
+    double kmod = kf, dlnkmod = dlnkf, dkdM = 0.;
+    int dense = 0, single = -1;
+    const int tb = (int)((info >> 8) & 0xffu) - 1;
+    const int fo = (int)((info >> 16) & 0xffu) - 1;
+    if (tb >= 0) {
+        double Me = cTot;
+        for (unsigned q = L.tbp[tb]; q < L.tbp[tb + 1]; q++) Me += scg[L.tbs[q]].x * L.tbe[q];
+        kmod = kf * Me;
+        dkdM = kf;
+        dense = 1;
+    }
+    if (fo >= 0) {
+        const double *fp = L.fop + 8 * fo;
+        const double kinf = exp(fp[0] + fp[1] * logT - fp[2] * uT);
+        const double dlnkinf = (fp[1] + fp[2] * uT) * uT;
+        double Mc;
+        single = L.fos[fo];
+        if (single < 0) {
+            Mc = cTot;
+            for (unsigned q = L.fol[fo]; q < L.fol[fo + 1]; q++) Mc += scg[L.fos2[q]].x * L.foe[q];
+            dense = 2;
+        } else {
+            Mc = scg[single].x + 1.e-16;
+        }
+        const double Pr = kf * Mc / kinf;
+        double F = 1., dlgF_dlgPr = 0., dlnF_dT = 0.;
+        if (fp[7] != 0.) {   // Troe: log10 F = L / (1 + x^2), x = s / (n - 0.14 s), s = log10 Pr + c
+            const double a = fp[3];
+            const double e3 = exp(-T / fp[4]), e1 = exp(-T / fp[5]);
+            double Fc = (1. - a) * e3 + a * e1;
+            double dFc = -(1. - a) / fp[4] * e3 - a / fp[5] * e1;
+            if (fp[6] != 0.) {
+                const double e2 = exp(-fp[6] / T);
+                Fc += e2;
+                dFc += fp[6] * uT * uT * e2;
+            }
+            const double Lf = (Fc < 1.e-300) ? -300. : log10(Fc);
+            const double dL = (Fc < 1.e-300) ? 0. : dFc / (Fc * LN10);
+            double lgF, dlgF_dL;
+            if (Pr > 1.e-32) {
+                const double n = 0.75 - 1.27 * Lf, cc = -0.4 - 0.67 * Lf;
+                const double sT = log10(Pr) + cc;
+                const double den = n - 0.14 * sT;
+                const double x = sT / den;
+                const double q = 1. + x * x;
+                lgF = Lf / q;
+                const double dlgF_dx = -Lf * 2. * x / (q * q);
+                dlgF_dlgPr = dlgF_dx * n / (den * den);
+                dlgF_dL = 1. / q + dlgF_dx * (1.27 * sT - 0.67 * n) / (den * den);
+            } else {
+                const double q = 1. + (1. / 0.14) * (1. / 0.14);
+                lgF = Lf / q;
+                dlgF_dL = 1. / q;
+            }
+            F = pow(10., lgF);
+            dlnF_dT = dlgF_dlgPr * (dlnkf - dlnkinf) + LN10 * dlgF_dL * dL;
+        }
+        kmod = kinf * (Pr / (1. + Pr)) * F;
+        dkdM = (kf * F / (1. + Pr)) * (1. / (1. + Pr) + dlgF_dlgPr);   // d kmod / d M
+        dlnkmod = dlnkinf + (dlnkf - dlnkinf) / (1. + Pr) + dlnF_dT;
+    }
+    if (info & 4u) {
+        const unsigned sl[6] = {w.y & 0xffu, (w.y >> 8) & 0xffu, (w.y >> 16) & 0xffu,
+                                w.z & 0xffu, (w.z >> 8) & 0xffu, (w.z >> 16) & 0xffu};
+        double cv[6];
+#pragma unroll
+        for (int e = 0; e < 6; e++) cv[e] = scg[sl[e]].x;
+        const double pf = (cv[0] * cv[1]) * cv[2];
+        double g[6];
+        g[0] = kmod * (cv[1] * cv[2]);
+        g[1] = kmod * (cv[0] * cv[2]);
+        g[2] = kmod * (cv[0] * cv[1]);
+        double uK = 0., pb = 0., dh = 0.;
+        g[3] = g[4] = g[5] = 0.;
+        if (info & 1u) {
+            const double dg = (scg[sl[3]].y + scg[sl[4]].y + scg[sl[5]].y) - (scg[sl[0]].y + scg[sl[1]].y + scg[sl[2]].y);
+            uK = exp(dg - logPRT * ed.y);
+            pb = (cv[3] * cv[4]) * cv[5];
+            dh = (sH[sl[3]] + sH[sl[4]] + sH[sl[5]]) - (sH[sl[0]] + sH[sl[1]] + sH[sl[2]]);
+            const double kb = -kmod * uK;
+            g[3] = kb * (cv[4] * cv[5]);
+            g[4] = kb * (cv[3] * cv[5]);
+            g[5] = kb * (cv[3] * cv[4]);
+        }
+        const double net = pf - pb * uK;
+        const double r = kmod * net;
+        const double drdT = kmod * (dlnkmod * net - pb * uK * (ed.y - dh) * uT);
+#pragma unroll
+        for (int o = 0; o < 6; o++) {
+            acc.R(sl[o], o < 3 ? -r : r, o < 3 ? -drdT : drdT);
+        }
+#pragma unroll
+        for (int d = 0; d < 6; d++) {
+            if (sl[d] == dummy) continue;
+#pragma unroll
+            for (int o = 0; o < 6; o++)
+                if (sl[o] != dummy) acc.D(sl[d], sl[o], o < 3 ? -g[d] : g[d]);
+        }
+        if (dense) {
+            const double qv = dkdM * net;
+            for (int i = 0; i < ns; i++)
+#pragma unroll
+                for (int o = 0; o < 6; o++)
+                    if (sl[o] != dummy) acc.D(i, sl[o], o < 3 ? -qv : qv);
+            const unsigned q0 = dense == 1 ? L.tbp[tb] : L.fol[fo], q1 = dense == 1 ? L.tbp[tb + 1] : L.fol[fo + 1];
+            for (unsigned q = q0; q < q1; q++) {
+                const unsigned i = dense == 1 ? L.tbs[q] : L.fos2[q];
+                const double e1 = qv * (dense == 1 ? L.tbe[q] : L.foe[q]);
+#pragma unroll
+                for (int o = 0; o < 6; o++)
+                    if (sl[o] != dummy) acc.D(i, sl[o], o < 3 ? -e1 : e1);
+            }
+        } else if (single >= 0) {
+            const double qv = dkdM * net;
+#pragma unroll
+            for (int o = 0; o < 6; o++)
+                if (sl[o] != dummy) acc.D(single, sl[o], o < 3 ? -qv : qv);
+        }
+        return;
+    }
+    // general stoichiometry: d(prod c^nu)/dc_i = nu_i c_i^(nu_i - 1) prod_{others}
+    const unsigned lst = w.w;
+    const unsigned e0 = lst & 0xffffu, nre = (lst >> 16) & 0xffu, npr = lst >> 24;
+    double pf = 1., pb = 1., dg = 0., dh = 0., uK = 0.;
+    for (unsigned q = 0; q < nre; q++) {
+        const unsigned e = L.ent[e0 + q];
+        pf *= conc_pow(scg[e & 0xfffu].x, L.nu(e));
+    }
+    if (info & 1u) {
+        for (unsigned q = 0; q < nre + npr; q++) {
+            const unsigned e = L.ent[e0 + q];
+            const double nu = q < nre ? -L.nu(e) : L.nu(e);
+            dg += nu * scg[e & 0xfffu].y;
+            dh += nu * sH[e & 0xfffu];
+            if (q >= nre) pb *= conc_pow(scg[e & 0xfffu].x, nu);
+        }
+        uK = exp(dg - logPRT * ed.y);
+    }
+    const double net = pf - pb * uK;
+    const double r = kmod * net;
+    const double drdT = kmod * (dlnkmod * net - pb * uK * (ed.y - dh) * uT);
+    for (unsigned q = 0; q < nre + npr; q++) {
+        const unsigned e = L.ent[e0 + q];
+        const double nu = L.nu(e);
+        acc.R(e & 0xfffu, q < nre ? -nu * r : nu * r, q < nre ? -nu * drdT : nu * drdT);
+    }
+    for (unsigned d = 0; d < nre + npr; d++) {
+        const bool fwd = d < nre;
+        if (!fwd && !(info & 1u)) break;
+        const unsigned ed_ = L.ent[e0 + d];
+        const unsigned id = ed_ & 0xfffu;
+        const double nud = L.nu(ed_), cd = scg[id].x;
+        // d(c^nu)/dc; at c = 0 a fractional order has no finite slope: 0 is used
+        double der = nud == 1. ? 1. : ((cd > 0. || nud > 1.) ? nud * conc_pow(cd, nud - 1.) : 0.);
+        const unsigned qa = fwd ? 0 : nre, qb = fwd ? nre : nre + npr;
+        for (unsigned q = qa; q < qb; q++)
+            if (q != d) der *= conc_pow(scg[L.ent[e0 + q] & 0xfffu].x, L.nu(L.ent[e0 + q]));
+        const double gv = fwd ? kmod * der : -kmod * uK * der;
+        for (unsigned o = 0; o < nre + npr; o++) {
+            const unsigned eo = L.ent[e0 + o];
+            const double nu = L.nu(eo);
+            acc.D(id, eo & 0xfffu, o < nre ? -nu * gv : nu * gv);
+        }
+    }
+    if (dense || single >= 0) {
+        const double qv = dkdM * net;
+        for (unsigned o = 0; o < nre + npr; o++) {
+            const unsigned eo = L.ent[e0 + o];
+            const double sg = o < nre ? -L.nu(eo) : L.nu(eo);
+            const unsigned ko = eo & 0xfffu;
+            if (single >= 0) {
+                acc.D(single, ko, sg * qv);
+                continue;
+            }
+            for (int i = 0; i < ns; i++) acc.D(i, ko, sg * qv);
+            const unsigned q0 = dense == 1 ? L.tbp[tb] : L.fol[fo], q1 = dense == 1 ? L.tbp[tb + 1] : L.fol[fo + 1];
+            for (unsigned q = q0; q < q1; q++) {
+                const unsigned i = dense == 1 ? L.tbs[q] : L.fos2[q];
+                acc.D(i, ko, sg * qv * (dense == 1 ? L.tbe[q] : L.foe[q]));
+            }
+        }
+    }
+}
+
 template <int NMAX>
 struct TeamCtx {
     double V[NVEC][1];
@@ -270,105 +576,8 @@ struct TeamCtx {
     __device__ __forceinline__ double reduce(double v) { return wave_sum(v); }
     __device__ __forceinline__ int count(bool b) { return __popcll(__ballot(b)); }
 
-    // One reaction (KineticsMap_CHEMKIN.hpp:1192-1386, 1683-1768), with the formation
-    // rates (StoichiometricMap.hpp:656-730) accumulated reaction-major into LDS.
-    // 1/Kc = exp(sum_k nu_k g_k - dn ln(Patm/RT)) with g = h/RT - s/R, which equals
-    // the reference's exp(-dS/R + dH/RT - dn ln(Patm/RT)).
     __device__ __forceinline__ void reaction(int j, double T, double logT, double uT, double logPRT, double cTot) {
-        const uint4 w = L.W[j];
-        const double2 ab = ((const double2 *)L.P)[2 * j];
-        const double2 ed = ((const double2 *)L.P)[2 * j + 1];
-        const unsigned info = w.x;
-        double kf = exp(ab.x + ab.y * logT - ed.x * uT);
-        if (info & 2u) kf = -kf;
-        double kmod = kf;
-        const int tb = (int)((info >> 8) & 0xffu) - 1;
-        if (tb >= 0) {
-            double Me = cTot;
-            for (unsigned q = L.tbp[tb]; q < L.tbp[tb + 1]; q++) Me += scg[L.tbs[q]].x * L.tbe[q];
-            kmod *= Me;
-        }
-        const int fo = (int)((info >> 16) & 0xffu) - 1;
-        if (fo >= 0) {
-            const double *fp = L.fop + 8 * fo;
-            const double kinf = exp(fp[0] + fp[1] * logT - fp[2] * uT);
-            double Mc;
-            if (L.fos[fo] < 0) {
-                Mc = cTot;
-                for (unsigned q = L.fol[fo]; q < L.fol[fo + 1]; q++) Mc += scg[L.fos2[q]].x * L.foe[q];
-            } else {
-                Mc = scg[L.fos[fo]].x + 1.e-16;
-            }
-            const double Pr = kf * Mc / kinf;
-            double wF = 1.;
-            if (fp[7] != 0.) {   // Troe
-                const double a = fp[3];
-                double F = (1. - a) * exp(-T / fp[4]) + a * exp(-T / fp[5]);
-                if (fp[6] != 0.) F += exp(-fp[6] / T);
-                const double lf = (F < 1.e-300) ? -300. : log10(F);
-                double q;
-                if (Pr > 1.e-32) {
-                    const double nT = 0.75 - 1.27 * lf;
-                    const double cT = -0.4 - 0.67 * lf;
-                    const double sT = log10(Pr) + cT;
-                    q = sT / (nT - 0.14 * sT);
-                } else {
-                    q = 1. / 0.14;
-                }
-                wF = pow(10., lf / (1. + q * q));
-            }
-            kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
-        }
-        if (info & 4u) {
-            // fixed-format record: three reactant and three product slots
-            const unsigned r0 = w.y & 0xffu, r1 = (w.y >> 8) & 0xffu, r2 = (w.y >> 16) & 0xffu;
-            const unsigned p0 = w.z & 0xffu, p1 = (w.z >> 8) & 0xffu, p2 = (w.z >> 16) & 0xffu;
-            const double2 a0 = scg[r0], a1 = scg[r1], a2 = scg[r2];
-            const double2 b0 = scg[p0], b1 = scg[p1], b2 = scg[p2];
-            double net = (a0.x * a1.x) * a2.x;
-            if (info & 1u) {
-                const double dg = (b0.y + b1.y + b2.y) - (a0.y + a1.y + a2.y);
-                net -= ((b0.x * b1.x) * b2.x) * exp(dg - logPRT * ed.y);
-            }
-            const double r = net * kmod;
-            atomicAdd(&sR[r0], -r);
-            atomicAdd(&sR[r1], -r);
-            atomicAdd(&sR[r2], -r);
-            atomicAdd(&sR[p0], r);
-            atomicAdd(&sR[p1], r);
-            atomicAdd(&sR[p2], r);
-            return;
-        }
-        // general stoichiometry (any orders, any number of species)
-        const unsigned lst = w.w;
-        const unsigned e0 = lst & 0xffffu, nre = (lst >> 16) & 0xffu, npr = lst >> 24;
-        double pf = 1.;
-        for (unsigned q = 0; q < nre; q++) {
-            const unsigned e = L.ent[e0 + q];
-            pf *= conc_pow(scg[e & 0xfffu].x, L.nu(e));
-        }
-        double net = pf;
-        if (info & 1u) {
-            double dg = 0., pb = 1.;
-            for (unsigned q = 0; q < nre; q++) {
-                const unsigned e = L.ent[e0 + q];
-                dg -= L.nu(e) * scg[e & 0xfffu].y;
-            }
-            for (unsigned q = 0; q < npr; q++) {
-                const unsigned e = L.ent[e0 + nre + q];
-                const double2 cg = scg[e & 0xfffu];
-                const double nu = L.nu(e);
-                dg += nu * cg.y;
-                pb *= conc_pow(cg.x, nu);
-            }
-            net -= pb * exp(dg - logPRT * ed.y);
-        }
-        const double r = net * kmod;
-        for (unsigned q = 0; q < nre + npr; q++) {
-            const unsigned e = L.ent[e0 + q];
-            const double nu = L.nu(e);
-            atomicAdd(&sR[e & 0xfffu], q < nre ? -nu * r : nu * r);
-        }
+        rxn_rate(L, scg, j, T, logT, uT, logPRT, cTot, [&](unsigned k, double v) { atomicAdd(&sR[k], v); });
     }
 
     __device__ __forceinline__ void rhs(int src, int dst) {
@@ -424,204 +633,16 @@ struct TeamCtx {
         V[dst][0] = dy;
     }
 
-    // ---- analytic Jacobian ------------------------------------------------------
-    // One reaction's contributions to R_k (sR), dR_k/dT at fixed c (sDT) and
-    // dR_k/dc_i (LU region, [i*65 + k]).  kmod is the effective rate constant
-    // (Arrhenius x M for third-body reactions, k_inf Pr/(1+Pr) F for fall-off),
-    // dkdM its derivative with respect to the collider concentration M.
+    struct JacAcc {
+        double *sR, *sDT, *Dm;
+        __device__ __forceinline__ void R(unsigned k, double r, double drdT) const {
+            atomicAdd(&sR[k], r);
+            atomicAdd(&sDT[k], drdT);
+        }
+        __device__ __forceinline__ void D(unsigned i, unsigned k, double v) const { atomicAdd(&Dm[i * 65 + k], v); }
+    };
     __device__ __forceinline__ void reaction_jac(int j, double T, double logT, double uT, double logPRT, double cTot) {
-        double *D = LU;
-        const uint4 w = L.W[j];
-        const double2 ab = ((const double2 *)L.P)[2 * j];
-        const double2 ed = ((const double2 *)L.P)[2 * j + 1];
-        const unsigned info = w.x;
-        double kf = exp(ab.x + ab.y * logT - ed.x * uT);
-        if (info & 2u) kf = -kf;
-        const double dlnkf = (ab.y + ed.x * uT) * uT;
-        double kmod = kf, dlnkmod = dlnkf, dkdM = 0.;
-        int dense = 0, single = -1;
-        const int tb = (int)((info >> 8) & 0xffu) - 1;
-        const int fo = (int)((info >> 16) & 0xffu) - 1;
-        if (tb >= 0) {
-            double Me = cTot;
-            for (unsigned q = L.tbp[tb]; q < L.tbp[tb + 1]; q++) Me += scg[L.tbs[q]].x * L.tbe[q];
-            kmod = kf * Me;
-            dkdM = kf;
-            dense = 1;
-        }
-        if (fo >= 0) {
-            const double *fp = L.fop + 8 * fo;
-            const double kinf = exp(fp[0] + fp[1] * logT - fp[2] * uT);
-            const double dlnkinf = (fp[1] + fp[2] * uT) * uT;
-            double Mc;
-            single = L.fos[fo];
-            if (single < 0) {
-                Mc = cTot;
-                for (unsigned q = L.fol[fo]; q < L.fol[fo + 1]; q++) Mc += scg[L.fos2[q]].x * L.foe[q];
-                dense = 2;
-            } else {
-                Mc = scg[single].x + 1.e-16;
-            }
-            const double Pr = kf * Mc / kinf;
-            double F = 1., dlgF_dlgPr = 0., dlnF_dT = 0.;
-            if (fp[7] != 0.) {   // Troe: log10 F = L / (1 + x^2), x = s / (n - 0.14 s), s = log10 Pr + c
-                const double a = fp[3];
-                const double e3 = exp(-T / fp[4]), e1 = exp(-T / fp[5]);
-                double Fc = (1. - a) * e3 + a * e1;
-                double dFc = -(1. - a) / fp[4] * e3 - a / fp[5] * e1;
-                if (fp[6] != 0.) {
-                    const double e2 = exp(-fp[6] / T);
-                    Fc += e2;
-                    dFc += fp[6] * uT * uT * e2;
-                }
-                const double Lf = (Fc < 1.e-300) ? -300. : log10(Fc);
-                const double dL = (Fc < 1.e-300) ? 0. : dFc / (Fc * LN10);
-                double lgF, dlgF_dL;
-                if (Pr > 1.e-32) {
-                    const double n = 0.75 - 1.27 * Lf, cc = -0.4 - 0.67 * Lf;
-                    const double sT = log10(Pr) + cc;
-                    const double den = n - 0.14 * sT;
-                    const double x = sT / den;
-                    const double q = 1. + x * x;
-                    lgF = Lf / q;
-                    const double dlgF_dx = -Lf * 2. * x / (q * q);
-                    dlgF_dlgPr = dlgF_dx * n / (den * den);
-                    dlgF_dL = 1. / q + dlgF_dx * (1.27 * sT - 0.67 * n) / (den * den);
-                } else {
-                    const double q = 1. + (1. / 0.14) * (1. / 0.14);
-                    lgF = Lf / q;
-                    dlgF_dL = 1. / q;
-                }
-                F = pow(10., lgF);
-                dlnF_dT = dlgF_dlgPr * (dlnkf - dlnkinf) + LN10 * dlgF_dL * dL;
-            }
-            kmod = kinf * (Pr / (1. + Pr)) * F;
-            dkdM = (kf * F / (1. + Pr)) * (1. / (1. + Pr) + dlgF_dlgPr);   // d kmod / d M
-            dlnkmod = dlnkinf + (dlnkf - dlnkinf) / (1. + Pr) + dlnF_dT;
-        }
-        if (info & 4u) {
-            const unsigned sl[6] = {w.y & 0xffu, (w.y >> 8) & 0xffu, (w.y >> 16) & 0xffu,
-                                    w.z & 0xffu, (w.z >> 8) & 0xffu, (w.z >> 16) & 0xffu};
-            double cv[6];
-#pragma unroll
-            for (int e = 0; e < 6; e++) cv[e] = scg[sl[e]].x;
-            const double pf = (cv[0] * cv[1]) * cv[2];
-            double g[6];
-            g[0] = kmod * (cv[1] * cv[2]);
-            g[1] = kmod * (cv[0] * cv[2]);
-            g[2] = kmod * (cv[0] * cv[1]);
-            double uK = 0., pb = 0., dh = 0.;
-            g[3] = g[4] = g[5] = 0.;
-            if (info & 1u) {
-                const double dg = (scg[sl[3]].y + scg[sl[4]].y + scg[sl[5]].y) - (scg[sl[0]].y + scg[sl[1]].y + scg[sl[2]].y);
-                uK = exp(dg - logPRT * ed.y);
-                pb = (cv[3] * cv[4]) * cv[5];
-                dh = (sH[sl[3]] + sH[sl[4]] + sH[sl[5]]) - (sH[sl[0]] + sH[sl[1]] + sH[sl[2]]);
-                const double kb = -kmod * uK;
-                g[3] = kb * (cv[4] * cv[5]);
-                g[4] = kb * (cv[3] * cv[5]);
-                g[5] = kb * (cv[3] * cv[4]);
-            }
-            const double net = pf - pb * uK;
-            const double r = kmod * net;
-            const double drdT = kmod * (dlnkmod * net - pb * uK * (ed.y - dh) * uT);
-#pragma unroll
-            for (int o = 0; o < 6; o++) {
-                atomicAdd(&sR[sl[o]], o < 3 ? -r : r);
-                atomicAdd(&sDT[sl[o]], o < 3 ? -drdT : drdT);
-            }
-#pragma unroll
-            for (int d = 0; d < 6; d++) {
-                if (sl[d] == LSM_DUMMY_SP) continue;
-#pragma unroll
-                for (int o = 0; o < 6; o++)
-                    if (sl[o] != LSM_DUMMY_SP) atomicAdd(&D[sl[d] * 65 + sl[o]], o < 3 ? -g[d] : g[d]);
-            }
-            if (dense) {
-                const double qv = dkdM * net;
-                for (int i = 0; i < ns; i++)
-#pragma unroll
-                    for (int o = 0; o < 6; o++)
-                        if (sl[o] != LSM_DUMMY_SP) atomicAdd(&D[i * 65 + sl[o]], o < 3 ? -qv : qv);
-                const unsigned q0 = dense == 1 ? L.tbp[tb] : L.fol[fo], q1 = dense == 1 ? L.tbp[tb + 1] : L.fol[fo + 1];
-                for (unsigned q = q0; q < q1; q++) {
-                    const unsigned i = dense == 1 ? L.tbs[q] : L.fos2[q];
-                    const double e1 = qv * (dense == 1 ? L.tbe[q] : L.foe[q]);
-#pragma unroll
-                    for (int o = 0; o < 6; o++)
-                        if (sl[o] != LSM_DUMMY_SP) atomicAdd(&D[i * 65 + sl[o]], o < 3 ? -e1 : e1);
-                }
-            } else if (single >= 0) {
-                const double qv = dkdM * net;
-#pragma unroll
-                for (int o = 0; o < 6; o++)
-                    if (sl[o] != LSM_DUMMY_SP) atomicAdd(&D[single * 65 + sl[o]], o < 3 ? -qv : qv);
-            }
-            return;
-        }
-        // general stoichiometry: d(prod c^nu)/dc_i = nu_i c_i^(nu_i - 1) prod_{others}
-        const unsigned lst = w.w;
-        const unsigned e0 = lst & 0xffffu, nre = (lst >> 16) & 0xffu, npr = lst >> 24;
-        double pf = 1., pb = 1., dg = 0., dh = 0., uK = 0.;
-        for (unsigned q = 0; q < nre; q++) {
-            const unsigned e = L.ent[e0 + q];
-            pf *= conc_pow(scg[e & 0xfffu].x, L.nu(e));
-        }
-        if (info & 1u) {
-            for (unsigned q = 0; q < nre + npr; q++) {
-                const unsigned e = L.ent[e0 + q];
-                const double nu = q < nre ? -L.nu(e) : L.nu(e);
-                dg += nu * scg[e & 0xfffu].y;
-                dh += nu * sH[e & 0xfffu];
-                if (q >= nre) pb *= conc_pow(scg[e & 0xfffu].x, nu);
-            }
-            uK = exp(dg - logPRT * ed.y);
-        }
-        const double net = pf - pb * uK;
-        const double r = kmod * net;
-        const double drdT = kmod * (dlnkmod * net - pb * uK * (ed.y - dh) * uT);
-        for (unsigned q = 0; q < nre + npr; q++) {
-            const unsigned e = L.ent[e0 + q];
-            const double nu = L.nu(e);
-            atomicAdd(&sR[e & 0xfffu], q < nre ? -nu * r : nu * r);
-            atomicAdd(&sDT[e & 0xfffu], q < nre ? -nu * drdT : nu * drdT);
-        }
-        for (unsigned d = 0; d < nre + npr; d++) {
-            const bool fwd = d < nre;
-            if (!fwd && !(info & 1u)) break;
-            const unsigned ed_ = L.ent[e0 + d];
-            const unsigned id = ed_ & 0xfffu;
-            const double nud = L.nu(ed_), cd = scg[id].x;
-            // d(c^nu)/dc; at c = 0 a fractional order has no finite slope: 0 is used
-            double der = nud == 1. ? 1. : ((cd > 0. || nud > 1.) ? nud * conc_pow(cd, nud - 1.) : 0.);
-            const unsigned qa = fwd ? 0 : nre, qb = fwd ? nre : nre + npr;
-            for (unsigned q = qa; q < qb; q++)
-                if (q != d) der *= conc_pow(scg[L.ent[e0 + q] & 0xfffu].x, L.nu(L.ent[e0 + q]));
-            const double gv = fwd ? kmod * der : -kmod * uK * der;
-            for (unsigned o = 0; o < nre + npr; o++) {
-                const unsigned eo = L.ent[e0 + o];
-                const double nu = L.nu(eo);
-                atomicAdd(&D[id * 65 + (eo & 0xfffu)], o < nre ? -nu * gv : nu * gv);
-            }
-        }
-        if (dense || single >= 0) {
-            const double qv = dkdM * net;
-            for (unsigned o = 0; o < nre + npr; o++) {
-                const unsigned eo = L.ent[e0 + o];
-                const double sg = o < nre ? -L.nu(eo) : L.nu(eo);
-                const unsigned ko = eo & 0xfffu;
-                if (single >= 0) {
-                    atomicAdd(&D[single * 65 + ko], sg * qv);
-                    continue;
-                }
-                for (int i = 0; i < ns; i++) atomicAdd(&D[i * 65 + ko], sg * qv);
-                const unsigned q0 = dense == 1 ? L.tbp[tb] : L.fol[fo], q1 = dense == 1 ? L.tbp[tb + 1] : L.fol[fo + 1];
-                for (unsigned q = q0; q < q1; q++) {
-                    const unsigned i = dense == 1 ? L.tbs[q] : L.fos2[q];
-                    atomicAdd(&D[i * 65 + ko], sg * qv * (dense == 1 ? L.tbe[q] : L.foe[q]));
-                }
-            }
-        }
+        rxn_jac(L, scg, sH, ns, LSM_DUMMY_SP, j, T, logT, uT, logPRT, cTot, JacAcc{sR, sDT, LU});
     }
 
     // Analytic Jacobian of the constant-pressure reactor at the state in VB:
@@ -801,14 +822,16 @@ struct TeamCtx {
             int pk[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                lk[u] = (ps > k + u) ? LU[(k + u) * 64 + lane] : 0.;
+                const double v = LU[(k + u) * 64 + lane];   // unconditional load, then mask
+                lk[u] = (ps > k + u) ? v : 0.;
                 pk[u] = __builtin_amdgcn_readlane(plv, k + u);
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) b = fma(-lk[u], readlane_d(b, pk[u]), b);
         }
         for (; k < ne; k++) {
-            const double lk = (ps > k) ? LU[k * 64 + lane] : 0.;
+            const double v = LU[k * 64 + lane];
+            const double lk = (ps > k) ? v : 0.;
             b = fma(-lk, readlane_d(b, __builtin_amdgcn_readlane(plv, k)), b);
         }
         double xr = 0.;
@@ -818,7 +841,8 @@ struct TeamCtx {
             int pk[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                uk[u] = (ps < k - u) ? LU[(k - u) * 64 + lane] : 0.;
+                const double v = LU[(k - u) * 64 + lane];
+                uk[u] = (ps < k - u) ? v : 0.;
                 pk[u] = __builtin_amdgcn_readlane(plv, k - u);
                 dk[u] = readlane_d(dinv, pk[u]);
             }
@@ -830,7 +854,8 @@ struct TeamCtx {
             }
         }
         for (; k >= 0; k--) {
-            const double uk = (ps < k) ? LU[k * 64 + lane] : 0.;
+            const double v = LU[k * 64 + lane];
+            const double uk = (ps < k) ? v : 0.;
             const int p = __builtin_amdgcn_readlane(plv, k);
             const double xk = readlane_d(b, p) * readlane_d(dinv, p);
             b = fma(-uk, xk, b);
@@ -873,7 +898,8 @@ struct TeamCtx {
 template <int NMAX>
 __global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
                                                    const double *yin, double *yout, int *status, double *q,
-                                                   int *counters, unsigned *queue, double *jws) {
+                                                   int *counters, unsigned *queue, const int *order,
+                                                   double *jws) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     {   // stage the mechanism image (once per workgroup)
         const int4 *src = (const int4 *)M.img.src;
@@ -891,8 +917,11 @@ __global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t
         unsigned cell_u = 0;
         if (lane == 0) cell_u = atomicAdd(queue, 1u);
         cell_u = __shfl(cell_u, 0, 64);
-        const int64_t cell = (int64_t)__builtin_amdgcn_readfirstlane((int)cell_u);
-        if (cell >= n) break;
+        const int64_t ticket = (int64_t)__builtin_amdgcn_readfirstlane((int)cell_u);
+        if (ticket >= n) break;
+        // visiting order (e.g. most expensive cells first); entries outside [0, n) are skipped
+        const int64_t cell = order ? (int64_t)__builtin_amdgcn_readfirstlane(order[ticket]) : ticket;
+        if (cell < 0 || cell >= n) continue;
         {   // opaque per-cell copies: keep lane/size-derived constants from being hoisted
             // out of the cell loop (they would stay live in registers for the whole kernel)
             int l = threadIdx.x & 63, a = M.ne, b = M.ns, r = M.nr;
@@ -942,6 +971,624 @@ __global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t
     }
 #ifdef LSM_PHASE_TIMERS
     if (lane == 0 && K.phase_cycles)
+        for (int i = 0; i < 7; i++) atomicAdd(&K.phase_cycles[i], c.tp[i]);
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// Wide context: one cell per 256-thread workgroup (ne <= 256).  Thread e owns
+// element e of every vector (registers); sums are wave DPP reductions combined
+// across the four waves through LDS (one barrier, fixed order, identical result
+// in every thread).  The Newton matrix G = I - h l0 J is factorised in FP32 in
+// LDS (row-major, physical row swaps); the Jacobian stays FP64 in an HBM/L2
+// workspace, which also serves the FP64 residuals of iterative refinement and
+// J*v.  Formation rates are accumulated per wave and summed in wave order, so
+// the RHS is bitwise reproducible.
+// ---------------------------------------------------------------------------
+struct WideCtx {
+    double V[NVEC][1];
+    const KParams &K;
+    LdsMech L;
+    int tid, wave, lane, ne, ns, nr, lda;
+    unsigned dummy;
+    double qr;
+    double hr0;               // h l0 of the current factorisation (for the residuals)
+    int rp;                   // reduction buffer parity
+    double2 *scg;             // LDS [256]  (c_k, g_k)
+    double *sR, *sDT;         // LDS [4][256] per-wave accumulators
+    double *sH;               // LDS [256]  h_k = H_k/RT
+    double *red;              // LDS [2][8] reduction / broadcast slots, then the queue slot
+    double *bs;               // LDS [256]  solve and mat-vec exchange
+    double *dinv;             // LDS [256]  1 / pivot, by row
+    int *perm;                // LDS [256]  row chosen as pivot at each elimination step
+    unsigned long long *pkey; // LDS [2]    pivot keys (parity of the elimination step)
+    float *fsc;               // LDS [1536] factorisation scratch
+    float *dsc;               // LDS [256]  1 / pivot of step k (U rows are stored scaled by it)
+    float *A;                 // LDS [ne + 1][lda] LU factors (spare last row); dR/dc in an analytic Jacobian
+    double *J;                // HBM [ne][ne] column-major, J[j*ne + i] = d f_i / d y_j
+#ifdef LSM_PHASE_TIMERS
+    unsigned long long tp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int in_jac = 0;
+#endif
+
+    __device__ WideCtx(const DevMech &m, const KParams &k, unsigned char *lds, double *jac)
+        : K(k), L((const unsigned char *)m.img.src, m.img, m.ns_pad), tid(threadIdx.x), wave(threadIdx.x >> 6),
+          lane(threadIdx.x & 63), ne(m.ne), ns(m.ns), nr(m.nr), lda(m.ne | 1), dummy((unsigned)m.img.dummy), qr(0.),
+          hr0(0.), rp(0), J(jac) {
+        double *d = (double *)lds;
+        scg = (double2 *)d;
+        sR = d + 512;
+        sDT = d + 1536;
+        sH = d + 2560;
+        red = d + 2816;             // [2][8] reductions, [16] queue slot
+        bs = d + 2840;
+        dinv = d + 3096;
+        pkey = (unsigned long long *)(d + 3352);
+        perm = (int *)(d + 3354);
+        fsc = (float *)(d + 3482);  // [1536]: [256,512) 1/pivot by step
+        dsc = fsc + 256;
+        A = (float *)(d + 4250);
+    }
+    __host__ __device__ static size_t lds_bytes(int ne) { return 8 * 4250 + 4 * (size_t)(ne + 1) * (size_t)(ne | 1); }
+
+    template <class F>
+    __device__ __forceinline__ void each(F f) { f(0); }
+    __device__ __forceinline__ double &x(int vid, int e) { return V[vid][e]; }
+    __device__ __forceinline__ double ymin(int) const {
+        return tid < ns ? 0. : (tid == ns ? 200. : NEG_INF);
+    }
+    __device__ __forceinline__ double ymax(int) const {
+        return tid < ns ? 1. : (tid == ns ? 6000. : POS_INF);
+    }
+    // Sum over the workgroup; slot 4 of the same buffer carries a broadcast value
+    // (written by thread `src`, -1 for none).
+    __device__ __forceinline__ double wg_sum(double v, double *bc = nullptr, double bv = 0., int src = -1) {
+        v = wave_sum(v);
+        double *b = red + 8 * rp;
+        rp ^= 1;
+        if (lane == 0) b[wave] = v;
+        if (tid == src) b[4] = bv;
+        __syncthreads();
+        if (bc) *bc = b[4];
+        return (b[0] + b[1]) + (b[2] + b[3]);
+    }
+    __device__ __forceinline__ void wg_sum2(double &a, double &c) {
+        wave_sum2(a, c);
+        double *b = red + 8 * rp;
+        rp ^= 1;
+        if (lane == 0) { b[wave] = a; b[4 + wave] = c; }
+        __syncthreads();
+        a = (b[0] + b[1]) + (b[2] + b[3]);
+        c = (b[4] + b[5]) + (b[6] + b[7]);
+    }
+    __device__ __forceinline__ double reduce(double v) { return wg_sum(v); }
+    __device__ __forceinline__ int count(bool b) { return (int)wg_sum(b ? 1. : 0.); }
+
+    __device__ __forceinline__ void rhs(int src, int dst) {
+        LSM_TIC(t0);
+        rhs_impl(src, dst);
+#ifdef LSM_PHASE_TIMERS
+        if (!in_jac) LSM_TOC(0, t0);
+#endif
+    }
+
+    // NASA-7 of species k at T (thread k): cp/R, h/RT, s/R and d(cp/R)/dT.
+    __device__ __forceinline__ void thermo(int k, double T, double &cp, double &h, double &s, double &dcp) const {
+        const int P = L.SP;
+        const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, uT = 1. / T, logT = log(T);
+        const double *a = L.sp + ((T > L.sp[14 * P + k]) ? 0 : 7 * P) + k;
+        const double a0 = a[0], a1 = a[P], a2 = a[2 * P], a3 = a[3 * P], a4 = a[4 * P];
+        cp = a0 + T * a1 + T2 * a2 + T3 * a3 + T4 * a4;
+        dcp = a1 + T * (2. * a2) + T2 * (3. * a3) + T3 * (4. * a4);
+        h = a0 + T * (0.50 * a1) + T2 * ((1. / 3.) * a2) + T3 * (0.25 * a3) + T4 * (0.20 * a4) + uT * a[5 * P];
+        s = a0 * logT + T * a1 + T2 * (0.50 * a2) + T3 * ((1. / 3.) * a3) + T4 * (0.25 * a4) + a[6 * P];
+    }
+
+    // BatchReactorHomogeneousConstantPressure::Equations (:175-247)
+    __device__ __forceinline__ void rhs_impl(int src, int dst) {
+        const int k = tid;
+        const bool isp = k < ns;
+        const double yv = V[src][0];
+        const double mwk = isp ? L.sp[15 * L.SP + k] : 1.;
+        double T;
+        const double MW = 1. / wg_sum(isp ? yv / mwk : 0., &T, yv, ns);
+        const double xk = isp ? (MW * yv) / mwk : 0.;
+        const double cTotP = K.pressure / RGAS / T;
+        const double ck = cTotP * xk;
+        const double rho = cTotP * MW;
+        double cp = 0., h = 0., s = 0., dcp = 0.;
+        if (isp) {
+            thermo(k, T, cp, h, s, dcp);
+            scg[k] = make_double2(ck, h - s);
+        }
+        if (k == (int)dummy) scg[k] = make_double2(1., 0.);
+#pragma unroll
+        for (int w = 0; w < 4; w++) sR[w * 256 + k] = 0.;
+        double cTot = ck, cpx = isp ? cp * xk : 0.;
+        wg_sum2(cTot, cpx);                        // also publishes scg and the cleared sR
+        const double logT = log(T), uT = 1. / T, logPRT = log(101325. / RGAS / T);
+        double *sRw = sR + 256 * wave;
+        for (int j = tid; j < nr; j += WIDE_THREADS)
+            rxn_rate(L, scg, j, T, logT, uT, logPRT, cTot, [&](unsigned kk, double v) { atomicAdd(&sRw[kk], v); });
+        __syncthreads();
+        const double Rk = isp ? ((sR[k] + sR[256 + k]) + (sR[512 + k] + sR[768 + k])) : 0.;
+        double dy = isp ? (mwk * Rk) / rho : 0.;
+        if (K.energy) {
+            const double hs = wg_sum(isp ? Rk * h : 0.);
+            qr = -hs * RGAS * T;
+            const double CpMixMass = (cpx * RGAS) / MW;
+            if (k == ns) dy = qr / (rho * CpMixMass);
+        }
+        V[dst][0] = k < ne ? dy : 0.;
+    }
+
+    // KernelDense::NumericalJacobian, max-constraint branch (KernelDense.hpp:160-183)
+    __device__ __forceinline__ void jacobian_fd(double h) {
+        const double ZERO_DER = sqrt(TINY_F), ETA2 = sqrt(EPS_D), BETA = 1.e+3 * EPS_D;
+        const double fw = V[F_][0] * V[OOE_][0];
+        double hf = BETA * fabs(h) * sqrt(wg_sum(fw * fw) / (double)ne) * (double)ne;
+        if (hf < 1.e-10) hf = 1.;
+        for (int j = 0; j < ne; j++) {
+            double *b = red + 8 * rp;
+            rp ^= 1;
+            if (tid == j) { b[0] = V[VB_][0]; b[1] = V[OOE_][0]; }
+            __syncthreads();
+            const double yh = b[0], ej = b[1];
+            const double a = fabs(yh), bb = fabs(1. / ej);
+            double hJ = ETA2 * (a > bb ? a : bb);
+            hJ = fmax(hJ, hf / ej);
+            hJ = fmax(hJ, ZERO_DER);
+            const double ymx = j < ns ? 1. : 6000.;
+            if (yh + hJ > ymx) hJ = -hJ;
+            if (tid == j) V[VB_][0] = yh + hJ;
+            rhs(VB_, AUX_);
+            if (tid == j) V[VB_][0] = yh;
+            const double hInv = 1. / hJ;
+            if (tid < ne) J[(size_t)j * ne + tid] = hInv * (V[AUX_][0] - V[F_][0]);
+        }
+    }
+
+    struct WideJacAcc {
+        double *sRw, *sDTw;
+        float *Dm;
+        int lda;
+        __device__ __forceinline__ void R(unsigned k, double r, double drdT) const {
+            atomicAdd(&sRw[k], r);
+            atomicAdd(&sDTw[k], drdT);
+        }
+        __device__ __forceinline__ void D(unsigned i, unsigned k, double v) const {
+            atomicAdd(&Dm[i * lda + k], (float)v);
+        }
+    };
+
+    // Analytic Jacobian (same closed forms as the team kernel).  dR/dc is gathered
+    // in FP32 in the LU area of LDS: it only feeds the FP32 Newton matrix.
+    __device__ __forceinline__ void jacobian_analytic() {
+        const int k = tid;
+        const bool isp = k < ns;
+        const double yv = V[VB_][0];
+        const int P = L.SP;
+        const double mwk = isp ? L.sp[15 * P + k] : 1.;
+        double T;
+        const double MW = 1. / wg_sum(isp ? yv / mwk : 0., &T, yv, ns);
+        const double Yk = isp ? yv : 0.;
+        const double xk = (MW * Yk) / mwk;
+        const double cTotP = K.pressure / RGAS / T;
+        const double ck = cTotP * xk;
+        const double rho = cTotP * MW;
+        const double uT = 1. / T, logT = log(T);
+        double cp = 0., h = 0., s = 0., dcp = 0.;
+        if (isp) {
+            thermo(k, T, cp, h, s, dcp);
+            scg[k] = make_double2(ck, h - s);
+            sH[k] = h;
+        }
+        if (k == (int)dummy) { scg[k] = make_double2(1., 0.); sH[k] = 0.; }
+#pragma unroll
+        for (int w = 0; w < 4; w++) { sR[w * 256 + k] = 0.; sDT[w * 256 + k] = 0.; }
+        for (int idx = tid; idx < ns * lda; idx += WIDE_THREADS) A[idx] = 0.f;
+        double cTot = ck, cpx = isp ? cp * xk : 0.;
+        wg_sum2(cTot, cpx);
+        const double logPRT = log(101325. / RGAS / T);
+        const WideJacAcc acc{sR + 256 * wave, sDT + 256 * wave, A, lda};
+        for (int j = tid; j < nr; j += WIDE_THREADS) rxn_jac(L, scg, sH, ns, dummy, j, T, logT, uT, logPRT, cTot, acc);
+        __syncthreads();
+        const double Rk = isp ? ((sR[k] + sR[256 + k]) + (sR[512 + k] + sR[768 + k])) : 0.;
+        const double dRdT = isp ? ((sDT[k] + sDT[256 + k]) + (sDT[512 + k] + sDT[768 + k])) : 0.;
+        double S = 0.;
+        if (isp)
+            for (int i = 0; i < ns; i++) S += (double)A[i * lda + k] * scg[i].x;
+        const double fk = isp ? (mwk * Rk) / rho : 0.;
+        if (isp) {
+            for (int j = 0; j < ns; j++) {
+                const double mwj = L.sp[15 * P + j];
+                J[(size_t)j * ne + k] = (mwk / (rho * mwj)) * (rho * (double)A[j * lda + k] - MW * S) + fk * MW / mwj;
+            }
+            J[(size_t)ns * ne + k] = (mwk / rho) * (dRdT - S * uT) + fk * uT;
+        }
+        const double cpm = (cpx * RGAS) / MW;
+        const double Phi = rho * cpm;
+        double hs = isp ? h * Rk : 0., hS = isp ? h * S : 0.;
+        wg_sum2(hs, hS);
+        const double QR = -hs * RGAS * T;
+        const double fT = K.energy ? QR / Phi : 0.;
+        double Acol = 0.;
+        if (isp)
+            for (int i = 0; i < ns; i++) Acol += sH[i] * (double)A[k * lda + i];
+        double dQ = isp ? -(cp * RGAS * Rk + h * RGAS * T * (dRdT - S * uT)) : 0.;
+        double dP = isp ? Yk * dcp * RGAS / mwk : 0.;
+        wg_sum2(dQ, dP);
+        if (K.energy) {
+            if (isp) {
+                const double dQdY = -(RGAS * T / mwk) * (rho * Acol - MW * hS);
+                const double dPdY = (rho / mwk) * (cp * RGAS - MW * cpm);
+                J[(size_t)k * ne + ns] = (dQdY - fT * dPdY) / Phi;
+            }
+            if (k == ns) J[(size_t)ns * ne + ns] = (dQ - fT * (-Phi * uT + rho * dP)) / Phi;
+        } else {
+            if (isp) J[(size_t)k * ne + ns] = 0.;
+            if (k == ns) J[(size_t)ns * ne + ns] = 0.;
+        }
+        __syncthreads();   // A is reused by the factorisation; J columns complete
+    }
+
+    __device__ __forceinline__ int jacobian(double h) {
+        LSM_TIC(t0);
+#ifdef LSM_PHASE_TIMERS
+        in_jac = 1;
+#endif
+        int nrhs = 0;
+        if (K.jac_mode == 1) {
+            jacobian_analytic();
+        } else {
+            jacobian_fd(h);
+            nrhs = ne;
+            __syncthreads();
+        }
+#ifdef LSM_PHASE_TIMERS
+        in_jac = 0;
+#endif
+        LSM_TOC(1, t0);
+        return nrhs;
+    }
+
+    // 64-bit pivot key: |a| (FP32 bits) high, inverted row low, so that the largest
+    // magnitude wins and ties go to the lowest row (Eigen PartialPivLU's choice).
+    __device__ __forceinline__ static unsigned long long piv_key(float a, int row) {
+        return ((unsigned long long)__float_as_uint(fabsf(a)) << 32) | (unsigned)(0x7fffffff - row);
+    }
+
+    // G = I - hr0 J and its LU with partial pivoting (KernelDense.hpp:196-221), in
+    // FP32 in LDS.  Rows stay in place ("virtual" pivoting): row i records the
+    // step ps(i) at which it became the pivot row, plv[k] the row chosen at step k.
+    // Lane l of every wave keeps rows l, l+64, l+128 (rows >= ne map to a spare
+    // row); wave w updates the columns k+1+w, k+5+w, ... of step k, so no entry is
+    // written by two waves, and one barrier per step suffices:
+    //  * every wave reads column k of its rows and forms the same multipliers;
+    //  * the multipliers of column k are written one step later (nobody reads
+    //    column k then);
+    //  * wave 0 owns column k+1 and proposes its pivot (largest |entry| among rows
+    //    not yet chosen, ties to the lowest row) before the barrier.
+    // Loads are masked by multiplication, not by condition: a select on a load
+    // result lets the compiler sink the load into a branch with its own wait.
+    __device__ __forceinline__ int wave_pivot(const int rr[3], const int psr[3], int col) const {
+        unsigned long long key = 0ull;
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const float v = A[rr[m] * lda + col];
+            const unsigned long long kk = piv_key(v, rr[m]);
+            key = umax64(key, psr[m] < 0 ? kk : 0ull);
+        }
+        key = wave_max_u64(key);
+        return 0x7fffffff - (int)(unsigned)(key & 0xffffffffull);
+    }
+    __device__ __forceinline__ void factor_impl(double hr) {
+        hr0 = hr;
+        const int n = ne;
+        int *pslot = (int *)pkey;                // [2] pivot of the next step (parity)
+        if (tid < n) {
+            const size_t t = tid;
+            int j = 0;
+            for (; j + 4 <= n; j += 4) {
+                const double j0 = J[(size_t)j * n + t], j1 = J[(size_t)(j + 1) * n + t];
+                const double j2 = J[(size_t)(j + 2) * n + t], j3 = J[(size_t)(j + 3) * n + t];
+                A[tid * lda + j] = (float)((tid == j ? 1. : 0.) - hr * j0);
+                A[tid * lda + j + 1] = (float)((tid == j + 1 ? 1. : 0.) - hr * j1);
+                A[tid * lda + j + 2] = (float)((tid == j + 2 ? 1. : 0.) - hr * j2);
+                A[tid * lda + j + 3] = (float)((tid == j + 3 ? 1. : 0.) - hr * j3);
+            }
+            for (; j < n; j++) A[tid * lda + j] = (float)((tid == j ? 1. : 0.) - hr * J[(size_t)j * n + t]);
+        } else {
+            for (int j = tid - n; j < lda; j += WIDE_THREADS - n) A[n * lda + j] = 0.f;   // spare row: finite
+        }
+        int rr[3], psr[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const int r = lane + 64 * m;
+            rr[m] = r < n ? r : n;                   // row n: spare
+            psr[m] = r < n ? -1 : 0x7fffffff;
+        }
+        __syncthreads();
+        int p = wave_pivot(rr, psr, 0);
+        float lprev[3] = {0.f, 0.f, 0.f};
+        bool wprev[3] = {false, false, false};
+        for (int k = 0; k < n; k++) {
+            if (k > 0) p = pslot[k & 1];
+            const float piv = A[p * lda + k];
+            float c[3];
+#pragma unroll
+            for (int m = 0; m < 3; m++) c[m] = A[rr[m] * lda + k];
+            if (k > 0 && wave == ((k - 1) & 3)) {
+#pragma unroll
+                for (int m = 0; m < 3; m++)
+                    if (wprev[m]) A[rr[m] * lda + k - 1] = lprev[m];
+            }
+            const float rpiv = piv != 0.f ? 1.f / piv : 0.f;
+            float l[3];
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                if (rr[m] == p) psr[m] = k;
+                wprev[m] = psr[m] < 0;
+                l[m] = wprev[m] ? c[m] * rpiv : 0.f;
+                lprev[m] = l[m];
+            }
+            if (tid == 0) {
+                perm[k] = p;
+                dinv[p] = piv != 0.f ? 1. / (double)piv : 0.;
+            }
+            const float *prow = A + p * lda;
+            int j = k + 1 + wave;
+            for (; j + 12 < n; j += 16) {            // four columns per pass: loads, then FMAs and stores
+                float u[4], a[4][3];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    u[q] = prow[j + 4 * q];
+#pragma unroll
+                    for (int m = 0; m < 3; m++) a[q][m] = A[rr[m] * lda + j + 4 * q];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int m = 0; m < 3; m++) A[rr[m] * lda + j + 4 * q] = fmaf(-l[m], u[q], a[q][m]);
+            }
+            for (; j < n; j += 4) {
+                const float u = prow[j];
+                float a[3];
+#pragma unroll
+                for (int m = 0; m < 3; m++) a[m] = A[rr[m] * lda + j];
+#pragma unroll
+                for (int m = 0; m < 3; m++) A[rr[m] * lda + j] = fmaf(-l[m], u, a[m]);
+            }
+            if (wave == 0 && k + 1 < n) {
+                const int pn = wave_pivot(rr, psr, k + 1);
+                if (lane == 0) pslot[(k + 1) & 1] = pn;
+            }
+            __syncthreads();
+        }
+        // Rows to pivot order (row k <- the row chosen at step k), U rows scaled by
+        // 1/pivot (unit diagonal), 32 columns at a time through registers.
+        for (int c0 = 0; c0 < n; c0 += 32) {
+            float v[22];                          // 22 x 256 >= 176 rows x 32 columns
+#pragma unroll
+            for (int q = 0; q < 22; q++) {
+                const int e = tid + 256 * q;
+                const int i = e >> 5, j = c0 + (e & 31);
+                const int ic = i < n ? i : n - 1, jc = j < n ? j : n - 1;
+                const int src = perm[ic];
+                const float a = A[src * lda + jc];
+                v[q] = (j > i) ? a * (float)dinv[src] : a;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 22; q++) {
+                const int e = tid + 256 * q;
+                const int i = e >> 5, j = c0 + (e & 31);
+                if (i < n && j < n) A[i * lda + j] = v[q];
+            }
+            __syncthreads();
+        }
+        if (tid < n) dsc[tid] = (float)dinv[perm[tid]];
+        __syncthreads();
+    }
+    __device__ __forceinline__ void factor(double hr) { LSM_TIC(t0); factor_impl(hr); LSM_TOC(2, t0); }
+
+    // Forward/backward substitution in FP32 on the factors in pivot order (L unit
+    // lower, U scaled to a unit diagonal, scales in dsc), by wave 0; the vector in
+    // bs (FP64, natural order) is overwritten with the solution.  Lane l holds
+    // rows l, l+64, l+128 (three registers, one per 64-row group, so the pivot
+    // element of step k is lane k & 63 of group k >> 6); the factors of 8 steps
+    // are fetched ahead of the dependent b -> readlane -> fma chain.  Accuracy
+    // comes from the FP64 residuals of iterative refinement.
+    template <int MK>
+    __device__ __forceinline__ void fwd_group(float (&b)[3], const int (&rr)[3], int k0, int k1) const {
+        for (int kb = k0; kb < k1; kb += 8) {
+            float lv[8][3];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int k = kb + u < k1 ? kb + u : k1 - 1;
+#pragma unroll
+                for (int m = MK; m < 3; m++) {
+                    const float v = A[rr[m] * lda + k];
+                    lv[u][m] = v * (((m > MK || lane > (k & 63)) && kb + u < k1) ? 1.f : 0.f);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                if (kb + u >= k1) break;            // past the group: lane k & 63 holds no row of it
+                const float xk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b[MK]), (kb + u) & 63));
+#pragma unroll
+                for (int m = MK; m < 3; m++) b[m] = fmaf(-lv[u][m], xk, b[m]);
+            }
+        }
+    }
+    template <int MK>
+    __device__ __forceinline__ void bwd_group(float (&b)[3], const int (&rr)[3], int k1, int k0) {
+        for (int kb = k1 - 1; kb >= k0; kb -= 8) {
+            float uv[8][3];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int k = kb - u >= k0 ? kb - u : k0;
+#pragma unroll
+                for (int m = 0; m <= MK; m++) {
+                    const float v = A[rr[m] * lda + k];
+                    uv[u][m] = v * (((m < MK || lane < (k & 63)) && kb - u >= k0) ? 1.f : 0.f);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                if (kb - u < k0) break;
+                const int k = kb - u;
+                const float xk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b[MK]), k & 63));
+#pragma unroll
+                for (int m = 0; m <= MK; m++) b[m] = fmaf(-uv[u][m], xk, b[m]);
+                if (lane == 0) bs[k] = (double)xk;
+            }
+        }
+    }
+    __device__ __forceinline__ void lu_solve_bs() {
+        if (wave == 0) {
+            const int n = ne;
+            float b[3];
+            int rr[3];
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                const int r = lane + 64 * m;
+                const int rc = r < n ? r : n - 1;
+                rr[m] = r < n ? r : n;
+                const double v = bs[perm[rc]];
+                b[m] = r < n ? (float)v : 0.f;
+            }
+            const int e0 = n < 64 ? n : 64, e1 = n < 128 ? n : 128;
+            fwd_group<0>(b, rr, 0, e0);
+            if (n > 64) fwd_group<1>(b, rr, 64, e1);
+            if (n > 128) fwd_group<2>(b, rr, 128, n);
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                const int r = lane + 64 * m;
+                const float d = dsc[r < n ? r : n - 1];
+                b[m] *= d;
+            }
+            if (n > 128) bwd_group<2>(b, rr, n, 128);
+            if (n > 64) bwd_group<1>(b, rr, e1, 64);
+            bwd_group<0>(b, rr, e0, 0);
+        }
+        __syncthreads();
+    }
+
+    // Row `tid` of J times the vector in bs (loads unconditional, 4 partial sums).
+    __device__ __forceinline__ double jrow_dot() const {
+        const int n = ne;
+        const int row = tid < n ? tid : n - 1;
+        double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+        int j = 0;
+        for (; j + 4 <= n; j += 4) {
+            a0 = fma(J[(size_t)j * n + row], bs[j], a0);
+            a1 = fma(J[(size_t)(j + 1) * n + row], bs[j + 1], a1);
+            a2 = fma(J[(size_t)(j + 2) * n + row], bs[j + 2], a2);
+            a3 = fma(J[(size_t)(j + 3) * n + row], bs[j + 3], a3);
+        }
+        for (; j < n; j++) a0 = fma(J[(size_t)j * n + row], bs[j], a0);
+        return (a0 + a1) + (a2 + a3);
+    }
+
+    // G^-1 b with K.refine sweeps of iterative refinement (residual in FP64 with the
+    // FP64 Jacobian: r = b - (x - hr0 J x)).
+    __device__ __forceinline__ void solve_impl(int vid) {
+        const double b = V[vid][0];
+        if (tid < ne) bs[tid] = b;
+        __syncthreads();
+        lu_solve_bs();
+        double x = tid < ne ? bs[tid] : 0.;
+        for (int it = 0; it < K.refine; it++) {
+            const double jx = jrow_dot();
+            const double r = b - (x - hr0 * jx);
+            __syncthreads();
+            if (tid < ne) bs[tid] = r;
+            __syncthreads();
+            lu_solve_bs();
+            if (tid < ne) x += bs[tid];
+            __syncthreads();
+            if (tid < ne) bs[tid] = x;
+            __syncthreads();
+        }
+        V[vid][0] = tid < ne ? x : 0.;
+    }
+    __device__ __forceinline__ void solve(int vid) { LSM_TIC(t0); solve_impl(vid); LSM_TOC(3, t0); }
+
+    __device__ __forceinline__ void jac_times(int src, int dst) {
+        LSM_TIC(t0);
+        if (tid < ne) bs[tid] = V[src][0];
+        __syncthreads();
+        const double acc = jrow_dot();
+        __syncthreads();
+        V[dst][0] = tid < ne ? acc : 0.;
+        LSM_TOC(4, t0);
+    }
+};
+
+// One cell per workgroup, persistent over an atomic queue; same harness as the
+// team kernel (chemistry_DI.H:92-328).
+__global__ void __launch_bounds__(256) wide_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
+                                                   const double *yin, double *yout, int *status, double *q,
+                                                   int *counters, unsigned *queue, const int *order,
+                                                   double *jws) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    WideCtx c(M, K, lds_raw, jws + (size_t)blockIdx.x * M.ne * M.ne);
+    const int tid = threadIdx.x, ns = M.ns, ne = M.ne;
+    int *cell_slot = (int *)(c.red + 16);
+    for (;;) {
+        if (tid == 0) {
+            const unsigned t = atomicAdd(queue, 1u);
+            cell_slot[0] = (int64_t)t < n ? 1 : 0;
+            cell_slot[1] = (int64_t)t < n ? (order ? order[t] : (int)t) : -1;
+        }
+        __syncthreads();
+        const int more = cell_slot[0];
+        const int64_t cell = (int64_t)cell_slot[1];
+        __syncthreads();
+        if (!more) break;
+        if (cell < 0 || cell >= n) continue;      // visiting order entries outside [0, n) are skipped
+        LSM_TIC(tcell);
+        const double yi = tid < ne ? yin[(int64_t)tid * n + cell] : 0.;
+        double T0;
+        c.wg_sum(0., &T0, yi, ns);
+        double y;
+        int st = 0;
+        BdfCounters cnt = BdfCounters{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (T0 > K.min_temperature) {                                  // chemistry_DI.H:97
+            double y0 = yi;                                            // :101-128
+            if (tid < ns && y0 < 0.) y0 = 0.;
+            const double sum = c.wg_sum(tid < ns ? y0 : 0.);
+            if (tid < ns) y0 /= sum;
+            c.V[Y0_][0] = tid < ne ? y0 : 0.;
+            st = bdf_integrate(c, K.bk, K.bk.t0 + dt[cell], cnt);       // :138-157
+            y = st > 0 ? c.V[Y_][0] : c.V[Y0_][0];
+            if (st > 0 && q != nullptr && tid == 0) q[cell] = c.qr;     // :168
+        } else {
+            y = yi;
+        }
+        const double sum = c.wg_sum(tid < ns ? y : 0.);               // :219 normalizeMassFractions
+        if (tid < ns) y /= sum;
+        if (!K.energy && tid == ns) y = yi;
+        if (tid < ne) yout[(int64_t)tid * n + cell] = y;
+        if (tid == 0 && status) status[cell] = st;
+#ifdef LSM_PHASE_TIMERS
+        c.tp[5] += clock64() - tcell;
+        c.tp[6] += 1;
+#endif
+        if (counters != nullptr && tid < LSM_NCOUNTERS_DEV) {
+            const unsigned vals[12] = {cnt.steps, cnt.fcalls, cnt.fcalls_jac, cnt.jacobians, cnt.factorizations,
+                                       cnt.solves, cnt.conv_failures, cnt.err_failures, cnt.max_order,
+                                       cnt.rhs_total, cnt.clamps, cnt.newton};
+            unsigned v = 0;
+#pragma unroll
+            for (int i = 0; i < 12; i++)
+                if (tid == i) v = vals[i];
+            counters[cell * LSM_NCOUNTERS_DEV + tid] = (int)v;
+        }
+    }
+#ifdef LSM_PHASE_TIMERS
+    if (tid == 0 && K.phase_cycles)
         for (int i = 0; i < 7; i++) atomicAdd(&K.phase_cycles[i], c.tp[i]);
 #endif
 }
@@ -1203,27 +1850,35 @@ int team_waves_per_block(const DevMech &m, size_t lds_per_block) {
 
 template <int NMAX>
 static int launch_team_n(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
-                         double *yout, int *status, double *q, int *counters, unsigned *queue, int grid, int wpb,
-                         double *jws, hipStream_t s) {
+                         double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order,
+                         int grid, int wpb, double *jws, hipStream_t s) {
     const size_t lds = (size_t)m.img.bytes + (size_t)wpb * team_wave_bytes(m);
-    if (lds > 65536 &&
-        hipFuncSetAttribute((const void *)team_kernel<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-            hipSuccess)
-        return -1;
+    static size_t lds_set = 65536;   // largest dynamic LDS size granted so far (not re-set under graph capture)
+    if (lds > lds_set) {
+        if (hipFuncSetAttribute((const void *)team_kernel<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds) != hipSuccess)
+            return -1;
+        lds_set = lds;
+    }
     hipLaunchKernelGGL(team_kernel<NMAX>, dim3(grid), dim3(64 * wpb), lds, s, m, k, n, dt, yin, yout, status, q,
-                       counters, queue, jws);
+                       counters, queue, order, jws);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
-                int *status, double *q, int *counters, unsigned *queue, int grid, int wpb, double *jws, void *stream) {
+                int *status, double *q, int *counters, unsigned *queue, const int *order, int grid, int wpb,
+                double *jws, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     if (wpb < 1 || wpb > 4) return -2;
-    if (m.ne <= 16) return launch_team_n<16>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
-    if (m.ne <= 32) return launch_team_n<32>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
-    if (m.ne <= 48) return launch_team_n<48>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
-    if (m.ne <= 56) return launch_team_n<56>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
-    if (m.ne <= 64) return launch_team_n<64>(m, k, n, dt, yin, yout, status, q, counters, queue, grid, wpb, jws, s);
+#define LSM_TEAM_CASE(NM)                                                                                           \
+    if (m.ne <= NM) return launch_team_n<NM>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, \
+                                             jws, s);
+    LSM_TEAM_CASE(16)
+    LSM_TEAM_CASE(32)
+    LSM_TEAM_CASE(48)
+    LSM_TEAM_CASE(56)
+    LSM_TEAM_CASE(64)
+#undef LSM_TEAM_CASE
     return -2;
 }
 
@@ -1242,6 +1897,26 @@ int launch_equations(const DevMech &m, const KParams &k, int64_t n, const double
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+size_t wide_lds_bytes(const DevMech &m) { return WideCtx::lds_bytes(m.ne); }
+size_t wide_jac_doubles(const DevMech &m) { return (size_t)m.ne * m.ne; }
+
+int launch_wide(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
+                int *status, double *q, int *counters, unsigned *queue, const int *order, int grid, double *jws,
+                void *stream) {
+    if (!m.img.wide_ok || m.ne > WIDE_MAX_NEQ) return -2;
+    const size_t lds = wide_lds_bytes(m);
+    static size_t lds_set = 65536;
+    if (lds > lds_set) {
+        if (hipFuncSetAttribute((const void *)wide_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+            hipSuccess)
+            return -1;
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL(wide_kernel, dim3(grid), dim3(WIDE_THREADS), lds, (hipStream_t)stream, m, k, n, dt, yin, yout,
+                       status, q, counters, queue, order, jws);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 size_t plain_work_doubles(const DevMech &m, int64_t slots) {
     const int64_t ne = m.ne;
     const int64_t per = (int64_t)NVEC * ne + 2 * ne * ne + 3 * (int64_t)m.ns + m.nr + ne;  // piv: 2 ints = 1 double
@@ -1254,6 +1929,41 @@ int launch_plain(const DevMech &m, const KParams &k, int64_t n, const double *dt
     const int grid = (int)((slots + bs - 1) / bs);
     hipLaunchKernelGGL(plain_kernel, dim3(grid), dim3(bs), 0, (hipStream_t)stream, m, k, n, dt, yin, yout, status,
                        q, counters, work, slots);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Sort keys for a cost order: key = BDF steps of the cell in a previous call.
+__global__ void order_keys_kernel(const int *counters, int64_t n, unsigned *keys, int *vals) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        keys[i] = counters ? (unsigned)counters[i * LSM_NCOUNTERS_DEV] : 0u;
+        vals[i] = (int)i;
+    }
+}
+
+// Strided sample of a batch (cells i * stride), compacted: ys [ne][m], dts [m].
+__global__ void gather_cells_kernel(const double *y, const double *dt, int64_t n, int ne, int64_t m, int64_t stride,
+                                    double *ys, double *dts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) {
+        const int64_t src = i * stride;
+        for (int e = 0; e < ne; e++) ys[(int64_t)e * m + i] = y[(int64_t)e * n + src];
+        dts[i] = dt[src];
+    }
+}
+
+int launch_order_keys(const int *counters, int64_t n, unsigned *keys, int *vals, void *stream) {
+    const int bs = 256;
+    hipLaunchKernelGGL(order_keys_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, (hipStream_t)stream,
+                       counters, n, keys, vals);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_gather_cells(const double *y, const double *dt, int64_t n, int ne, int64_t m, int64_t stride, double *ys,
+                        double *dts, void *stream) {
+    const int bs = 256;
+    hipLaunchKernelGGL(gather_cells_kernel, dim3((unsigned)((m + bs - 1) / bs)), dim3(bs), 0, (hipStream_t)stream, y,
+                       dt, n, ne, m, stride, ys, dts);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -1,0 +1,48 @@
+"""Writes tests/golden/cfg4_oracle.npz (run from the repository root:
+``python tests/golden/make_cfg4_ref.py``; a few minutes on 8 cores).
+
+Six cells of the seeded 160-species / 1000-reaction workload (the four cheapest
+and two median-cost cells of a 32-cell sample of the field), integrated over
+dt = 1e-5 s by the oracle at rtol 1e-6 / atol 1e-12 and rtol 1e-8 / atol 1e-14,
+with the oracle's self-envelope at each (largest deviation, in units of
+rtol |y| + atol, of three runs on inputs perturbed by 1e-15 relative), and at
+rtol 1e-10 / atol 1e-16 as the tight solution.  The GPU tests of the wide
+(FP32-LU) integrator compare against these without re-running the oracle, whose
+cost on this mechanism is seconds per cell.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
+
+import pyoracle  # noqa: E402
+from laminarsmoke_amd.fields import workload_cells  # noqa: E402
+
+sys.path.insert(0, os.path.dirname(HERE))
+from parity import oracle_envelope  # noqa: E402
+
+
+def main():
+    pyoracle.build()
+    mech, y = workload_cells(4, n=32)
+    _, st, _, c = pyoracle.chemistry_step(mech, y, 1e-5, rtol=1e-6, atol=1e-12, threads=8)
+    order = np.argsort(c[:, 0], kind="stable")
+    sel = np.sort(np.concatenate([order[:4], order[15:17]]))
+    ys = np.ascontiguousarray(y[:, sel])
+    out = dict(y_in=ys, sel=sel, dt=1e-5)
+    for tag, rt, at in (("r6", 1e-6, 1e-12), ("r8", 1e-8, 1e-14), ("tight", 1e-10, 1e-16)):
+        yo, sto, _, co = pyoracle.chemistry_step(mech, ys, 1e-5, rtol=rt, atol=at, threads=8)
+        out[f"y_{tag}"], out[f"status_{tag}"], out[f"steps_{tag}"] = yo, sto, co[:, 0]
+        if tag != "tight":
+            out[f"env_{tag}"] = oracle_envelope(pyoracle, mech, ys, 1e-5, rt, at, yo)
+            print("envelope", tag, out[f"env_{tag}"], flush=True)
+        print(tag, sto, co[:, 0], flush=True)
+    np.savez(os.path.join(HERE, "cfg4_oracle.npz"), **out)
+
+
+if __name__ == "__main__":
+    main()

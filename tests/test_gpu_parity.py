@@ -21,7 +21,10 @@ def gpu_step(mech, y, dt, method, rtol=RTOL, atol=ATOL, **kw):
     ode = api.ODE_Parameters()
     ode.SetRelativeTolerance(rtol)
     ode.SetAbsoluteTolerance(atol)
-    step = api.ChemistryStep(mech, ode, method=method, **kw)
+    jacobian = "numerical"
+    if method.endswith("-analytic"):
+        method, jacobian = method[:-len("-analytic")], "analytic"
+    step = api.ChemistryStep(mech, ode, method=method, jacobian=jacobian, **kw)
     try:
         return step.integrate(y, dt)
     finally:
@@ -42,7 +45,7 @@ def _check(oracle, mech, y, method, what):
     return yg, yo, q_g, q_o
 
 
-@pytest.mark.parametrize("method", ["team", "plain"])
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic", "wide-analytic"])
 def test_cfg1_parity(hip_device, oracle, method):
     mech, y = workload_cells(1, n=64)
     print(f"cfg1 parity method={method} cells={y.shape[1]}", flush=True)
@@ -51,7 +54,7 @@ def test_cfg1_parity(hip_device, oracle, method):
     np.testing.assert_allclose(q_g, q_o, rtol=1e-3)
 
 
-@pytest.mark.parametrize("method", ["team", "plain"])
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic", "wide-analytic"])
 def test_cfg2_sample_parity(hip_device, oracle, method):
     mech = workload_mechanism(53)
     yfull, flame = coflow_field(mech, 512, 512, 22)

@@ -1,0 +1,77 @@
+"""GPU: the wide integrator (one cell per 256-thread workgroup, FP32 Newton-matrix
+LU with FP64 iterative refinement) on the 160-species / 1000-reaction mechanism,
+against the oracle's results stored in tests/golden/cfg4_oracle.npz (made by
+tests/golden/make_cfg4_ref.py: rtol 1e-6 and 1e-8, and a tight rtol 1e-10
+solution), plus the refinement sweeps on the 14-species field.
+
+Criterion: identical status, and per cell max_k |y_gpu - y_oracle| / (rtol |y| +
+atol) <= max(10, 3 x the oracle's self-envelope on that cell), the envelope being
+stored with the fixture (the same rule as the FP64 parity tests: the FP32
+factorisation perturbs the Newton iterates by about its refined accuracy, far less
+than an integration tolerance, but on ill-conditioned cells any perturbation moves
+the step sequence).  The errors of both against the tight solution and the largest
+|dT| and |dY_k| against the oracle are printed (the precision study's figures).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from laminarsmoke_amd import api
+from laminarsmoke_amd.fields import workload_cells
+from laminarsmoke_amd.mechanism import workload_mechanism
+from parity import ratio, assert_parity, check_against_truth
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cfg4_oracle.npz")
+
+
+def wide_step(mech, y, dt, rtol, atol, jacobian="numerical", refine=1):
+    ode = api.ODE_Parameters()
+    ode.SetRelativeTolerance(rtol)
+    ode.SetAbsoluteTolerance(atol)
+    step = api.ChemistryStep(mech, ode, method="wide", jacobian=jacobian)
+    try:
+        step.set_refinement(refine)
+        return step.integrate(y, dt)
+    finally:
+        step.close()
+
+
+@pytest.mark.parametrize("tag,rtol,atol", [("r6", 1e-6, 1e-12), ("r8", 1e-8, 1e-14)])
+@pytest.mark.parametrize("jacobian", ["numerical", "analytic"])
+def test_cfg4_against_oracle(hip_device, tag, rtol, atol, jacobian):
+    d = np.load(GOLDEN)
+    mech = workload_mechanism(160)
+    y = d["y_in"]
+    print(f"cfg4 wide {jacobian} rtol={rtol} cells={y.shape[1]}", flush=True)
+    yg, st, q, cnt = wide_step(mech, y, float(d["dt"]), rtol, atol, jacobian)
+    yo, st_o, tight = d[f"y_{tag}"], d[f"status_{tag}"], d["y_tight"]
+    assert np.array_equal(st, st_o), (st, st_o)
+    cell, _ = assert_parity(yg, yo, rtol, atol, d[f"env_{tag}"], what=f"cfg4/{jacobian}/{tag}")
+    e_ref = ratio(yo, tight, rtol, atol).max(axis=0)
+    e_gpu = ratio(yg, tight, rtol, atol).max(axis=0)
+    np.testing.assert_allclose(yg[:mech.ns].sum(0), 1.0, atol=1e-14)
+    dT = np.abs(yg[-1] - yo[-1]).max()
+    dY = np.abs(yg[:-1] - yo[:-1]).max()
+    print(f"cfg4/{jacobian}/rtol {rtol}: vs oracle max {cell.max():.3g} (envelope max {d[f'env_{tag}'].max():.3g}); "
+          f"error vs tight max {e_gpu.max():.3g} median {np.median(e_gpu):.3g} "
+          f"(oracle {e_ref.max():.3g} / {np.median(e_ref):.3g}); vs oracle max|dT| {dT:.3g} K, max|dY| {dY:.3g}; "
+          f"steps {cnt[:, 0]} (oracle {d[f'steps_{tag}']})", flush=True)
+
+
+@pytest.mark.parametrize("refine", [0, 1, 2])
+def test_refinement_sweeps_cfg1(hip_device, oracle, refine):
+    """FP32 factorisation with 0-2 refinement sweeps: accuracy against a tight
+    solution no worse than the reference's (the solves' accuracy changes the
+    Newton iterates, so this is the criterion of the other precision variants)."""
+    mech, y = workload_cells(1, n=32)
+    rtol, atol, dt = 1e-7, 1e-12, 1e-5
+    print(f"cfg1 wide refine={refine}", flush=True)
+    yo, st_o, _, _ = oracle.chemistry_step(mech, y, dt, rtol=rtol, atol=atol, threads=8)
+    truth, *_ = oracle.chemistry_step(mech, y, dt, rtol=1e-12, atol=1e-20, threads=8)
+    yg, st, _, cnt = wide_step(mech, y, dt, rtol, atol, refine=refine)
+    check_against_truth(yo, st_o, yg, st, truth, rtol, atol, f"cfg1/wide/refine{refine}")
+    print(f"cfg1/wide/refine {refine}: max ratio vs oracle {ratio(yg, yo, rtol, atol).max():.3g}, steps "
+          f"{np.median(cnt[:, 0])}, factorisations {cnt[:, 4].mean():.1f}", flush=True)
