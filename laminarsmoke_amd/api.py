@@ -63,6 +63,14 @@ def lib():
         L.lsm_cost_order.argtypes = [vp, i64, vp, vp, vp]
         L.lsm_autotune.argtypes = [vp, i64, vp, vp, i64, vp, vp]
         L.lsm_get_launch.argtypes = [vp, vp]
+        L.lsm_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int32, ctypes.POINTER(vp)]
+        L.lsm_server_destroy.argtypes = [vp]
+        L.lsm_server_add_case.argtypes = [vp, vp, i64, vp, ctypes.POINTER(ctypes.c_int32)]
+        L.lsm_server_submit.argtypes = [vp, ctypes.c_int32, ctypes.c_double, vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.lsm_server_poll.argtypes = [vp, vp, vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.lsm_server_drain.argtypes = [vp]
+        L.lsm_server_read_state.argtypes = [vp, ctypes.c_int32, vp]
+        L.lsm_server_stats.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -212,3 +220,69 @@ class ChemistryStep:
 
 def counters_dict(cnt: np.ndarray) -> dict:
     return {name: cnt[:, i] for i, name in enumerate(COUNTER_NAMES)}
+
+
+class Server:
+    """Device-resident serving of chemistry steps from several CFD cases
+    (include/lsm_serve.h): HBM-budget admission, overlapped cases, per-case order."""
+
+    def __init__(self, hbm_budget_bytes, max_inflight=4):
+        h = ctypes.c_void_p()
+        _check(lib().lsm_server_create(int(hbm_budget_bytes), int(max_inflight), ctypes.byref(h)))
+        self.s = h
+        self.steps = {}      # case id -> ChemistryStep (kept alive)
+        self.neq = {}
+
+    def add_case(self, step: ChemistryStep, y0):
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        cid = ctypes.c_int32()
+        _check(lib().lsm_server_add_case(self.s, step.h, y0.shape[1], y0.ctypes.data, ctypes.byref(cid)))
+        self.steps[cid.value] = step
+        self.neq[cid.value] = y0.shape
+        return cid.value
+
+    def submit(self, case_id, dt, y=None):
+        """Queues a chemistry step; y (optional): new fields for the case (host array [ns+1, n])."""
+        t = ctypes.c_uint64()
+        if y is not None:
+            y = np.ascontiguousarray(y, dtype=np.float64)
+            if y.shape != self.neq[case_id]:
+                raise ValueError(f"y has shape {y.shape}, case has {self.neq[case_id]}")
+        _check(lib().lsm_server_submit(self.s, int(case_id), float(dt), y.ctypes.data if y is not None else None,
+                                       ctypes.byref(t)))
+        return t.value
+
+    def poll(self, capacity=256):
+        tk = np.zeros(capacity, dtype=np.uint64)
+        lat = np.zeros(capacity)
+        dev = np.zeros(capacity)
+        nd = ctypes.c_int32()
+        _check(lib().lsm_server_poll(self.s, tk.ctypes.data, lat.ctypes.data, dev.ctypes.data, capacity,
+                                     ctypes.byref(nd)))
+        k = nd.value
+        return [(int(tk[i]), float(lat[i]), float(dev[i])) for i in range(k)]
+
+    def drain(self):
+        _check(lib().lsm_server_drain(self.s))
+
+    def state(self, case_id):
+        y = np.empty(self.neq[case_id])
+        _check(lib().lsm_server_read_state(self.s, int(case_id), y.ctypes.data))
+        return y
+
+    def stats(self):
+        out = np.zeros(7, dtype=np.int64)
+        _check(lib().lsm_server_stats(self.s, out.ctypes.data))
+        return dict(bytes_in_use=int(out[0]), peak_bytes=int(out[1]), evictions=int(out[2]), admitted=int(out[3]),
+                    finished=int(out[4]), cases=int(out[5]), handle_bytes=int(out[6]))
+
+    def close(self):
+        if self.s:
+            lib().lsm_server_destroy(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -15,7 +15,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "liblsm_hip.so")
 ARCH = os.environ.get("LSM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["lsm_kernels.hip", "lsm_api.cpp"]
+SOURCES = ["lsm_kernels.hip", "lsm_api.cpp", "lsm_serve.cpp"]
 HEADERS = ["lsm_bdf.hpp", "lsm_internal.hpp"]
 
 # Packed-FP32 instructions are switched off for every HIP translation unit.

@@ -229,6 +229,7 @@ struct lsm_handle {
     int wide_grid = 0;
     double *d_wjws = nullptr;   // wide kernel Jacobians, [wide_grid][ne][ne]
     int team_max_waves = 0;     // waves the team Jacobian workspace is sized for
+    size_t tables_bytes = 0;
     // cost-order sort buffers
     unsigned *d_keys = nullptr, *d_keys_out = nullptr;
     int *d_vals = nullptr;
@@ -236,6 +237,18 @@ struct lsm_handle {
     size_t sort_tmp_bytes = 0;
     int64_t sort_cap = 0;
 };
+
+int lsm_fail(int code, const char *msg) { return fail(code, msg); }
+
+size_t lsm_handle_device_bytes(const lsm_handle *h) {
+    size_t b = h->tables_bytes + 256;
+    b += sizeof(double) * team_jac_doubles(h->dm) * (size_t)h->team_max_waves;
+    b += sizeof(double) * wide_jac_doubles(h->dm) * (size_t)h->wide_grid;
+    b += h->d_work ? (size_t)h->work_slots * plain_work_doubles(h->dm, 1) * sizeof(double) : 0;
+    return b;
+}
+
+int lsm_handle_neq(const lsm_handle *h) { return h->dm.ne; }
 
 extern "C" {
 
@@ -573,6 +586,7 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     const bool img_ok = build_image(m, perm, tb_of, fo_of, d.ns_pad, d.img, image);
     size_t o_img = img_ok ? b.add(image) : 0;
     HIP_OK(hipMalloc(&h->d_tables, b.bytes.size()));
+    h->tables_bytes = b.bytes.size();
     HIP_OK(hipMemcpy(h->d_tables, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice));
     char *base = (char *)h->d_tables;
     auto D = [&](size_t o) { return (const double *)(base + o); };

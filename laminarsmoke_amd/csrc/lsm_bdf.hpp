@@ -184,15 +184,21 @@ __device__ __forceinline__ unsigned find_correction(C &c, const BdfConst &K, Bdf
 template <class C>
 __device__ __forceinline__ void multi_value_step(C &c, const BdfConst &K, BdfState &s, BdfCounters &n) {
     const int p = s.p;
+    // Updates under the order p are written as selects, not branches: with
+    // branches the compiler moves part of the Nordsieck array to scratch memory.
     c.each([&](int e) {
 #pragma unroll
-        for (int i = 0; i <= MAX_ORDER; i++)
-            if (i <= p) c.x(V0 + i, e) = c.x(Z0 + i, e);
+        for (int i = 0; i <= MAX_ORDER; i++) {
+            const double z = c.x(Z0 + i, e), v = c.x(V0 + i, e);
+            c.x(V0 + i, e) = (i <= p) ? z : v;
+        }
 #pragma unroll
         for (int i = 0; i < MAX_ORDER; i++)
 #pragma unroll
-            for (int j = MAX_ORDER - 1; j >= i; j--)
-                if (i < p && j <= p - 1) c.x(V0 + j, e) += c.x(V0 + j + 1, e);
+            for (int j = MAX_ORDER - 1; j >= i; j--) {
+                const double a = c.x(V0 + j, e), b = c.x(V0 + j + 1, e);
+                c.x(V0 + j, e) = (i < p && j <= p - 1) ? a + b : a;
+            }
     });
     const double dne = (double)c.ne;
     int cl = 0;
@@ -201,13 +207,15 @@ __device__ __forceinline__ void multi_value_step(C &c, const BdfConst &K, BdfSta
         const double lo = c.ymin(e), hi = c.ymax(e);
         const bool clo = v0 < lo && fabs(v0 - lo) * c.x(OOE_, e) < dne;
         const bool chi = v0 > hi && fabs(v0 - hi) * c.x(OOE_, e) < dne;
-        if (clo || chi) {
-            const double b = clo ? lo : hi;
-            c.x(V0, e) = b;
-            c.x(V0 + 1, e) = b - c.x(Z0, e);
+        const bool cc = clo || chi;
+        const double bnd = clo ? lo : hi;
+        const double v1 = c.x(V0 + 1, e), z0 = c.x(Z0, e);
+        c.x(V0, e) = cc ? bnd : v0;
+        c.x(V0 + 1, e) = cc ? bnd - z0 : v1;
 #pragma unroll
-            for (int i = 2; i <= MAX_ORDER; i++)
-                if (i <= p) c.x(V0 + i, e) = 0.;
+        for (int i = 2; i <= MAX_ORDER; i++) {
+            const double vi = c.x(V0 + i, e);
+            c.x(V0 + i, e) = (cc && i <= p) ? 0. : vi;
         }
         cl += c.count(clo || chi);
     });
@@ -241,24 +249,34 @@ __device__ __forceinline__ void multi_value_step(C &c, const BdfConst &K, BdfSta
         c.x(V0, e) += va;
         c.x(V0 + 1, e) += b;
 #pragma unroll
-        for (int i = 2; i <= MAX_ORDER; i++)
-            if (i <= p) c.x(V0 + i, e) += b * rr[i];
+        for (int i = 2; i <= MAX_ORDER; i++) {
+            const double vi = c.x(V0 + i, e);
+            c.x(V0 + i, e) = (i <= p) ? vi + b * rr[i] : vi;
+        }
         // error estimates v_[p+1], v_[p+2] (eq. 29.169)
         double vp = 0., zp = 0., zp1 = 0.;
 #pragma unroll
-        for (int i = 1; i <= MAX_ORDER; i++)
-            if (i == p) { vp = c.x(V0 + i, e); zp = c.x(Z0 + i, e); zp1 = c.x(Z0 + i + 1, e); }
+        for (int i = 1; i <= MAX_ORDER; i++) {
+            const double a = c.x(V0 + i, e), z = c.x(Z0 + i, e), z1 = c.x(Z0 + i + 1, e);
+            vp = (i == p) ? a : vp;
+            zp = (i == p) ? z : zp;
+            zp1 = (i == p) ? z1 : zp1;
+        }
         const double vp1 = (vp - zp) / ((double)p + 1.);
         (void)inv_p1;
         (void)inv_p2;
 #pragma unroll
-        for (int i = 2; i <= MAX_ORDER + 1; i++)
-            if (i == p + 1) c.x(V0 + i, e) = vp1;
+        for (int i = 2; i <= MAX_ORDER + 1; i++) {
+            const double vi = c.x(V0 + i, e);
+            c.x(V0 + i, e) = (i == p + 1) ? vp1 : vi;
+        }
         if (do_p2) {
             const double vp2 = (vp1 - zp1) / ((double)p + 2.);
 #pragma unroll
-            for (int i = 3; i <= MAX_ORDER + 2; i++)
-                if (i == p + 2) c.x(V0 + i, e) = vp2;
+            for (int i = 3; i <= MAX_ORDER + 2; i++) {
+                const double vi = c.x(V0 + i, e);
+                c.x(V0 + i, e) = (i == p + 2) ? vp2 : vi;
+            }
         }
     });
     n.clamps += cl;

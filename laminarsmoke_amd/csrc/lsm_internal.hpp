@@ -104,6 +104,7 @@ int launch_equations(const DevMech &m, const KParams &k, int64_t n, const double
                      int grid, int waves_per_block, double *jws, void *stream);
 int launch_reset_queue(unsigned *queue, void *stream);
 int launch_order_keys(const int *counters, int64_t n, unsigned *keys, int *vals, void *stream);
+int launch_fill(double *p, int64_t n, double v, void *stream);
 int launch_gather_cells(const double *y, const double *dt, int64_t n, int ne, int64_t m, int64_t stride, double *ys,
                         double *dts, void *stream);
 
@@ -118,3 +119,10 @@ int launch_wide(const DevMech &m, const KParams &k, int64_t n, const double *dt,
                 void *stream);
 
 }  // namespace lsm
+
+struct lsm_handle;
+// Device bytes a handle holds (tables, workspaces), for the serving budget.
+size_t lsm_handle_device_bytes(const lsm_handle *h);
+int lsm_handle_neq(const lsm_handle *h);
+// Sets the thread's last-error text and returns code.
+int lsm_fail(int code, const char *msg);

@@ -896,7 +896,7 @@ struct TeamCtx {
 
 
 template <int NMAX>
-__global__ void __launch_bounds__(256) team_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
+__global__ void __launch_bounds__(256, (NMAX <= 16 ? 2 : 1)) team_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
                                                    const double *yin, double *yout, int *status, double *q,
                                                    int *counters, unsigned *queue, const int *order,
                                                    double *jws) {
@@ -1964,6 +1964,17 @@ int launch_gather_cells(const double *y, const double *dt, int64_t n, int ne, in
     const int bs = 256;
     hipLaunchKernelGGL(gather_cells_kernel, dim3((unsigned)((m + bs - 1) / bs)), dim3(bs), 0, (hipStream_t)stream, y,
                        dt, n, ne, m, stride, ys, dts);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void fill_kernel(double *p, int64_t n, double v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int launch_fill(double *p, int64_t n, double v, void *stream) {
+    const int bs = 256;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, (hipStream_t)stream, p, n, v);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

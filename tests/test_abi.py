@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     names = []
-    for h in ("laminarsmoke_amd.h",):
+    for h in sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")):
         text = open(os.path.join(ROOT, "include", h)).read()
         names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(lsm_\w+)\s*\(", text, flags=re.M)
     return sorted(set(names))
@@ -24,7 +24,7 @@ def declared_functions():
 def test_library_exports_every_declared_symbol():
     lib = api.lib()
     names = declared_functions()
-    assert len(names) >= 10
+    assert len(names) >= 25
     for n in names:
         assert hasattr(lib, n), n
     assert lib.lsm_version().startswith(b"laminarsmoke_amd")
