@@ -63,6 +63,21 @@ def cpu_baseline(mech, y, n_sample, threads, dt, rtol, atol):
                        f"with {os.cpu_count()} logical CPUs")
 
 
+def measured_traffic(workload_name, method, jacobian):
+    """HBM bytes per launch measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE, copy-calibrated) for the default
+    configuration, from the committed profile summary; None for any other configuration."""
+    if (workload_name, method, jacobian) != ("cfg2-53sp-coflow-512x512", "team", "analytic"):
+        return None
+    path = os.path.join(ROOT, "profiles", "team_kernel_cfg2_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,7 +194,8 @@ def main():
                    "hip_graph": graph is not None},
         "p50_batch_ms": float(np.percentile(lat_ms, 50)), "p99_batch_ms": float(np.percentile(lat_ms, 99)),
         "roofline": {"bound": "mfma", "pipe": "fp64", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
+                     "traffic": measured_traffic(name, launch["method"], args.jacobian),
                      "algorithmic_flops_per_launch": work["flops"],
                      "special_ops_per_launch": work["special_ops"],
                      "kernel_ms": kernel_s * 1e3},
