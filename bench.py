@@ -19,7 +19,9 @@ Before the timed steps:
     of the timed steps (most expensive cells first), as the previous CFD step's
     counters would.
 With --graph, the step (queue reset + integrator) is captured once in a HIP
-graph and replayed.
+graph and replayed.  With --step-cap K every cell stops after K BDF steps, so
+that kernels can be compared at equal work where full integrations of the slow
+plain kernel would take hours (160-species mechanism).
 
 Prints one JSON line with:
   * cell-integrations/s and the p50/p99 batch latency;
@@ -91,6 +93,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-order", action="store_true")
     ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--step-cap", type=int, default=0,
+                    help="@MaximumNumberOfSteps for every cell (equal-work kernel comparisons; cells stop with "
+                         "status -1 at the cap)")
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -122,6 +127,8 @@ def main():
     ode.SetType("OpenSMOKE_GPU")
     ode.SetRelativeTolerance(rtol)
     ode.SetAbsoluteTolerance(atol)
+    if args.step_cap > 0:
+        ode.SetMaximumNumberOfSteps(args.step_cap)
     step = api.ChemistryStep(mech, ode, method=args.method, jacobian=args.jacobian)
     stream = torch.cuda.current_stream(dev)
 
@@ -191,7 +198,7 @@ def main():
                    "newton_lu": "fp32+refinement" if launch["method"] == "wide" else "fp64",
                    "waves_per_workgroup": launch["waves_per_block"], "grid": launch["grid"],
                    "visit_order": "cost (previous step's counters)" if use_order else "cell index",
-                   "hip_graph": graph is not None},
+                   "hip_graph": graph is not None, "step_cap": args.step_cap or None},
         "p50_batch_ms": float(np.percentile(lat_ms, 50)), "p99_batch_ms": float(np.percentile(lat_ms, 99)),
         "roofline": {"bound": "mfma", "pipe": "fp64", "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
