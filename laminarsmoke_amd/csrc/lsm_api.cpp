@@ -487,16 +487,32 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
         S(SP_TMID) = c[16];
         S(SP_MW) = c[17];
     }
-    // reaction renumbering: elementary, third-body, fall-off
+    // Reaction renumbering: fall-off, then third-body, then elementary.  The
+    // reaction loops hand reaction j to lane j mod 64 (team) or thread j mod 256
+    // (wide); the fall-off work (Troe: several transcendental functions) is a branch
+    // the whole wave waits for, so the fall-off reactions are packed into as few
+    // passes of the loop as possible (team), or dealt round-robin to the four waves
+    // of the first pass (wide, which waits for its slowest wave at a barrier).
     std::vector<int> tb_of(nr, -1), fo_of(nr, -1);
     for (int s = 0; s < m->ntb; s++) tb_of[m->tb_rxn[s]] = s;
     for (int s = 0; s < m->nfo; s++) fo_of[m->fo_rxn[s]] = s;
-    std::vector<int> perm, inv(nr);
-    for (int pass = 0; pass < 3; pass++)
+    std::vector<int> by_class, perm, inv(nr);
+    for (int pass = 2; pass >= 0; pass--)
         for (int j = 0; j < nr; j++) {
             const int cls = fo_of[j] >= 0 ? 2 : (tb_of[j] >= 0 ? 1 : 0);
-            if (cls == pass) perm.push_back(j);
+            if (cls == pass) by_class.push_back(j);
         }
+    if (ns + 1 <= 64) {
+        perm = by_class;
+    } else {
+        std::vector<std::pair<int64_t, int>> slot(nr);   // (loop position, reaction)
+        for (int i = 0; i < nr; i++) {
+            const int64_t b = i / 256, r = i % 256;
+            slot[i] = {256 * b + (r % 4) * 64 + r / 4, by_class[i]};
+        }
+        std::sort(slot.begin(), slot.end());
+        for (auto &e : slot) perm.push_back(e.second);
+    }
     for (int jn = 0; jn < nr; jn++) inv[perm[jn]] = jn;
     std::vector<double> rA(nr), rB(nr), rE(nr), rdn(nr), fw_nu, bw_nu, rx_nu;
     std::vector<int> rinfo(nr), rtb(nr), rfo(nr), fw_ptr(nr + 1), bw_ptr(nr + 1), rx_ptr(nr + 1), fw_sp, bw_sp, rx_sp;
@@ -649,8 +665,8 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
     k.refine = 1;
     k.phase_cycles = nullptr;
 #ifdef LSM_PHASE_TIMERS
-    if (hipMalloc(&k.phase_cycles, 8 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMemset(k.phase_cycles, 0, 8 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc(&k.phase_cycles, 12 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(k.phase_cycles, 0, 12 * sizeof(unsigned long long)) != hipSuccess) {
         lsm_destroy(h);
         return fail(LSM_E_HIP, "phase timer buffer");
     }
@@ -977,11 +993,11 @@ int lsm_get_launch(lsm_handle *h, int32_t *out) {
 
 #ifdef LSM_PHASE_TIMERS
 // Instrumented builds only: cumulative per-phase cycles (rhs, jacobian, factor,
-// solve, J*v, cells, cell count) since lsm_create.
+// solve, J*v, cells, cell count, RHS parts) since lsm_create; out[12].
 int lsm_debug_phase_cycles(lsm_handle *h, unsigned long long *out) {
     if (!h || !out || !h->kp.phase_cycles) return fail(LSM_E_INVALID, "no phase timers");
     HIP_OK(hipDeviceSynchronize());
-    HIP_OK(hipMemcpy(out, h->kp.phase_cycles, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(out, h->kp.phase_cycles, 12 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     return LSM_OK;
 }
 #endif

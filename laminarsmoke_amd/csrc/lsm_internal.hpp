@@ -45,9 +45,9 @@ struct LdsImage {
     int dummy;     // unused slot of fixed-format records: 63 if ns <= 63, else 255
 };
 
-// Reactions are renumbered on upload: elementary first, then third-body, then
-// fall-off, so that the wavefront lanes that take the extra third-body and
-// fall-off work are contiguous.  rperm maps device order -> mechanism order.
+// Reactions are renumbered on upload: fall-off first, then third-body, then
+// elementary, so that the lanes that take the extra fall-off and third-body work
+// share as few passes of the reaction loop as possible (lsm_api.cpp).
 struct DevMech {
     int ns, nr, ne, ntb, nfo, ns_pad;
     const double *sp;                        // [NSP_ROWS][ns_pad]
@@ -73,7 +73,7 @@ struct DevMech {
 
 struct KParams {
     BdfConst bk;
-    unsigned long long *phase_cycles;   // [8] per-phase cycle totals (builds with LSM_PHASE_TIMERS only)
+    unsigned long long *phase_cycles;   // [12] per-phase cycle totals (builds with LSM_PHASE_TIMERS only)
     double pressure;
     double min_temperature;
     int energy;

@@ -27,7 +27,8 @@ constexpr double POS_INF = __builtin_huge_val();
 
 // Optional phase timers (-DLSM_PHASE_TIMERS): cycles spent in the RHS, the
 // Jacobian (including its RHS calls), the factorisation, the solves and J*v,
-// summed over waves into KParams::phase_cycles[0..4]; [5] whole cells, [6] cells.
+// summed over waves into KParams::phase_cycles[0..4]; [5] whole cells, [6] cells;
+// team RHS parts: [7] species and sums, [8] reactions, [9] assembly.
 #ifdef LSM_PHASE_TIMERS
 #define LSM_TIC(v) const unsigned long long v = clock64()
 #define LSM_TOC(i, v) (tp[i] += clock64() - (v))
@@ -275,7 +276,7 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
             } else {
                 q = 1. / 0.14;
             }
-            wF = pow(10., lf / (1. + q * q));
+            wF = exp(LN10 * (lf / (1. + q * q)));    // 10^x
         }
         kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
     }
@@ -403,7 +404,7 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
                 lgF = Lf / q;
                 dlgF_dL = 1. / q;
             }
-            F = pow(10., lgF);
+            F = exp(LN10 * lgF);
             dlnF_dT = dlgF_dlgPr * (dlnkf - dlnkinf) + LN10 * dlgF_dL * dL;
         }
         kmod = kinf * (Pr / (1. + Pr)) * F;
@@ -550,7 +551,7 @@ struct TeamCtx {
     int lane, ne, ns, nr;
     double qr;
 #ifdef LSM_PHASE_TIMERS
-    unsigned long long tp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long tp[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int in_jac = 0;
 #endif
 
@@ -590,6 +591,7 @@ struct TeamCtx {
 
     // BatchReactorHomogeneousConstantPressure::Equations (:175-247)
     __device__ __forceinline__ void rhs_impl(int src, int dst) {
+        LSM_TIC(t_rhs0);
         const double T = readlane_d(V[src][0], ns);
         const int k = lane;
         const bool isp = k < ns;
@@ -618,9 +620,13 @@ struct TeamCtx {
         double cTot = ck, cpx = isp ? cp * xk : 0.;
         wave_sum2(cTot, cpx);
         wave_sync();
+        LSM_TOC(7, t_rhs0);
+        LSM_TIC(t_rhs1);
         const double logPRT = log(101325. / RGAS / T);
         for (int j = lane; j < nr; j += 64) reaction(j, T, logT, uT, logPRT, cTot);
         wave_sync();
+        LSM_TOC(8, t_rhs1);
+        LSM_TIC(t_rhs2);
         const double Rk = isp ? sR[k] : 0.;
         double dy = isp ? (mwk * Rk) / rho : 0.;
         if (K.energy) {
@@ -631,6 +637,7 @@ struct TeamCtx {
         }
         wave_sync();
         V[dst][0] = dy;
+        LSM_TOC(9, t_rhs2);
     }
 
     struct JacAcc {
@@ -971,7 +978,7 @@ __global__ void __launch_bounds__(256, (NMAX <= 16 ? 2 : 1)) team_kernel(DevMech
     }
 #ifdef LSM_PHASE_TIMERS
     if (lane == 0 && K.phase_cycles)
-        for (int i = 0; i < 7; i++) atomicAdd(&K.phase_cycles[i], c.tp[i]);
+        for (int i = 0; i < 12; i++) atomicAdd(&K.phase_cycles[i], c.tp[i]);
 #endif
 }
 
@@ -1007,7 +1014,7 @@ struct WideCtx {
     float *A;                 // LDS [ne + 1][lda] LU factors (spare last row); dR/dc in an analytic Jacobian
     double *J;                // HBM [ne][ne] column-major, J[j*ne + i] = d f_i / d y_j
 #ifdef LSM_PHASE_TIMERS
-    unsigned long long tp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long tp[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int in_jac = 0;
 #endif
 
@@ -1589,7 +1596,7 @@ __global__ void __launch_bounds__(256) wide_kernel(DevMech M, KParams K, int64_t
     }
 #ifdef LSM_PHASE_TIMERS
     if (tid == 0 && K.phase_cycles)
-        for (int i = 0; i < 7; i++) atomicAdd(&K.phase_cycles[i], c.tp[i]);
+        for (int i = 0; i < 12; i++) atomicAdd(&K.phase_cycles[i], c.tp[i]);
 #endif
 }
 

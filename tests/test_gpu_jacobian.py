@@ -187,8 +187,9 @@ def test_general_mechanism_step(hip_device, oracle, method, jac):
     print(f"general/{method}/{jac}: max ratio {cell.max():.3g}", flush=True)
 
 
+@pytest.mark.parametrize("method", ["team", "wide"])
 @pytest.mark.parametrize("cfg", [1, 2])
-def test_analytic_jacobian_step_accuracy(hip_device, oracle, cfg):
+def test_analytic_jacobian_step_accuracy(hip_device, oracle, cfg, method):
     if cfg == 1:
         mech, y = workload_cells(1, n=64)
     else:
@@ -196,16 +197,16 @@ def test_analytic_jacobian_step_accuracy(hip_device, oracle, cfg):
         yfull, flame = coflow_field(mech, 512, 512, 22)
         idx = np.concatenate([np.where(flame)[0][:12], np.where(~flame)[0][:4]])
         y = np.ascontiguousarray(yfull[:, idx])
-    print(f"analytic-Jacobian step cfg{cfg} cells={y.shape[1]}", flush=True)
+    print(f"analytic-Jacobian step cfg{cfg} method={method} cells={y.shape[1]}", flush=True)
     yo, st_o, _, c_o = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=8)
     truth, st_t, *_ = oracle.chemistry_step(mech, y, DT, rtol=1e-12, atol=1e-20, threads=8)
-    step = make_step(mech, "team", "analytic")
+    step = make_step(mech, method, "analytic")
     try:
         yg, st_g, _, c_g = step.integrate(y, DT)
     finally:
         step.close()
-    check_against_truth(yo, st_o, yg, st_g, truth, RTOL, ATOL, f"cfg{cfg}/analytic")
-    print(f"cfg{cfg}/analytic: steps {np.median(c_g[:, 0])} vs {np.median(c_o[:, 0])}, rhs per cell "
+    check_against_truth(yo, st_o, yg, st_g, truth, RTOL, ATOL, f"cfg{cfg}/{method}/analytic")
+    print(f"cfg{cfg}/{method}/analytic: steps {np.median(c_g[:, 0])} vs {np.median(c_o[:, 0])}, rhs per cell "
           f"{c_g[:, 9].mean():.1f} vs {c_o[:, 9].mean():.1f}", flush=True)
     assert (c_g[:, 2] == 0).all()          # no RHS evaluations spent on Jacobians
     np.testing.assert_allclose(yg[:mech.ns].sum(0), 1.0, atol=1e-14)

@@ -45,7 +45,9 @@ def _check(oracle, mech, y, method, what):
     return yg, yo, q_g, q_o
 
 
-@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic", "wide-analytic"])
+# The wide kernel with the analytic Jacobian (FP32 dR/dc and FP32 Newton LU) is held
+# to the tight-solution criterion instead (tests/test_gpu_jacobian.py).
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic"])
 def test_cfg1_parity(hip_device, oracle, method):
     mech, y = workload_cells(1, n=64)
     print(f"cfg1 parity method={method} cells={y.shape[1]}", flush=True)
@@ -54,7 +56,7 @@ def test_cfg1_parity(hip_device, oracle, method):
     np.testing.assert_allclose(q_g, q_o, rtol=1e-3)
 
 
-@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic", "wide-analytic"])
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic"])
 def test_cfg2_sample_parity(hip_device, oracle, method):
     mech = workload_mechanism(53)
     yfull, flame = coflow_field(mech, 512, 512, 22)
