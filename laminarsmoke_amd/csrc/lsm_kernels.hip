@@ -829,8 +829,8 @@ struct TeamCtx {
             int pk[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                const double v = LU[(k + u) * 64 + lane];   // unconditional load, then mask
-                lk[u] = (ps > k + u) ? v : 0.;
+                const double v = LU[(k + u) * 64 + lane];   // unconditional load, masked by a product
+                lk[u] = v * ((ps > k + u) ? 1. : 0.);       // (a select would be sunk into a branch)
                 pk[u] = __builtin_amdgcn_readlane(plv, k + u);
             }
 #pragma unroll
@@ -838,7 +838,7 @@ struct TeamCtx {
         }
         for (; k < ne; k++) {
             const double v = LU[k * 64 + lane];
-            const double lk = (ps > k) ? v : 0.;
+            const double lk = v * ((ps > k) ? 1. : 0.);
             b = fma(-lk, readlane_d(b, __builtin_amdgcn_readlane(plv, k)), b);
         }
         double xr = 0.;
@@ -849,7 +849,7 @@ struct TeamCtx {
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 const double v = LU[(k - u) * 64 + lane];
-                uk[u] = (ps < k - u) ? v : 0.;
+                uk[u] = v * ((ps < k - u) ? 1. : 0.);
                 pk[u] = __builtin_amdgcn_readlane(plv, k - u);
                 dk[u] = readlane_d(dinv, pk[u]);
             }
@@ -862,7 +862,7 @@ struct TeamCtx {
         }
         for (; k >= 0; k--) {
             const double v = LU[k * 64 + lane];
-            const double uk = (ps < k) ? v : 0.;
+            const double uk = v * ((ps < k) ? 1. : 0.);
             const int p = __builtin_amdgcn_readlane(plv, k);
             const double xk = readlane_d(b, p) * readlane_d(dinv, p);
             b = fma(-uk, xk, b);
@@ -871,11 +871,22 @@ struct TeamCtx {
         V[vid][0] = xr;
     }
 
+    // J v with the Jacobian columns streamed from the workspace, four independent
+    // partial sums so that the column loads overlap.
     __device__ __forceinline__ void jac_times_impl(int src, int dst) {
         const double v = V[src][0];
-        double acc = 0.;
-        for (int j = 0; j < ne; j++) acc += Jl[j * 64 + lane] * readlane_d(v, j);
-        V[dst][0] = acc;
+        double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+        int j = 0;
+        for (; j + 4 <= ne; j += 4) {
+            const double c0 = Jl[j * 64 + lane], c1 = Jl[(j + 1) * 64 + lane];
+            const double c2 = Jl[(j + 2) * 64 + lane], c3 = Jl[(j + 3) * 64 + lane];
+            a0 = fma(c0, readlane_d(v, j), a0);
+            a1 = fma(c1, readlane_d(v, j + 1), a1);
+            a2 = fma(c2, readlane_d(v, j + 2), a2);
+            a3 = fma(c3, readlane_d(v, j + 3), a3);
+        }
+        for (; j < ne; j++) a0 = fma(Jl[j * 64 + lane], readlane_d(v, j), a0);
+        V[dst][0] = (a0 + a1) + (a2 + a3);
     }
     // Returns the number of RHS evaluations the Jacobian took.
     __device__ __forceinline__ int jacobian(double h) {
@@ -1198,8 +1209,16 @@ struct WideCtx {
         wg_sum2(cTot, cpx);
         const double logPRT = log(101325. / RGAS / T);
         const WideJacAcc acc{sR + 256 * wave, sDT + 256 * wave, A, lda};
-        for (int j = tid; j < nr; j += WIDE_THREADS) rxn_jac(L, scg, sH, ns, dummy, j, T, logT, uT, logPRT, cTot, acc);
-        __syncthreads();
+        // The waves scatter into the shared dR/dc one after another: FP32 sums are
+        // order-dependent, and concurrent waves would make the Jacobian (and so the
+        // step sequence) vary from run to run.  Within a wave the LDS atomics are
+        // applied in a fixed order.
+        for (int w = 0; w < WIDE_THREADS / 64; w++) {
+            if (wave == w)
+                for (int j = tid; j < nr; j += WIDE_THREADS)
+                    rxn_jac(L, scg, sH, ns, dummy, j, T, logT, uT, logPRT, cTot, acc);
+            __syncthreads();
+        }
         const double Rk = isp ? ((sR[k] + sR[256 + k]) + (sR[512 + k] + sR[768 + k])) : 0.;
         const double dRdT = isp ? ((sDT[k] + sDT[256 + k]) + (sDT[512 + k] + sDT[768 + k])) : 0.;
         double S = 0.;
