@@ -84,6 +84,50 @@ def hot_cells(mech: Mechanism, n: int, seed: int, T_lo=800.0, T_hi=2400.0):
     return y
 
 
+_STOICH_O2 = {"H2": 0.5, "CO": 0.5, "CH4": 2.0, "CH2O": 1.0, "C2H4": 3.0, "C2H6": 3.5}
+_DILUENTS = ("N2", "AR", "HE")
+
+
+def premixed_cells(mech: Mechanism, n: int, seed: int, T_lo: float = 900.0, T_hi: float = 2200.0,
+                   radical_fraction: float = 0.5):
+    """Fuel/air cells for a real (CHEMKIN) mechanism.
+
+    The fuels of the mechanism among H2, CO, CH4, CH2O, C2H4, C2H6 in random
+    proportions (Dirichlet), O2 at an equivalence ratio log-uniform in [0.3, 3],
+    3.76 mol of diluent per mol of O2 (N2, with a random share of AR or HE where
+    the mechanism has them).  In ``radical_fraction`` of the cells every other
+    species (radicals, intermediates, products) gets a mass fraction log-uniform in
+    [1e-8, 1e-3], as in a flame front.  T ~ U[T_lo, T_hi] K.
+    """
+    rng = np.random.default_rng(seed)
+    sp = mech.species
+    fuels = [s for s in _STOICH_O2 if s in sp]
+    dil = [s for s in _DILUENTS if s in sp]
+    if not fuels or "O2" not in sp or not dil:
+        raise ValueError("premixed_cells needs a fuel, O2 and a diluent in the mechanism")
+    x = np.zeros((mech.ns, n))
+    share = rng.dirichlet(np.ones(len(fuels)), size=n).T if len(fuels) > 1 else np.ones((1, n))
+    phi = np.exp(rng.uniform(np.log(0.3), np.log(3.0), n))
+    o2 = sum(share[i] * _STOICH_O2[f] for i, f in enumerate(fuels)) / phi
+    for i, f in enumerate(fuels):
+        x[sp.index(f)] = share[i]
+    x[sp.index("O2")] = o2
+    d = rng.dirichlet(np.array([4.0] + [1.0] * (len(dil) - 1)), size=n).T if len(dil) > 1 else np.ones((1, n))
+    for i, s in enumerate(dil):
+        x[sp.index(s)] += 3.76 * o2 * d[i]
+    y = x * mech.mw[:, None]
+    y /= y.sum(axis=0, keepdims=True)
+    others = [k for k, s in enumerate(sp) if s not in fuels and s != "O2" and s not in dil]
+    hot = rng.random(n) < radical_fraction
+    y[np.ix_(others, np.where(hot)[0])] = np.exp(rng.uniform(np.log(1e-8), np.log(1e-3),
+                                                            size=(len(others), int(hot.sum()))))
+    y /= y.sum(axis=0, keepdims=True)
+    out = np.empty((mech.neq, n))
+    out[:mech.ns] = y
+    out[mech.ns] = rng.uniform(T_lo, T_hi, n)
+    return out
+
+
 # The five workloads (cells are generated lazily; sizes are the full sizes).
 WORKLOADS = {
     1: dict(name="cfg1-14sp-4096", ns=14, cells=4096, dt=1e-5, rtol=1e-7, atol=1e-12),

@@ -73,6 +73,8 @@ class Mechanism:
     fo_sp: np.ndarray
     fo_eff_m1: np.ndarray
     name: str = "mechanism"
+    element_names: tuple = ELEMENTS    # columns of `elements`
+    atomic_weights: object = None      # [n_elements] kg/kmol (default: C, H, O, N)
     _struct: object = field(default=None, repr=False)
 
     @property
@@ -101,6 +103,9 @@ class Mechanism:
                      "tb_ptr", "tb_sp", "fo_rxn", "fo_type", "fo_single", "fo_ptr", "fo_sp"):
             setattr(self, name, i32(getattr(self, name)))
         self.elements = np.ascontiguousarray(self.elements, dtype=np.float64)
+        if self.atomic_weights is None:
+            self.atomic_weights = ATOMIC_WEIGHTS
+        self.atomic_weights = np.asarray(self.atomic_weights, dtype=np.float64)
         self.validate()
 
     def validate(self):
@@ -155,8 +160,8 @@ class Mechanism:
         return nu
 
     def element_mass_matrix(self) -> np.ndarray:
-        """[4, ns] kg of element e per kg of species k (for element-mass balances)."""
-        return (self.elements * ATOMIC_WEIGHTS[None, :]).T / self.mw[None, :]
+        """[n_elements, ns] kg of element e per kg of species k (for element-mass balances)."""
+        return (self.elements * self.atomic_weights[None, :]).T / self.mw[None, :]
 
 
 # ----------------------------------------------------------------------------------

@@ -1,0 +1,192 @@
+"""CHEMKIN reader (laminarsmoke_amd/chemkin.py) on the reference's H2/CO and H2 mechanisms.
+
+The fixtures under tests/golden/ are the reference's own input files
+(run/kinetic-mechanisms/POLIMI_H2CO_1412, POLIMI_H2_1412; the thermodynamics file cut
+down to the species used).  What pins the reader:
+
+* structure read off the files by eye: species and reaction counts, which reactions
+  are irreversible, third-body, fall-off (Troe / Lindemann), efficiencies;
+* every reaction conserves every element (exactly, in the atom counts);
+* NASA-7 data against published JANAF values (cp at 1000 K, formation enthalpy and
+  entropy at 298.15 K) - a misplaced coefficient column fails these by far;
+* rate constants against the file's numbers converted with physical unit factors
+  (1 cm3/mol = 1e-3 m3/kmol, 1 cal = 4.18443 J as the reference defines it);
+* option and unit handling on small inline mechanisms, and a clear error for the
+  options the kernels do not evaluate (the reference's CH4 skeletal mechanism uses SRI,
+  its one-step global mechanisms use FORD).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from laminarsmoke_amd.chemkin import ChemkinError, load_chemkin, read_chemkin, read_thermo
+from laminarsmoke_amd.mechanism import FALLOFF_LINDEMANN, FALLOFF_TROE, RXN_NEGATIVE_A, RXN_REVERSIBLE
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+THERMO = os.path.join(GOLDEN, "POLIMI_H2CO_1412_thermo.CKT")
+R = 8.3144621                       # J/mol/K (math/PhysicalConstants.h:20)
+CAL = 4.18443                       # J/cal (math/Conversions.h:112)
+
+
+@pytest.fixture(scope="module")
+def h2co():
+    return load_chemkin(os.path.join(GOLDEN, "POLIMI_H2CO_1412.CKI"), THERMO)
+
+
+@pytest.fixture(scope="module")
+def h2():
+    return load_chemkin(os.path.join(GOLDEN, "POLIMI_H2_1412.CKI"), THERMO)
+
+
+def test_h2co_structure(h2co):
+    m = h2co
+    assert m.ns == 14 and m.nr == 33
+    assert m.species[:4] == ["H2", "O2", "H2O", "H2O2"] and m.species[-3:] == ["HE", "AR", "N2"]
+    irrev = np.where((m.rflags & RXN_REVERSIBLE) == 0)[0].tolist()
+    assert irrev == [26, 30, 31, 32]                  # HCO+HO2=>H+OH+CO2 and the last three
+    assert not np.any(m.rflags & RXN_NEGATIVE_A)
+    assert m.tb_rxn.tolist() == [7, 8, 9, 13, 21]      # H2+M, O2+M, H+OH+M, O+OH+M, HCO+M
+    assert m.fo_rxn.tolist() == [2, 12, 16]
+    assert m.fo_type.tolist() == [FALLOFF_TROE, FALLOFF_TROE, FALLOFF_LINDEMANN]
+    np.testing.assert_array_equal(m.fo_troe[0], [0.5, 1e-30, 1e30, 0.0])       # 3-parameter TROE
+    np.testing.assert_array_equal(m.fo_troe[1], [0.7346, 94.0, 1756.0, 5182.0])
+    # efficiencies of H+O2(+M): H2O 18, H2 2.5, N2 1.26, AR .8, HE .8, CO 1.2, CO2 2.4 (stored as eff-1)
+    sp = [m.species[k] for k in m.fo_sp[m.fo_ptr[0]:m.fo_ptr[1]]]
+    eff = dict(zip(sp, m.fo_eff_m1[m.fo_ptr[0]:m.fo_ptr[1]] + 1.0))
+    assert eff == pytest.approx(dict(H2O=18.0, H2=2.5, N2=1.26, AR=0.8, HE=0.8, CO=1.2, CO2=2.4))
+    # OH+OH=O+H2O: the repeated reactant is one entry with coefficient 2
+    j = 6
+    assert m.reac_nu[m.reac_ptr[j]:m.reac_ptr[j + 1]].tolist() == [2.0]
+    # the DUPLICATE pair CO+OH=CO2+H is kept as two reactions
+    assert m.lnA[17] != m.lnA[18] and m.reac_sp[m.reac_ptr[17]:m.reac_ptr[18]].tolist() == \
+        m.reac_sp[m.reac_ptr[18]:m.reac_ptr[19]].tolist()
+
+
+def test_h2_structure(h2):
+    assert h2.ns == 9 and h2.nr == 20
+    assert h2.element_names == ("H", "O", "N")
+
+
+@pytest.mark.parametrize("which", ["h2co", "h2"])
+def test_element_balance(request, which):
+    m = request.getfixturevalue(which)
+    bal = m.elements.T @ m.stoich_matrix()
+    assert np.abs(bal).max() == 0.0
+
+
+def test_molecular_weights(h2co):
+    mw = dict(zip(h2co.species, h2co.mw))
+    assert mw["H2O"] == pytest.approx(18.015, abs=2e-3)
+    assert mw["CO2"] == pytest.approx(44.009, abs=2e-3)
+    assert mw["AR"] == pytest.approx(39.948, abs=1e-5)       # single-precision table values
+    assert mw["HE"] == pytest.approx(4.003, abs=1e-5)
+
+
+# JANAF (NIST-JANAF Thermochemical Tables): cp(1000 K) J/mol/K, dHf(298.15) kJ/mol, S(298.15) J/mol/K
+JANAF = {
+    "H2": (30.203, 0.0, 130.680), "O2": (34.870, 0.0, 205.147), "N2": (32.697, 0.0, 191.609),
+    "H2O": (41.268, -241.826, 188.834), "CO": (33.183, -110.527, 197.653),
+    "CO2": (54.308, -393.522, 213.795), "AR": (20.786, 0.0, 154.845),
+}
+
+
+def test_thermo_against_janaf(h2co, oracle):
+    for T in (298.15, 1000.0):
+        cp, h, s = oracle.species_thermo(h2co, T)
+        for name, (cp1000, hf, s298) in JANAF.items():
+            k = h2co.species.index(name)
+            if T == 1000.0:
+                assert cp[k] * R == pytest.approx(cp1000, rel=5e-3), name
+            else:
+                assert h[k] * R * T / 1e3 == pytest.approx(hf, abs=0.5), name
+                assert s[k] * R == pytest.approx(s298, rel=3e-3), name
+
+
+def test_rate_constants(h2co, oracle):
+    m = h2co
+    T = 1300.0
+    c = np.linspace(1e-3, 3e-3, m.ns)                 # kmol/m3
+    k = oracle.kinetics(m, T, c)
+    cgs2 = 1e-6 * 1e3                                  # cm3/mol/s -> m3/kmol/s
+    # H+O2=OH+O  .9600E+15 -.200 16625.0
+    assert k["kf"][0] == pytest.approx(0.96e15 * cgs2 * T ** -0.2 * np.exp(-16625.0 * CAL / (R * T)), rel=1e-12)
+    # H2+M=H+H+M  .1115E+15 0 96081: second order counting M (cm3/mol/s)
+    j = 7
+    kf = 0.1115e15 * cgs2 * np.exp(-96081.0 * CAL / (R * T))
+    assert k["kf"][j] == pytest.approx(kf, rel=1e-12)
+    ck = dict(zip(m.species, c))
+    M = c.sum() + 1.5 * ck["H2"] + 11.0 * ck["H2O"] - 0.5 * ck["AR"] + 0.9 * ck["CO"] + 2.8 * ck["CO2"]
+    assert k["kmod"][j] == pytest.approx(kf * M, rel=1e-12)
+    # CO+O(+M)=CO2(+M): k_inf .9640E+11 0 3800, LOW .2070E+27 -3.34 7610 (Lindemann)
+    j = 16
+    kinf = 0.964e11 * cgs2 * np.exp(-3800.0 * CAL / (R * T))
+    k0 = 0.207e27 * 1e-6 * T ** -3.34 * np.exp(-7610.0 * CAL / (R * T))
+    M = c.sum() + 11.0 * ck["H2O"] + 1.0 * ck["H2"] + 0.5 * ck["CO"] + 1.0 * ck["CO2"] - 0.5 * ck["AR"]
+    Pr = k0 * M / kinf
+    assert k["kf"][j] == pytest.approx(k0, rel=1e-12)
+    assert k["kmod"][j] == pytest.approx(kinf * Pr / (1.0 + Pr), rel=1e-12)
+
+
+def test_thermo_reader_fields():
+    th = read_thermo(open(THERMO).read())
+    coef, (tl, thi, tm), comp = th["AR"]
+    assert (tl, thi, tm) == (300.0, 3500.0, 1490.0) and comp == {"AR": 1.0}
+    assert coef[0] == 2.5 and coef[5] == -745.375 and coef[13] == 4.366
+    assert th["H2O2"][2] == {"H": 2.0, "O": 2.0}
+
+
+def _small(reactions, units=""):
+    kin = f"ELEMENTS H O N END\nSPECIES H2 O2 H2O H O OH HO2 N2 END\nREACTIONS {units}\n{reactions}\nEND\n"
+    return read_chemkin(kin, open(THERMO).read())
+
+
+def test_energy_units():
+    base = _small("H+O2<=>OH+O 1.0E13 0.0 16.625", "KCAL/MOLE").E_over_R[0]
+    assert base == pytest.approx(16625.0 * CAL / R, rel=1e-14)
+    assert _small("H+O2=OH+O 1.0E13 0.0 8000.0", "KELVINS").E_over_R[0] == pytest.approx(8000.0, rel=1e-14)
+    assert _small("H+O2=OH+O 1.0E13 0.0 69.565", "KJOULES/MOLE").E_over_R[0] == \
+        pytest.approx(69565.0 / R, rel=1e-14)
+
+
+def test_coefficients_negative_A_and_collider():
+    m = _small("2OH=>O+H2O 1.0E12 0.0 0.0\nH+OH=H2O -1.0E12 0.0 0.0\n"
+               "H+O2(+N2)=HO2(+N2) 1.0E12 0.5 0.0\n LOW/ 1.0E18 -1.0 0.0/")
+    assert m.reac_nu[m.reac_ptr[0]:m.reac_ptr[1]].tolist() == [2.0]
+    assert m.rflags[0] == 0 and m.rflags[1] == RXN_NEGATIVE_A | RXN_REVERSIBLE
+    assert m.lnA[1] == pytest.approx(np.log(1e12 * 1e-3))
+    assert m.fo_single.tolist() == [m.species.index("N2")] and m.fo_type.tolist() == [FALLOFF_LINDEMANN]
+    assert m.lnA[2] == pytest.approx(np.log(1e18 * 1e-6)) and m.fo_inf[0, 0] == pytest.approx(np.log(1e12 * 1e-3))
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0\n LOW/ 1.0E18 -1.0 0.0/\n SRI/ 0.5 100 1000/", "SRI"),
+    ("H2+O2=>H2O+O 1.0E12 0.0 0.0\n FORD /H2 1.5/", "FORD"),
+    ("H+O2=OH+O 1.0E12 0.0 0.0\n REV/ 1.0E11 0.0 0.0/", "REV"),
+    ("H+O2=OH+O 1.0E12 0.0 0.0\n PLOG/ 1.0 1.0E12 0.0 0.0/", "PLOG"),
+    ("H+O2=OH+XX 1.0E12 0.0 0.0", "unknown species"),
+    ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0", "without LOW"),
+])
+def test_unsupported_options_raise(text, msg):
+    with pytest.raises(ChemkinError, match=msg):
+        _small(text)
+
+
+def test_integration_conserves_elements_and_enthalpy(h2co, oracle):
+    """Stoichiometric H2/CO/air at 1200 K ignites within 1 ms (oracle); elements and enthalpy conserved."""
+    m = h2co
+    x = np.zeros(m.ns)
+    sp = m.species.index
+    x[sp("H2")], x[sp("CO")], x[sp("O2")], x[sp("N2")] = 0.5, 0.5, 0.5, 0.5 * 3.76
+    y0 = x * m.mw / (x @ m.mw)
+    y = np.concatenate([y0, [1200.0]])[:, None]
+    yf, st, q, cnt = oracle.chemistry_step(m, y, 1e-3, rtol=1e-8, atol=1e-14)
+    assert st[0] > 0
+    assert yf[-1, 0] > 2000.0
+    em = m.element_mass_matrix()
+    np.testing.assert_allclose(em @ yf[:m.ns, 0], em @ y0, atol=1e-12)
+    cp0, h0, _ = oracle.species_thermo(m, 1200.0)
+    cpf, hf, _ = oracle.species_thermo(m, yf[-1, 0])
+    H0 = (y0 / m.mw) @ h0 * R * 1200.0
+    Hf = (yf[:m.ns, 0] / m.mw) @ hf * R * yf[-1, 0]
+    assert abs(Hf - H0) <= 1e-6 * abs(cp0 @ (y0 / m.mw)) * R * 1200.0
