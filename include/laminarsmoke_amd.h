@@ -85,6 +85,14 @@ typedef struct lsm_reactor_options {
                                    (utilities.H:159)                                          */
     double t0;                  /* start time of the chemistry step, t0 = t - dt of the CFD
                                    step (Policy_MomentumTransportReaction.H:62)               */
+    int32_t constant_volume;    /* !constPressureBatchReactor: constant-volume reactor
+                                   (BatchReactorHomogeneousConstantVolume.H:150-199, selected
+                                   at chemistry_DI.H:171-208); numerical Jacobian only        */
+    int32_t strang_compact;     /* strangAlgorithm == STRANG_COMPACT: y_out receives the
+                                   sources RR_k = rho (Y_k,out - Y_k,in)/dt [kg/m3/s] (rows
+                                   0..ns-1) and RT = rho cp (T_out - T_in)/dt [W/m3] (row ns),
+                                   rho and cp of the final state at `pressure`; zero when
+                                   dt <= 1e-14 (chemistry_DI.H:236-282)                       */
 } lsm_reactor_options;
 
 int lsm_reactor_options_default(lsm_reactor_options *r);
@@ -198,6 +206,17 @@ int lsm_integrate_batch_device_ordered(lsm_handle *h, int64_t n_cells, const dou
                                        const double *d_y_in, double *d_y_out, int32_t *d_status,
                                        double *d_q, int32_t *d_counters, const int32_t *d_order,
                                        void *stream);
+
+/* Constant-volume variant: as lsm_integrate_batch / _device_ordered (d_order may be
+ * NULL), with the per-cell density rho0 [kg/m3] of the constant-volume reactor,
+ * rhoCells in BatchReactorHomogeneousConstantVolume::SetReactor(V, P, rho)
+ * (chemistry_DI.H:173).  rho NULL: the ideal-gas density of each cell's initial state
+ * at the reactor pressure.  Constant-pressure handles ignore rho. */
+int lsm_integrate_batch_rho(lsm_handle *h, int64_t n_cells, const double *dt, const double *rho,
+                            const double *y_in, double *y_out, int32_t *status, double *q, int32_t *counters);
+int lsm_integrate_batch_device_rho(lsm_handle *h, int64_t n_cells, const double *d_dt, const double *d_rho,
+                                   const double *d_y_in, double *d_y_out, int32_t *d_status, double *d_q,
+                                   int32_t *d_counters, const int32_t *d_order, void *stream);
 
 /* Visiting order from the per-cell counters of a previous call on the same cells
  * (the cpuChemistry field of chemistry_DI.H:170 plays this role in the reference):

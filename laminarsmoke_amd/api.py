@@ -60,6 +60,8 @@ def lib():
         L.lsm_integrate_batch.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.lsm_integrate_batch_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_integrate_batch_device_ordered.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lsm_integrate_batch_rho.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.lsm_integrate_batch_device_rho.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_cost_order.argtypes = [vp, i64, vp, vp, vp]
         L.lsm_autotune.argtypes = [vp, i64, vp, vp, i64, vp, vp]
         L.lsm_get_launch.argtypes = [vp, vp]
@@ -124,11 +126,15 @@ class ODE_Parameters:
         return self.c.maximum_order
 
 
-def reactor_options(pressure=101325.0, min_temperature=0.0, energy=True, mass_fractions_tol=1e-4, t0=0.0):
+def reactor_options(pressure=101325.0, min_temperature=0.0, energy=True, mass_fractions_tol=1e-4, t0=0.0,
+                    constant_volume=False, compact=False):
+    """lsm_reactor_options; constant_volume: constant-volume reactor; compact: Strang "compact"
+    output (y_out holds the sources RR and RT instead of the final state)."""
     r = LsmReactorOptions()
     _check(lib().lsm_reactor_options_default(ctypes.byref(r)))
     r.pressure, r.min_temperature, r.energy_equation = pressure, min_temperature, int(bool(energy))
     r.mass_fractions_tol, r.t0 = mass_fractions_tol, t0
+    r.constant_volume, r.strang_compact = int(bool(constant_volume)), int(bool(compact))
     return r
 
 
@@ -143,8 +149,12 @@ class ChemistryStep:
         _check(lib().lsm_create(ctypes.byref(mech.struct()), ctypes.byref(self.ode.c),
                                 ctypes.byref(self.reactor), ctypes.byref(h)))
         self.h = h
-        self.set_method(method)
-        self.set_jacobian(jacobian)
+        try:
+            self.set_method(method)
+            self.set_jacobian(jacobian)
+        except LsmError:
+            self.close()
+            raise
 
     def set_method(self, method):
         _check(lib().lsm_set_method(self.h, METHODS[method] if isinstance(method, str) else int(method)))
@@ -177,8 +187,9 @@ class ChemistryStep:
         except Exception:
             pass
 
-    def integrate(self, y, dt, q=None, with_counters=True):
-        """Host arrays: y[ns+1, n] -> (y_out, status, q, counters[n, 12])."""
+    def integrate(self, y, dt, q=None, with_counters=True, rho=None):
+        """Host arrays: y[ns+1, n] -> (y_out, status, q, counters[n, 12]).  rho: per-cell densities of a
+        constant-volume reactor (default: ideal-gas density of the input state)."""
         y = np.ascontiguousarray(y, dtype=np.float64)
         neq, n = y.shape
         if neq != self.mech.neq:
@@ -189,13 +200,19 @@ class ChemistryStep:
         qa = np.zeros(n) if q is None else np.ascontiguousarray(q, dtype=np.float64).copy()
         cnt = np.zeros((n, N_COUNTERS), dtype=np.int32) if with_counters else None
         p = lambda a: a.ctypes.data if a is not None else None
-        _check(lib().lsm_integrate_batch(self.h, n, p(dt), p(y), p(yout), p(st), p(qa), p(cnt)))
+        if rho is not None:
+            rho = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, dtype=np.float64), (n,)))
+        _check(lib().lsm_integrate_batch_rho(self.h, n, p(dt), p(rho), p(y), p(yout), p(st), p(qa), p(cnt)))
         return yout, st, qa, cnt
 
     def integrate_device(self, n, dt_ptr, y_in_ptr, y_out_ptr, status_ptr=None, q_ptr=None, counters_ptr=None,
-                         stream=None, order_ptr=None):
-        """Device pointers (ints), enqueued on `stream` (int handle or None); `order_ptr`: visiting order."""
-        if order_ptr:
+                         stream=None, order_ptr=None, rho_ptr=None):
+        """Device pointers (ints), enqueued on `stream` (int handle or None); `order_ptr`: visiting order;
+        `rho_ptr`: per-cell densities of a constant-volume reactor."""
+        if rho_ptr:
+            _check(lib().lsm_integrate_batch_device_rho(self.h, int(n), dt_ptr, rho_ptr, y_in_ptr, y_out_ptr,
+                                                        status_ptr, q_ptr, counters_ptr, order_ptr, stream))
+        elif order_ptr:
             _check(lib().lsm_integrate_batch_device_ordered(self.h, int(n), dt_ptr, y_in_ptr, y_out_ptr, status_ptr,
                                                             q_ptr, counters_ptr, order_ptr, stream))
         else:

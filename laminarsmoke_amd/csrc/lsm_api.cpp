@@ -315,6 +315,8 @@ int lsm_reactor_options_default(lsm_reactor_options *r) {
     r->energy_equation = 1;
     r->mass_fractions_tol = 1.e-4;
     r->t0 = 0.;
+    r->constant_volume = 0;
+    r->strang_compact = 0;
     return LSM_OK;
 }
 
@@ -663,6 +665,9 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
     k.full_pivoting = p.full_pivoting ? 1 : 0;
     k.jac_mode = LSM_JACOBIAN_NUMERICAL;
     k.refine = 1;
+    k.const_volume = r.constant_volume ? 1 : 0;
+    k.compact = r.strang_compact ? 1 : 0;
+    k.rho = nullptr;
     k.phase_cycles = nullptr;
 #ifdef LSM_PHASE_TIMERS
     if (hipMalloc(&k.phase_cycles, 12 * sizeof(unsigned long long)) != hipSuccess ||
@@ -714,6 +719,9 @@ int lsm_set_method(lsm_handle *h, int32_t method) {
 int lsm_set_jacobian(lsm_handle *h, int32_t type) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (type != LSM_JACOBIAN_NUMERICAL && type != LSM_JACOBIAN_ANALYTIC) return fail(LSM_E_INVALID, "unknown Jacobian type");
+    if (type == LSM_JACOBIAN_ANALYTIC && h->kp.const_volume)
+        return fail(LSM_E_UNSUPPORTED, "the analytic Jacobian is built for the constant-pressure reactor; "
+                                       "constant-volume handles use the numerical Jacobian");
     h->kp.jac_mode = type;
     return LSM_OK;
 }
@@ -753,41 +761,49 @@ static int pick_method(const lsm_handle *h) {
     return h->wide_grid > 0 ? LSM_METHOD_WIDE : LSM_METHOD_PLAIN;
 }
 
-static int integrate_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in, double *d_y_out,
-                            int32_t *d_status, double *d_q, int32_t *d_counters, const int32_t *d_order,
-                            void *stream);
+static int integrate_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_rho, const double *d_y_in,
+                            double *d_y_out, int32_t *d_status, double *d_q, int32_t *d_counters,
+                            const int32_t *d_order, void *stream);
 
 int lsm_integrate_batch_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in, double *d_y_out,
                                int32_t *d_status, double *d_q, int32_t *d_counters, void *stream) {
-    return integrate_device(h, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, nullptr, stream);
+    return integrate_device(h, n, d_dt, nullptr, d_y_in, d_y_out, d_status, d_q, d_counters, nullptr, stream);
 }
 
 int lsm_integrate_batch_device_ordered(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in,
                                        double *d_y_out, int32_t *d_status, double *d_q, int32_t *d_counters,
                                        const int32_t *d_order, void *stream) {
     if (!d_order) return fail(LSM_E_INVALID, "null order");
-    return integrate_device(h, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, d_order, stream);
+    return integrate_device(h, n, d_dt, nullptr, d_y_in, d_y_out, d_status, d_q, d_counters, d_order, stream);
 }
 
-static int integrate_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in, double *d_y_out,
-                            int32_t *d_status, double *d_q, int32_t *d_counters, const int32_t *d_order,
-                            void *stream) {
+int lsm_integrate_batch_device_rho(lsm_handle *h, int64_t n, const double *d_dt, const double *d_rho,
+                                   const double *d_y_in, double *d_y_out, int32_t *d_status, double *d_q,
+                                   int32_t *d_counters, const int32_t *d_order, void *stream) {
+    return integrate_device(h, n, d_dt, d_rho, d_y_in, d_y_out, d_status, d_q, d_counters, d_order, stream);
+}
+
+static int integrate_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_rho, const double *d_y_in,
+                            double *d_y_out, int32_t *d_status, double *d_q, int32_t *d_counters,
+                            const int32_t *d_order, void *stream) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
     if (n == 0) return LSM_OK;
     if (!d_dt || !d_y_in || !d_y_out) return fail(LSM_E_INVALID, "null dt / y pointer");
     if (n > (int64_t)0x7fffffff) return fail(LSM_E_INVALID, "more than 2^31-1 cells in one call");
     const int method = pick_method(h);
+    KParams kp = h->kp;                          // per-call: the constant-volume densities
+    kp.rho = h->kp.const_volume ? d_rho : nullptr;
     if (method == LSM_METHOD_TEAM) {
         if (launch_reset_queue(h->d_queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
-        int rc = launch_team(h->dm, h->kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue, d_order,
+        int rc = launch_team(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue, d_order,
                              h->team_grid, h->team_wpb, h->d_jws, stream);
         if (rc) return fail(LSM_E_HIP, std::string("team kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
         return LSM_OK;
     }
     if (method == LSM_METHOD_WIDE) {
         if (launch_reset_queue(h->d_queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
-        int rc = launch_wide(h->dm, h->kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue, d_order,
+        int rc = launch_wide(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue, d_order,
                              h->wide_grid, h->d_wjws, stream);
         if (rc) return fail(LSM_E_HIP, std::string("wide kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
         return LSM_OK;
@@ -804,7 +820,7 @@ static int integrate_device(lsm_handle *h, int64_t n, const double *d_dt, const 
         HIP_OK(hipMalloc(&h->d_work, (size_t)slots * per));
         h->work_slots = slots;
     }
-    int rc = launch_plain(h->dm, h->kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_work, h->work_slots,
+    int rc = launch_plain(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_work, h->work_slots,
                           stream);
     if (rc) return fail(LSM_E_HIP, std::string("plain kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
     return LSM_OK;
@@ -812,6 +828,11 @@ static int integrate_device(lsm_handle *h, int64_t n, const double *d_dt, const 
 
 int lsm_integrate_batch(lsm_handle *h, int64_t n, const double *dt, const double *y_in, double *y_out,
                         int32_t *status, double *q, int32_t *counters) {
+    return lsm_integrate_batch_rho(h, n, dt, nullptr, y_in, y_out, status, q, counters);
+}
+
+int lsm_integrate_batch_rho(lsm_handle *h, int64_t n, const double *dt, const double *rho, const double *y_in,
+                            double *y_out, int32_t *status, double *q, int32_t *counters) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
     if (n == 0) return LSM_OK;
@@ -836,10 +857,20 @@ int lsm_integrate_batch(lsm_handle *h, int64_t n, const double *dt, const double
     HIP_OK(hipMemcpy(h->d_y, y_in, sizeof(double) * ne * n, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->d_dt, dt, sizeof(double) * n, hipMemcpyHostToDevice));
     if (q) HIP_OK(hipMemcpy(h->d_q, q, sizeof(double) * n, hipMemcpyHostToDevice));
-    int rc = lsm_integrate_batch_device(h, n, h->d_dt, h->d_y, h->d_y, status ? h->d_status : nullptr,
-                                        q ? h->d_q : nullptr, counters ? h->d_cnt : nullptr, nullptr);
-    if (rc) return rc;
-    HIP_OK(hipDeviceSynchronize());
+    double *d_rho = nullptr;
+    if (rho && h->kp.const_volume) {
+        HIP_OK(hipMalloc(&d_rho, sizeof(double) * n));
+        if (hipMemcpy(d_rho, rho, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(d_rho);
+            return fail(LSM_E_HIP, "copy rho");
+        }
+    }
+    int rc = integrate_device(h, n, h->d_dt, d_rho, h->d_y, h->d_y, status ? h->d_status : nullptr,
+                              q ? h->d_q : nullptr, counters ? h->d_cnt : nullptr, nullptr, nullptr);
+    if (rc) { if (d_rho) (void)hipFree(d_rho); return rc; }
+    const hipError_t sync = hipDeviceSynchronize();
+    if (d_rho) (void)hipFree(d_rho);
+    HIP_OK(sync);
     HIP_OK(hipMemcpy(y_out, h->d_y, sizeof(double) * ne * n, hipMemcpyDeviceToHost));
     if (status) HIP_OK(hipMemcpy(status, h->d_status, sizeof(int) * n, hipMemcpyDeviceToHost));
     if (q) HIP_OK(hipMemcpy(q, h->d_q, sizeof(double) * n, hipMemcpyDeviceToHost));
@@ -853,6 +884,8 @@ int lsm_equations_device(lsm_handle *h, int64_t n, const double *d_y, double *d_
     if (n == 0 || (!d_f && !d_jac)) return LSM_OK;
     if (!d_y) return fail(LSM_E_INVALID, "null y pointer");
     if (h->team_wpb < 1) return fail(LSM_E_UNSUPPORTED, "equations need ns+1 <= 64 and a mechanism image that fits in LDS");
+    if (d_jac && h->kp.const_volume)
+        return fail(LSM_E_UNSUPPORTED, "the analytic Jacobian is built for the constant-pressure reactor");
     if (launch_equations(h->dm, h->kp, n, d_y, d_f, d_jac, h->team_grid, h->team_wpb, h->d_jws, stream))
         return fail(LSM_E_HIP, std::string("equations kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
     return LSM_OK;

@@ -80,6 +80,10 @@ struct KParams {
     int full_pivoting;
     int jac_mode;                       // 0: forward differences (KernelDense), 1: analytic
     int refine;                         // wide kernel: iterative-refinement sweeps after each FP32 solve
+    int const_volume;                   // constant-volume reactor (BatchReactorHomogeneousConstantVolume)
+    int compact;                        // Strang "compact" output: y_out holds RR [kg/m3/s] and RT [W/m3]
+    const double *rho;                  // constant volume: per-cell density [kg/m3] of this call (may be null:
+                                        // ideal-gas density of the initial state at `pressure`)
 };
 
 // Host-side launchers (lsm_kernels.hip).

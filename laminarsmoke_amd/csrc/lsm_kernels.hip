@@ -550,13 +550,14 @@ struct TeamCtx {
     const KParams &K;
     int lane, ne, ns, nr;
     double qr;
+    double rho0;                      // constant-volume reactor: density of the current cell
 #ifdef LSM_PHASE_TIMERS
     unsigned long long tp[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int in_jac = 0;
 #endif
 
     __device__ TeamCtx(const DevMech &m, const KParams &k, const unsigned char *img, double *wave_lds, double *jac)
-        : L(img, m.img, m.ns_pad), K(k), lane(threadIdx.x & 63), ne(m.ne), ns(m.ns), nr(m.nr), qr(0.) {
+        : L(img, m.img, m.ns_pad), K(k), lane(threadIdx.x & 63), ne(m.ne), ns(m.ns), nr(m.nr), qr(0.), rho0(0.) {
         Jl = jac;
         scg = (double2 *)wave_lds;
         sR = wave_lds + 128;
@@ -589,7 +590,8 @@ struct TeamCtx {
 #endif
     }
 
-    // BatchReactorHomogeneousConstantPressure::Equations (:175-247)
+    // BatchReactorHomogeneousConstantPressure::Equations (:175-247), or with K.const_volume
+    // BatchReactorHomogeneousConstantVolume::Equations (BatchReactorHomogeneousConstantVolume.H:150-199)
     __device__ __forceinline__ void rhs_impl(int src, int dst) {
         LSM_TIC(t_rhs0);
         const double T = readlane_d(V[src][0], ns);
@@ -600,9 +602,9 @@ struct TeamCtx {
         const double mwk = isp ? L.sp[15 * P + k] : 1.;
         const double MW = 1. / wave_sum(Yk / mwk);
         const double xk = (MW * Yk) / mwk;
-        const double cTotP = K.pressure / RGAS / T;
+        const double cTotP = K.const_volume ? rho0 / MW : K.pressure / RGAS / T;
         const double ck = cTotP * xk;
-        const double rho = cTotP * MW;
+        const double rho = K.const_volume ? rho0 : cTotP * MW;
         const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, uT = 1. / T, logT = log(T);
         double cp = 0., h = 0.;
         if (isp) {
@@ -629,7 +631,14 @@ struct TeamCtx {
         LSM_TIC(t_rhs2);
         const double Rk = isp ? sR[k] : 0.;
         double dy = isp ? (mwk * Rk) / rho : 0.;
-        if (K.energy) {
+        if (K.energy && K.const_volume) {
+            // dT/dt = (QR + R T sum_k R_k) / (rho0 cv), cv = (CpMixMolar - R) / MW
+            double hs = isp ? Rk * h : 0., rs = Rk;
+            wave_sum2(hs, rs);
+            qr = -hs * RGAS * T;
+            const double CvMixMass = (cpx * RGAS - RGAS) / MW;
+            if (k == ns) dy = (qr + RGAS * T * rs) / (rho * CvMixMass);
+        } else if (K.energy) {
             const double hs = wave_sum(isp ? Rk * h : 0.);
             qr = -hs * RGAS * T;
             const double CpMixMass = (cpx * RGAS) / MW;
@@ -638,6 +647,35 @@ struct TeamCtx {
         wave_sync();
         V[dst][0] = dy;
         LSM_TOC(9, t_rhs2);
+    }
+
+    // Constant-volume reactor: density of the cell (rhoCells, chemistry_DI.H:173), or
+    // the ideal-gas density of the initial state y0 (lane value) at the reactor pressure.
+    __device__ __forceinline__ void set_density(double y0, double T0, const double *rho_cells, int64_t cell) {
+        const double mwk = lane < ns ? L.sp[15 * L.SP + lane] : 1.;
+        const double MW0 = 1. / wave_sum(lane < ns ? y0 / mwk : 0.);
+        rho0 = rho_cells ? rho_cells[cell] : K.pressure * MW0 / (RGAS * T0);
+    }
+
+    // Strang "compact" sources of the final state y (chemistry_DI.H:236-282):
+    // RR_k = rho (Y_k - Y_k,in) / dt, RT = rho cp (T - T_in) / dt, rho and cp of the final
+    // state at the reactor pressure; zero for dt <= 1e-14, RT zero without energy equation.
+    __device__ __forceinline__ double compact_sources(double y, double yin, double Tin, double dtc) {
+        const bool isp = lane < ns;
+        const double Tf = readlane_d(y, ns);
+        const int P = L.SP;
+        const double mwk = isp ? L.sp[15 * P + lane] : 1.;
+        double cp = 0.;
+        if (isp) {
+            const double *a = L.sp + ((Tf > L.sp[14 * P + lane]) ? 0 : 7 * P) + lane;
+            cp = a[0] + Tf * (a[P] + Tf * (a[2 * P] + Tf * (a[3 * P] + Tf * a[4 * P])));
+        }
+        double u = isp ? y / mwk : 0., cy = isp ? cp * (y / mwk) : 0.;
+        wave_sum2(u, cy);
+        const double rho = K.pressure * (1. / u) / (RGAS * Tf);
+        const bool ok = dtc > 1.e-14;
+        if (isp) return ok ? rho * (y - yin) / dtc : 0.;
+        return (ok && K.energy) ? rho * (cy * RGAS) * (Tf - Tin) / dtc : 0.;
     }
 
     struct JacAcc {
@@ -959,6 +997,7 @@ __global__ void __launch_bounds__(256, (NMAX <= 16 ? 2 : 1)) team_kernel(DevMech
             const double sum = wave_sum(lane < ns ? y0 : 0.);
             if (lane < ns) y0 /= sum;
             c.V[Y0_][0] = lane < ne ? y0 : 0.;
+            if (K.const_volume) c.set_density(y0, T0, K.rho, cell);     // :173
             st = bdf_integrate(c, K.bk, K.bk.t0 + dt[cell], cnt);       // :138-157
             y = st > 0 ? c.V[Y_][0] : c.V[Y0_][0];
             if (st > 0 && q != nullptr && lane == 0) q[cell] = c.qr;    // :168
@@ -968,6 +1007,7 @@ __global__ void __launch_bounds__(256, (NMAX <= 16 ? 2 : 1)) team_kernel(DevMech
         const double sum = wave_sum(lane < ns ? y : 0.);               // :219 normalizeMassFractions
         if (lane < ns) y /= sum;
         if (!K.energy && lane == ns) y = yi;
+        if (K.compact) y = c.compact_sources(y, yi, T0, dt[cell]);     // :236-282
         if (lane < ne) yout[(int64_t)lane * n + cell] = y;
         if (lane == 0) {
             if (status) status[cell] = st;
@@ -1010,6 +1050,7 @@ struct WideCtx {
     int tid, wave, lane, ne, ns, nr, lda;
     unsigned dummy;
     double qr;
+    double rho0;              // constant-volume reactor: density of the current cell
     double hr0;               // h l0 of the current factorisation (for the residuals)
     int rp;                   // reduction buffer parity
     double2 *scg;             // LDS [256]  (c_k, g_k)
@@ -1032,7 +1073,7 @@ struct WideCtx {
     __device__ WideCtx(const DevMech &m, const KParams &k, unsigned char *lds, double *jac)
         : K(k), L((const unsigned char *)m.img.src, m.img, m.ns_pad), tid(threadIdx.x), wave(threadIdx.x >> 6),
           lane(threadIdx.x & 63), ne(m.ne), ns(m.ns), nr(m.nr), lda(m.ne | 1), dummy((unsigned)m.img.dummy), qr(0.),
-          hr0(0.), rp(0), J(jac) {
+          rho0(0.), hr0(0.), rp(0), J(jac) {
         double *d = (double *)lds;
         scg = (double2 *)d;
         sR = d + 512;
@@ -1102,7 +1143,8 @@ struct WideCtx {
         s = a0 * logT + T * a1 + T2 * (0.50 * a2) + T3 * ((1. / 3.) * a3) + T4 * (0.25 * a4) + a[6 * P];
     }
 
-    // BatchReactorHomogeneousConstantPressure::Equations (:175-247)
+    // BatchReactorHomogeneousConstantPressure::Equations (:175-247), or with K.const_volume
+    // BatchReactorHomogeneousConstantVolume::Equations (BatchReactorHomogeneousConstantVolume.H:150-199)
     __device__ __forceinline__ void rhs_impl(int src, int dst) {
         const int k = tid;
         const bool isp = k < ns;
@@ -1111,9 +1153,9 @@ struct WideCtx {
         double T;
         const double MW = 1. / wg_sum(isp ? yv / mwk : 0., &T, yv, ns);
         const double xk = isp ? (MW * yv) / mwk : 0.;
-        const double cTotP = K.pressure / RGAS / T;
+        const double cTotP = K.const_volume ? rho0 / MW : K.pressure / RGAS / T;
         const double ck = cTotP * xk;
-        const double rho = cTotP * MW;
+        const double rho = K.const_volume ? rho0 : cTotP * MW;
         double cp = 0., h = 0., s = 0., dcp = 0.;
         if (isp) {
             thermo(k, T, cp, h, s, dcp);
@@ -1131,13 +1173,43 @@ struct WideCtx {
         __syncthreads();
         const double Rk = isp ? ((sR[k] + sR[256 + k]) + (sR[512 + k] + sR[768 + k])) : 0.;
         double dy = isp ? (mwk * Rk) / rho : 0.;
-        if (K.energy) {
+        if (K.energy && K.const_volume) {
+            // dT/dt = (QR + R T sum_k R_k) / (rho0 cv), cv = (CpMixMolar - R) / MW
+            double hs = isp ? Rk * h : 0., rs = Rk;
+            wg_sum2(hs, rs);
+            qr = -hs * RGAS * T;
+            const double CvMixMass = (cpx * RGAS - RGAS) / MW;
+            if (k == ns) dy = (qr + RGAS * T * rs) / (rho * CvMixMass);
+        } else if (K.energy) {
             const double hs = wg_sum(isp ? Rk * h : 0.);
             qr = -hs * RGAS * T;
             const double CpMixMass = (cpx * RGAS) / MW;
             if (k == ns) dy = qr / (rho * CpMixMass);
         }
         V[dst][0] = k < ne ? dy : 0.;
+    }
+
+    // Constant-volume reactor: density of the cell, or the ideal-gas density of the
+    // initial state y0 (thread value) at the reactor pressure.
+    __device__ __forceinline__ void set_density(double y0, double T0, const double *rho_cells, int64_t cell) {
+        const double mwk = tid < ns ? L.sp[15 * L.SP + tid] : 1.;
+        const double MW0 = 1. / wg_sum(tid < ns ? y0 / mwk : 0.);
+        rho0 = rho_cells ? rho_cells[cell] : K.pressure * MW0 / (RGAS * T0);
+    }
+
+    // Strang "compact" sources of the final state (chemistry_DI.H:236-282); T_f is
+    // passed in (thread ns holds it).
+    __device__ __forceinline__ double compact_sources(double y, double yin, double Tf, double Tin, double dtc) {
+        const bool isp = tid < ns;
+        const double mwk = isp ? L.sp[15 * L.SP + tid] : 1.;
+        double cp = 0., h, s, dcp;
+        if (isp) thermo(tid, Tf, cp, h, s, dcp);
+        double u = isp ? y / mwk : 0., cy = isp ? cp * (y / mwk) : 0.;
+        wg_sum2(u, cy);
+        const double rho = K.pressure * (1. / u) / (RGAS * Tf);
+        const bool ok = dtc > 1.e-14;
+        if (isp) return ok ? rho * (y - yin) / dtc : 0.;
+        return (ok && K.energy) ? rho * (cy * RGAS) * (Tf - Tin) / dtc : 0.;
     }
 
     // KernelDense::NumericalJacobian, max-constraint branch (KernelDense.hpp:160-183)
@@ -1587,15 +1659,18 @@ __global__ void __launch_bounds__(256) wide_kernel(DevMech M, KParams K, int64_t
             const double sum = c.wg_sum(tid < ns ? y0 : 0.);
             if (tid < ns) y0 /= sum;
             c.V[Y0_][0] = tid < ne ? y0 : 0.;
+            if (K.const_volume) c.set_density(y0, T0, K.rho, cell);     // :173
             st = bdf_integrate(c, K.bk, K.bk.t0 + dt[cell], cnt);       // :138-157
             y = st > 0 ? c.V[Y_][0] : c.V[Y0_][0];
             if (st > 0 && q != nullptr && tid == 0) q[cell] = c.qr;     // :168
         } else {
             y = yi;
         }
-        const double sum = c.wg_sum(tid < ns ? y : 0.);               // :219 normalizeMassFractions
+        double Tf;
+        const double sum = c.wg_sum(tid < ns ? y : 0., &Tf, y, ns);   // :219 normalizeMassFractions
         if (tid < ns) y /= sum;
-        if (!K.energy && tid == ns) y = yi;
+        if (!K.energy) { if (tid == ns) y = yi; Tf = T0; }
+        if (K.compact) y = c.compact_sources(y, yi, Tf, T0, dt[cell]); // :236-282
         if (tid < ne) yout[(int64_t)tid * n + cell] = y;
         if (tid == 0 && status) status[cell] = st;
 #ifdef LSM_PHASE_TIMERS
@@ -1640,6 +1715,7 @@ __global__ void __launch_bounds__(256) equations_kernel(DevMech M, KParams K, in
     const int lane = c.lane, ne = M.ne;
     for (int64_t cell = (int64_t)gw; cell < n; cell += (int64_t)gridDim.x * wpb) {
         c.V[VB_][0] = lane < ne ? yin[(int64_t)lane * n + cell] : 0.;
+        if (K.const_volume) c.set_density(c.V[VB_][0], readlane_d(c.V[VB_][0], M.ns), K.rho, cell);
         c.rhs(VB_, F_);
         if (f != nullptr && lane < ne) f[(int64_t)lane * n + cell] = c.V[F_][0];
         if (J != nullptr) {
@@ -1666,9 +1742,10 @@ struct PlainCtx {
     int64_t slot;
     int ne, ns, nr;
     double qr;
+    double rho0;      // constant-volume reactor: density of the current cell
 
     __device__ PlainCtx(const DevMech &m, const KParams &k, double *work, int64_t slots, int64_t sl)
-        : M(m), K(k), S(slots), slot(sl), ne(m.ne), ns(m.ns), nr(m.nr), qr(0.) {
+        : M(m), K(k), S(slots), slot(sl), ne(m.ne), ns(m.ns), nr(m.nr), qr(0.), rho0(0.) {
         W = work;
         Jm = W + (int64_t)NVEC * ne * S;
         Gm = Jm + (int64_t)ne * ne * S;
@@ -1697,8 +1774,8 @@ struct PlainCtx {
         double u = 0.;
         for (int k = 0; k < ns; k++) u += x(src, k) / M.sp[SP_MW * M.ns_pad + k];
         const double MW = 1. / u;
-        const double cTotP = K.pressure / RGAS / T;
-        const double rho = cTotP * MW;
+        const double cTotP = K.const_volume ? rho0 / MW : K.pressure / RGAS / T;
+        const double rho = K.const_volume ? rho0 : cTotP * MW;
         const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, uT = 1. / T, logT = log(T);
         double cTot = 0., cpx = 0.;
         for (int k = 0; k < ns; k++) {
@@ -1717,15 +1794,19 @@ struct PlainCtx {
             sr[(int64_t)j * S + slot] = reaction_rate(
                 M, j, T, logT, uT, logPRT, cTot, [&](int i) { return sc[(int64_t)i * S + slot]; },
                 [&](int i) { return sh[(int64_t)i * S + slot]; }, [&](int i) { return ss[(int64_t)i * S + slot]; });
-        double hs = 0.;
+        double hs = 0., rs = 0.;
         for (int k = 0; k < ns; k++) {
             double Rk = 0.;
             for (int e = M.f_ptr[k]; e < M.f_ptr[k + 1]; e++) Rk += M.f_nu[e] * sr[(int64_t)M.f_rx[e] * S + slot];
             x(dst, k) = (M.sp[SP_MW * M.ns_pad + k] * Rk) / rho;
             hs += Rk * sh[k * S + slot];
+            rs += Rk;
         }
         double dT = 0.;
-        if (K.energy) {
+        if (K.energy && K.const_volume) {      // BatchReactorHomogeneousConstantVolume.H:185-196
+            qr = -hs * RGAS * T;
+            dT = (qr + RGAS * T * rs) / (rho * ((cpx * RGAS - RGAS) / MW));
+        } else if (K.energy) {
             qr = -hs * RGAS * T;
             dT = qr / (rho * ((cpx * RGAS) / MW));
         }
@@ -1834,6 +1915,11 @@ __global__ void __launch_bounds__(256) plain_kernel(DevMech M, KParams K, int64_
             }
             for (int i = 0; i < ns; i++) c.x(Y0_, i) /= sum;
             c.x(Y0_, ns) = T0;
+            if (K.const_volume) {                                       // chemistry_DI.H:173
+                double u = 0.;
+                for (int i = 0; i < ns; i++) u += c.x(Y0_, i) / M.sp[SP_MW * M.ns_pad + i];
+                c.rho0 = K.rho ? K.rho[cell] : K.pressure * (1. / u) / (RGAS * T0);
+            }
             st = bdf_integrate(c, K.bk, K.bk.t0 + dt[cell], cnt);
             const int src = st > 0 ? Y_ : Y0_;
             for (int i = 0; i < ne; i++) c.x(AUX_, i) = c.x(src, i);
@@ -1843,8 +1929,30 @@ __global__ void __launch_bounds__(256) plain_kernel(DevMech M, KParams K, int64_
         }
         double sum = 0.;
         for (int i = 0; i < ns; i++) sum += c.x(AUX_, i);
-        for (int i = 0; i < ns; i++) yout[(int64_t)i * n + cell] = c.x(AUX_, i) / sum;
-        yout[(int64_t)ns * n + cell] = K.energy ? c.x(AUX_, ns) : T0;
+        for (int i = 0; i < ns; i++) c.x(AUX_, i) /= sum;
+        if (!K.energy) c.x(AUX_, ns) = T0;
+        if (K.compact) {                       // Strang "compact" sources, chemistry_DI.H:236-282
+            const double Tf = c.x(AUX_, ns);
+            const double T2 = Tf * Tf, T3 = T2 * Tf, T4 = T3 * Tf, uT = 1. / Tf, logT = log(Tf);
+            double u = 0., cy = 0.;
+            for (int i = 0; i < ns; i++) {
+                double cp, h, s;
+                nasa7(M, i, Tf, T2, T3, T4, uT, logT, cp, h, s);
+                const double yw = c.x(AUX_, i) / M.sp[SP_MW * M.ns_pad + i];
+                u += yw;
+                cy += cp * yw;
+            }
+            const double rho = K.pressure * (1. / u) / (RGAS * Tf);
+            const double dtc = dt[cell];
+            const bool ok = dtc > 1.e-14;
+            for (int i = 0; i < ns; i++) {
+                const double yi = yin[(int64_t)i * n + cell];
+                yout[(int64_t)i * n + cell] = ok ? rho * (c.x(AUX_, i) - yi) / dtc : 0.;
+            }
+            yout[(int64_t)ns * n + cell] = (ok && K.energy) ? rho * (cy * RGAS) * (Tf - T0) / dtc : 0.;
+        } else {
+            for (int i = 0; i < ne; i++) yout[(int64_t)i * n + cell] = c.x(AUX_, i);
+        }
         if (status) status[cell] = st;
         if (counters) {
             int *o = counters + cell * LSM_NCOUNTERS_DEV;

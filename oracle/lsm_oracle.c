@@ -356,6 +356,8 @@ typedef struct {
     kin_ws_t w;
     double P;
     int energy;
+    int cv;         /* constant-volume reactor */
+    double rho0;    /* its density (SetReactor, BatchReactorHomogeneousConstantVolume.H:118-123) */
     double QR;      /* heat release of the last RHS call (reactor QR_, :216-221) */
     long nrhs;
 } reactor_t;
@@ -372,14 +374,26 @@ static void reactor_rhs(void *vctx, const double *y, double *dy) {
     for (int i = 0; i < ns; i++) u += y[i] / M->MW[i];
     const double MW = 1. / u;
     for (int i = 0; i < ns; i++) w->x[i] = (MW * y[i]) / M->MW[i];
-    const double cTot = R->P / LSM_R_J_KMOL / T;
+    /* constant pressure: cTot = P/(R T); constant volume: cTot = rho0/MW
+     * (BatchReactorHomogeneousConstantVolume.H:167-170) */
+    const double cTot = R->cv ? R->rho0 / MW : R->P / LSM_R_J_KMOL / T;
     for (int i = 0; i < ns; i++) w->c[i] = cTot * w->x[i];
-    const double rho = cTot * MW;
+    const double rho = R->cv ? R->rho0 : cTot * MW;
     thermo_eval(M, T, w->cp, w->h, w->s);
     kinetics_eval(M, w, T, w->c);
     for (int i = 0; i < ns; i++) dy[i] = M->MW[i] * w->R[i] / rho;
     dy[ns] = 0.;
-    if (R->energy) {
+    if (R->energy && R->cv) {
+        /* BatchReactorHomogeneousConstantVolume.H:185-196:
+         * dT/dt = (QR + R T sum_k R_k) / (rho0 cv), cv = (CpMixMolar - R)/MW */
+        double cpd = 0.;
+        for (int i = 0; i < ns; i++) cpd += w->cp[i] * w->x[i];
+        const double CvMixMass = (cpd * LSM_R_J_KMOL - LSM_R_J_KMOL) / MW;
+        double hd = 0., sumR = 0.;
+        for (int i = 0; i < ns; i++) { hd += w->R[i] * w->h[i]; sumR += w->R[i]; }
+        R->QR = -hd * LSM_R_J_KMOL * T;
+        dy[ns] = (R->QR + LSM_R_J_KMOL * T * sumR) / (rho * CvMixMass);
+    } else if (R->energy) {
         double cpd = 0.;
         for (int i = 0; i < ns; i++) cpd += w->cp[i] * w->x[i];
         const double CpMixMolar = cpd * LSM_R_J_KMOL;
@@ -391,16 +405,25 @@ static void reactor_rhs(void *vctx, const double *y, double *dy) {
     }
 }
 
-double lsm_oracle_rhs(const lsm_mechanism *m, double P, int energy, const double *y, double *f) {
+static double rhs_once(const lsm_mechanism *m, double P, int cv, double rho0, int energy, const double *y,
+                       double *f) {
     mech_t M;
     reactor_t R;
     mech_init(&M, m);
-    R.M = &M; R.P = P; R.energy = energy; R.QR = 0.; R.nrhs = 0;
+    R.M = &M; R.P = P; R.energy = energy; R.cv = cv; R.rho0 = rho0; R.QR = 0.; R.nrhs = 0;
     kin_ws_alloc(&R.w, m->ns, m->nr);
     reactor_rhs(&R, y, f);
     kin_ws_free(&R.w);
     mech_free(&M);
     return R.QR;
+}
+
+double lsm_oracle_rhs(const lsm_mechanism *m, double P, int energy, const double *y, double *f) {
+    return rhs_once(m, P, 0, 0., energy, y, f);
+}
+
+double lsm_oracle_rhs_cv(const lsm_mechanism *m, double rho0, int energy, const double *y, double *f) {
+    return rhs_once(m, 101325., 1, rho0, energy, y, f);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1095,6 +1118,7 @@ static void cell_ctx_init(cell_ctx_t *c, const lsm_mechanism *m, const lsm_oracl
     const int ns = m->ns, ne = ns + 1;
     mech_init(&c->M, m);
     c->R.M = &c->M; c->R.P = o->pressure; c->R.energy = o->energy_equation; c->R.QR = 0.; c->R.nrhs = 0;
+    c->R.cv = o->constant_volume ? 1 : 0; c->R.rho0 = 0.;
     kin_ws_alloc(&c->R.w, ns, m->nr);
     c->ymin = malloc(8 * ne); c->ymax = malloc(8 * ne); c->y0 = malloc(8 * ne);
     bdf_alloc(&c->s, ne);
@@ -1114,8 +1138,29 @@ static void normalize_mass_fractions(double *y, int ns) {
     for (int i = 0; i < ns; i++) y[i] /= sum;
 }
 
-static int integrate_cell(cell_ctx_t *c, const lsm_oracle_options *o, double dt, const double *yin,
-                          double *yout, double *q, int32_t *counters) {
+/* Strang "compact" sources of the final state (chemistry_DI.H:236-282):
+ * RR_i = rho (yf_i - Y_i)/dt, RT = rho cp (Tf - T)/dt with rho and cp of the final
+ * state at the thermodynamic pressure, Y and T the cell's input fields; zero for
+ * dt <= 1e-14 (and RT zero without the energy equation). */
+static void compact_sources(cell_ctx_t *c, const lsm_oracle_options *o, double dt, const double *yin, double *yout) {
+    const mech_t *M = &c->M;
+    const int ns = M->ns;
+    const double Tf = yout[ns];
+    double u = 0.;
+    for (int i = 0; i < ns; i++) u += yout[i] / M->MW[i];
+    const double MW = 1. / u;
+    thermo_eval(M, Tf, c->R.w.cp, c->R.w.h, c->R.w.s);
+    double cpd = 0.;
+    for (int i = 0; i < ns; i++) cpd += c->R.w.cp[i] * ((MW * yout[i]) / M->MW[i]);
+    const double cp = cpd * LSM_R_J_KMOL / MW;
+    const double rho = o->pressure * MW / LSM_R_J_KMOL / Tf;
+    const int ok = dt > 1.e-14;
+    for (int i = 0; i < ns; i++) yout[i] = ok ? rho * (yout[i] - yin[i]) / dt : 0.;
+    yout[ns] = (ok && o->energy_equation) ? rho * cp * (Tf - yin[ns]) / dt : 0.;
+}
+
+static int integrate_cell(cell_ctx_t *c, const lsm_oracle_options *o, double dt, double rho_cell,
+                          const double *yin, double *yout, double *q, int32_t *counters) {
     const int ns = c->M.ns, ne = ns + 1;
     int status = 0;
     if (yin[ns] > o->min_temperature) {                         /* chemistry_DI.H:97 */
@@ -1123,6 +1168,11 @@ static int integrate_cell(cell_ctx_t *c, const lsm_oracle_options *o, double dt,
         for (int i = 0; i < ns; i++) { c->y0[i] = yin[i] < 0. ? 0. : yin[i]; sum += c->y0[i]; }
         for (int i = 0; i < ns; i++) c->y0[i] /= sum;
         c->y0[ns] = yin[ns];
+        if (c->R.cv) {                                          /* SetReactor(V, P, rho), :173 */
+            double u = 0.;
+            for (int i = 0; i < ns; i++) u += c->y0[i] / c->M.MW[i];
+            c->R.rho0 = rho_cell > 0. ? rho_cell : o->pressure * (1. / u) / LSM_R_J_KMOL / c->y0[ns];
+        }
         bdf_fresh(&c->s);                                       /* fresh solver per cell */
         c->s.rhs = reactor_rhs; c->s.ctx = &c->R;
         c->s.rel_tol = o->rel_tol; c->s.abs_tol = o->abs_tol; c->s.full_pivoting = o->full_pivoting;
@@ -1144,6 +1194,7 @@ static int integrate_cell(cell_ctx_t *c, const lsm_oracle_options *o, double dt,
     }
     normalize_mass_fractions(yout, ns);                         /* :219 */
     if (!o->energy_equation) yout[ns] = yin[ns];                /* :231-233 */
+    if (o->strang_compact) compact_sources(c, o, dt, yin, yout);
     return status;
 }
 
@@ -1151,7 +1202,7 @@ int lsm_oracle_integrate_cell(const lsm_mechanism *m, const lsm_oracle_options *
                               const double *y_in, double *y_out, double *q, int32_t *counters) {
     cell_ctx_t c;
     cell_ctx_init(&c, m, o);
-    const int st = integrate_cell(&c, o, dt, y_in, y_out, q, counters);
+    const int st = integrate_cell(&c, o, dt, o->rho ? o->rho[0] : 0., y_in, y_out, q, counters);
     cell_ctx_free(&c);
     return st;
 }
@@ -1183,7 +1234,7 @@ static void *worker(void *arg) {
         for (int64_t cell = b; cell < e; cell++) {
             for (int i = 0; i < ne; i++) yi[i] = P->yin[(int64_t)i * P->n + cell];
             double qv = P->q ? P->q[cell] : 0.;
-            const int st = integrate_cell(&c, P->o, P->dt[cell], yi, yo, &qv,
+            const int st = integrate_cell(&c, P->o, P->dt[cell], P->o->rho ? P->o->rho[cell] : 0., yi, yo, &qv,
                                           P->counters ? P->counters + cell * LSM_ORACLE_NCOUNTERS : NULL);
             for (int i = 0; i < ne; i++) P->yout[(int64_t)i * P->n + cell] = yo[i];
             if (P->q) P->q[cell] = qv;

@@ -39,6 +39,14 @@ typedef struct lsm_oracle_options {
     int32_t energy_equation;   /* energyEquation                               */
     double mass_fractions_tol; /* massFractionsTol (warning threshold)         */
     double t0;                 /* start time of the chemistry step             */
+    int32_t constant_volume;   /* constPressureBatchReactor false: constant-volume
+                                  reactor (BatchReactorHomogeneousConstantVolume) */
+    int32_t strang_compact;    /* strangAlgorithm STRANG_COMPACT: y_out holds the
+                                  sources RR [kg/m3/s] and RT [W/m3]
+                                  (chemistry_DI.H:236-282)                     */
+    const double *rho;         /* constant volume: per-cell density rho0 [kg/m3]
+                                  (rhoCells); NULL: ideal-gas density of the
+                                  initial state at `pressure`                  */
 } lsm_oracle_options;
 
 typedef void (*lsm_oracle_rhs_fn)(void *ctx, const double *y, double *f);
@@ -64,6 +72,11 @@ void lsm_oracle_kinetics(const lsm_mechanism *m, double T, const double *c,
 /* Constant-pressure RHS (BatchReactorHomogeneousConstantPressure::Equations);
  * y = [Y_0..Y_{ns-1}, T], returns QR [J/m3/s]. */
 double lsm_oracle_rhs(const lsm_mechanism *m, double P, int energy, const double *y, double *f);
+
+/* Constant-volume RHS (BatchReactorHomogeneousConstantVolume::Equations,
+ * BatchReactorHomogeneousConstantVolume.H:150-199) at density rho0 [kg/m3];
+ * returns QR [J/m3/s]. */
+double lsm_oracle_rhs_cv(const lsm_mechanism *m, double rho0, int energy, const double *y, double *f);
 
 /* Dense LU, row-major n x n in place; rp/cp row/column permutations (cp may be
  * NULL for partial pivoting).  Returns 0, or k+1 if pivot k was exactly zero. */

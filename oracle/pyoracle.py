@@ -27,7 +27,8 @@ class OracleOptions(ctypes.Structure):
         ("full_pivoting", ctypes.c_int32), ("max_steps", ctypes.c_int32),
         ("pressure", ctypes.c_double), ("min_temperature", ctypes.c_double),
         ("energy_equation", ctypes.c_int32), ("mass_fractions_tol", ctypes.c_double),
-        ("t0", ctypes.c_double),
+        ("t0", ctypes.c_double), ("constant_volume", ctypes.c_int32), ("strang_compact", ctypes.c_int32),
+        ("rho", _d),
     ]
 
 
@@ -53,6 +54,7 @@ def lib():
             build()
         L = ctypes.CDLL(LIB_PATH)
         L.lsm_oracle_rhs.restype = ctypes.c_double
+        L.lsm_oracle_rhs_cv.restype = ctypes.c_double
         L.lsm_oracle_bdf.restype = ctypes.c_int
         L.lsm_oracle_integrate_cell.restype = ctypes.c_int
         L.lsm_oracle_lu_factor.restype = ctypes.c_int
@@ -67,9 +69,11 @@ def _p(a, t=ctypes.c_double):
 
 
 def options(rtol=1e-7, atol=1e-12, full_pivoting=False, max_steps=0, pressure=101325.0,
-            min_temperature=0.0, energy=True, mass_fractions_tol=1e-4, t0=0.0) -> OracleOptions:
+            min_temperature=0.0, energy=True, mass_fractions_tol=1e-4, t0=0.0, constant_volume=False,
+            compact=False, rho=None) -> OracleOptions:
+    """rho: per-cell densities of the constant-volume reactor (kept alive by the caller)."""
     return OracleOptions(rtol, atol, int(full_pivoting), int(max_steps), pressure, min_temperature,
-                         int(energy), mass_fractions_tol, t0)
+                         int(energy), mass_fractions_tol, t0, int(constant_volume), int(compact), _p(rho))
 
 
 def species_thermo(mech, T):
@@ -88,10 +92,14 @@ def kinetics(mech, T, c):
     return dict(kf=kf, uKeq=uK, kmod=km, r=r, R=R)
 
 
-def rhs(mech, y, P=101325.0, energy=True):
+def rhs(mech, y, P=101325.0, energy=True, rho0=None):
+    """Constant-pressure RHS, or the constant-volume RHS at density rho0 when given."""
     y = np.ascontiguousarray(y, dtype=np.float64)
     f = np.zeros_like(y)
-    qr = lib().lsm_oracle_rhs(ctypes.byref(mech.struct()), ctypes.c_double(P), int(energy), _p(y), _p(f))
+    if rho0 is None:
+        qr = lib().lsm_oracle_rhs(ctypes.byref(mech.struct()), ctypes.c_double(P), int(energy), _p(y), _p(f))
+    else:
+        qr = lib().lsm_oracle_rhs_cv(ctypes.byref(mech.struct()), ctypes.c_double(rho0), int(energy), _p(y), _p(f))
     return f, qr
 
 
@@ -143,10 +151,14 @@ def integrate_cell(mech, y, dt, **kw):
     return st, out, q.value, cnt
 
 
-def chemistry_step(mech, y_soa, dt, threads=1, q=None, **kw):
-    """SoA y[neq, n] -> (y_out, status, q, counters[n, 12])."""
+def chemistry_step(mech, y_soa, dt, threads=1, q=None, rho=None, **kw):
+    """SoA y[neq, n] -> (y_out, status, q, counters[n, 12]).  rho: per-cell densities of the
+    constant-volume reactor (constant_volume=True), default the ideal-gas density of the input."""
     y_soa = np.ascontiguousarray(y_soa, dtype=np.float64)
     neq, n = y_soa.shape
+    if rho is not None:
+        rho = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, dtype=np.float64), (n,)))
+        kw["rho"] = rho
     dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (n,)))
     yout = np.zeros_like(y_soa)
     st = np.zeros(n, dtype=np.int32)
