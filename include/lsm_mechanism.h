@@ -30,9 +30,10 @@
  *   tb_*             third-body reactions: M = cTot + sum_k (eff_k - 1) c_k
  *                    (KineticsMap_CHEMKIN.hpp:1388-1398); efficiencies are
  *                    stored as eff - 1, like the map does (hpp:655-661).
- *   fo_*             fall-off reactions (Lindemann or Troe), k_inf Arrhenius
- *                    parameters, Troe (a, T***, T*, T**) and either a single
- *                    collider species or an M efficiency list (hpp:1496-1550).
+ *   fo_*             fall-off reactions (Lindemann, Troe or SRI), k_inf Arrhenius
+ *                    parameters, Troe (a, T***, T*, T**) or SRI (a, b, c, d, e)
+ *                    and either a single collider species or an M efficiency
+ *                    list (hpp:1496-1550).
  *
  * All indices are 0-based.  The arrays are owned by the caller and are only
  * read during lsm_create()/oracle calls.
@@ -55,7 +56,9 @@ enum {
 
 enum {
     LSM_FALLOFF_LINDEMANN = 0,
-    LSM_FALLOFF_TROE = 1
+    LSM_FALLOFF_TROE = 1,
+    LSM_FALLOFF_SRI = 2      /* F = d [a exp(-b/T) + exp(-T/c)]^X T^e,
+                                X = 1/(1 + log10(Pr)^2) (KineticsMap_CHEMKIN.hpp:1283-1287, 1538-1544) */
 };
 
 typedef struct lsm_mechanism {
@@ -86,6 +89,9 @@ typedef struct lsm_mechanism {
     const int32_t *fo_ptr;     /* [nfo+1]                                    */
     const int32_t *fo_sp;      /* [fo_ptr[nfo]]                              */
     const double *fo_eff_m1;   /* [fo_ptr[nfo]] efficiency - 1               */
+    const double *fo_sri;      /* [nfo*5] a, b, c, d, e of LSM_FALLOFF_SRI entries
+                                  (d = 1, e = 0 for the 3-parameter form); may be
+                                  NULL when no reaction is SRI; d > 0, c != 0   */
 } lsm_mechanism;
 
 /* Physical constants of the reference (math/PhysicalConstants.h:24). */

@@ -12,12 +12,12 @@ Supported:
   coefficients, repeated species on one side merged (``OH+OH`` -> 2 OH);
 * third-body reactions (``+M``) with efficiencies;
 * fall-off reactions (``(+M)`` or ``(+species)``) with ``LOW`` and optional
-  3- or 4-parameter ``TROE``, with efficiencies;
+  3- or 4-parameter ``TROE`` or 3- or 5-parameter ``SRI``, with efficiencies;
 * ``DUPLICATE``; negative pre-exponential factors on elementary reactions;
 * energy units CAL/MOLE (default), KCAL/MOLE, JOULES/MOLE, KJOULES/MOLE, KELVINS;
   quantity unit MOLES.
 
-Anything else the kernels do not evaluate (``REV``, ``SRI``, ``HIGH``, ``PLOG``,
+Anything else the kernels do not evaluate (``REV``, ``HIGH``, ``PLOG``,
 ``CHEB``, ``FORD``/``RORD``, ``LT``, ``MOLECULES``, ...) raises
 :class:`ChemkinError` naming the reaction, rather than being dropped.
 
@@ -44,8 +44,8 @@ import re
 
 import numpy as np
 
-from .mechanism import (FALLOFF_LINDEMANN, FALLOFF_TROE, RXN_NEGATIVE_A, RXN_REVERSIBLE, J_FROM_CAL,
-                        R_J_KMOL, Mechanism)
+from .mechanism import (FALLOFF_LINDEMANN, FALLOFF_SRI, FALLOFF_TROE, RXN_NEGATIVE_A, RXN_REVERSIBLE,
+                        J_FROM_CAL, R_J_KMOL, Mechanism)
 
 # kernel/thermo/AtomicElementMap.hpp:41-60 (the values are stored as floats there)
 ATOMIC_WEIGHTS = {
@@ -58,7 +58,7 @@ E_CONVERSION = {               # J/kmol per unit (ReactionPolicy_CHEMKIN.hpp:148
     "CAL/MOLE": 1e3 * J_FROM_CAL, "KCAL/MOLE": 1e6 * J_FROM_CAL,
     "JOULES/MOLE": 1e3, "KJOULES/MOLE": 1e6, "KELVINS": R_J_KMOL,
 }
-_UNSUPPORTED = {"REV", "SRI", "HIGH", "PLOG", "CHEB", "TCHEB", "PCHEB", "FORD", "RORD", "LT", "RLT",
+_UNSUPPORTED = {"REV", "HIGH", "PLOG", "CHEB", "TCHEB", "PCHEB", "FORD", "RORD", "LT", "RLT",
                 "JAN", "FIT1", "TDEP", "EXCI", "MOME", "XSMI", "USRPROG", "LOWMX", "LOWSP", "TROEMX",
                 "TROESP", "SRIMX", "SRISP", "PLOGMX", "PLOGSP", "COLLEFF", "UNITS"}
 
@@ -247,7 +247,7 @@ def _parse_reaction_line(line: str, species_set: set, idx: int):
     if tb_l and co_l:
         raise ChemkinError(f"{what}: both +M and (+M)")
     return dict(eq=eq, what=what, reac=r, prod=p, rev=rev, A=A, beta=beta, E=E,
-                third=tb_l, collider=co_l, low=None, troe=None, eff={}, dup=False)
+                third=tb_l, collider=co_l, low=None, troe=None, sri=None, eff={}, dup=False)
 
 
 def read_kinetics(text: str):
@@ -299,6 +299,11 @@ def read_kinetics(text: str):
                             if len(vals) not in (3, 4):
                                 raise ChemkinError(f"{cur['what']}: TROE needs 3 or 4 values")
                             cur["troe"] = [float(v) for v in vals] + ([0.0] if len(vals) == 3 else [])
+                        elif kw == "SRI":
+                            # 3 parameters: d = 1, e = 0 (KineticsMap_CHEMKIN.hpp:749-767)
+                            if len(vals) not in (3, 5):
+                                raise ChemkinError(f"{cur['what']}: SRI needs 3 or 5 values")
+                            cur["sri"] = [float(v) for v in vals] + ([1.0, 0.0] if len(vals) == 3 else [])
                         elif kw in _UNSUPPORTED:
                             raise ChemkinError(f"{cur['what']}: the {kw} option is not supported")
                         elif kw in sset:
@@ -352,14 +357,19 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
     lnA, beta, EoR, flags, reac, prod = [], [], [], [], [], []
     tb_rxn, tb_sp, tb_eff = [], [], []
     fo_rxn, fo_type, fo_inf, fo_troe, fo_single, fo_sp, fo_eff = [], [], [], [], [], [], []
+    fo_sri = []
     for j, r in enumerate(reactions):
         what = r["what"]
         order = sum(nu for _, nu in r["reac"])
         falloff = r["collider"] is not None
         if falloff and r["low"] is None:
             raise ChemkinError(f"{what}: (+M) reaction without LOW (HIGH/chemically activated is not supported)")
-        if not falloff and (r["low"] is not None or r["troe"] is not None):
-            raise ChemkinError(f"{what}: LOW/TROE on a reaction without (+M)")
+        if not falloff and (r["low"] is not None or r["troe"] is not None or r["sri"] is not None):
+            raise ChemkinError(f"{what}: LOW/TROE/SRI on a reaction without (+M)")
+        if r["troe"] is not None and r["sri"] is not None:
+            raise ChemkinError(f"{what}: both TROE and SRI")
+        if r["sri"] is not None and (r["sri"][3] <= 0.0 or r["sri"][2] == 0.0):
+            raise ChemkinError(f"{what}: SRI needs d > 0 and c != 0")
         if falloff:
             A0, b0, E0 = r["low"]
             if r["A"] < 0.0 or A0 < 0.0:
@@ -370,8 +380,10 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
             EoR.append(E0 * econv / R_J_KMOL)
             fo_rxn.append(j)
             fo_inf.append((np.log(r["A"] * 1e3 ** (1.0 - order)), r["beta"], r["E"] * econv / R_J_KMOL))
-            fo_type.append(FALLOFF_TROE if r["troe"] is not None else FALLOFF_LINDEMANN)
+            fo_type.append(FALLOFF_TROE if r["troe"] is not None else
+                           FALLOFF_SRI if r["sri"] is not None else FALLOFF_LINDEMANN)
             fo_troe.append(tuple(r["troe"]) if r["troe"] is not None else (0.0, 1.0, 1.0, 0.0))
+            fo_sri.append(tuple(r["sri"]) if r["sri"] is not None else (0.0, 0.0, 1.0, 1.0, 0.0))
             if r["collider"] == "M":
                 fo_single.append(-1)
                 items = sorted((idx[s], e - 1.0) for s, e in r["eff"].items())
@@ -430,6 +442,7 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
         fo_sp=np.array([s for x in fo_sp for s in x], dtype=np.int32),
         fo_eff_m1=np.array([e for x in fo_eff for e in x], dtype=np.float64),
         name=name, element_names=tuple(el_names), atomic_weights=aw,
+        fo_sri=np.array(fo_sri, dtype=np.float64).reshape(nfo, 5),
     )
 
 

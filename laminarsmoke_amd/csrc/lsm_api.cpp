@@ -342,8 +342,14 @@ static int check_mechanism(const lsm_mechanism *m) {
     }
     for (int s = 0; s < m->nfo; s++) {
         if (m->fo_rxn[s] < 0 || m->fo_rxn[s] >= m->nr) return fail(LSM_E_INVALID, "fall-off reaction index");
-        if (m->fo_type[s] != LSM_FALLOFF_LINDEMANN && m->fo_type[s] != LSM_FALLOFF_TROE)
-            return fail(LSM_E_UNSUPPORTED, "fall-off type (only Lindemann and Troe)");
+        if (m->fo_type[s] != LSM_FALLOFF_LINDEMANN && m->fo_type[s] != LSM_FALLOFF_TROE &&
+            m->fo_type[s] != LSM_FALLOFF_SRI)
+            return fail(LSM_E_UNSUPPORTED, "fall-off type (Lindemann, Troe or SRI)");
+        if (m->fo_type[s] == LSM_FALLOFF_SRI) {
+            if (!m->fo_sri) return fail(LSM_E_INVALID, "SRI fall-off reaction without fo_sri parameters");
+            const double *t = m->fo_sri + 5 * s;
+            if (!(t[3] > 0.) || t[2] == 0.) return fail(LSM_E_INVALID, "SRI parameters need d > 0 and c != 0");
+        }
         if (m->fo_single[s] >= m->ns) return fail(LSM_E_INVALID, "fall-off collider species index");
         for (int q = m->fo_ptr[s]; q < m->fo_ptr[s + 1]; q++)
             if (m->fo_sp[q] < 0 || m->fo_sp[q] >= m->ns) return fail(LSM_E_INVALID, "fall-off species index");
@@ -443,9 +449,17 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
     std::vector<unsigned> fol(m->nfo + 1), fos2;
     std::vector<double> foe;
     for (int s = 0; s < m->nfo; s++) {
+        // [0..2] k_inf; Troe: [3..6] a, T***, T*, T**, [7] = 1; SRI: [3..6] a, b, c, e,
+        // [7] = -d (d > 0); Lindemann: [7] = 0
         for (int i = 0; i < 3; i++) fop[8 * s + i] = m->fo_inf[3 * s + i];
-        for (int i = 0; i < 4; i++) fop[8 * s + 3 + i] = m->fo_troe[4 * s + i];
-        fop[8 * s + 7] = (double)m->fo_type[s];
+        if (m->fo_type[s] == LSM_FALLOFF_SRI) {
+            const double *t = m->fo_sri + 5 * s;
+            fop[8 * s + 3] = t[0]; fop[8 * s + 4] = t[1]; fop[8 * s + 5] = t[2]; fop[8 * s + 6] = t[4];
+            fop[8 * s + 7] = -t[3];
+        } else {
+            for (int i = 0; i < 4; i++) fop[8 * s + 3 + i] = m->fo_troe[4 * s + i];
+            fop[8 * s + 7] = (double)m->fo_type[s];
+        }
         fos[s] = m->fo_single[s];
         fol[s] = (unsigned)fos2.size();
         for (int q = m->fo_ptr[s]; q < m->fo_ptr[s + 1]; q++) { fos2.push_back((unsigned)m->fo_sp[q]); foe.push_back(m->fo_eff_m1[q]); }
@@ -584,7 +598,13 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
         fo_type[s] = m->fo_type[s];
         fo_single[s] = m->fo_single[s];
         for (int i = 0; i < 3; i++) fo_par[8 * s + i] = m->fo_inf[3 * s + i];
-        for (int i = 0; i < 4; i++) fo_par[8 * s + 3 + i] = m->fo_troe[4 * s + i];
+        if (m->fo_type[s] == LSM_FALLOFF_SRI) {        // a, b, c, e, d
+            const double *t = m->fo_sri + 5 * s;
+            fo_par[8 * s + 3] = t[0]; fo_par[8 * s + 4] = t[1]; fo_par[8 * s + 5] = t[2]; fo_par[8 * s + 6] = t[4];
+            fo_par[8 * s + 7] = t[3];
+        } else {
+            for (int i = 0; i < 4; i++) fo_par[8 * s + 3 + i] = m->fo_troe[4 * s + i];
+        }
         fo_ptr[s] = (int)fo_sp.size();
         for (int q = m->fo_ptr[s]; q < m->fo_ptr[s + 1]; q++) { fo_sp.push_back(m->fo_sp[q]); fo_eff.push_back(m->fo_eff_m1[q]); }
     }

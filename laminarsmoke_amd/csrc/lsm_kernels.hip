@@ -167,6 +167,11 @@ __device__ __forceinline__ double reaction_rate(const DevMech &M, int j, double 
                 q = 1. / 0.14;
             }
             wF = pow(10., lf / (1. + q * q));
+        } else if (M.fo_type[fo] == LSM_FALLOFF_SRI) {       // hpp:1283-1287, 1538-1544
+            const double Fs = fp[3] * exp(-fp[4] / T) + exp(-T / fp[5]);
+            const double lp = log10(Pr);
+            wF = pow(Fs, 1. / (1. + lp * lp)) * fp[7];
+            if (fp[6] != 0.) wF *= pow(T, fp[6]);
         }
         kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
     }
@@ -262,7 +267,7 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
         }
         const double Pr = kf * Mc / kinf;
         double wF = 1.;
-        if (fp[7] != 0.) {   // Troe
+        if (fp[7] > 0.) {    // Troe
             const double a = fp[3];
             double F = (1. - a) * exp(-T / fp[4]) + a * exp(-T / fp[5]);
             if (fp[6] != 0.) F += exp(-fp[6] / T);
@@ -277,6 +282,10 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
                 q = 1. / 0.14;
             }
             wF = exp(LN10 * (lf / (1. + q * q)));    // 10^x
+        } else if (fp[7] < 0.) {   // SRI: d [a exp(-b/T) + exp(-T/c)]^X T^e, X = 1/(1 + log10(Pr)^2)
+            const double Fs = fp[3] * exp(-fp[4] * uT) + exp(-T / fp[5]);
+            const double lp = log10(Pr);
+            wF = exp(log(Fs) / (1. + lp * lp) + fp[6] * logT) * (-fp[7]);
         }
         kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
     }
@@ -376,7 +385,19 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
         }
         const double Pr = kf * Mc / kinf;
         double F = 1., dlgF_dlgPr = 0., dlnF_dT = 0.;
-        if (fp[7] != 0.) {   // Troe: log10 F = L / (1 + x^2), x = s / (n - 0.14 s), s = log10 Pr + c
+        if (fp[7] < 0.) {    // SRI: ln F = X ln Fs + e ln T + ln d, X = 1/(1 + u^2), u = log10 Pr
+            const double eb = exp(-fp[4] * uT), ec = exp(-T / fp[5]);
+            const double Fs = fp[3] * eb + ec;
+            const double dFs = fp[3] * fp[4] * uT * uT * eb - ec / fp[5];
+            const double u = log10(Pr);
+            const double q = 1. + u * u;
+            const double X = 1. / q;
+            const double lnFs = log(Fs);
+            F = exp(X * lnFs + fp[6] * logT) * (-fp[7]);
+            dlgF_dlgPr = (Pr > 0.) ? -2. * u / (q * q) * lnFs / LN10 : 0.;   // d log10 F / d log10 Pr
+            dlnF_dT = dlgF_dlgPr * (dlnkf - dlnkinf) + X * dFs / Fs + fp[6] * uT;
+        }
+        if (fp[7] > 0.) {    // Troe: log10 F = L / (1 + x^2), x = s / (n - 0.14 s), s = log10 Pr + c
             const double a = fp[3];
             const double e3 = exp(-T / fp[4]), e1 = exp(-T / fp[5]);
             double Fc = (1. - a) * e3 + a * e1;

@@ -41,6 +41,7 @@ RXN_REVERSIBLE = 1
 RXN_NEGATIVE_A = 2
 FALLOFF_LINDEMANN = 0
 FALLOFF_TROE = 1
+FALLOFF_SRI = 2
 
 
 @dataclass
@@ -73,6 +74,7 @@ class Mechanism:
     fo_sp: np.ndarray
     fo_eff_m1: np.ndarray
     name: str = "mechanism"
+    fo_sri: object = None             # [nfo, 5] SRI a, b, c, d, e (rows of FALLOFF_SRI reactions)
     element_names: tuple = ELEMENTS    # columns of `elements`
     atomic_weights: object = None      # [n_elements] kg/kmol (default: C, H, O, N)
     _struct: object = field(default=None, repr=False)
@@ -103,6 +105,9 @@ class Mechanism:
                      "tb_ptr", "tb_sp", "fo_rxn", "fo_type", "fo_single", "fo_ptr", "fo_sp"):
             setattr(self, name, i32(getattr(self, name)))
         self.elements = np.ascontiguousarray(self.elements, dtype=np.float64)
+        if self.fo_sri is None:
+            self.fo_sri = np.tile([0.0, 0.0, 1.0, 1.0, 0.0], (len(self.fo_rxn), 1))
+        self.fo_sri = np.ascontiguousarray(np.asarray(self.fo_sri, dtype=np.float64).reshape(len(self.fo_rxn), 5))
         if self.atomic_weights is None:
             self.atomic_weights = ATOMIC_WEIGHTS
         self.atomic_weights = np.asarray(self.atomic_weights, dtype=np.float64)
@@ -145,6 +150,7 @@ class Mechanism:
             s.fo_inf, s.fo_troe = P(self.fo_inf, d), P(self.fo_troe, d)
             s.fo_single, s.fo_ptr, s.fo_sp = P(self.fo_single, i), P(self.fo_ptr, i), P(self.fo_sp, i)
             s.fo_eff_m1 = P(self.fo_eff_m1, d)
+            s.fo_sri = P(self.fo_sri, d)
             self._struct = s
         return self._struct
 

@@ -266,6 +266,10 @@ static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, const double *
             double F = (t[3] != 0.) ? (1. - t[0]) * exp(-T / t[1]) + t[0] * exp(-T / t[2]) + exp(-t[3] / T)
                                     : (1. - t[0]) * exp(-T / t[1]) + t[0] * exp(-T / t[2]);
             w->logFc[k] = (F < 1.e-300) ? -300. : log10(F);
+        } else if (m->fo_type[k] == LSM_FALLOFF_SRI) {
+            /* hpp:1283-1287: a exp(-b/T) + exp(-T/c), kept in the logFcent slot */
+            const double *t = m->fo_sri + 5 * k;
+            w->logFc[k] = t[0] * exp(-t[1] / T) + exp(-T / t[2]);
         }
     }
     /* ReactionRates(c), hpp:1683-1690: cTot recomputed as sum(c) */
@@ -302,6 +306,12 @@ static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, const double *
                 const double q = 1. / 0.14;
                 wF = pow(10., lf / (1. + q * q));
             }
+        } else if (m->fo_type[k] == LSM_FALLOFF_SRI) {   /* hpp:1538-1544 */
+            const double *t = m->fo_sri + 5 * k;
+            const double lp = log10(Pr);
+            const double xSRI = 1. / (1. + lp * lp);
+            wF = pow(w->logFc[k], xSRI) * t[3];
+            if (t[4] != 0.) wF *= pow(T, t[4]);
         }
         w->kmod[j] *= w->finf[k] * (Pr / (1. + Pr)) * wF / w->kf[j];
     }

@@ -1,9 +1,12 @@
-"""Copies the reference's CHEMKIN input for the H2/CO and H2 mechanisms into fixtures.
+"""Copies the reference's CHEMKIN input for the H2/CO, H2 and CH4 skeletal mechanisms
+into fixtures.
 
 Data only: the kinetics files of run/kinetic-mechanisms/POLIMI_H2CO_1412 and
 POLIMI_H2_1412, and the NASA-7 entries of the species they use, cut out of the
 shared POLIMI_TOT_NOX_1412.CKT thermodynamics file (its THERMO header line and
-global temperature line, the 4-line entries of those species, END).
+global temperature line, the 4-line entries of those species, END); the kinetics
+and thermodynamics files of POLIMI_CH4_SKELETAL_1412 (the mechanism of the
+MethaneCoflowFlame/02-skeletal-mechanism tutorial) as they are.
 
     python tests/golden/make_chemkin_fixtures.py [/root/reference]
 """
@@ -39,6 +42,11 @@ def main(ref="/root/reference"):
     with open(os.path.join(HERE, "POLIMI_H2CO_1412_thermo.CKT"), "w") as f:
         f.write("\n".join(out) + "\n")
     print(f"{len(species)} species: {species}")
+    ch4 = os.path.join(base, "POLIMI_CH4_SKELETAL_1412")
+    for src, dst in (("POLIMI_CH4_SKELETAL_1412.CKI", "POLIMI_CH4_SKELETAL_1412.CKI"),
+                     ("POLIMI_TOT_NOX_1412.CKT", "POLIMI_CH4_SKELETAL_1412_thermo.CKT")):
+        with open(os.path.join(ch4, src)) as f, open(os.path.join(HERE, dst), "w") as g:
+            g.write(f.read())
 
 
 if __name__ == "__main__":

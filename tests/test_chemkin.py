@@ -1,8 +1,9 @@
-"""CHEMKIN reader (laminarsmoke_amd/chemkin.py) on the reference's H2/CO and H2 mechanisms.
+"""CHEMKIN reader (laminarsmoke_amd/chemkin.py) on the reference's H2/CO, H2 and CH4
+skeletal mechanisms.
 
 The fixtures under tests/golden/ are the reference's own input files
-(run/kinetic-mechanisms/POLIMI_H2CO_1412, POLIMI_H2_1412; the thermodynamics file cut
-down to the species used).  What pins the reader:
+(run/kinetic-mechanisms/POLIMI_H2CO_1412, POLIMI_H2_1412 with the thermodynamics file
+cut down to the species used; POLIMI_CH4_SKELETAL_1412 as it is).  What pins the reader:
 
 * structure read off the files by eye: species and reaction counts, which reactions
   are irreversible, third-body, fall-off (Troe / Lindemann), efficiencies;
@@ -11,9 +12,11 @@ down to the species used).  What pins the reader:
   entropy at 298.15 K) - a misplaced coefficient column fails these by far;
 * rate constants against the file's numbers converted with physical unit factors
   (1 cm3/mol = 1e-3 m3/kmol, 1 cal = 4.18443 J as the reference defines it);
+* the SRI fall-off rate of the CH4 skeletal mechanism against the textbook SRI form
+  evaluated from the file's numbers;
 * option and unit handling on small inline mechanisms, and a clear error for the
-  options the kernels do not evaluate (the reference's CH4 skeletal mechanism uses SRI,
-  its one-step global mechanisms use FORD).
+  options the kernels do not evaluate (the reference's one-step global mechanisms
+  use FORD).
 """
 import os
 
@@ -21,7 +24,8 @@ import numpy as np
 import pytest
 
 from laminarsmoke_amd.chemkin import ChemkinError, load_chemkin, read_chemkin, read_thermo
-from laminarsmoke_amd.mechanism import FALLOFF_LINDEMANN, FALLOFF_TROE, RXN_NEGATIVE_A, RXN_REVERSIBLE
+from laminarsmoke_amd.mechanism import (FALLOFF_LINDEMANN, FALLOFF_SRI, FALLOFF_TROE, RXN_NEGATIVE_A,
+                                        RXN_REVERSIBLE)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 THERMO = os.path.join(GOLDEN, "POLIMI_H2CO_1412_thermo.CKT")
@@ -37,6 +41,12 @@ def h2co():
 @pytest.fixture(scope="module")
 def h2():
     return load_chemkin(os.path.join(GOLDEN, "POLIMI_H2_1412.CKI"), THERMO)
+
+
+@pytest.fixture(scope="module")
+def ch4():
+    return load_chemkin(os.path.join(GOLDEN, "POLIMI_CH4_SKELETAL_1412.CKI"),
+                        os.path.join(GOLDEN, "POLIMI_CH4_SKELETAL_1412_thermo.CKT"))
 
 
 def test_h2co_structure(h2co):
@@ -68,7 +78,16 @@ def test_h2_structure(h2):
     assert h2.element_names == ("H", "O", "N")
 
 
-@pytest.mark.parametrize("which", ["h2co", "h2"])
+def test_ch4_structure(ch4):
+    m = ch4
+    assert m.ns == 24 and m.nr == 155 and len(m.tb_rxn) == 9 and len(m.fo_rxn) == 8
+    assert m.species[:3] == ["CH4", "O2", "CO2"] and m.species[-1] == "N2"
+    k = m.fo_type.tolist().index(FALLOFF_SRI)
+    assert m.fo_type.tolist().count(FALLOFF_SRI) == 1 and m.fo_rxn[k] == 20      # H+CH3(+M)=CH4(+M)
+    np.testing.assert_array_equal(m.fo_sri[k], [0.45, 797.0, 979.0, 1.0, 0.0])
+
+
+@pytest.mark.parametrize("which", ["h2co", "h2", "ch4"])
 def test_element_balance(request, which):
     m = request.getfixturevalue(which)
     bal = m.elements.T @ m.stoich_matrix()
@@ -128,6 +147,31 @@ def test_rate_constants(h2co, oracle):
     assert k["kmod"][j] == pytest.approx(kinf * Pr / (1.0 + Pr), rel=1e-12)
 
 
+def test_sri_rate(ch4, oracle):
+    """H+CH3(+M)=CH4(+M) 1.2E15 -0.4 0 / LOW 6.4E23 -1.8 0 / SRI 0.45 797 979 1 0 /
+    efficiencies H2 2, CO 2, CO2 3, H2O 5."""
+    m = ch4
+    j = 20
+    ck = {s: 1e-3 * (1 + 0.1 * i) for i, s in enumerate(m.species)}
+    c = np.array([ck[s] for s in m.species])
+    for T in (800.0, 1500.0, 2300.0):
+        k = oracle.kinetics(m, T, c)
+        kinf = 1.2e15 * 1e-3 * T ** -0.4
+        k0 = 6.4e23 * 1e-6 * T ** -1.8
+        M = c.sum() + ck["H2"] + ck["CO"] + 2.0 * ck["CO2"] + 4.0 * ck["H2O"]
+        Pr = k0 * M / kinf
+        X = 1.0 / (1.0 + np.log10(Pr) ** 2)
+        F = (0.45 * np.exp(-797.0 / T) + np.exp(-T / 979.0)) ** X
+        assert k["kf"][j] == pytest.approx(k0, rel=1e-12)
+        assert k["kmod"][j] == pytest.approx(kinf * Pr / (1.0 + Pr) * F, rel=1e-12)
+
+
+def test_sri_five_parameter_form():
+    m = _small("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0\n LOW/ 1.0E18 -1.0 0.0/\n SRI/ 0.5 100 1000 2.0 0.3/")
+    assert m.fo_type.tolist() == [FALLOFF_SRI]
+    np.testing.assert_array_equal(m.fo_sri[0], [0.5, 100.0, 1000.0, 2.0, 0.3])
+
+
 def test_thermo_reader_fields():
     th = read_thermo(open(THERMO).read())
     coef, (tl, thi, tm), comp = th["AR"]
@@ -160,7 +204,8 @@ def test_coefficients_negative_A_and_collider():
 
 
 @pytest.mark.parametrize("text,msg", [
-    ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0\n LOW/ 1.0E18 -1.0 0.0/\n SRI/ 0.5 100 1000/", "SRI"),
+    ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0\n LOW/ 1.0E18 -1.0 0.0/\n SRI/ 0.5 100 1000 0.0 0.0/", "SRI needs d > 0"),
+    ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0\n HIGH/ 1.0E18 -1.0 0.0/", "HIGH"),
     ("H2+O2=>H2O+O 1.0E12 0.0 0.0\n FORD /H2 1.5/", "FORD"),
     ("H+O2=OH+O 1.0E12 0.0 0.0\n REV/ 1.0E11 0.0 0.0/", "REV"),
     ("H+O2=OH+O 1.0E12 0.0 0.0\n PLOG/ 1.0 1.0E12 0.0 0.0/", "PLOG"),

@@ -1,5 +1,6 @@
-"""GPU: the reference's own H2/CO and H2 mechanisms (CHEMKIN input read by
-laminarsmoke_amd.chemkin) through every kernel, against the oracle.
+"""GPU: the reference's own H2/CO, H2 and CH4 skeletal (24 species, 155 reactions, one
+SRI fall-off) mechanisms (CHEMKIN input read by laminarsmoke_amd.chemkin) through every
+kernel, against the oracle.
 
 * right-hand side and analytic Jacobian against the oracle (central differences of
   the oracle's right-hand side for the Jacobian);
@@ -27,7 +28,10 @@ RTOL, ATOL, DT = 1e-7, 1e-12, 1e-4
 
 
 def mechanism(name):
-    return load_chemkin(os.path.join(GOLDEN, name + ".CKI"), os.path.join(GOLDEN, "POLIMI_H2CO_1412_thermo.CKT"))
+    thermo = os.path.join(GOLDEN, name + "_thermo.CKT")
+    if not os.path.exists(thermo):
+        thermo = os.path.join(GOLDEN, "POLIMI_H2CO_1412_thermo.CKT")
+    return load_chemkin(os.path.join(GOLDEN, name + ".CKI"), thermo)
 
 
 def make_step(mech, method, jacobian):
@@ -37,7 +41,7 @@ def make_step(mech, method, jacobian):
     return api.ChemistryStep(mech, ode, method=method, jacobian=jacobian)
 
 
-@pytest.mark.parametrize("name", ["POLIMI_H2CO_1412", "POLIMI_H2_1412"])
+@pytest.mark.parametrize("name", ["POLIMI_H2CO_1412", "POLIMI_H2_1412", "POLIMI_CH4_SKELETAL_1412"])
 def test_equations_and_jacobian(hip_device, oracle, name):
     mech = mechanism(name)
     y = premixed_cells(mech, 8, seed=3)
@@ -53,7 +57,7 @@ def test_equations_and_jacobian(hip_device, oracle, name):
 
 @pytest.mark.parametrize("method,jac", [("team", "numerical"), ("plain", "numerical"), ("wide", "numerical"),
                                         ("team", "analytic"), ("wide", "analytic")])
-@pytest.mark.parametrize("name", ["POLIMI_H2CO_1412", "POLIMI_H2_1412"])
+@pytest.mark.parametrize("name", ["POLIMI_H2CO_1412", "POLIMI_H2_1412", "POLIMI_CH4_SKELETAL_1412"])
 def test_step(hip_device, oracle, name, method, jac):
     mech = mechanism(name)
     y = premixed_cells(mech, 48, seed=5)
