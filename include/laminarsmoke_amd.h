@@ -53,7 +53,10 @@ typedef struct lsm_ode_parameters {
     double initial_step;               /* -1: solver default                                   */
     int32_t maximum_order;             /* -1: solver default (5)                               */
     int32_t maximum_number_of_steps;   /* -1: solver default (500000)                          */
-    int32_t full_pivoting;             /* 0/1                                                  */
+    int32_t full_pivoting;             /* 0/1: FullPivLU (KernelDense.hpp:209-237).  The plain
+                                          kernel pivots fully; the team and wide kernels
+                                          always pivot by rows (partial pivoting), which changes
+                                          the Newton solves only at rounding level           */
     char linear_algebra[32];           /* "Eigen" (the only value KernelDense accepts)         */
 } lsm_ode_parameters;
 

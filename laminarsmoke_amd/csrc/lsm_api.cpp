@@ -726,12 +726,10 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
 int lsm_set_method(lsm_handle *h, int32_t method) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (method < LSM_METHOD_AUTO || method > LSM_METHOD_WIDE) return fail(LSM_E_INVALID, "unknown method");
-    if (method == LSM_METHOD_TEAM && (h->team_wpb < 1 || h->kp.full_pivoting))
-        return fail(LSM_E_UNSUPPORTED, "team kernel needs ns+1 <= 64, a mechanism image that fits in LDS "
-                                       "and partial pivoting");
-    if (method == LSM_METHOD_WIDE && (h->wide_grid < 1 || h->kp.full_pivoting))
-        return fail(LSM_E_UNSUPPORTED, "wide kernel needs ns+1 <= 176 "
-                                       "and partial pivoting");
+    if (method == LSM_METHOD_TEAM && h->team_wpb < 1)
+        return fail(LSM_E_UNSUPPORTED, "team kernel needs ns+1 <= 64 and a mechanism image that fits in LDS");
+    if (method == LSM_METHOD_WIDE && h->wide_grid < 1)
+        return fail(LSM_E_UNSUPPORTED, "wide kernel needs ns+1 <= 176");
     h->method = method;
     return LSM_OK;
 }
@@ -776,7 +774,6 @@ int lsm_destroy(lsm_handle *h) {
 
 static int pick_method(const lsm_handle *h) {
     if (h->method != LSM_METHOD_AUTO) return h->method;
-    if (h->kp.full_pivoting) return LSM_METHOD_PLAIN;
     if (h->team_wpb > 0) return LSM_METHOD_TEAM;
     return h->wide_grid > 0 ? LSM_METHOD_WIDE : LSM_METHOD_PLAIN;
 }

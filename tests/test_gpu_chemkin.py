@@ -55,6 +55,32 @@ def test_equations_and_jacobian(hip_device, oracle, name):
     check_jacobian(oracle, mech, y, Ja, name)
 
 
+@pytest.mark.parametrize("method", ["auto", "plain"])
+def test_tutorial_settings(hip_device, oracle, method):
+    """The MethaneCoflowFlame/02-skeletal-mechanism tutorial's OdeHomogeneous dictionary
+    (rtol 1e-6, atol 1e-12, maximumOrder 5, fullPivoting true) on its mechanism: `auto`
+    picks the team kernel (row pivoting), `plain` pivots fully like the oracle."""
+    mech = mechanism("POLIMI_CH4_SKELETAL_1412")
+    y = premixed_cells(mech, 48, seed=8)
+    ode = api.ODE_Parameters()
+    ode.SetupFromDictionary('OdeHomogeneous { odeSolver "OpenSMOKE"; relTolerance 1e-6; absTolerance 1e-12; '
+                            'maximumOrder 5; fullPivoting true; }')
+    print(f"tutorial settings, method={method}", flush=True)
+    kw = dict(rtol=1e-6, atol=1e-12, full_pivoting=True)
+    yo, st_o, *_ = oracle.chemistry_step(mech, y, DT, threads=8, **kw)
+    env = oracle_envelope(oracle, mech, y, DT, 1e-6, 1e-12, yo, full_pivoting=True)
+    s = api.ChemistryStep(mech, ode, method=method)
+    try:
+        used = s.launch()["method"]
+        yg, st_g, *_ = s.integrate(y, DT)
+    finally:
+        s.close()
+    assert used == ("team" if method == "auto" else "plain")
+    assert np.array_equal(st_g, st_o)
+    cell, _ = assert_parity(yg, yo, 1e-6, 1e-12, env, what=f"tutorial/{method}")
+    print(f"tutorial/{used}: max ratio {cell.max():.3g} (envelope max {env.max():.3g})", flush=True)
+
+
 @pytest.mark.parametrize("method,jac", [("team", "numerical"), ("plain", "numerical"), ("wide", "numerical"),
                                         ("team", "analytic"), ("wide", "analytic")])
 @pytest.mark.parametrize("name", ["POLIMI_H2CO_1412", "POLIMI_H2_1412", "POLIMI_CH4_SKELETAL_1412"])
