@@ -28,6 +28,16 @@
 //   void jac_times(int src, int dst)
 #pragma once
 
+// Phase timers of the BDF bookkeeping (builds with LSM_PHASE_TIMERS only): the
+// context's add_time(slot, cycles) accumulates them.
+#ifdef LSM_PHASE_TIMERS
+#define LSM_BDF_TIC(v) const unsigned long long v = clock64()
+#define LSM_BDF_TOC(c, i, v) (c).add_time(i, clock64() - (v))
+#else
+#define LSM_BDF_TIC(v)
+#define LSM_BDF_TOC(c, i, v)
+#endif
+
 namespace lsm {
 
 enum VecId {
@@ -438,7 +448,10 @@ __device__ __forceinline__ void driver_step(C &c, const BdfConst &K, BdfState &s
         if (fabs(s.h) < EPS_F * 0.) { s.status = ST_SMALLH; return; }   // min_step_size_ = 0
         s.t = s.tInMeshPoint + s.h;
         if (s.t == s.tInMeshPoint) { s.status = ST_SMALLH; return; }
+        LSM_BDF_TIC(t_mvs);
         multi_value_step(c, K, s, n);
+        LSM_BDF_TOC(c, 10, t_mvs);
+        LSM_BDF_TIC(t_tail);
         if (!s.convOK) {
             s.iterConvergenceFailure++;
             n.conv_failures++;
@@ -492,9 +505,11 @@ __device__ __forceinline__ void driver_step(C &c, const BdfConst &K, BdfState &s
                         if (i <= p + 2) c.x(Z0 + i, e) = c.x(V0 + i, e);
                 });
                 check_constraints(c, Z0, n);
+                LSM_BDF_TOC(c, 11, t_tail);
                 break;
             }
         }
+        LSM_BDF_TOC(c, 11, t_tail);
     }
 }
 

@@ -574,6 +574,7 @@ struct TeamCtx {
     double rho0;                      // constant-volume reactor: density of the current cell
 #ifdef LSM_PHASE_TIMERS
     unsigned long long tp[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    __device__ __forceinline__ void add_time(int i, unsigned long long d) { tp[i] += d; }
     int in_jac = 0;
 #endif
 
@@ -1088,6 +1089,7 @@ struct WideCtx {
     double *J;                // HBM [ne][ne] column-major, J[j*ne + i] = d f_i / d y_j
 #ifdef LSM_PHASE_TIMERS
     unsigned long long tp[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    __device__ __forceinline__ void add_time(int i, unsigned long long d) { tp[i] += d; }
     int in_jac = 0;
 #endif
 
@@ -1764,6 +1766,7 @@ struct PlainCtx {
     int ne, ns, nr;
     double qr;
     double rho0;      // constant-volume reactor: density of the current cell
+    __device__ __forceinline__ void add_time(int, unsigned long long) {}
 
     __device__ PlainCtx(const DevMech &m, const KParams &k, double *work, int64_t slots, int64_t sl)
         : M(m), K(k), S(slots), slot(sl), ne(m.ne), ns(m.ns), nr(m.nr), qr(0.), rho0(0.) {
