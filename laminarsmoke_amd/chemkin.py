@@ -401,7 +401,9 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
             ordr = order + (1.0 if r["third"] else 0.0)
             A = r["A"]
             fl = RXN_NEGATIVE_A if A < 0.0 else 0
-            lnA.append(np.log(abs(A) * 1e3 ** (1.0 - ordr)) if A != 0.0 else -np.inf)
+            # A = 0 is kept as ln(1e-100), like the preprocessor's writer
+            # (preprocessing/PreProcessorKineticsPolicy_CHEMKIN.hpp:2142)
+            lnA.append(np.log(abs(A) * 1e3 ** (1.0 - ordr)) if A != 0.0 else np.log(1e-100))
             beta.append(r["beta"])
             EoR.append(r["E"] * econv / R_J_KMOL)
             if r["third"]:
@@ -416,8 +418,6 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
         flags.append(fl)
         reac.append(sorted((idx[s], nu) for s, nu in r["reac"]))
         prod.append(sorted((idx[s], nu) for s, nu in r["prod"]))
-        if not np.isfinite(lnA[-1]):
-            raise ChemkinError(f"{what}: zero pre-exponential factor")
 
     def csr(lists):
         ptr = np.zeros(len(lists) + 1, dtype=np.int32)

@@ -1,0 +1,409 @@
+"""OpenSMOKE++ ``kinetics.xml`` (the preprocessed mechanism a laminarSMOKE case loads)
+<-> Mechanism tables.
+
+Reader: the sections read by
+  * ThermodynamicsMap::ImportElementsFromXMLFile (maps/ThermodynamicsMap.hpp:84-116):
+    NumberOfElements, NamesOfElements, AtomicComposition (optional here);
+  * ThermodynamicsMap_CHEMKIN::ImportSpeciesFromXMLFile / ImportCoefficientsFromXMLFile
+    (maps/ThermodynamicsMap_CHEMKIN.hpp:359-410): NumberOfSpecies, NamesOfSpecies,
+    Thermodynamics type="NASA" / NASA-coefficients, 18 numbers per species in the
+    order of SetCoefficients (:263-343), which is the order of Mechanism.nasa;
+  * KineticsMap_CHEMKIN::ImportCoefficientsFromXMLFile
+    (maps/KineticsMap_CHEMKIN.hpp:311-1165): Kinetics type="OpenSMOKE"
+    version="04-22-2013", NumberOfReactions, the 1-based index lists (Irreversible,
+    Reversible, Reversible-Thermodynamics, Reversible-Explicit, ThreeBody, FallOff
+    with Lindemann/Troe/SRI, CABR, ...), KineticParameters (Direct lnA /
+    negative-lnA / Beta / E_over_R; ThreeBody per-reaction species and efficiency
+    lists; FallOff k_inf lnA / Beta / E_over_R and per-reaction "lindemann" |
+    "troe n a b c [d]" | "sri n a b c [d e]" followed by "species k" or "thirdbody"
+    and the species / efficiency lists);
+  * StoichiometricMap::ReadFromASCIIFile (maps/StoichiometricMap.hpp:78-254): the
+    species-major lists by stoichiometric class (1, 2, 3, 0.5, other) for reactants
+    (numDir/jDir/valueDir5), products of all reactions (numRevTot/jRevTot/
+    valueRevTot5) and of reversible ones (numRevEq/...), changeOfMoles and the
+    explicit-reaction-order flag.
+Every list is "n v1 ... vn" (OpenSMOKEVector::Load, math/OpenSMOKEVector.hpp:509-520).
+
+Writer: the same layout as the reference preprocessor's
+(preprocessing/PreProcessorKineticsPolicy_CHEMKIN.hpp:1877-2330,
+preprocessing/PreProcessorSpecies.hpp:362-392,
+kernel/kinetics/ReactionPolicy_CHEMKIN.hpp:2584-2620), so that tables read from
+CHEMKIN input can be stored the way laminarSMOKE cases keep them.
+
+Forms the kernels do not evaluate (explicitly reversible, CABR, Chebyshev, PLOG,
+extended fall-off, FIT1, Janev-Langer, Landau-Teller, explicit reaction orders)
+raise KineticsXmlError.
+"""
+from __future__ import annotations
+
+import xml.etree.ElementTree as ET
+
+import numpy as np
+
+from .chemkin import ATOMIC_WEIGHTS
+from .mechanism import FALLOFF_LINDEMANN, FALLOFF_SRI, FALLOFF_TROE, RXN_NEGATIVE_A, RXN_REVERSIBLE, Mechanism
+
+VERSION = "04-22-2013"
+_UNSUPPORTED_LISTS = ("Reversible-Explicit", "CABR", "Chebyshev", "PressureLog", "ExtendedPressureLog",
+                      "ExtendedFallOff", "FIT1", "JanevLanger", "LandauTeller")
+
+
+class KineticsXmlError(ValueError):
+    pass
+
+
+# ----------------------------------------------------------------------------------
+# reading
+# ----------------------------------------------------------------------------------
+class _Tokens:
+    def __init__(self, text):
+        self.t = (text or "").split()
+        self.i = 0
+
+    def word(self):
+        if self.i >= len(self.t):
+            raise KineticsXmlError("unexpected end of a list")
+        self.i += 1
+        return self.t[self.i - 1]
+
+    def vec(self, conv=float):
+        n = int(self.word())
+        return [conv(self.word()) for _ in range(n)]
+
+    def more(self):
+        return self.i < len(self.t)
+
+
+def _node(parent, tag, required=True):
+    n = parent.find(tag)
+    if n is None and required:
+        raise KineticsXmlError(f"<{tag}> not found")
+    return n
+
+
+def _ilist(parent, tag, required=True):
+    n = _node(parent, tag, required)
+    return [] if n is None else _Tokens(n.text).vec(int)
+
+
+def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
+    root = ET.fromstring(text)
+    if root.tag != "opensmoke":
+        root = _node(root, "opensmoke")
+    ns = int(_node(root, "NumberOfSpecies").text.split()[0])
+    species = _node(root, "NamesOfSpecies").text.split()[:ns]
+    if len(species) != ns:
+        raise KineticsXmlError("NamesOfSpecies is shorter than NumberOfSpecies")
+
+    th = _node(root, "Thermodynamics")
+    if th.get("type") != "NASA":
+        raise KineticsXmlError("Thermodynamics type not supported")
+    coef = np.array(_node(th, "NASA-coefficients").text.split(), dtype=np.float64)
+    if coef.size < 18 * ns:
+        raise KineticsXmlError("NASA-coefficients: fewer than 18 numbers per species")
+    nasa = coef[:18 * ns].reshape(ns, 18)
+
+    el_node = root.find("NumberOfElements")
+    if el_node is not None:
+        ne_ = int(el_node.text.split()[0])
+        el_names = tuple(_node(root, "NamesOfElements").text.split()[:ne_])
+        atoms = np.array(_node(root, "AtomicComposition").text.split(), dtype=np.float64)[:ns * ne_].reshape(ns, ne_)
+        aw = np.array([ATOMIC_WEIGHTS.get(e.upper(), 0.0) for e in el_names])
+    else:
+        el_names, atoms, aw = ("C", "H", "O", "N"), np.zeros((ns, 4)), None
+
+    kin = _node(root, "Kinetics")
+    if kin.get("type") != "OpenSMOKE" or kin.get("version") != VERSION:
+        raise KineticsXmlError("The current kinetic scheme is not supported (type/version)")
+    nr = int(_node(kin, "NumberOfReactions").text.split()[0])
+    reversible = set(_ilist(kin, "Reversible-Thermodynamics"))
+    for tag in _UNSUPPORTED_LISTS:
+        n = kin.find(tag)
+        if n is not None and _Tokens(n.text).vec(int):
+            raise KineticsXmlError(f"{tag} reactions are not supported")
+    tb_idx = _ilist(kin, "ThreeBody")
+    fo_node = _node(kin, "FallOff")
+    fo_idx = _Tokens(fo_node.text).vec(int)
+
+    kp = _node(kin, "KineticParameters")
+    direct = _node(kp, "Direct")
+    lnA = np.array(_Tokens(_node(direct, "lnA").text).vec())
+    neg = set(_ilist(direct, "negative-lnA", required=False))
+    beta = np.array(_Tokens(_node(direct, "Beta").text).vec())
+    EoR = np.array(_Tokens(_node(direct, "E_over_R").text).vec())
+    if not (len(lnA) == len(beta) == len(EoR) == nr):
+        raise KineticsXmlError("Direct parameters do not have NumberOfReactions entries")
+    flags = np.zeros(nr, dtype=np.int32)
+    for j in range(1, nr + 1):
+        if j in reversible:
+            flags[j - 1] |= RXN_REVERSIBLE
+        if j in neg:
+            flags[j - 1] |= RXN_NEGATIVE_A
+
+    tb_sp, tb_eff = [], []
+    if tb_idx:
+        nodes = list(_node(kp, "ThreeBody").findall("reaction"))
+        if len(nodes) != len(tb_idx):
+            raise KineticsXmlError("ThreeBody parameters do not match the ThreeBody list")
+        for nd in nodes:
+            t = _Tokens(nd.text)
+            sp = [k - 1 for k in t.vec(int)]
+            ef = [e - 1.0 for e in t.vec()]
+            tb_sp.append(sp)
+            tb_eff.append(ef)
+
+    fo_type, fo_inf, fo_troe, fo_sri, fo_single, fo_sp, fo_eff = [], [], [], [], [], [], []
+    if fo_idx:
+        fk = _node(kp, "FallOff")
+        a = _Tokens(_node(fk, "lnA").text).vec()
+        b = _Tokens(_node(fk, "Beta").text).vec()
+        e = _Tokens(_node(fk, "E_over_R").text).vec()
+        nodes = list(_node(fk, "Parameters").findall("reaction"))
+        if not (len(a) == len(b) == len(e) == len(nodes) == len(fo_idx)):
+            raise KineticsXmlError("FallOff parameters do not match the FallOff list")
+        for k, nd in enumerate(nodes):
+            fo_inf.append((a[k], b[k], e[k]))
+            t = _Tokens(nd.text)
+            kind = t.word()
+            troe, sri = (0.0, 1.0, 1.0, 0.0), (0.0, 0.0, 1.0, 1.0, 0.0)
+            if kind == "lindemann":
+                fo_type.append(FALLOFF_LINDEMANN)
+            elif kind == "troe":
+                c = t.vec()
+                if len(c) not in (3, 4):
+                    raise KineticsXmlError("troe needs 3 or 4 parameters")
+                troe = tuple(c) + ((0.0,) if len(c) == 3 else ())          # KineticsMap_CHEMKIN.hpp:736-746
+                fo_type.append(FALLOFF_TROE)
+            elif kind == "sri":
+                c = t.vec()
+                if len(c) not in (3, 5):
+                    raise KineticsXmlError("sri needs 3 or 5 parameters")
+                sri = tuple(c) + ((1.0, 0.0) if len(c) == 3 else ())       # KineticsMap_CHEMKIN.hpp:749-767
+                fo_type.append(FALLOFF_SRI)
+            else:
+                raise KineticsXmlError(f"unknown fall-off form {kind!r}")
+            fo_troe.append(troe)
+            fo_sri.append(sri)
+            how = t.word()
+            if how == "species":
+                fo_single.append(int(t.word()) - 1)
+                fo_sp.append([])
+                fo_eff.append([])
+            else:
+                fo_single.append(-1)
+                fo_sp.append([k - 1 for k in t.vec(int)])
+                fo_eff.append([v - 1.0 for v in t.vec()])
+
+    st = _node(kin, "Stoichiometry")
+    if st.get("type") != "OpenSMOKE" or st.get("version") != VERSION:
+        raise KineticsXmlError("The current stoichiometric data are not supported")
+    reac, prod = _read_stoichiometry(_Tokens(st.text), ns, nr)
+
+    def csr(lists):
+        ptr = np.zeros(len(lists) + 1, dtype=np.int32)
+        ptr[1:] = np.cumsum([len(x) for x in lists])
+        return ptr
+
+    nfo = len(fo_idx)
+    return Mechanism(
+        species=list(species), elements=atoms, nasa=nasa,
+        lnA=lnA, beta=beta, E_over_R=EoR, rflags=flags,
+        reac_ptr=csr(reac), reac_sp=np.array([s for x in reac for s, _ in x], dtype=np.int32),
+        reac_nu=np.array([v for x in reac for _, v in x], dtype=np.float64),
+        prod_ptr=csr(prod), prod_sp=np.array([s for x in prod for s, _ in x], dtype=np.int32),
+        prod_nu=np.array([v for x in prod for _, v in x], dtype=np.float64),
+        tb_rxn=np.array([j - 1 for j in tb_idx], dtype=np.int32), tb_ptr=csr(tb_sp),
+        tb_sp=np.array([s for x in tb_sp for s in x], dtype=np.int32),
+        tb_eff_m1=np.array([v for x in tb_eff for v in x], dtype=np.float64),
+        fo_rxn=np.array([j - 1 for j in fo_idx], dtype=np.int32), fo_type=np.array(fo_type, dtype=np.int32),
+        fo_inf=np.array(fo_inf, dtype=np.float64).reshape(nfo, 3),
+        fo_troe=np.array(fo_troe, dtype=np.float64).reshape(nfo, 4),
+        fo_single=np.array(fo_single, dtype=np.int32), fo_ptr=csr(fo_sp),
+        fo_sp=np.array([s for x in fo_sp for s in x], dtype=np.int32),
+        fo_eff_m1=np.array([v for x in fo_eff for v in x], dtype=np.float64),
+        fo_sri=np.array(fo_sri, dtype=np.float64).reshape(nfo, 5),
+        name=name, element_names=tuple(e.upper() for e in el_names), atomic_weights=aw,
+    )
+
+
+_CLASS_NU = (1.0, 2.0, 3.0, 0.5)
+
+
+def _read_stoichiometry(t: _Tokens, ns: int, nr: int):
+    """Reaction-major (species, nu) lists from StoichiometricMap's species-major ASCII form."""
+    num = {side: [t.vec(int) for _ in range(5)] for side in ("dir", "revtot", "reveq")}
+    lists = {}
+    for side in ("dir", "revtot", "reveq"):
+        js = [t.vec(int) for _ in range(5)]
+        v5 = t.vec()
+        lists[side] = (js, v5)
+    t.vec()                                   # changeOfMoles (recomputed from the stoichiometry)
+    explicit = t.word() if t.more() else "0"
+    if explicit not in ("0", "false"):
+        raise KineticsXmlError("explicit reaction orders (FORD/RORD) are not supported")
+    out = {}
+    for side in ("dir", "revtot"):
+        per = [dict() for _ in range(nr)]
+        js, v5 = lists[side]
+        for c in range(5):
+            counts = num[side][c]
+            if len(counts) != ns:
+                raise KineticsXmlError("stoichiometry counts do not have NumberOfSpecies entries")
+            q = 0
+            for k in range(ns):
+                for _ in range(counts[k]):
+                    j = js[c][q] - 1
+                    nu = _CLASS_NU[c] if c < 4 else v5[q]
+                    if not (0 <= j < nr):
+                        raise KineticsXmlError("stoichiometry reaction index out of range")
+                    if k in per[j]:
+                        raise KineticsXmlError(f"species {k + 1} listed twice on one side of reaction {j + 1}")
+                    per[j][k] = nu
+                    q += 1
+            if q != len(js[c]):
+                raise KineticsXmlError("stoichiometry index list length does not match its counts")
+        out[side] = [sorted(d.items()) for d in per]
+    return out["dir"], out["revtot"]
+
+
+def load_kinetics_xml(path: str) -> Mechanism:
+    with open(path) as f:
+        return read_kinetics_xml(f.read(), name=path)
+
+
+# ----------------------------------------------------------------------------------
+# writing
+# ----------------------------------------------------------------------------------
+def _vec(values, fmt="{:.16e}", per_line=1):
+    vals = list(values)
+    out = [str(len(vals))]
+    for i in range(0, len(vals), per_line):
+        out.append(" ".join(fmt.format(v) for v in vals[i:i + per_line]))
+    return "\n".join(out)
+
+
+def _ivec(values):
+    vals = [int(v) for v in values]
+    lines = [str(len(vals))]
+    for i in range(0, len(vals), 30):
+        lines.append(" ".join(str(v) for v in vals[i:i + 30]))
+    return "\n".join(lines)
+
+
+def _tag(tag, body, attrs=""):
+    return f"<{tag}{attrs}>\n{body}\n</{tag}>"
+
+
+def _stoichiometry(m: Mechanism) -> str:
+    ns, nr = m.ns, m.nr
+    rev = (m.rflags & RXN_REVERSIBLE) != 0
+
+    def classify(ptr, sp, nu, only=None):
+        entries = [[[] for _ in range(ns)] for _ in range(5)]       # class -> species -> [(j, nu)]
+        for j in range(nr):
+            if only is not None and not only[j]:
+                continue
+            for p in range(ptr[j], ptr[j + 1]):
+                v = float(nu[p])
+                c = _CLASS_NU.index(v) if v in _CLASS_NU else 4
+                entries[c][int(sp[p])].append((j + 1, v))
+        nums = [[len(entries[c][k]) for k in range(ns)] for c in range(5)]
+        js = [[j for k in range(ns) for j, _ in entries[c][k]] for c in range(5)]
+        v5 = [v for k in range(ns) for _, v in entries[4][k]]
+        return nums, js, v5
+
+    d = classify(m.reac_ptr, m.reac_sp, m.reac_nu)
+    t = classify(m.prod_ptr, m.prod_sp, m.prod_nu)
+    e = classify(m.prod_ptr, m.prod_sp, m.prod_nu, only=rev)
+    parts = [_ivec(x) for x in d[0]] + [_ivec(x) for x in t[0]] + [_ivec(x) for x in e[0]]
+    for blk in (d, t, e):
+        parts += [_ivec(x) for x in blk[1]] + [_vec(blk[2])]
+    dn = [float(m.prod_nu[m.prod_ptr[j]:m.prod_ptr[j + 1]].sum() - m.reac_nu[m.reac_ptr[j]:m.reac_ptr[j + 1]].sum())
+          for j in range(nr)]
+    parts += [_vec(dn), "0"]
+    return "\n".join(parts)
+
+
+def write_kinetics_xml(m: Mechanism) -> str:
+    ns, nr = m.ns, m.nr
+    idx = lambda mask: [j + 1 for j in range(nr) if mask[j]]
+    rev = (m.rflags & RXN_REVERSIBLE) != 0
+    fo_of = {int(j): k for k, j in enumerate(m.fo_rxn)}
+    lines = ['<?xml version="1.0" encoding="utf-8"?>', '<opensmoke version="0.1a">']
+    lines.append(_tag("NumberOfElements", str(len(m.element_names))))
+    lines.append(_tag("NamesOfElements", " ".join(m.element_names)))
+    lines.append(_tag("AtomicComposition", "\n".join(" ".join(f"{v:.10g}" for v in row) for row in m.elements)))
+    lines.append(_tag("NumberOfSpecies", str(ns)))
+    lines.append(_tag("NamesOfSpecies", " ".join(m.species)))
+    lines.append(_tag("Thermodynamics", _tag("NASA-coefficients",
+                                             "\n".join(" ".join(f"{v:.16e}" for v in row) for row in m.nasa)),
+                      ' type="NASA"'))
+    k = [f'<Kinetics type="OpenSMOKE" version="{VERSION}">', _tag("NumberOfReactions", str(nr))]
+    k.append(_tag("Irreversible", _ivec(idx(~rev))))
+    k.append(_tag("Reversible", _ivec(idx(rev))))
+    k.append(_tag("Reversible-Thermodynamics", _ivec(idx(rev))))
+    k.append(_tag("Reversible-Explicit", _ivec([])))
+    k.append(_tag("ThreeBody", _ivec(m.tb_rxn + 1)))
+    fo_types = m.fo_type.tolist()
+    k.append("<FallOff>\n" + _ivec(m.fo_rxn + 1) + "\n" +
+             _tag("Lindemann", _ivec([int(m.fo_rxn[i]) + 1 for i, t in enumerate(fo_types) if t == FALLOFF_LINDEMANN])) + "\n" +
+             _tag("Troe", _ivec([int(m.fo_rxn[i]) + 1 for i, t in enumerate(fo_types) if t == FALLOFF_TROE])) + "\n" +
+             _tag("SRI", _ivec([int(m.fo_rxn[i]) + 1 for i, t in enumerate(fo_types) if t == FALLOFF_SRI])) +
+             "\n</FallOff>")
+    k.append("<CABR>\n0\n" + _tag("Lindemann", "0") + "\n" + _tag("Troe", "0") + "\n" + _tag("SRI", "0") + "\n</CABR>")
+    neg = [j + 1 for j in range(nr) if m.rflags[j] & RXN_NEGATIVE_A]
+    direct = "\n".join([_tag("lnA", _vec(m.lnA)), _tag("negative-lnA", _ivec(neg)),
+                        _tag("Beta", _vec(m.beta)), _tag("E_over_R", _vec(m.E_over_R))])
+    kp = [_tag("Direct", direct)]
+    if len(m.tb_rxn):
+        rs = []
+        for s in range(len(m.tb_rxn)):
+            a, b = m.tb_ptr[s], m.tb_ptr[s + 1]
+            body = _ivec(m.tb_sp[a:b] + 1) + "\n" + _vec(m.tb_eff_m1[a:b] + 1.0, per_line=30)
+            rs.append(_tag("reaction", body, f' i="{int(m.tb_rxn[s]) + 1}"'))
+        kp.append(_tag("ThreeBody", "\n".join(rs)))
+    if len(m.fo_rxn):
+        ps = []
+        for s in range(len(m.fo_rxn)):
+            t = fo_types[s]
+            if t == FALLOFF_LINDEMANN:
+                head = "lindemann"
+            elif t == FALLOFF_TROE:
+                tr = list(m.fo_troe[s])
+                head = "troe " + " ".join([str(4 if tr[3] != 0.0 else 3)] + [f"{v:.16e}" for v in tr[:4 if tr[3] != 0.0 else 3]])
+            else:
+                head = "sri 5 " + " ".join(f"{v:.16e}" for v in m.fo_sri[s])
+            if m.fo_single[s] >= 0:
+                tail = f"species {int(m.fo_single[s]) + 1}"
+            else:
+                a, b = m.fo_ptr[s], m.fo_ptr[s + 1]
+                tail = "thirdbody\n" + _ivec(m.fo_sp[a:b] + 1) + "\n" + _vec(m.fo_eff_m1[a:b] + 1.0, per_line=30)
+            ps.append(_tag("reaction", head + "\n" + tail, f' i="{int(m.fo_rxn[s]) + 1}"'))
+        kp.append(_tag("FallOff", "\n".join([_tag("lnA", _vec(m.fo_inf[:, 0])), _tag("Beta", _vec(m.fo_inf[:, 1])),
+                                             _tag("E_over_R", _vec(m.fo_inf[:, 2])),
+                                             _tag("Parameters", "\n".join(ps))])))
+    k.append(_tag("KineticParameters", "\n".join(kp)))
+    k.append(_tag("Stoichiometry", _stoichiometry(m), f' type="OpenSMOKE" version="{VERSION}"'))
+    k.append("</Kinetics>")
+    lines += k
+    lines.append("</opensmoke>")
+    return "\n".join(lines) + "\n"
+
+
+def main(argv=None):
+    """python -m laminarsmoke_amd.kinetics_xml KINETICS.CKI THERMO.CKT OUT.xml"""
+    import sys
+    from .chemkin import load_chemkin
+    args = sys.argv[1:] if argv is None else argv
+    if len(args) != 3:
+        print(main.__doc__)
+        return 2
+    m = load_chemkin(args[0], args[1])
+    with open(args[2], "w") as f:
+        f.write(write_kinetics_xml(m))
+    print(f"{args[2]}: {m.ns} species, {m.nr} reactions")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
