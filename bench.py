@@ -12,6 +12,13 @@ field, which stays resident in HBM.  Other workloads:
   --workload 4: the 160-species / 1000-reaction mechanism (wide kernel, FP32
                 Newton LU with FP64 refinement) on a subsample of its
                 131072-cell field, at rtol 1e-6.
+  --workload ch4: the reference's CH4 skeletal mechanism (POLIMI_CH4_SKELETAL_1412,
+                24 species / 155 reactions, CHEMKIN input from tests/golden) with
+                the OdeHomogeneous settings of its MethaneCoflowFlame tutorial
+                (rtol 1e-6, atol 1e-12, fullPivoting true: team kernel with row
+                pivoting) on a 512x512 coflow field of premixed fuel/air cells
+                (40 % flame band 1200-2200 K with radicals, the rest 300-600 K),
+                dt = 1e-5 s.
 
 Before the timed steps:
   * the launch shape is autotuned on a 16384-cell sample;
@@ -50,14 +57,15 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline(mech, y, n_sample, threads, dt, rtol, atol):
+def cpu_baseline(mech, y, n_sample, threads, dt, rtol, atol, full_pivoting=False):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     pyoracle.build()
     idx = np.sort(np.random.default_rng(2024).choice(y.shape[1], n_sample, replace=False))
     ys = np.ascontiguousarray(y[:, idx])
     t0 = time.perf_counter()
-    _, st, _, cnt = pyoracle.chemistry_step(mech, ys, dt, rtol=rtol, atol=atol, threads=threads)
+    _, st, _, cnt = pyoracle.chemistry_step(mech, ys, dt, rtol=rtol, atol=atol, threads=threads,
+                                            full_pivoting=full_pivoting)
     el = time.perf_counter() - t0
     return dict(value=n_sample / el, unit="cell-integrations/s", cores=threads, kind="port",
                 sample=f"{n_sample} cells drawn at random (seed 2024) from the same field, "
@@ -85,7 +93,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", type=int, default=2, choices=[2, 3, 4])
+    ap.add_argument("--workload", default="2", choices=["2", "3", "4", "ch4"])
     ap.add_argument("--cells", type=int, default=None, help="subsample of the field (default: all of it; "
                                                                "workload 4: 4096)")
     ap.add_argument("--method", default="auto", choices=["team", "plain", "wide", "auto"])
@@ -109,10 +117,26 @@ def main():
     from laminarsmoke_amd import api, workmodel
     from laminarsmoke_amd.fields import workload_cells, WORKLOADS
 
-    w = WORKLOADS[args.workload]
-    rtol, atol, dt = w["rtol"], w["atol"], w["dt"]
-    cells = args.cells if args.cells is not None else (4096 if args.workload == 4 else None)
-    mech, y = workload_cells(args.workload, n=cells)
+    full_pivoting = False
+    if args.workload == "ch4":
+        from laminarsmoke_amd.chemkin import load_chemkin
+        from laminarsmoke_amd.fields import coflow_premixed_field
+        golden = os.path.join(ROOT, "tests", "golden")
+        mech = load_chemkin(os.path.join(golden, "POLIMI_CH4_SKELETAL_1412.CKI"),
+                            os.path.join(golden, "POLIMI_CH4_SKELETAL_1412_thermo.CKT"))
+        w = dict(name="ch4-skeletal-coflow-512x512", rtol=1e-6, atol=1e-12, dt=1e-5)
+        rtol, atol, dt = w["rtol"], w["atol"], w["dt"]
+        full_pivoting = True
+        cells = args.cells
+        y, _ = coflow_premixed_field(mech, 512, 512, 77)
+        if cells is not None and cells < y.shape[1]:
+            y = np.ascontiguousarray(y[:, np.linspace(0, y.shape[1] - 1, cells).astype(np.int64)])
+    else:
+        args.workload = int(args.workload)
+        w = WORKLOADS[args.workload]
+        rtol, atol, dt = w["rtol"], w["atol"], w["dt"]
+        cells = args.cells if args.cells is not None else (4096 if args.workload == 4 else None)
+        mech, y = workload_cells(args.workload, n=cells)
     n = y.shape[1]
     dev = torch.device("cuda:0")
     y_in = torch.from_numpy(y).to(dev)
@@ -127,6 +151,8 @@ def main():
     ode.SetType("OpenSMOKE_GPU")
     ode.SetRelativeTolerance(rtol)
     ode.SetAbsoluteTolerance(atol)
+    if full_pivoting:
+        ode.SetFullPivoting(True)
     if args.step_cap > 0:
         ode.SetMaximumNumberOfSteps(args.step_cap)
     step = api.ChemistryStep(mech, ode, method=args.method, jacobian=args.jacobian)
@@ -196,6 +222,8 @@ def main():
         "config": {"workload": name, "mechanism": f"{mech.ns} species / {mech.nr} reactions", "cells": n,
                    "dt_s": dt, "rtol": rtol, "atol": atol, "method": launch["method"], "jacobian": args.jacobian,
                    "newton_lu": "fp32+refinement" if launch["method"] == "wide" else "fp64",
+                   "pivoting": ("full (plain kernel)" if launch["method"] == "plain" else
+                                "rows (fullPivoting requested)") if full_pivoting else "rows",
                    "waves_per_workgroup": launch["waves_per_block"], "grid": launch["grid"],
                    "visit_order": "cost (previous step's counters)" if use_order else "cell index",
                    "hip_graph": graph is not None, "step_cap": args.step_cap or None},
@@ -214,8 +242,8 @@ def main():
                   "autotune": tune},
     }
     if not args.no_cpu:
-        ns = args.cpu_sample if args.cpu_sample is not None else (8 if args.workload == 4 else 96)
-        out["cpu_baseline"] = cpu_baseline(mech, y, min(ns, n), args.cpu_threads, dt, rtol, atol)
+        ns = args.cpu_sample if args.cpu_sample is not None else {4: 8, "ch4": 4096}.get(args.workload, 96)
+        out["cpu_baseline"] = cpu_baseline(mech, y, min(ns, n), args.cpu_threads, dt, rtol, atol, full_pivoting)
     step.close()
     print(json.dumps(out), flush=True)
 

@@ -128,6 +128,25 @@ def premixed_cells(mech: Mechanism, n: int, seed: int, T_lo: float = 900.0, T_hi
     return out
 
 
+def coflow_premixed_field(mech: Mechanism, nx: int, ny: int, seed: int, flame_fraction: float = 0.4):
+    """The coflow geometry of :func:`coflow_field` for a real (CHEMKIN) mechanism: the
+    flame band holds :func:`premixed_cells` at 1200-2200 K with a radical pool, the rest
+    fresh fuel/air at 300-600 K."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    z = (np.arange(ny) + 0.5) / ny
+    X, Z = np.meshgrid(x, z, indexing="ij")
+    r_f = 0.12 + 0.25 * np.sqrt(Z)
+    phi = -np.abs(X - r_f) + 0.02 * rng.standard_normal(X.shape)
+    flame = (phi >= np.quantile(phi, 1.0 - flame_fraction)).ravel()
+    n = nx * ny
+    y = np.empty((mech.neq, n))
+    nf = int(flame.sum())
+    y[:, flame] = premixed_cells(mech, nf, seed + 1, 1200.0, 2200.0, radical_fraction=1.0)
+    y[:, ~flame] = premixed_cells(mech, n - nf, seed + 2, 300.0, 600.0, radical_fraction=0.0)
+    return y, flame
+
+
 # The five workloads (cells are generated lazily; sizes are the full sizes).
 WORKLOADS = {
     1: dict(name="cfg1-14sp-4096", ns=14, cells=4096, dt=1e-5, rtol=1e-7, atol=1e-12),
