@@ -22,11 +22,13 @@
  *                    constant (hpp:1215-1222, 1737-1742);
  *                    LSM_RXN_NEGATIVE_A: k is multiplied by -1 (hpp:1208-1210).
  *   reac_* / prod_*  reaction-major CSR of (species, coefficient).  The
- *                    coefficient is both the stoichiometric coefficient and the
- *                    reaction order (elementary reactions), as in
- *                    StoichiometricMap (maps/StoichiometricMap.hpp:512-598,
+ *                    coefficient is the stoichiometric coefficient and, unless
+ *                    ord_* gives explicit forward orders, the reaction order,
+ *                    as in StoichiometricMap (maps/StoichiometricMap.hpp:512-598,
  *                    656-730, 916-1001).  A species may appear at most once per
- *                    side.
+ *                    side.  A side with an order below 1 is non-elementary: each
+ *                    of its factors is pow(c, lambda) for lambda >= 1 and the
+ *                    smoothed form of StoichiometricMap.hpp:617-654 below 1.
  *   tb_*             third-body reactions: M = cTot + sum_k (eff_k - 1) c_k
  *                    (KineticsMap_CHEMKIN.hpp:1388-1398); efficiencies are
  *                    stored as eff - 1, like the map does (hpp:655-661).
@@ -92,6 +94,12 @@ typedef struct lsm_mechanism {
     const double *fo_sri;      /* [nfo*5] a, b, c, d, e of LSM_FALLOFF_SRI entries
                                   (d = 1, e = 0 for the 3-parameter form); may be
                                   NULL when no reaction is SRI; d > 0, c != 0   */
+    const int32_t *ord_ptr;    /* [nr+1] explicit forward reaction orders (FORD),
+                                  reaction-major CSR; an empty row means the
+                                  orders are the reactant coefficients; NULL: no
+                                  reaction has explicit orders                  */
+    const int32_t *ord_sp;     /* [ord_ptr[nr]] species                         */
+    const double *ord_val;     /* [ord_ptr[nr]] order lambda                    */
 } lsm_mechanism;
 
 /* Physical constants of the reference (math/PhysicalConstants.h:24). */

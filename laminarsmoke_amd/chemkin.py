@@ -14,11 +14,14 @@ Supported:
 * fall-off reactions (``(+M)`` or ``(+species)``) with ``LOW`` and optional
   3- or 4-parameter ``TROE`` or 3- or 5-parameter ``SRI``, with efficiencies;
 * ``DUPLICATE``; negative pre-exponential factors on elementary reactions;
+* explicit forward reaction orders (``FORD``): the orders are the reactant
+  coefficients with the named species' orders replaced or added
+  (ReactionPolicy_CHEMKIN.hpp:511-545), and the A conversion uses their sum;
 * energy units CAL/MOLE (default), KCAL/MOLE, JOULES/MOLE, KJOULES/MOLE, KELVINS;
   quantity unit MOLES.
 
 Anything else the kernels do not evaluate (``REV``, ``HIGH``, ``PLOG``,
-``CHEB``, ``FORD``/``RORD``, ``LT``, ``MOLECULES``, ...) raises
+``CHEB``, ``RORD``, ``LT``, ``MOLECULES``, ...) raises
 :class:`ChemkinError` naming the reaction, rather than being dropped.
 
 Unit conversions follow ReactionPolicy_CHEMKIN::ConvertUnits /
@@ -58,7 +61,7 @@ E_CONVERSION = {               # J/kmol per unit (ReactionPolicy_CHEMKIN.hpp:148
     "CAL/MOLE": 1e3 * J_FROM_CAL, "KCAL/MOLE": 1e6 * J_FROM_CAL,
     "JOULES/MOLE": 1e3, "KJOULES/MOLE": 1e6, "KELVINS": R_J_KMOL,
 }
-_UNSUPPORTED = {"REV", "HIGH", "PLOG", "CHEB", "TCHEB", "PCHEB", "FORD", "RORD", "LT", "RLT",
+_UNSUPPORTED = {"REV", "HIGH", "PLOG", "CHEB", "TCHEB", "PCHEB", "RORD", "LT", "RLT",
                 "JAN", "FIT1", "TDEP", "EXCI", "MOME", "XSMI", "USRPROG", "LOWMX", "LOWSP", "TROEMX",
                 "TROESP", "SRIMX", "SRISP", "PLOGMX", "PLOGSP", "COLLEFF", "UNITS"}
 
@@ -247,7 +250,7 @@ def _parse_reaction_line(line: str, species_set: set, idx: int):
     if tb_l and co_l:
         raise ChemkinError(f"{what}: both +M and (+M)")
     return dict(eq=eq, what=what, reac=r, prod=p, rev=rev, A=A, beta=beta, E=E,
-                third=tb_l, collider=co_l, low=None, troe=None, sri=None, eff={}, dup=False)
+                third=tb_l, collider=co_l, low=None, troe=None, sri=None, eff={}, ford={}, dup=False)
 
 
 def read_kinetics(text: str):
@@ -304,6 +307,11 @@ def read_kinetics(text: str):
                             if len(vals) not in (3, 5):
                                 raise ChemkinError(f"{cur['what']}: SRI needs 3 or 5 values")
                             cur["sri"] = [float(v) for v in vals] + ([1.0, 0.0] if len(vals) == 3 else [])
+                        elif kw == "FORD":
+                            # FORD /species order/ (ReactionPolicy_CHEMKIN.hpp:511-545)
+                            if len(vals) != 2 or vals[0] not in sset:
+                                raise ChemkinError(f"{cur['what']}: FORD needs a species and an order")
+                            cur["ford"][vals[0]] = float(vals[1])
                         elif kw in _UNSUPPORTED:
                             raise ChemkinError(f"{cur['what']}: the {kw} option is not supported")
                         elif kw in sset:
@@ -358,9 +366,16 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
     tb_rxn, tb_sp, tb_eff = [], [], []
     fo_rxn, fo_type, fo_inf, fo_troe, fo_single, fo_sp, fo_eff = [], [], [], [], [], [], []
     fo_sri = []
+    ord_rows = []
     for j, r in enumerate(reactions):
         what = r["what"]
-        order = sum(nu for _, nu in r["reac"])
+        lam = dict(r["reac"])
+        if r["ford"]:
+            lam.update(r["ford"])
+            ord_rows.append(sorted((idx[s], v) for s, v in lam.items()))
+        else:
+            ord_rows.append([])
+        order = sum(lam.values())             # sumLambdaReactants_ (ReactionPolicy_CHEMKIN.hpp:1360-1366)
         falloff = r["collider"] is not None
         if falloff and r["low"] is None:
             raise ChemkinError(f"{what}: (+M) reaction without LOW (HIGH/chemically activated is not supported)")
@@ -443,6 +458,8 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
         fo_eff_m1=np.array([e for x in fo_eff for e in x], dtype=np.float64),
         name=name, element_names=tuple(el_names), atomic_weights=aw,
         fo_sri=np.array(fo_sri, dtype=np.float64).reshape(nfo, 5),
+        ord_ptr=csr(ord_rows), ord_sp=np.array([s for x in ord_rows for s, _ in x], dtype=np.int32),
+        ord_val=np.array([v for x in ord_rows for _, v in x], dtype=np.float64),
     )
 
 

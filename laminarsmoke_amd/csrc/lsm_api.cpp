@@ -361,6 +361,16 @@ static int check_mechanism(const lsm_mechanism *m) {
         if (seen[j]) return fail(LSM_E_INVALID, "reaction listed twice among third-body / fall-off reactions");
         seen[j] = 1;
     }
+    if (m->ord_ptr) {
+        if (m->ord_ptr[0] != 0) return fail(LSM_E_INVALID, "ord_ptr[0] must be 0");
+        for (int j = 0; j < m->nr; j++) {
+            if (m->ord_ptr[j + 1] < m->ord_ptr[j]) return fail(LSM_E_INVALID, "ord_ptr must not decrease");
+            if (m->ord_ptr[j + 1] > m->ord_ptr[j] && (!m->ord_sp || !m->ord_val))
+                return fail(LSM_E_INVALID, "explicit orders without ord_sp / ord_val");
+            for (int q = m->ord_ptr[j]; q < m->ord_ptr[j + 1]; q++)
+                if (m->ord_sp[q] < 0 || m->ord_sp[q] >= m->ns) return fail(LSM_E_INVALID, "order species index");
+        }
+    }
     return LSM_OK;
 }
 
@@ -378,6 +388,33 @@ struct Image {
         return (int)off;
     }
 };
+
+// Forward orders of reaction j: the explicit orders when given, else the reactant
+// coefficients (sorted by species); and whether each side is non-elementary (an
+// order below 1 takes the smoothed form, StoichiometricMap.hpp:256-300, 617-654).
+static std::vector<std::pair<int, double>> forward_orders(const lsm_mechanism *m, int j) {
+    std::vector<std::pair<int, double>> o;
+    if (m->ord_ptr && m->ord_ptr[j + 1] > m->ord_ptr[j])
+        for (int q = m->ord_ptr[j]; q < m->ord_ptr[j + 1]; q++) o.push_back({m->ord_sp[q], m->ord_val[q]});
+    else
+        for (int q = m->reac_ptr[j]; q < m->reac_ptr[j + 1]; q++) o.push_back({m->reac_sp[q], m->reac_nu[q]});
+    std::stable_sort(o.begin(), o.end(), [](const std::pair<int, double> &a, const std::pair<int, double> &b) {
+        return a.first < b.first;
+    });
+    return o;
+}
+static bool explicit_orders(const lsm_mechanism *m, int j) {
+    return m->ord_ptr && m->ord_ptr[j + 1] > m->ord_ptr[j];
+}
+static bool smooth_forward(const lsm_mechanism *m, int j) {
+    for (auto &e : forward_orders(m, j)) if (e.second < 1.) return true;
+    return false;
+}
+static bool smooth_reverse(const lsm_mechanism *m, int j) {
+    if (!(m->rflags[j] & LSM_RXN_REVERSIBLE)) return false;
+    for (int q = m->prod_ptr[j]; q < m->prod_ptr[j + 1]; q++) if (m->prod_nu[q] < 1.) return true;
+    return false;
+}
 
 static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, const std::vector<int> &tb_of,
                         const std::vector<int> &fo_of, int ns_pad, LdsImage &L, std::vector<char> &out) {
@@ -413,10 +450,14 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
         auto bysp = [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; };
         std::stable_sort(re.begin(), re.end(), bysp);
         std::stable_sort(pr.begin(), pr.end(), bysp);
-        if (re.size() > 255 || pr.size() > 255 || ent.size() + re.size() + pr.size() > 65535) return false;
+        const bool xo = explicit_orders(m, j), sf = smooth_forward(m, j), sb = smooth_reverse(m, j);
+        const std::vector<std::pair<int, double>> od = xo ? forward_orders(m, j) : std::vector<std::pair<int, double>>();
+        if (re.size() > 255 || pr.size() > 255 || od.size() > 255 ||
+            ent.size() + re.size() + pr.size() + od.size() > 65535) return false;
         W[4 * jn + 3] = (unsigned)ent.size() | ((unsigned)re.size() << 16) | ((unsigned)pr.size() << 24);
         for (auto &e : re) ent.push_back((unsigned)e.first | (code_of(e.second) << 12));
         for (auto &e : pr) ent.push_back((unsigned)e.first | (code_of(e.second) << 12));
+        for (auto &e : od) ent.push_back((unsigned)e.first | (code_of(e.second) << 12));
         // fixed-format record: integer orders, at most 3 species per side (multiplicity counted)
         auto pack = [&](const std::vector<std::pair<int, double>> &side, unsigned &word) {
             std::vector<unsigned> sl;
@@ -430,11 +471,14 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
             return true;
         };
         unsigned wr = 0, wp = 0;
-        const bool fast = pack(re, wr) && pack(pr, wp);
+        const bool fast = !xo && !sf && !sb && pack(re, wr) && pack(pr, wp);
         W[4 * jn + 1] = wr;
         W[4 * jn + 2] = wp;
-        W[4 * jn + 0] = (unsigned)(m->rflags[j] & 3) | (fast ? 4u : 0u) | ((unsigned)(tb_of[j] + 1) << 8) |
-                        ((unsigned)(fo_of[j] + 1) << 16);
+        // bits: 0 reversible, 1 negative A, 2 fixed format, 3/4 non-elementary forward/reverse
+        // side, 8-15 third-body slot + 1, 16-23 fall-off slot + 1, 24-31 explicit orders
+        W[4 * jn + 0] = (unsigned)(m->rflags[j] & 3) | (fast ? 4u : 0u) | (sf ? 8u : 0u) | (sb ? 16u : 0u) |
+                        ((unsigned)(tb_of[j] + 1) << 8) | ((unsigned)(fo_of[j] + 1) << 16) |
+                        ((unsigned)od.size() << 24);
     }
     if (nuv.size() > (1u << 19)) return false;
     std::vector<unsigned> tbp(m->ntb + 1), tbs;
@@ -535,7 +579,8 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     for (int jn = 0; jn < nr; jn++) {
         const int j = perm[jn];
         rA[jn] = m->lnA[j]; rB[jn] = m->beta[j]; rE[jn] = m->E_over_R[j];
-        rinfo[jn] = m->rflags[j]; rtb[jn] = tb_of[j]; rfo[jn] = fo_of[j];
+        rinfo[jn] = (m->rflags[j] & 3) | (smooth_forward(m, j) ? 8 : 0) | (smooth_reverse(m, j) ? 16 : 0);
+        rtb[jn] = tb_of[j]; rfo[jn] = fo_of[j];
         std::vector<std::pair<int, double>> re, pr;
         double dn = 0.;
         for (int q = m->reac_ptr[j]; q < m->reac_ptr[j + 1]; q++) { re.push_back({m->reac_sp[q], m->reac_nu[q]}); }
@@ -547,7 +592,7 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
         std::stable_sort(re.begin(), re.end(), bysp);
         std::stable_sort(pr.begin(), pr.end(), bysp);
         fw_ptr[jn] = (int)fw_sp.size();
-        for (auto &e : re) { fw_sp.push_back(e.first); fw_nu.push_back(e.second); }
+        for (auto &e : forward_orders(m, j)) { fw_sp.push_back(e.first); fw_nu.push_back(e.second); }
         bw_ptr[jn] = (int)bw_sp.size();
         for (auto &e : pr) { bw_sp.push_back(e.first); bw_nu.push_back(e.second); }
         std::vector<std::pair<int, double>> mx;

@@ -122,6 +122,32 @@ __device__ __forceinline__ double conc_pow(double c, double nu) {
     return pow(c, nu);
 }
 
+// A factor c^lambda of a non-elementary side (an order below 1 on that side):
+// StoichiometricMap::ProductOfConcentrationsForNonElementaryReactions
+// (StoichiometricMap.hpp:617-654) - c^lambda for lambda >= 1; below 1 a tanh blend of
+// (c + m/1e9)^lambda and the line c*^(lambda-1) c through 0, c* = 1e-8 kmol/m3.
+__device__ __forceinline__ double order_pow(double c, double lambda) {
+    if (lambda >= 1.) return pow(c, lambda);
+    const double Cs = 1.e-8, ALFA = 1.e-5;
+    const double H = 1.50 * log(ALFA / (1. - ALFA));
+    const double K = 2.00 * log((1. - ALFA) / ALFA) / Cs;
+    const double m = (tanh(K * c + H) + 1.) / 2.;
+    return m * pow(c + m / 1.e9, lambda) + (1. - m) * pow(Cs, lambda - 1.) * c;
+}
+
+// d order_pow / dc
+__device__ __forceinline__ double order_pow_der(double c, double lambda) {
+    if (lambda >= 1.) return lambda == 1. ? 1. : ((c > 0. || lambda > 1.) ? lambda * pow(c, lambda - 1.) : 0.);
+    const double Cs = 1.e-8, ALFA = 1.e-5;
+    const double H = 1.50 * log(ALFA / (1. - ALFA));
+    const double K = 2.00 * log((1. - ALFA) / ALFA) / Cs;
+    const double th = tanh(K * c + H);
+    const double m = (th + 1.) / 2., dm = 0.5 * K * (1. - th * th);
+    const double a = c + m / 1.e9;
+    const double csl = pow(Cs, lambda - 1.);
+    return dm * pow(a, lambda) + m * lambda * pow(a, lambda - 1.) * (1. + dm / 1.e9) - dm * csl * c + (1. - m) * csl;
+}
+
 // Net rate of reaction j [kmol/m3/s].  C, H, S return c_k, h_k/RT, s_k/R.
 template <class FC, class FH, class FS>
 __device__ __forceinline__ double reaction_rate(const DevMech &M, int j, double T, double logT, double uT,
@@ -177,7 +203,9 @@ __device__ __forceinline__ double reaction_rate(const DevMech &M, int j, double 
     }
     // products of concentrations (StoichiometricMap.hpp:512-598)
     double pf = 1.;
-    for (int q = M.fw_ptr[j]; q < M.fw_ptr[j + 1]; q++) pf *= conc_pow(C(M.fw_sp[q]), M.fw_nu[q]);
+    const bool sf = info & 8, sb = info & 16;     // non-elementary sides
+    for (int q = M.fw_ptr[j]; q < M.fw_ptr[j + 1]; q++)
+        pf *= sf ? order_pow(C(M.fw_sp[q]), M.fw_nu[q]) : conc_pow(C(M.fw_sp[q]), M.fw_nu[q]);
     double net = pf;
     if (info & LSM_RXN_REVERSIBLE) {
         // reaction dH/RT, dS/R (StoichiometricMap.hpp:916-1001) and 1/Kc (hpp:1215-1222)
@@ -190,7 +218,8 @@ __device__ __forceinline__ double reaction_rate(const DevMech &M, int j, double 
         }
         const double uKeq = exp(-ds + dh - logPRT * M.rdn[j]);
         double pb = 1.;
-        for (int q = M.bw_ptr[j]; q < M.bw_ptr[j + 1]; q++) pb *= conc_pow(C(M.bw_sp[q]), M.bw_nu[q]);
+        for (int q = M.bw_ptr[j]; q < M.bw_ptr[j + 1]; q++)
+            pb *= sb ? order_pow(C(M.bw_sp[q]), M.bw_nu[q]) : conc_pow(C(M.bw_sp[q]), M.bw_nu[q]);
         net -= pb * uKeq;
     }
     return net * kmod;
@@ -309,13 +338,18 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
         acc(p2, r);
         return;
     }
-    // general stoichiometry (any orders, any number of species)
+    // general stoichiometry (any orders, any number of species); explicit forward
+    // orders, when given, follow the product entries
     const unsigned lst = w.w;
     const unsigned e0 = lst & 0xffffu, nre = (lst >> 16) & 0xffu, npr = lst >> 24;
+    const unsigned nord = info >> 24;
+    const bool sf = info & 8u, sb = info & 16u;
+    const unsigned fb = nord ? e0 + nre + npr : e0, fn = nord ? nord : nre;
     double pf = 1.;
-    for (unsigned q = 0; q < nre; q++) {
-        const unsigned e = L.ent[e0 + q];
-        pf *= conc_pow(scg[e & 0xfffu].x, L.nu(e));
+    for (unsigned q = 0; q < fn; q++) {
+        const unsigned e = L.ent[fb + q];
+        const double cq = scg[e & 0xfffu].x;
+        pf *= sf ? order_pow(cq, L.nu(e)) : conc_pow(cq, L.nu(e));
     }
     double net = pf;
     if (info & 1u) {
@@ -329,7 +363,7 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
             const double2 cg = scg[e & 0xfffu];
             const double nu = L.nu(e);
             dg += nu * cg.y;
-            pb *= conc_pow(cg.x, nu);
+            pb *= sb ? order_pow(cg.x, nu) : conc_pow(cg.x, nu);
         }
         net -= pb * exp(dg - logPRT * ed.y);
     }
@@ -491,13 +525,18 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
         }
         return;
     }
-    // general stoichiometry: d(prod c^nu)/dc_i = nu_i c_i^(nu_i - 1) prod_{others}
+    // general stoichiometry: d(prod f(c))/dc_i = f'(c_i) prod_{others} f(c), f = c^nu or,
+    // on a non-elementary side, order_pow; explicit forward orders follow the products
     const unsigned lst = w.w;
     const unsigned e0 = lst & 0xffffu, nre = (lst >> 16) & 0xffu, npr = lst >> 24;
+    const unsigned nord = info >> 24;
+    const bool sf = info & 8u, sb = info & 16u;
+    const unsigned fb = nord ? e0 + nre + npr : e0, fn = nord ? nord : nre;
     double pf = 1., pb = 1., dg = 0., dh = 0., uK = 0.;
-    for (unsigned q = 0; q < nre; q++) {
-        const unsigned e = L.ent[e0 + q];
-        pf *= conc_pow(scg[e & 0xfffu].x, L.nu(e));
+    for (unsigned q = 0; q < fn; q++) {
+        const unsigned e = L.ent[fb + q];
+        const double cq = scg[e & 0xfffu].x;
+        pf *= sf ? order_pow(cq, L.nu(e)) : conc_pow(cq, L.nu(e));
     }
     if (info & 1u) {
         for (unsigned q = 0; q < nre + npr; q++) {
@@ -505,7 +544,7 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
             const double nu = q < nre ? -L.nu(e) : L.nu(e);
             dg += nu * scg[e & 0xfffu].y;
             dh += nu * sH[e & 0xfffu];
-            if (q >= nre) pb *= conc_pow(scg[e & 0xfffu].x, nu);
+            if (q >= nre) pb *= sb ? order_pow(scg[e & 0xfffu].x, nu) : conc_pow(scg[e & 0xfffu].x, nu);
         }
         uK = exp(dg - logPRT * ed.y);
     }
@@ -517,17 +556,24 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
         const double nu = L.nu(e);
         acc.R(e & 0xfffu, q < nre ? -nu * r : nu * r, q < nre ? -nu * drdT : nu * drdT);
     }
-    for (unsigned d = 0; d < nre + npr; d++) {
-        const bool fwd = d < nre;
+    for (unsigned d = 0; d < fn + npr; d++) {
+        const bool fwd = d < fn;
         if (!fwd && !(info & 1u)) break;
-        const unsigned ed_ = L.ent[e0 + d];
+        const unsigned base = fwd ? fb : e0 + nre;          // entries of this side
+        const unsigned cnt = fwd ? fn : npr, dd = fwd ? d : d - fn;
+        const bool sm = fwd ? sf : sb;
+        const unsigned ed_ = L.ent[base + dd];
         const unsigned id = ed_ & 0xfffu;
         const double nud = L.nu(ed_), cd = scg[id].x;
         // d(c^nu)/dc; at c = 0 a fractional order has no finite slope: 0 is used
-        double der = nud == 1. ? 1. : ((cd > 0. || nud > 1.) ? nud * conc_pow(cd, nud - 1.) : 0.);
-        const unsigned qa = fwd ? 0 : nre, qb = fwd ? nre : nre + npr;
-        for (unsigned q = qa; q < qb; q++)
-            if (q != d) der *= conc_pow(scg[L.ent[e0 + q] & 0xfffu].x, L.nu(L.ent[e0 + q]));
+        double der = sm ? order_pow_der(cd, nud)
+                        : (nud == 1. ? 1. : ((cd > 0. || nud > 1.) ? nud * conc_pow(cd, nud - 1.) : 0.));
+        for (unsigned q = 0; q < cnt; q++)
+            if (q != dd) {
+                const unsigned eq = L.ent[base + q];
+                const double cq = scg[eq & 0xfffu].x;
+                der *= sm ? order_pow(cq, L.nu(eq)) : conc_pow(cq, L.nu(eq));
+            }
         const double gv = fwd ? kmod * der : -kmod * uK * der;
         for (unsigned o = 0; o < nre + npr; o++) {
             const unsigned eo = L.ent[e0 + o];

@@ -30,9 +30,10 @@ preprocessing/PreProcessorSpecies.hpp:362-392,
 kernel/kinetics/ReactionPolicy_CHEMKIN.hpp:2584-2620), so that tables read from
 CHEMKIN input can be stored the way laminarSMOKE cases keep them.
 
-Forms the kernels do not evaluate (explicitly reversible, CABR, Chebyshev, PLOG,
-extended fall-off, FIT1, Janev-Langer, Landau-Teller, explicit reaction orders)
-raise KineticsXmlError.
+Explicit forward reaction orders (the lambda lists after the flag) are read into
+Mechanism.ord_*; explicit reverse orders that differ from the product coefficients,
+and forms the kernels do not evaluate (explicitly reversible, CABR, Chebyshev, PLOG,
+extended fall-off, FIT1, Janev-Langer, Landau-Teller), raise KineticsXmlError.
 """
 from __future__ import annotations
 
@@ -197,7 +198,7 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
     st = _node(kin, "Stoichiometry")
     if st.get("type") != "OpenSMOKE" or st.get("version") != VERSION:
         raise KineticsXmlError("The current stoichiometric data are not supported")
-    reac, prod = _read_stoichiometry(_Tokens(st.text), ns, nr)
+    reac, prod, orders = _read_stoichiometry(_Tokens(st.text), ns, nr, reversible)
 
     def csr(lists):
         ptr = np.zeros(len(lists) + 1, dtype=np.int32)
@@ -222,6 +223,8 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
         fo_sp=np.array([s for x in fo_sp for s in x], dtype=np.int32),
         fo_eff_m1=np.array([v for x in fo_eff for v in x], dtype=np.float64),
         fo_sri=np.array(fo_sri, dtype=np.float64).reshape(nfo, 5),
+        ord_ptr=csr(orders), ord_sp=np.array([s for x in orders for s, _ in x], dtype=np.int32),
+        ord_val=np.array([v for x in orders for _, v in x], dtype=np.float64),
         name=name, element_names=tuple(e.upper() for e in el_names), atomic_weights=aw,
     )
 
@@ -229,8 +232,10 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
 _CLASS_NU = (1.0, 2.0, 3.0, 0.5)
 
 
-def _read_stoichiometry(t: _Tokens, ns: int, nr: int):
-    """Reaction-major (species, nu) lists from StoichiometricMap's species-major ASCII form."""
+def _read_stoichiometry(t: _Tokens, ns: int, nr: int, reversible):
+    """Reaction-major (species, nu) lists from StoichiometricMap's species-major ASCII form,
+    and the explicit forward orders of the reactions whose orders differ from their
+    reactant coefficients."""
     num = {side: [t.vec(int) for _ in range(5)] for side in ("dir", "revtot", "reveq")}
     lists = {}
     for side in ("dir", "revtot", "reveq"):
@@ -239,10 +244,15 @@ def _read_stoichiometry(t: _Tokens, ns: int, nr: int):
         lists[side] = (js, v5)
     t.vec()                                   # changeOfMoles (recomputed from the stoichiometry)
     explicit = t.word() if t.more() else "0"
-    if explicit not in ("0", "false"):
-        raise KineticsXmlError("explicit reaction orders (FORD/RORD) are not supported")
+    if explicit not in ("0", "false", "1", "true"):
+        raise KineticsXmlError("bad explicit-reaction-order flag")
+    if explicit in ("1", "true"):             # lambda lists (StoichiometricMap.hpp:194-247)
+        num["ldir"] = [t.vec(int) for _ in range(5)]
+        num["lrev"] = [t.vec(int) for _ in range(5)]
+        lists["ldir"] = ([t.vec(int) for _ in range(5)], t.vec())
+        lists["lrev"] = ([t.vec(int) for _ in range(5)], t.vec())
     out = {}
-    for side in ("dir", "revtot"):
+    for side in [s for s in ("dir", "revtot", "reveq", "ldir", "lrev") if s in lists]:
         per = [dict() for _ in range(nr)]
         js, v5 = lists[side]
         for c in range(5):
@@ -263,7 +273,15 @@ def _read_stoichiometry(t: _Tokens, ns: int, nr: int):
             if q != len(js[c]):
                 raise KineticsXmlError("stoichiometry index list length does not match its counts")
         out[side] = [sorted(d.items()) for d in per]
-    return out["dir"], out["revtot"]
+    orders = [[] for _ in range(nr)]
+    if "ldir" in out:
+        for j in range(nr):
+            if out["ldir"][j] != out["dir"][j]:
+                orders[j] = out["ldir"][j]
+            rev = out["reveq"][j] if (j + 1) in reversible else []
+            if (j + 1) in reversible and out["lrev"][j] != rev:
+                raise KineticsXmlError(f"explicit reverse orders (RORD) of reaction {j + 1} are not supported")
+    return out["dir"], out["revtot"], orders
 
 
 def load_kinetics_xml(path: str) -> Mechanism:
@@ -320,7 +338,26 @@ def _stoichiometry(m: Mechanism) -> str:
         parts += [_ivec(x) for x in blk[1]] + [_vec(blk[2])]
     dn = [float(m.prod_nu[m.prod_ptr[j]:m.prod_ptr[j + 1]].sum() - m.reac_nu[m.reac_ptr[j]:m.reac_ptr[j + 1]].sum())
           for j in range(nr)]
-    parts += [_vec(dn), "0"]
+    parts += [_vec(dn)]
+    if m.ord_ptr[-1] == 0:
+        parts.append("0")
+    else:
+        # lambda lists: forward orders (explicit where given, else the coefficients), reverse
+        # orders = the product coefficients of reversible reactions
+        optr, osp, oval = [0], [], []
+        for j in range(nr):
+            a, b = (m.ord_ptr[j], m.ord_ptr[j + 1]) if m.ord_ptr[j + 1] > m.ord_ptr[j] else (None, None)
+            if a is None:
+                osp += m.reac_sp[m.reac_ptr[j]:m.reac_ptr[j + 1]].tolist()
+                oval += m.reac_nu[m.reac_ptr[j]:m.reac_ptr[j + 1]].tolist()
+            else:
+                osp += m.ord_sp[a:b].tolist()
+                oval += m.ord_val[a:b].tolist()
+            optr.append(len(osp))
+        ld = classify(np.array(optr), np.array(osp), np.array(oval))
+        parts.append("1")
+        parts += [_ivec(x) for x in ld[0]] + [_ivec(x) for x in e[0]]
+        parts += [_ivec(x) for x in ld[1]] + [_vec(ld[2])] + [_ivec(x) for x in e[1]] + [_vec(e[2])]
     return "\n".join(parts)
 
 

@@ -75,6 +75,9 @@ class Mechanism:
     fo_eff_m1: np.ndarray
     name: str = "mechanism"
     fo_sri: object = None             # [nfo, 5] SRI a, b, c, d, e (rows of FALLOFF_SRI reactions)
+    ord_ptr: object = None            # [nr+1] explicit forward orders (FORD), CSR; empty row: orders = coefficients
+    ord_sp: object = None
+    ord_val: object = None
     element_names: tuple = ELEMENTS    # columns of `elements`
     atomic_weights: object = None      # [n_elements] kg/kmol (default: C, H, O, N)
     _struct: object = field(default=None, repr=False)
@@ -108,6 +111,11 @@ class Mechanism:
         if self.fo_sri is None:
             self.fo_sri = np.tile([0.0, 0.0, 1.0, 1.0, 0.0], (len(self.fo_rxn), 1))
         self.fo_sri = np.ascontiguousarray(np.asarray(self.fo_sri, dtype=np.float64).reshape(len(self.fo_rxn), 5))
+        if self.ord_ptr is None:
+            self.ord_ptr, self.ord_sp, self.ord_val = np.zeros(self.nr + 1), [], []
+        self.ord_ptr = np.ascontiguousarray(self.ord_ptr, dtype=np.int32)
+        self.ord_sp = np.ascontiguousarray(self.ord_sp, dtype=np.int32)
+        self.ord_val = np.ascontiguousarray(self.ord_val, dtype=np.float64)
         if self.atomic_weights is None:
             self.atomic_weights = ATOMIC_WEIGHTS
         self.atomic_weights = np.asarray(self.atomic_weights, dtype=np.float64)
@@ -151,6 +159,7 @@ class Mechanism:
             s.fo_single, s.fo_ptr, s.fo_sp = P(self.fo_single, i), P(self.fo_ptr, i), P(self.fo_sp, i)
             s.fo_eff_m1 = P(self.fo_eff_m1, d)
             s.fo_sri = P(self.fo_sri, d)
+            s.ord_ptr, s.ord_sp, s.ord_val = P(self.ord_ptr, i), P(self.ord_sp, i), P(self.ord_val, d)
             self._struct = s
         return self._struct
 

@@ -46,6 +46,12 @@ typedef struct {
      * rx_* for dH/dS (signed nu), fw_* reactants, bw_* products               */
     int *rx_ptr, *rx_sp, *fw_ptr, *fw_sp, *bw_ptr, *bw_sp;
     double *rx_nu, *fw_nu, *bw_nu;
+    /* forward reaction orders (the reactant list, or the explicit FORD orders) and
+     * the non-elementary flags of each side (an order below 1,
+     * StoichiometricMap.hpp:256-300) */
+    int *od_ptr, *od_sp;
+    double *od_nu;
+    char *sm_f, *sm_b;
 } mech_t;
 
 static int nu_class(double nu) {
@@ -62,6 +68,7 @@ static void mech_free(mech_t *M) {
     free(M->fptr); free(M->frx); free(M->fnu); free(M->fcls);
     free(M->rx_ptr); free(M->rx_sp); free(M->rx_nu); free(M->fw_ptr); free(M->fw_sp);
     free(M->fw_nu); free(M->bw_ptr); free(M->bw_sp); free(M->bw_nu);
+    free(M->od_ptr); free(M->od_sp); free(M->od_nu); free(M->sm_f); free(M->sm_b);
 }
 
 /* stable sort of (sp, nu) pairs by species */
@@ -150,6 +157,23 @@ static void mech_init(mech_t *M, const lsm_mechanism *m) {
         sort_pairs(ex - M->rx_ptr[j], M->rx_sp + M->rx_ptr[j], M->rx_nu + M->rx_ptr[j]);
     }
     M->fw_ptr[nr] = ef; M->bw_ptr[nr] = eb; M->rx_ptr[nr] = ex;
+    const int nod = m->ord_ptr ? m->ord_ptr[nr] : 0;
+    M->od_ptr = malloc(sizeof(int) * (nr + 1));
+    M->od_sp = malloc(sizeof(int) * (nre + nod + 1)); M->od_nu = malloc(sizeof(double) * (nre + nod + 1));
+    M->sm_f = calloc(nr + 1, 1); M->sm_b = calloc(nr + 1, 1);
+    int eo = 0;
+    for (int j = 0; j < nr; j++) {
+        M->od_ptr[j] = eo;
+        if (m->ord_ptr && m->ord_ptr[j + 1] > m->ord_ptr[j]) {
+            for (int p = m->ord_ptr[j]; p < m->ord_ptr[j + 1]; p++) { M->od_sp[eo] = m->ord_sp[p]; M->od_nu[eo++] = m->ord_val[p]; }
+        } else {
+            for (int p = M->fw_ptr[j]; p < M->fw_ptr[j + 1]; p++) { M->od_sp[eo] = M->fw_sp[p]; M->od_nu[eo++] = M->fw_nu[p]; }
+        }
+        sort_pairs(eo - M->od_ptr[j], M->od_sp + M->od_ptr[j], M->od_nu + M->od_ptr[j]);
+        for (int p = M->od_ptr[j]; p < eo; p++) if (M->od_nu[p] < 1.) M->sm_f[j] = 1;
+        for (int p = M->bw_ptr[j]; p < M->bw_ptr[j + 1]; p++) if (M->bw_nu[p] < 1.) M->sm_b[j] = 1;
+    }
+    M->od_ptr[nr] = eo;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -220,6 +244,20 @@ static double conc_pow(double c1, double nu) {
     case 4: return sqrt(c1);
     default: return pow(c1, nu);
     }
+}
+
+/* One factor of the product of concentrations of a non-elementary side:
+ * StoichiometricMap::ProductOfConcentrationsForNonElementaryReactions
+ * (StoichiometricMap.hpp:617-654): c^lambda for lambda >= 1, below 1 a blend of
+ * (c + m/delta)^lambda and a linear branch through 0 around c* = 1e-8 kmol/m3. */
+static double order_pow(double C, double lambda) {
+    if (lambda >= 1.) return pow(C, lambda);
+    const double Cstar = 1.e-8, ALFA = 1.e-5;
+    const double H = 1.50 * log(ALFA / (1. - ALFA));
+    const double K = 2.00 * log((1. - ALFA) / ALFA) / Cstar;
+    const double delta = 1.e9;
+    const double mm = (tanh(K * C + H) + 1.) / 2.;
+    return mm * pow(C + mm / delta, lambda) + (1. - mm) * pow(Cstar, lambda - 1.) * C;
 }
 
 /* h, s must already hold h/RT and s/R at T. */
@@ -318,9 +356,11 @@ static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, const double *
     /* ProductOfConcentrations StoichiometricMap.hpp:512-598 (species order) */
     for (int j = 0; j < nr; j++) {
         double pf = 1., pb = 1.;
-        for (int p = M->fw_ptr[j]; p < M->fw_ptr[j + 1]; p++) pf *= conc_pow(c[M->fw_sp[p]], M->fw_nu[p]);
+        for (int p = M->od_ptr[j]; p < M->od_ptr[j + 1]; p++)
+            pf *= M->sm_f[j] ? order_pow(c[M->od_sp[p]], M->od_nu[p]) : conc_pow(c[M->od_sp[p]], M->od_nu[p]);
         if (m->rflags[j] & LSM_RXN_REVERSIBLE)
-            for (int p = M->bw_ptr[j]; p < M->bw_ptr[j + 1]; p++) pb *= conc_pow(c[M->bw_sp[p]], M->bw_nu[p]);
+            for (int p = M->bw_ptr[j]; p < M->bw_ptr[j + 1]; p++)
+                pb *= M->sm_b[j] ? order_pow(c[M->bw_sp[p]], M->bw_nu[p]) : conc_pow(c[M->bw_sp[p]], M->bw_nu[p]);
         w->pf[j] = pf; w->pb[j] = pb;
     }
     /* hpp:1737-1767 */

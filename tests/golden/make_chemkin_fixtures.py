@@ -6,7 +6,9 @@ POLIMI_H2_1412, and the NASA-7 entries of the species they use, cut out of the
 shared POLIMI_TOT_NOX_1412.CKT thermodynamics file (its THERMO header line and
 global temperature line, the 4-line entries of those species, END); the kinetics
 and thermodynamics files of POLIMI_CH4_SKELETAL_1412 (the mechanism of the
-MethaneCoflowFlame/02-skeletal-mechanism tutorial) as they are.
+MethaneCoflowFlame/02-skeletal-mechanism tutorial) as they are; the one-step global
+mechanisms GLOBAL_CH4_1step and GLOBAL_H2_1step with their thermodynamics cut to
+their species.
 
     python tests/golden/make_chemkin_fixtures.py [/root/reference]
 """
@@ -42,6 +44,26 @@ def main(ref="/root/reference"):
     with open(os.path.join(HERE, "POLIMI_H2CO_1412_thermo.CKT"), "w") as f:
         f.write("\n".join(out) + "\n")
     print(f"{len(species)} species: {species}")
+    # one-step global mechanisms (FORD orders), their thermodynamics cut to their species
+    for mech in ("GLOBAL_CH4_1step", "GLOBAL_H2_1step"):
+        text = open(os.path.join(base, mech, mech + ".CKI")).read()
+        with open(os.path.join(HERE, mech + ".CKI"), "w") as f:
+            f.write(text)
+        sp = read_kinetics(text)[1]
+        lines = open(os.path.join(base, mech, mech + ".CKT")).read().splitlines()
+        h = next(i for i, ln in enumerate(lines) if ln.strip().upper().startswith("THERMO"))
+        out = lines[h:h + 2]
+        i = h + 2
+        while i < len(lines):
+            ln = lines[i]
+            if len(ln) >= 80 and ln[79] == "1" and ln[:18].split()[0].upper() in sp:
+                out += lines[i:i + 4]
+                i += 4
+            else:
+                i += 1
+        out.append("END")
+        with open(os.path.join(HERE, mech + "_thermo.CKT"), "w") as f:
+            f.write("\n".join(out) + "\n")
     ch4 = os.path.join(base, "POLIMI_CH4_SKELETAL_1412")
     for src, dst in (("POLIMI_CH4_SKELETAL_1412.CKI", "POLIMI_CH4_SKELETAL_1412.CKI"),
                      ("POLIMI_TOT_NOX_1412.CKT", "POLIMI_CH4_SKELETAL_1412_thermo.CKT")):

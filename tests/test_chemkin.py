@@ -14,9 +14,12 @@ cut down to the species used; POLIMI_CH4_SKELETAL_1412 as it is).  What pins the
   (1 cm3/mol = 1e-3 m3/kmol, 1 cal = 4.18443 J as the reference defines it);
 * the SRI fall-off rate of the CH4 skeletal mechanism against the textbook SRI form
   evaluated from the file's numbers;
+* the one-step global mechanisms (explicit FORD orders): the orders, the A conversion
+  with the order sum, and the rate k [CH4][O2];
+* orders below 1 (fractional coefficients or FORD) take the reference's smoothed
+  form: (c + 1e-9)^lambda well above 1.5e-5 kmol/m3, linear in c well below 9e-6;
 * option and unit handling on small inline mechanisms, and a clear error for the
-  options the kernels do not evaluate (the reference's one-step global mechanisms
-  use FORD).
+  options the kernels do not evaluate.
 """
 import os
 
@@ -206,7 +209,7 @@ def test_coefficients_negative_A_and_collider():
 @pytest.mark.parametrize("text,msg", [
     ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0\n LOW/ 1.0E18 -1.0 0.0/\n SRI/ 0.5 100 1000 0.0 0.0/", "SRI needs d > 0"),
     ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0\n HIGH/ 1.0E18 -1.0 0.0/", "HIGH"),
-    ("H2+O2=>H2O+O 1.0E12 0.0 0.0\n FORD /H2 1.5/", "FORD"),
+    ("H2+O2=>H2O+O 1.0E12 0.0 0.0\n RORD /H2 1.5/", "RORD"),
     ("H+O2=OH+O 1.0E12 0.0 0.0\n REV/ 1.0E11 0.0 0.0/", "REV"),
     ("H+O2=OH+O 1.0E12 0.0 0.0\n PLOG/ 1.0 1.0E12 0.0 0.0/", "PLOG"),
     ("H+O2=OH+XX 1.0E12 0.0 0.0", "unknown species"),
@@ -235,3 +238,35 @@ def test_integration_conserves_elements_and_enthalpy(h2co, oracle):
     H0 = (y0 / m.mw) @ h0 * R * 1200.0
     Hf = (yf[:m.ns, 0] / m.mw) @ hf * R * yf[-1, 0]
     assert abs(Hf - H0) <= 1e-6 * abs(cp0 @ (y0 / m.mw)) * R * 1200.0
+
+
+@pytest.mark.parametrize("name,fuel,a", [("GLOBAL_CH4_1step", "CH4", 1.0e15), ("GLOBAL_H2_1step", "H2", 1.0e15)])
+def test_global_mechanisms(oracle, name, fuel, a):
+    m = load_chemkin(os.path.join(GOLDEN, name + ".CKI"), os.path.join(GOLDEN, name + "_thermo.CKT"))
+    assert m.nr == 1 and m.rflags[0] == 0
+    orders = dict(zip([m.species[k] for k in m.ord_sp], m.ord_val))
+    assert orders == {fuel: 1.0, "O2": 1.0}
+    assert m.lnA[0] == pytest.approx(np.log(a * 1e-3), rel=1e-15)       # orders sum to 2: cm3/mol/s
+    T = 1500.0
+    c = np.linspace(2e-3, 4e-3, m.ns)
+    k = oracle.kinetics(m, T, c)
+    kf = a * 1e-3 * np.exp(-20000.0 * CAL / (R * T))
+    cf, co = c[m.species.index(fuel)], c[m.species.index("O2")]
+    assert k["r"][0] == pytest.approx(kf * cf * co, rel=1e-12)
+
+
+def test_orders_below_one_are_smoothed(oracle):
+    """A + 0.5 B => C: the B factor is (c + 1e-9)^0.5 at c = 1e-2 and linear in c at 1e-12."""
+    m = _small("H2+0.5O2=>H2O 1.0E10 0.0 0.0")
+    sp = m.species.index
+    T = 1000.0
+    c = np.full(m.ns, 1e-2)
+    r_hi = oracle.kinetics(m, T, c)["r"][0]
+    kf = 1.0e10 * 10 ** (-3 * 0.5) * 1.0      # order 1.5: (cm3/mol)^0.5 -> (m3/kmol)^0.5
+    assert r_hi == pytest.approx(kf * 1e-2 * (1e-2 + 1e-9) ** 0.5, rel=1e-12)
+    assert abs(r_hi / (kf * 1e-2 * np.sqrt(1e-2)) - 1.0) > 1e-8
+    c[sp("O2")] = 1e-12
+    r1 = oracle.kinetics(m, T, c)["r"][0]
+    c[sp("O2")] = 2e-12
+    r2 = oracle.kinetics(m, T, c)["r"][0]
+    assert r2 / r1 == pytest.approx(2.0, rel=1e-6)

@@ -1,6 +1,7 @@
 """GPU: the reference's own H2/CO, H2 and CH4 skeletal (24 species, 155 reactions, one
-SRI fall-off) mechanisms (CHEMKIN input read by laminarsmoke_amd.chemkin) through every
-kernel, against the oracle.
+SRI fall-off) mechanisms and its one-step global CH4 and H2 mechanisms (explicit FORD
+orders) - CHEMKIN input read by laminarsmoke_amd.chemkin - through every kernel, against
+the oracle.
 
 * right-hand side and analytic Jacobian against the oracle (central differences of
   the oracle's right-hand side for the Jacobian);
@@ -41,7 +42,8 @@ def make_step(mech, method, jacobian):
     return api.ChemistryStep(mech, ode, method=method, jacobian=jacobian)
 
 
-@pytest.mark.parametrize("name", ["POLIMI_H2CO_1412", "POLIMI_H2_1412", "POLIMI_CH4_SKELETAL_1412"])
+@pytest.mark.parametrize("name", ["POLIMI_H2CO_1412", "POLIMI_H2_1412", "POLIMI_CH4_SKELETAL_1412", "GLOBAL_CH4_1step",
+                                  "GLOBAL_H2_1step"])
 def test_equations_and_jacobian(hip_device, oracle, name):
     mech = mechanism(name)
     y = premixed_cells(mech, 8, seed=3)
@@ -83,7 +85,8 @@ def test_tutorial_settings(hip_device, oracle, method):
 
 @pytest.mark.parametrize("method,jac", [("team", "numerical"), ("plain", "numerical"), ("wide", "numerical"),
                                         ("team", "analytic"), ("wide", "analytic")])
-@pytest.mark.parametrize("name", ["POLIMI_H2CO_1412", "POLIMI_H2_1412", "POLIMI_CH4_SKELETAL_1412"])
+@pytest.mark.parametrize("name", ["POLIMI_H2CO_1412", "POLIMI_H2_1412", "POLIMI_CH4_SKELETAL_1412", "GLOBAL_CH4_1step",
+                                  "GLOBAL_H2_1step"])
 def test_step(hip_device, oracle, name, method, jac):
     mech = mechanism(name)
     y = premixed_cells(mech, 48, seed=5)

@@ -9,6 +9,8 @@
   checked against the values written into the file.
 * Tables written and read back are identical, bit for bit, for the reference's CHEMKIN
   mechanisms and the synthetic workload mechanisms.
+* Explicit forward orders (FORD of the one-step global mechanisms) travel in the
+  lambda lists and come back as the same order tables.
 * Forms the kernels do not evaluate are refused.
 No kinetics.xml file ships with the reference (its mechanisms are CHEMKIN input), so
 the format is pinned by the reader code it follows, not by a reference file.
@@ -140,13 +142,14 @@ def test_hand_written_file():
 def _same(a, b):
     for f in ("nasa", "lnA", "beta", "E_over_R", "rflags", "reac_ptr", "reac_sp", "reac_nu", "prod_ptr", "prod_sp",
               "prod_nu", "tb_rxn", "tb_ptr", "tb_sp", "tb_eff_m1", "fo_rxn", "fo_type", "fo_inf", "fo_troe",
-              "fo_single", "fo_ptr", "fo_sp", "fo_eff_m1", "fo_sri", "elements"):
+              "fo_single", "fo_ptr", "fo_sp", "fo_eff_m1", "fo_sri", "ord_ptr", "ord_sp", "ord_val", "elements"):
         x, y = getattr(a, f), getattr(b, f)
         assert x.shape == y.shape and np.array_equal(x, y), f
     assert a.species == b.species
 
 
-@pytest.mark.parametrize("name", ["POLIMI_H2CO_1412", "POLIMI_H2_1412", "POLIMI_CH4_SKELETAL_1412", 14, 53, 160])
+@pytest.mark.parametrize("name", ["POLIMI_H2CO_1412", "POLIMI_H2_1412", "POLIMI_CH4_SKELETAL_1412", "GLOBAL_CH4_1step",
+                                  "GLOBAL_H2_1step", 14, 53, 160])
 def test_round_trip(name):
     if isinstance(name, int):
         m = workload_mechanism(name)
@@ -161,7 +164,7 @@ def test_round_trip(name):
 @pytest.mark.parametrize("old,new,msg", [
     ("<Reversible-Explicit> 0 </Reversible-Explicit>", "<Reversible-Explicit> 1 3 </Reversible-Explicit>",
      "Reversible-Explicit"),
-    ("0.7\n0\n</Stoichiometry>", "0.7\n1\n</Stoichiometry>", "explicit reaction orders"),
+    ("0.7\n0\n</Stoichiometry>", "0.7\n1\n</Stoichiometry>", "unexpected end"),   # flag without lambda lists
     ('version="04-22-2013">\n<NumberOfReactions>', 'version="01-01-2000">\n<NumberOfReactions>', "not supported"),
 ])
 def test_refused(old, new, msg):

@@ -83,12 +83,14 @@ def test_arrhenius_and_negative_A(oracle):
 
 
 def test_reaction_orders(oracle):
-    """2A + C -> B: rate k c_A^2 c_C; 0.5-order handled as sqrt."""
+    """2A + C -> B: rate k c_A^2 c_C.  An order below 1 makes the side non-elementary
+    (StoichiometricMap.hpp:256-300): its factor is (c + 1e-9)^0.5 at c = 0.7, where the
+    smoothing weight is 1 (StoichiometricMap.hpp:617-654), not sqrt(c)."""
     m = _two_species_mech(reac=[(0, 2), (2, 1)], prod=[(1, 1)], lnA=np.log(11.0), reversible=False)
     c = np.array([0.7, 0.1, 0.3])
     assert oracle.kinetics(m, 1000.0, c)["r"][0] == pytest.approx(11.0 * 0.49 * 0.3, rel=1e-14)
     m = _two_species_mech(reac=[(0, 0.5)], prod=[(1, 0.5)], lnA=np.log(2.0), reversible=False)
-    assert oracle.kinetics(m, 1000.0, c)["r"][0] == pytest.approx(2.0 * np.sqrt(0.7), rel=1e-14)
+    assert oracle.kinetics(m, 1000.0, c)["r"][0] == pytest.approx(2.0 * np.sqrt(0.7 + 1e-9), rel=1e-14)
 
 
 @pytest.mark.parametrize("T", [700.0, 1600.0, 2900.0])
