@@ -252,6 +252,29 @@ int lsm_equations(lsm_handle *h, int64_t n_cells, const double *y, double *f, do
 int lsm_equations_device(lsm_handle *h, int64_t n_cells, const double *d_y, double *d_f, double *d_jac,
                          void *stream);
 
+/* Thermodynamic part of the per-cell property refresh that follows the chemistry
+ * step in every PIMPLE iteration (solvers/laminarSMOKE/properties.H:41-131): mole
+ * fractions X, MWmix, cTot = p/(RT), psi = cTot MWmix/p, h, cp, cv and the species
+ * cp (CpSpecies, with massDiffusionInEnergyEquation), per cell.  The transport part of
+ * that loop (mu, lambda, Dmix) is not provided.
+ *   y: SoA [ns+1][n_cells] (Y, then T);  p: [n_cells] pressure, or NULL for the
+ *   handle's reactor pressure;  props: [LSM_NPROPS][n_cells];  x, cp_species:
+ *   [ns][n_cells] or NULL.  Host variant synchronous; _device takes device pointers
+ *   and a hipStream_t and does not synchronise. */
+enum {
+    LSM_PROP_MW = 0,      /* kg/kmol                          */
+    LSM_PROP_CTOT = 1,    /* kmol/m3                          */
+    LSM_PROP_PSI = 2,     /* s2/m2 (rho = psi p)              */
+    LSM_PROP_H = 3,       /* J/kg                             */
+    LSM_PROP_CP = 4,      /* J/kg/K                           */
+    LSM_PROP_CV = 5,      /* J/kg/K                           */
+    LSM_NPROPS = 6
+};
+int lsm_properties(lsm_handle *h, int64_t n_cells, const double *y, const double *p, double *props, double *x,
+                   double *cp_species);
+int lsm_properties_device(lsm_handle *h, int64_t n_cells, const double *d_y, const double *d_p, double *d_props,
+                          double *d_x, double *d_cp_species, void *stream);
+
 /* Text of the last error of this thread ("" if none). */
 const char *lsm_last_error(void);
 

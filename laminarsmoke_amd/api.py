@@ -61,6 +61,8 @@ def lib():
         L.lsm_integrate_batch_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_integrate_batch_device_ordered.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_integrate_batch_rho.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.lsm_properties.argtypes = [vp, i64, vp, vp, vp, vp, vp]
+        L.lsm_properties_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.lsm_integrate_batch_device_rho.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_cost_order.argtypes = [vp, i64, vp, vp, vp]
         L.lsm_autotune.argtypes = [vp, i64, vp, vp, i64, vp, vp]
@@ -204,6 +206,26 @@ class ChemistryStep:
             rho = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, dtype=np.float64), (n,)))
         _check(lib().lsm_integrate_batch_rho(self.h, n, p(dt), p(rho), p(y), p(yout), p(st), p(qa), p(cnt)))
         return yout, st, qa, cnt
+
+    PROPS = ("MW", "cTot", "psi", "h", "cp", "cv")
+
+    def properties(self, y, p=None, species=True):
+        """Host arrays: y[ns+1, n] -> (props[6, n] rows MW, cTot, psi, h, cp, cv; x[ns, n]; cp_k[ns, n])."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        neq, n = y.shape
+        if neq != self.mech.neq:
+            raise ValueError(f"y has {neq} rows, mechanism needs {self.mech.neq}")
+        out = np.empty((6, n))
+        x = np.empty((self.mech.ns, n)) if species else None
+        cpk = np.empty((self.mech.ns, n)) if species else None
+        if p is not None:
+            p = np.ascontiguousarray(np.broadcast_to(np.asarray(p, dtype=np.float64), (n,)))
+        q = lambda a: a.ctypes.data if a is not None else None
+        _check(lib().lsm_properties(self.h, n, q(y), q(p), q(out), q(x), q(cpk)))
+        return out, x, cpk
+
+    def properties_device(self, n, y_ptr, props_ptr, p_ptr=None, x_ptr=None, cpk_ptr=None, stream=None):
+        _check(lib().lsm_properties_device(self.h, int(n), y_ptr, p_ptr, props_ptr, x_ptr, cpk_ptr, stream))
 
     def integrate_device(self, n, dt_ptr, y_in_ptr, y_out_ptr, status_ptr=None, q_ptr=None, counters_ptr=None,
                          stream=None, order_ptr=None, rho_ptr=None):

@@ -940,6 +940,51 @@ int lsm_integrate_batch_rho(lsm_handle *h, int64_t n, const double *dt, const do
     return LSM_OK;
 }
 
+int lsm_properties_device(lsm_handle *h, int64_t n, const double *d_y, const double *d_p, double *d_props, double *d_x,
+                          double *d_cpk, void *stream) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
+    if (n == 0) return LSM_OK;
+    if (!d_y || !d_props) return fail(LSM_E_INVALID, "null y / props pointer");
+    if (launch_properties(h->dm, h->kp.pressure, n, d_y, d_p, d_props, d_x, d_cpk, h->num_cu, stream))
+        return fail(LSM_E_HIP, std::string("properties kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
+    return LSM_OK;
+}
+
+int lsm_properties(lsm_handle *h, int64_t n, const double *y, const double *p, double *props, double *x, double *cpk) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
+    if (n == 0) return LSM_OK;
+    if (!y || !props) return fail(LSM_E_INVALID, "null y / props pointer");
+    const size_t ns = (size_t)h->dm.ns;
+    double *dy = nullptr, *dp = nullptr, *dout = nullptr, *dx = nullptr, *dc = nullptr;
+    auto release = [&]() {
+        for (double *q : {dy, dp, dout, dx, dc}) if (q) (void)hipFree(q);
+    };
+    int rc = LSM_OK;
+    if (hipMalloc(&dy, sizeof(double) * (ns + 1) * n) != hipSuccess ||
+        hipMalloc(&dout, sizeof(double) * LSM_NPROPS * n) != hipSuccess ||
+        (p && hipMalloc(&dp, sizeof(double) * n) != hipSuccess) ||
+        (x && hipMalloc(&dx, sizeof(double) * ns * n) != hipSuccess) ||
+        (cpk && hipMalloc(&dc, sizeof(double) * ns * n) != hipSuccess)) {
+        release();
+        return fail(LSM_E_NOMEM, "properties buffers");
+    }
+    if (hipMemcpy(dy, y, sizeof(double) * (ns + 1) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        (p && hipMemcpy(dp, p, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(LSM_E_HIP, "copy inputs");
+    if (!rc) rc = lsm_properties_device(h, n, dy, dp, dout, dx, dc, nullptr);
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = fail(LSM_E_HIP, "properties kernel failed");
+    if (!rc && hipMemcpy(props, dout, sizeof(double) * LSM_NPROPS * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(LSM_E_HIP, "copy props");
+    if (!rc && x && hipMemcpy(x, dx, sizeof(double) * ns * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(LSM_E_HIP, "copy x");
+    if (!rc && cpk && hipMemcpy(cpk, dc, sizeof(double) * ns * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(LSM_E_HIP, "copy cp_species");
+    release();
+    return rc;
+}
+
 int lsm_equations_device(lsm_handle *h, int64_t n, const double *d_y, double *d_f, double *d_jac, void *stream) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (n < 0) return fail(LSM_E_INVALID, "negative cell count");

@@ -104,6 +104,8 @@ int launch_plain(const DevMech &m, const KParams &k, int64_t n, const double *dt
 size_t plain_work_doubles(const DevMech &m, int64_t slots);
 // Reactor right-hand side f (SoA [ne][n]) and analytic Jacobian J ([n][ne][ne]) at
 // the states yin; either output may be null.  Uses the team workspace jws.
+int launch_properties(const DevMech &m, double p_default, int64_t n, const double *y, const double *p, double *out,
+                      double *x, double *cpk, int num_cu, void *stream);
 int launch_equations(const DevMech &m, const KParams &k, int64_t n, const double *yin, double *f, double *J,
                      int grid, int waves_per_block, double *jws, void *stream);
 int launch_reset_queue(unsigned *queue, void *stream);

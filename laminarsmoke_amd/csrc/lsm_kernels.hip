@@ -2086,6 +2086,59 @@ int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt,
     return -2;
 }
 
+// Thermodynamic part of the per-cell property refresh (solvers/laminarSMOKE/properties.H:41-131):
+// one thread per cell, species rows streamed from HBM (two passes: MW, then the mole
+// fractions and the mixture sums).  out rows: MW [kg/kmol], cTot = p/(RT) [kmol/m3],
+// psi = cTot MW / p [s2/m2], h [J/kg], cp [J/kg/K], cv = (cp_molar - R)/MW [J/kg/K];
+// x (may be null): mole fractions; cpk (may be null): species cp_k / MW_k [J/kg/K].
+__global__ void __launch_bounds__(256) properties_kernel(DevMech M, double p_default, int64_t n,
+                                                         const double *__restrict__ y, const double *__restrict__ p,
+                                                         double *__restrict__ out, double *__restrict__ x,
+                                                         double *__restrict__ cpk) {
+    const double *sp = M.sp;
+    const int P = M.ns_pad, ns = M.ns;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const double T = y[(int64_t)ns * n + c];
+        const double pc = p ? p[c] : p_default;
+        double u = 0.;
+        for (int k = 0; k < ns; k++) u += y[(int64_t)k * n + c] / sp[SP_MW * P + k];
+        const double MW = 1. / u;
+        const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, uT = 1. / T;
+        double cpx = 0., hx = 0.;
+        for (int k = 0; k < ns; k++) {
+            const double mwk = sp[SP_MW * P + k];
+            const double xk = (MW * y[(int64_t)k * n + c]) / mwk;
+            const bool hi = T > sp[SP_TMID * P + k];
+            const int rc = hi ? SP_CP_HT : SP_CP_LT, rh = hi ? SP_DH_HT : SP_DH_LT;
+            const double cp = sp[rc * P + k] + T * sp[(rc + 1) * P + k] + T2 * sp[(rc + 2) * P + k] +
+                              T3 * sp[(rc + 3) * P + k] + T4 * sp[(rc + 4) * P + k];
+            const double h = sp[rh * P + k] + T * sp[(rh + 1) * P + k] + T2 * sp[(rh + 2) * P + k] +
+                             T3 * sp[(rh + 3) * P + k] + T4 * sp[(rh + 4) * P + k] + uT * sp[(rh + 5) * P + k];
+            cpx += xk * cp;
+            hx += xk * h;
+            if (x) x[(int64_t)k * n + c] = xk;
+            if (cpk) cpk[(int64_t)k * n + c] = (cp * RGAS) / mwk;
+        }
+        const double cTot = pc / RGAS / T;
+        out[c] = MW;
+        out[n + c] = cTot;
+        out[2 * n + c] = cTot * MW / pc;
+        out[3 * n + c] = (hx * RGAS * T) / MW;
+        out[4 * n + c] = (cpx * RGAS) / MW;
+        out[5 * n + c] = (cpx * RGAS - RGAS) / MW;
+    }
+}
+
+int launch_properties(const DevMech &m, double p_default, int64_t n, const double *y, const double *p, double *out,
+                      double *x, double *cpk, int num_cu, void *stream) {
+    if (n <= 0) return 0;
+    const int64_t blocks = (n + 255) / 256;
+    const int g = (int)(blocks < (int64_t)num_cu * 64 ? blocks : (int64_t)num_cu * 64);
+    hipLaunchKernelGGL(properties_kernel, dim3(g), dim3(256), 0, (hipStream_t)stream, m, p_default, n, y, p, out, x,
+                       cpk);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_equations(const DevMech &m, const KParams &k, int64_t n, const double *yin, double *f, double *J,
                      int grid, int wpb, double *jws, void *stream) {
     if (wpb < 1 || wpb > 4 || m.ne > 64) return -2;
