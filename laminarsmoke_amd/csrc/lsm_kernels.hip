@@ -2101,10 +2101,12 @@ __global__ void __launch_bounds__(256) properties_kernel(DevMech M, double p_def
         const double T = y[(int64_t)ns * n + c];
         const double pc = p ? p[c] : p_default;
         double u = 0.;
+#pragma unroll 4
         for (int k = 0; k < ns; k++) u += y[(int64_t)k * n + c] / sp[SP_MW * P + k];
         const double MW = 1. / u;
         const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, uT = 1. / T;
         double cpx = 0., hx = 0.;
+#pragma unroll 4
         for (int k = 0; k < ns; k++) {
             const double mwk = sp[SP_MW * P + k];
             const double xk = (MW * y[(int64_t)k * n + c]) / mwk;
