@@ -18,8 +18,10 @@
  *                    reactions these are the LOW-pressure (k0) parameters, as
  *                    after the preprocessor swap
  *                    (kernel/kinetics/ReactionPolicy_CHEMKIN.hpp:618-631).
- *   rflags           LSM_RXN_REVERSIBLE: reverse rate from the equilibrium
- *                    constant (hpp:1215-1222, 1737-1742);
+ *   rflags           LSM_RXN_REVERSIBLE: reversible reaction, reverse rate constant
+ *                    k/Kc from the equilibrium constant (hpp:1215-1222, 1737-1742),
+ *                    or, with LSM_RXN_EXPLICIT_REV too, the explicit reverse
+ *                    Arrhenius k_rev of rev_arr (REV, hpp:1242-1253, 1746-1750);
  *                    LSM_RXN_NEGATIVE_A: k is multiplied by -1 (hpp:1208-1210).
  *   reac_* / prod_*  reaction-major CSR of (species, coefficient).  The
  *                    coefficient is the stoichiometric coefficient and, unless
@@ -35,7 +37,15 @@
  *   fo_*             fall-off reactions (Lindemann, Troe or SRI), k_inf Arrhenius
  *                    parameters, Troe (a, T***, T*, T**) or SRI (a, b, c, d, e)
  *                    and either a single collider species or an M efficiency
- *                    list (hpp:1496-1550).
+ *                    list (hpp:1496-1550).  With LSM_FALLOFF_CHEMACT the entry is
+ *                    a chemically activated bimolecular reaction (CABR, HIGH
+ *                    keyword): the direct parameters are k0 (the reaction line),
+ *                    fo_inf holds k_inf (HIGH) and k = k0 F / (1 + Pr)
+ *                    (hpp:1292-1326, 1613-1659, 1721-1726).
+ *   pl_*             pressure-logarithmic (PLOG) reactions: their forward k is
+ *                    interpolated in ln P between per-pressure Arrhenius sets and
+ *                    replaces the direct parameters (hpp:1342-1349)
+ *                    (kernel/kinetics/PressureLogarithmicRateExpression.hpp).
  *
  * All indices are 0-based.  The arrays are owned by the caller and are only
  * read during lsm_create()/oracle calls.
@@ -52,15 +62,20 @@ extern "C" {
 #define LSM_NASA_STRIDE 18
 
 enum {
-    LSM_RXN_REVERSIBLE = 1,  /* thermodynamically reversible (1/Kc from NASA-7) */
-    LSM_RXN_NEGATIVE_A = 2   /* negative frequency factor: k -> -k            */
+    LSM_RXN_REVERSIBLE = 1,  /* reversible (reverse k = k/Kc from NASA-7)      */
+    LSM_RXN_NEGATIVE_A = 2,  /* negative frequency factor: k -> -k            */
+    LSM_RXN_EXPLICIT_REV = 4 /* with LSM_RXN_REVERSIBLE: reverse k = k_rev of
+                                rev_arr instead of k/Kc                       */
 };
 
 enum {
     LSM_FALLOFF_LINDEMANN = 0,
     LSM_FALLOFF_TROE = 1,
-    LSM_FALLOFF_SRI = 2      /* F = d [a exp(-b/T) + exp(-T/c)]^X T^e,
+    LSM_FALLOFF_SRI = 2,     /* F = d [a exp(-b/T) + exp(-T/c)]^X T^e,
                                 X = 1/(1 + log10(Pr)^2) (KineticsMap_CHEMKIN.hpp:1283-1287, 1538-1544) */
+    LSM_FALLOFF_CHEMACT = 8  /* bit ORed onto the form: chemically activated
+                                (CABR), k = k0 F/(1 + Pr); the Troe form has no
+                                Pr <= 1e-32 branch (hpp:1613-1659)            */
 };
 
 typedef struct lsm_mechanism {
@@ -100,6 +115,14 @@ typedef struct lsm_mechanism {
                                   reaction has explicit orders                  */
     const int32_t *ord_sp;     /* [ord_ptr[nr]] species                         */
     const double *ord_val;     /* [ord_ptr[nr]] order lambda                    */
+    const double *rev_arr;     /* [nr*3] explicit reverse lnA, beta, E_over_R; rows
+                                  of LSM_RXN_EXPLICIT_REV reactions are read, the
+                                  rest ignored; NULL when no reaction has REV     */
+    int32_t npl;               /* number of PLOG reactions                      */
+    const int32_t *pl_rxn;     /* [npl] reaction index                          */
+    const int32_t *pl_ptr;     /* [npl+1] pressure levels per reaction (>= 1)   */
+    const double *pl_par;      /* [pl_ptr[npl]*4] per level: P [Pa] (strictly
+                                  increasing), lnA, beta, E_over_R               */
 } lsm_mechanism;
 
 /* Physical constants of the reference (math/PhysicalConstants.h:24). */

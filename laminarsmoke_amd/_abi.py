@@ -15,7 +15,8 @@ class LsmMechanism(ctypes.Structure):
         ("ntb", ctypes.c_int32), ("tb_rxn", _i), ("tb_ptr", _i), ("tb_sp", _i), ("tb_eff_m1", _d),
         ("nfo", ctypes.c_int32), ("fo_rxn", _i), ("fo_type", _i), ("fo_inf", _d), ("fo_troe", _d),
         ("fo_single", _i), ("fo_ptr", _i), ("fo_sp", _i), ("fo_eff_m1", _d), ("fo_sri", _d),
-        ("ord_ptr", _i), ("ord_sp", _i), ("ord_val", _d),
+        ("ord_ptr", _i), ("ord_sp", _i), ("ord_val", _d), ("rev_arr", _d),
+        ("npl", ctypes.c_int32), ("pl_rxn", _i), ("pl_ptr", _i), ("pl_par", _d),
     ]
 
 

@@ -13,6 +13,15 @@ Supported:
 * third-body reactions (``+M``) with efficiencies;
 * fall-off reactions (``(+M)`` or ``(+species)``) with ``LOW`` and optional
   3- or 4-parameter ``TROE`` or 3- or 5-parameter ``SRI``, with efficiencies;
+* chemically activated reactions (``(+M)`` with ``HIGH`` instead of ``LOW``,
+  with optional ``TROE`` / ``SRI``): the reaction line is k0 (order), ``HIGH`` is
+  k_inf (order - 1) (ReactionPolicy_CHEMKIN.hpp:546-582, 1396-1407);
+* explicit reverse Arrhenius parameters (``REV``) on reversible elementary and
+  third-body reactions, converted with the product order (+1 for ``+M``)
+  (ReactionPolicy_CHEMKIN.hpp:882-902, 1359-1376, 1474-1478);
+* pressure-logarithmic reactions (``PLOG /P A beta E/`` lines, P in atm,
+  increasing) on reactions without ``M``: A converted like the reaction line and
+  floored at 1e-32 in the log (PressureLogarithmicRateExpression.hpp:71-102);
 * ``DUPLICATE``; negative pre-exponential factors on elementary reactions;
 * explicit forward reaction orders (``FORD``): the orders are the reactant
   coefficients with the named species' orders replaced or added
@@ -20,8 +29,8 @@ Supported:
 * energy units CAL/MOLE (default), KCAL/MOLE, JOULES/MOLE, KJOULES/MOLE, KELVINS;
   quantity unit MOLES.
 
-Anything else the kernels do not evaluate (``REV``, ``HIGH``, ``PLOG``,
-``CHEB``, ``RORD``, ``LT``, ``MOLECULES``, ...) raises
+Anything else the kernels do not evaluate (``CHEB``, ``RORD``, ``LT``,
+``MOLECULES``, ...) raises
 :class:`ChemkinError` naming the reaction, rather than being dropped.
 
 Unit conversions follow ReactionPolicy_CHEMKIN::ConvertUnits /
@@ -47,8 +56,8 @@ import re
 
 import numpy as np
 
-from .mechanism import (FALLOFF_LINDEMANN, FALLOFF_SRI, FALLOFF_TROE, RXN_NEGATIVE_A, RXN_REVERSIBLE,
-                        J_FROM_CAL, R_J_KMOL, Mechanism)
+from .mechanism import (FALLOFF_CHEMACT, FALLOFF_LINDEMANN, FALLOFF_SRI, FALLOFF_TROE, RXN_EXPLICIT_REV,
+                        RXN_NEGATIVE_A, RXN_REVERSIBLE, J_FROM_CAL, R_J_KMOL, Mechanism)
 
 # kernel/thermo/AtomicElementMap.hpp:41-60 (the values are stored as floats there)
 ATOMIC_WEIGHTS = {
@@ -61,7 +70,7 @@ E_CONVERSION = {               # J/kmol per unit (ReactionPolicy_CHEMKIN.hpp:148
     "CAL/MOLE": 1e3 * J_FROM_CAL, "KCAL/MOLE": 1e6 * J_FROM_CAL,
     "JOULES/MOLE": 1e3, "KJOULES/MOLE": 1e6, "KELVINS": R_J_KMOL,
 }
-_UNSUPPORTED = {"REV", "HIGH", "PLOG", "CHEB", "TCHEB", "PCHEB", "RORD", "LT", "RLT",
+_UNSUPPORTED = {"CHEB", "TCHEB", "PCHEB", "RORD", "LT", "RLT",
                 "JAN", "FIT1", "TDEP", "EXCI", "MOME", "XSMI", "USRPROG", "LOWMX", "LOWSP", "TROEMX",
                 "TROESP", "SRIMX", "SRISP", "PLOGMX", "PLOGSP", "COLLEFF", "UNITS"}
 
@@ -250,7 +259,8 @@ def _parse_reaction_line(line: str, species_set: set, idx: int):
     if tb_l and co_l:
         raise ChemkinError(f"{what}: both +M and (+M)")
     return dict(eq=eq, what=what, reac=r, prod=p, rev=rev, A=A, beta=beta, E=E,
-                third=tb_l, collider=co_l, low=None, troe=None, sri=None, eff={}, ford={}, dup=False)
+                third=tb_l, collider=co_l, low=None, high=None, troe=None, sri=None, eff={}, ford={}, dup=False,
+                rev_arr=None, plog=[])
 
 
 def read_kinetics(text: str):
@@ -298,6 +308,17 @@ def read_kinetics(text: str):
                             if len(vals) != 3:
                                 raise ChemkinError(f"{cur['what']}: LOW needs 3 values")
                             cur["low"] = [float(v) for v in vals]
+                        elif kw in ("HIGH", "REV"):
+                            key = "high" if kw == "HIGH" else "rev_arr"
+                            if len(vals) != 3:
+                                raise ChemkinError(f"{cur['what']}: {kw} needs 3 values")
+                            if cur[key] is not None:
+                                raise ChemkinError(f"{cur['what']}: the {kw} option is used more than once")
+                            cur[key] = [float(v) for v in vals]
+                        elif kw == "PLOG":
+                            if len(vals) != 4:
+                                raise ChemkinError(f"{cur['what']}: PLOG needs 4 values (P, A, beta, E)")
+                            cur["plog"].append([float(v) for v in vals])
                         elif kw == "TROE":
                             if len(vals) not in (3, 4):
                                 raise ChemkinError(f"{cur['what']}: TROE needs 3 or 4 values")
@@ -367,6 +388,8 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
     fo_rxn, fo_type, fo_inf, fo_troe, fo_single, fo_sp, fo_eff = [], [], [], [], [], [], []
     fo_sri = []
     ord_rows = []
+    rev_arr = np.zeros((len(reactions), 3))
+    pl_rxn, pl_lev = [], []
     for j, r in enumerate(reactions):
         what = r["what"]
         lam = dict(r["reac"])
@@ -377,26 +400,39 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
             ord_rows.append([])
         order = sum(lam.values())             # sumLambdaReactants_ (ReactionPolicy_CHEMKIN.hpp:1360-1366)
         falloff = r["collider"] is not None
-        if falloff and r["low"] is None:
-            raise ChemkinError(f"{what}: (+M) reaction without LOW (HIGH/chemically activated is not supported)")
-        if not falloff and (r["low"] is not None or r["troe"] is not None or r["sri"] is not None):
-            raise ChemkinError(f"{what}: LOW/TROE/SRI on a reaction without (+M)")
+        cabr = r["high"] is not None
+        if falloff and (r["low"] is None) == (r["high"] is None):
+            raise ChemkinError(f"{what}: a (+M) reaction needs exactly one of LOW and HIGH")
+        if not falloff and any(r[k] is not None for k in ("low", "high", "troe", "sri")):
+            raise ChemkinError(f"{what}: LOW/HIGH/TROE/SRI on a reaction without (+M)")
+        if r["rev_arr"] is not None and (not r["rev"] or falloff):
+            raise ChemkinError(f"{what}: REV needs a reversible reaction without (+M)")
+        if r["plog"] and (falloff or r["third"]):
+            raise ChemkinError(f"{what}: PLOG cannot be used on third-body or fall-off reactions")
         if r["troe"] is not None and r["sri"] is not None:
             raise ChemkinError(f"{what}: both TROE and SRI")
         if r["sri"] is not None and (r["sri"][3] <= 0.0 or r["sri"][2] == 0.0):
             raise ChemkinError(f"{what}: SRI needs d > 0 and c != 0")
         if falloff:
-            A0, b0, E0 = r["low"]
-            if r["A"] < 0.0 or A0 < 0.0:
+            if cabr:
+                # chemically activated: direct = the reaction line (order), k_inf = HIGH
+                # (order - 1) (ReactionPolicy_CHEMKIN.hpp:1396-1404)
+                (A0, b0, E0), (Ai, bi, Ei) = (r["A"], r["beta"], r["E"]), r["high"]
+                c0, ci = 1e3 ** (1.0 - order), 1e3 ** (2.0 - order)
+            else:
+                # direct parameters: LOW (order + 1); k_inf: the Arrhenius line (order)
+                (A0, b0, E0), (Ai, bi, Ei) = r["low"], (r["A"], r["beta"], r["E"])
+                c0, ci = 1e3 ** (-order), 1e3 ** (1.0 - order)
+            if Ai < 0.0 or A0 < 0.0:
                 raise ChemkinError(f"{what}: negative pre-exponential factor on a fall-off reaction")
-            # direct parameters: LOW (order + 1); k_inf: the Arrhenius line (order)
-            lnA.append(np.log(A0 * 1e3 ** (-order)))
+            lnA.append(np.log(A0 * c0))
             beta.append(b0)
             EoR.append(E0 * econv / R_J_KMOL)
             fo_rxn.append(j)
-            fo_inf.append((np.log(r["A"] * 1e3 ** (1.0 - order)), r["beta"], r["E"] * econv / R_J_KMOL))
-            fo_type.append(FALLOFF_TROE if r["troe"] is not None else
-                           FALLOFF_SRI if r["sri"] is not None else FALLOFF_LINDEMANN)
+            fo_inf.append((np.log(Ai * ci), bi, Ei * econv / R_J_KMOL))
+            fo_type.append((FALLOFF_TROE if r["troe"] is not None else
+                            FALLOFF_SRI if r["sri"] is not None else FALLOFF_LINDEMANN) |
+                           (FALLOFF_CHEMACT if cabr else 0))
             fo_troe.append(tuple(r["troe"]) if r["troe"] is not None else (0.0, 1.0, 1.0, 0.0))
             fo_sri.append(tuple(r["sri"]) if r["sri"] is not None else (0.0, 0.0, 1.0, 1.0, 0.0))
             if r["collider"] == "M":
@@ -428,6 +464,25 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
                 tb_eff.append([e for _, e in items])
             elif r["eff"]:
                 raise ChemkinError(f"{what}: efficiencies on a reaction without M")
+            if r["rev_arr"] is not None:
+                Ar, br, Er = r["rev_arr"]
+                if Ar < 0.0:
+                    raise ChemkinError(f"{what}: negative pre-exponential factor in REV")
+                prod_order = sum(nu for _, nu in r["prod"]) + (1.0 if r["third"] else 0.0)
+                rev_arr[j] = (np.log(Ar * 1e3 ** (1.0 - prod_order)) if Ar != 0.0 else np.log(1e-100), br,
+                              Er * econv / R_J_KMOL)
+                fl |= RXN_EXPLICIT_REV
+            if r["plog"]:
+                lev = []
+                for P, Ap, bp, Ep in r["plog"]:
+                    if Ap < 0.0:
+                        raise ChemkinError(f"{what}: PLOG needs non-negative frequency factors")
+                    lev.append((P * 101325.0, np.log(max(Ap * 1e3 ** (1.0 - order), 1e-32)), bp,
+                                Ep * econv / R_J_KMOL))
+                if any(b[0] <= a[0] for a, b in zip(lev[:-1], lev[1:])):
+                    raise ChemkinError(f"{what}: PLOG pressures must increase")
+                pl_rxn.append(j)
+                pl_lev.append(lev)
         if r["rev"]:
             fl |= RXN_REVERSIBLE
         flags.append(fl)
@@ -460,6 +515,8 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
         fo_sri=np.array(fo_sri, dtype=np.float64).reshape(nfo, 5),
         ord_ptr=csr(ord_rows), ord_sp=np.array([s for x in ord_rows for s, _ in x], dtype=np.int32),
         ord_val=np.array([v for x in ord_rows for _, v in x], dtype=np.float64),
+        rev_arr=rev_arr, pl_rxn=np.array(pl_rxn, dtype=np.int32), pl_ptr=csr(pl_lev),
+        pl_par=np.array([v for x in pl_lev for v in x], dtype=np.float64).reshape(-1, 4),
     )
 
 

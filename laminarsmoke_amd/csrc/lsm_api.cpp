@@ -340,12 +340,24 @@ static int check_mechanism(const lsm_mechanism *m) {
         for (int q = m->tb_ptr[s]; q < m->tb_ptr[s + 1]; q++)
             if (m->tb_sp[q] < 0 || m->tb_sp[q] >= m->ns) return fail(LSM_E_INVALID, "third-body species index");
     }
+    for (int j = 0; j < m->nr; j++) {
+        if (m->rflags[j] & ~(LSM_RXN_REVERSIBLE | LSM_RXN_NEGATIVE_A | LSM_RXN_EXPLICIT_REV))
+            return fail(LSM_E_UNSUPPORTED, "unknown reaction flag");
+        if (m->rflags[j] & LSM_RXN_EXPLICIT_REV) {
+            if (!(m->rflags[j] & LSM_RXN_REVERSIBLE))
+                return fail(LSM_E_INVALID, "explicit reverse parameters on an irreversible reaction");
+            if (!m->rev_arr) return fail(LSM_E_INVALID, "explicitly reversible reaction without rev_arr");
+        }
+    }
     for (int s = 0; s < m->nfo; s++) {
         if (m->fo_rxn[s] < 0 || m->fo_rxn[s] >= m->nr) return fail(LSM_E_INVALID, "fall-off reaction index");
-        if (m->fo_type[s] != LSM_FALLOFF_LINDEMANN && m->fo_type[s] != LSM_FALLOFF_TROE &&
-            m->fo_type[s] != LSM_FALLOFF_SRI)
+        const int form = m->fo_type[s] & ~LSM_FALLOFF_CHEMACT;
+        if (form != LSM_FALLOFF_LINDEMANN && form != LSM_FALLOFF_TROE && form != LSM_FALLOFF_SRI)
             return fail(LSM_E_UNSUPPORTED, "fall-off type (Lindemann, Troe or SRI)");
-        if (m->fo_type[s] == LSM_FALLOFF_SRI) {
+        // ReactionPolicy_CHEMKIN.hpp:1392-1407
+        if (m->rflags[m->fo_rxn[s]] & LSM_RXN_EXPLICIT_REV)
+            return fail(LSM_E_INVALID, "explicitly reversible fall-off / CABR reactions are not allowed");
+        if (form == LSM_FALLOFF_SRI) {
             if (!m->fo_sri) return fail(LSM_E_INVALID, "SRI fall-off reaction without fo_sri parameters");
             const double *t = m->fo_sri + 5 * s;
             if (!(t[3] > 0.) || t[2] == 0.) return fail(LSM_E_INVALID, "SRI parameters need d > 0 and c != 0");
@@ -360,6 +372,27 @@ static int check_mechanism(const lsm_mechanism *m) {
         const int j = s < m->ntb ? m->tb_rxn[s] : m->fo_rxn[s - m->ntb];
         if (seen[j]) return fail(LSM_E_INVALID, "reaction listed twice among third-body / fall-off reactions");
         seen[j] = 1;
+    }
+    if (m->npl < 0 || (m->npl > 0 && (!m->pl_rxn || !m->pl_ptr || !m->pl_par)))
+        return fail(LSM_E_INVALID, "PLOG tables");
+    {
+        std::vector<char> pl(m->nr, 0);
+        for (int s = 0; s < m->npl; s++) {
+            const int j = m->pl_rxn[s];
+            if (j < 0 || j >= m->nr) return fail(LSM_E_INVALID, "PLOG reaction index");
+            if (pl[j]) return fail(LSM_E_INVALID, "reaction listed twice among PLOG reactions");
+            pl[j] = 1;
+            if (m->pl_ptr[s + 1] - m->pl_ptr[s] < 1) return fail(LSM_E_INVALID, "PLOG reaction without levels");
+            for (int q = m->pl_ptr[s]; q < m->pl_ptr[s + 1]; q++) {
+                if (!(m->pl_par[4 * q] > 0.)) return fail(LSM_E_INVALID, "PLOG pressure must be positive");
+                // PressureLogarithmicRateExpression.hpp:97-101
+                if (q > m->pl_ptr[s] && !(m->pl_par[4 * q] > m->pl_par[4 * (q - 1)]))
+                    return fail(LSM_E_INVALID, "PLOG pressures must increase");
+            }
+        }
+        // PLOG cannot be combined with LOW / HIGH (ReactionPolicy_CHEMKIN.hpp:778-785)
+        for (int s = 0; s < m->nfo; s++)
+            if (pl[m->fo_rxn[s]]) return fail(LSM_E_INVALID, "PLOG on a fall-off / CABR reaction");
     }
     if (m->ord_ptr) {
         if (m->ord_ptr[0] != 0) return fail(LSM_E_INVALID, "ord_ptr[0] must be 0");
@@ -410,6 +443,10 @@ static bool smooth_forward(const lsm_mechanism *m, int j) {
     for (auto &e : forward_orders(m, j)) if (e.second < 1.) return true;
     return false;
 }
+static int plog_slot_of(const lsm_mechanism *m, int j) {
+    for (int s = 0; s < m->npl; s++) if (m->pl_rxn[s] == j) return s;
+    return -1;
+}
 static bool smooth_reverse(const lsm_mechanism *m, int j) {
     if (!(m->rflags[j] & LSM_RXN_REVERSIBLE)) return false;
     for (int q = m->prod_ptr[j]; q < m->prod_ptr[j + 1]; q++) if (m->prod_nu[q] < 1.) return true;
@@ -430,8 +467,16 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
         sp[(size_t)14 * ns_pad + k] = c[16];
         sp[(size_t)15 * ns_pad + k] = c[17];
     }
-    std::vector<double> P((size_t)4 * nr), nuv;
-    std::vector<unsigned> W((size_t)4 * nr), ent;
+    std::vector<double> P((size_t)4 * nr), nuv, rvp, pll;
+    std::vector<unsigned> W((size_t)4 * nr), ent, plp;
+    for (int s = 0; s < m->npl; s++) {
+        plp.push_back((unsigned)(pll.size() / 4));
+        for (int q = m->pl_ptr[s]; q < m->pl_ptr[s + 1]; q++) {
+            pll.push_back(std::log(m->pl_par[4 * q]));
+            for (int i = 1; i < 4; i++) pll.push_back(m->pl_par[4 * q + i]);
+        }
+    }
+    plp.push_back((unsigned)(pll.size() / 4));
     auto code_of = [&](double nu) -> unsigned {
         const double t = 2. * nu;
         if (t == std::floor(t) && t >= 1. && t <= 15.) return (unsigned)t;
@@ -447,6 +492,14 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
         for (int q = m->reac_ptr[j]; q < m->reac_ptr[j + 1]; q++) { re.push_back({m->reac_sp[q], m->reac_nu[q]}); d -= m->reac_nu[q]; }
         for (int q = m->prod_ptr[j]; q < m->prod_ptr[j + 1]; q++) { pr.push_back({m->prod_sp[q], m->prod_nu[q]}); d += m->prod_nu[q]; }
         P[4 * jn + 3] = d;
+        const int pls = plog_slot_of(m, j);
+        if (pls >= 0) P[4 * jn + 0] = (double)pls;
+        const bool xrev = (m->rflags[j] & LSM_RXN_EXPLICIT_REV) != 0;
+        if (xrev) {
+            P[4 * jn + 3] = (double)(rvp.size() / 4);
+            for (int i = 0; i < 3; i++) rvp.push_back(m->rev_arr[3 * j + i]);
+            rvp.push_back(0.);
+        }
         auto bysp = [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; };
         std::stable_sort(re.begin(), re.end(), bysp);
         std::stable_sort(pr.begin(), pr.end(), bysp);
@@ -471,12 +524,16 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
             return true;
         };
         unsigned wr = 0, wp = 0;
-        const bool fast = !xo && !sf && !sb && pack(re, wr) && pack(pr, wp);
+        const bool fast = !xo && !sf && !sb && !xrev && pack(re, wr) && pack(pr, wp);
         W[4 * jn + 1] = wr;
         W[4 * jn + 2] = wp;
         // bits: 0 reversible, 1 negative A, 2 fixed format, 3/4 non-elementary forward/reverse
-        // side, 8-15 third-body slot + 1, 16-23 fall-off slot + 1, 24-31 explicit orders
-        W[4 * jn + 0] = (unsigned)(m->rflags[j] & 3) | (fast ? 4u : 0u) | (sf ? 8u : 0u) | (sb ? 16u : 0u) |
+        // side, 5 explicit reverse, 6 chemically activated, 7 PLOG, 8-15 third-body slot + 1,
+        // 16-23 fall-off slot + 1, 24-31 explicit orders.  A PLOG constant replaces the
+        // sign-flipped Arrhenius one (KineticsMap_CHEMKIN.hpp:1208-1210, 1342-1349).
+        const bool cabr = fo_of[j] >= 0 && (m->fo_type[fo_of[j]] & LSM_FALLOFF_CHEMACT);
+        W[4 * jn + 0] = (unsigned)(m->rflags[j] & (pls >= 0 ? 1 : 3)) | (fast ? 4u : 0u) | (sf ? 8u : 0u) |
+                        (sb ? 16u : 0u) | (xrev ? 32u : 0u) | (cabr ? 64u : 0u) | (pls >= 0 ? 128u : 0u) |
                         ((unsigned)(tb_of[j] + 1) << 8) | ((unsigned)(fo_of[j] + 1) << 16) |
                         ((unsigned)od.size() << 24);
     }
@@ -496,13 +553,14 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
         // [0..2] k_inf; Troe: [3..6] a, T***, T*, T**, [7] = 1; SRI: [3..6] a, b, c, e,
         // [7] = -d (d > 0); Lindemann: [7] = 0
         for (int i = 0; i < 3; i++) fop[8 * s + i] = m->fo_inf[3 * s + i];
-        if (m->fo_type[s] == LSM_FALLOFF_SRI) {
+        const int form = m->fo_type[s] & 3;
+        if (form == LSM_FALLOFF_SRI) {
             const double *t = m->fo_sri + 5 * s;
             fop[8 * s + 3] = t[0]; fop[8 * s + 4] = t[1]; fop[8 * s + 5] = t[2]; fop[8 * s + 6] = t[4];
             fop[8 * s + 7] = -t[3];
         } else {
             for (int i = 0; i < 4; i++) fop[8 * s + 3 + i] = m->fo_troe[4 * s + i];
-            fop[8 * s + 7] = (double)m->fo_type[s];
+            fop[8 * s + 7] = (double)form;
         }
         fos[s] = m->fo_single[s];
         fol[s] = (unsigned)fos2.size();
@@ -514,6 +572,7 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
     L.o_ent = im.put(ent); L.o_nuv = im.put(nuv);
     L.o_tbp = im.put(tbp); L.o_tbs = im.put(tbs); L.o_tbe = im.put(tbe);
     L.o_fop = im.put(fop); L.o_fos = im.put(fos); L.o_fol = im.put(fol); L.o_fos2 = im.put(fos2); L.o_foe = im.put(foe);
+    L.o_rvp = im.put(rvp); L.o_plp = im.put(plp); L.o_pll = im.put(pll);
     im.b.resize((im.b.size() + 15) & ~size_t(15), 0);
     L.bytes = (int)im.b.size();
     L.team_ok = ns <= 63 ? 1 : 0;
@@ -527,6 +586,9 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     const int ns = m->ns, nr = m->nr;
     DevMech &d = h->dm;
     d.ns = ns; d.nr = nr; d.ne = ns + 1; d.ntb = m->ntb; d.nfo = m->nfo;
+    d.xk = m->npl > 0 ? 1 : 0;
+    for (int j = 0; j < nr; j++) if (m->rflags[j] & LSM_RXN_EXPLICIT_REV) d.xk = 1;
+    for (int s = 0; s < m->nfo; s++) if (m->fo_type[s] & LSM_FALLOFF_CHEMACT) d.xk = 1;
     d.ns_pad = (ns + 63) / 64 * 64;
     // species coefficient rows (SetCoefficients, ThermodynamicsMap_CHEMKIN.hpp:263-343)
     std::vector<double> sp((size_t)NSP_ROWS * d.ns_pad, 0.);
@@ -576,10 +638,23 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     for (int jn = 0; jn < nr; jn++) inv[perm[jn]] = jn;
     std::vector<double> rA(nr), rB(nr), rE(nr), rdn(nr), fw_nu, bw_nu, rx_nu;
     std::vector<int> rinfo(nr), rtb(nr), rfo(nr), fw_ptr(nr + 1), bw_ptr(nr + 1), rx_ptr(nr + 1), fw_sp, bw_sp, rx_sp;
+    std::vector<int> rpl(nr), pl_ptr(m->npl + 1);
+    std::vector<double> rv((size_t)3 * nr, 0.), pl_lev;
+    for (int s = 0; s < m->npl; s++) {
+        pl_ptr[s] = (int)(pl_lev.size() / 4);
+        for (int q = m->pl_ptr[s]; q < m->pl_ptr[s + 1]; q++) {
+            pl_lev.push_back(std::log(m->pl_par[4 * q]));
+            for (int i = 1; i < 4; i++) pl_lev.push_back(m->pl_par[4 * q + i]);
+        }
+    }
+    pl_ptr[m->npl] = (int)(pl_lev.size() / 4);
     for (int jn = 0; jn < nr; jn++) {
         const int j = perm[jn];
         rA[jn] = m->lnA[j]; rB[jn] = m->beta[j]; rE[jn] = m->E_over_R[j];
-        rinfo[jn] = (m->rflags[j] & 3) | (smooth_forward(m, j) ? 8 : 0) | (smooth_reverse(m, j) ? 16 : 0);
+        rinfo[jn] = (m->rflags[j] & 7) | (smooth_forward(m, j) ? 8 : 0) | (smooth_reverse(m, j) ? 16 : 0);
+        rpl[jn] = plog_slot_of(m, j);
+        if (m->rflags[j] & LSM_RXN_EXPLICIT_REV)
+            for (int i = 0; i < 3; i++) rv[3 * jn + i] = m->rev_arr[3 * j + i];
         rtb[jn] = tb_of[j]; rfo[jn] = fo_of[j];
         std::vector<std::pair<int, double>> re, pr;
         double dn = 0.;
@@ -640,10 +715,10 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     std::vector<int> fo_type(m->nfo), fo_single(m->nfo), fo_ptr(m->nfo + 1), fo_sp;
     std::vector<double> fo_par((size_t)8 * m->nfo, 0.), fo_eff;
     for (int s = 0; s < m->nfo; s++) {
-        fo_type[s] = m->fo_type[s];
+        fo_type[s] = m->fo_type[s];   // form | LSM_FALLOFF_CHEMACT
         fo_single[s] = m->fo_single[s];
         for (int i = 0; i < 3; i++) fo_par[8 * s + i] = m->fo_inf[3 * s + i];
-        if (m->fo_type[s] == LSM_FALLOFF_SRI) {        // a, b, c, e, d
+        if ((m->fo_type[s] & 3) == LSM_FALLOFF_SRI) {        // a, b, c, e, d
             const double *t = m->fo_sri + 5 * s;
             fo_par[8 * s + 3] = t[0]; fo_par[8 * s + 4] = t[1]; fo_par[8 * s + 5] = t[2]; fo_par[8 * s + 6] = t[4];
             fo_par[8 * s + 7] = t[3];
@@ -665,6 +740,7 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     size_t o_fot = b.add(fo_type), o_fos1 = b.add(fo_single), o_fopar = b.add(fo_par);
     size_t o_fop = b.add(fo_ptr), o_fos = b.add(fo_sp), o_foe = b.add(fo_eff);
     size_t o_fp = b.add(f_ptr), o_frx = b.add(f_rx), o_fnu = b.add(f_nu);
+    size_t o_rv = b.add(rv), o_rpl = b.add(rpl), o_plptr = b.add(pl_ptr), o_pllev = b.add(pl_lev);
     std::vector<char> image;
     const bool img_ok = build_image(m, perm, tb_of, fo_of, d.ns_pad, d.img, image);
     size_t o_img = img_ok ? b.add(image) : 0;
@@ -683,6 +759,7 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     d.fo_type = I(o_fot); d.fo_single = I(o_fos1); d.fo_par = D(o_fopar);
     d.fo_ptr = I(o_fop); d.fo_sp = I(o_fos); d.fo_eff = D(o_foe);
     d.f_ptr = I(o_fp); d.f_rx = I(o_frx); d.f_nu = D(o_fnu);
+    d.rv = D(o_rv); d.rpl = I(o_rpl); d.pl_ptr = I(o_plptr); d.pl_lev = D(o_pllev);
     d.img.src = img_ok ? (const void *)(base + o_img) : nullptr;
     return LSM_OK;
 }

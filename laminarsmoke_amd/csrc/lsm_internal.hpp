@@ -20,10 +20,16 @@ enum SpRow {
 // Compact mechanism image that the team kernel copies into LDS once per
 // workgroup (byte offsets from the image start; every block 16-byte aligned).
 //   sp   double [LSP_ROWS][ns_pad]  rows 0-6 high-T a1..a7, 7-13 low-T a1..a7, 14 T_mid, 15 MW
-//   P    double [nr][4]             lnA, beta, E/R, change of moles (device reaction order)
+//   P    double [nr][4]             lnA, beta, E/R, change of moles (device reaction order);
+//                                   a PLOG reaction holds its PLOG slot in lnA, an
+//                                   explicitly reversible one its rvp slot in the
+//                                   change of moles
 //   W    uint   [nr][4]             info, reactant bytes, product bytes, general list:
-//        info: bit0 reversible, bit1 negative A, bit2 fixed-format record, bits 8-15
-//              third-body slot+1, bits 16-23 fall-off slot+1;
+//        info: bit0 reversible, bit1 negative A, bit2 fixed-format record, bits 3/4
+//              non-elementary forward/reverse side, bit5 explicit reverse Arrhenius,
+//              bit6 chemically activated (CABR) fall-off entry, bit7 PLOG, bits 8-15
+//              third-body slot+1, bits 16-23 fall-off slot+1, bits 24-31 number of
+//              explicit forward orders (their entries follow the products);
 //        fixed-format record: up to 3 reactant and 3 product species, integer orders as
 //              repeated species, unused slots = LdsImage::dummy (c = 1, g = 0): 63
 //              (LSM_DUMMY_SP) when ns <= 63, else 255;
@@ -34,12 +40,17 @@ enum SpRow {
 //   tbp  uint [ntb+1], tbs uint [], tbe double []   third-body lists (eff - 1)
 //   fop  double [nfo][8]            lnA_inf, beta_inf, E/R_inf, a, T***, T*, T**, type
 //   fos  int [nfo] single collider or -1; fol uint [nfo+1]; fos2 uint []; foe double []
+//   rvp  double [nrev][4]           explicit reverse lnA, beta, E/R, 0
+//   plp  uint [npl+1], pll double [levels][4]   PLOG levels: ln p, lnA, beta, E/R
 enum { LSM_DUMMY_SP = 63 };
 enum { LSP_ROWS = 16 };
 struct LdsImage {
     const void *src;
     int bytes;
     int o_sp, o_P, o_W, o_ent, o_nuv, o_tbp, o_tbs, o_tbe, o_fop, o_fos, o_fol, o_fos2, o_foe;
+    int o_rvp;     // [nrev][4] explicit reverse lnA, beta, E/R, 0
+    int o_plp;     // [npl+1] first level of each PLOG reaction
+    int o_pll;     // [levels][4] ln p, lnA, beta, E/R
     int team_ok;   // the team kernel can use the image (ns <= 63)
     int wide_ok;   // the wide kernel can use the image (ns <= 254)
     int dummy;     // unused slot of fixed-format records: 63 if ns <= 63, else 255
@@ -50,6 +61,8 @@ struct LdsImage {
 // share as few passes of the reaction loop as possible (lsm_api.cpp).
 struct DevMech {
     int ns, nr, ne, ntb, nfo, ns_pad;
+    int xk;                                  // 1: explicitly reversible, chemically activated or PLOG
+                                             // reactions present (kernels built with those forms)
     const double *sp;                        // [NSP_ROWS][ns_pad]
     const double *rA, *rB, *rE, *rdn;        // [nr] lnA, beta, E/R, change of moles
     const int *rinfo;                        // [nr] LSM_RXN_* bits
@@ -68,6 +81,10 @@ struct DevMech {
     const double *fo_eff;
     const int *f_ptr, *f_rx;                 // species-major formation list
     const double *f_nu;                      // signed coefficient
+    const double *rv;                        // [nr][3] explicit reverse lnA, beta, E/R (zeros elsewhere)
+    const int *rpl;                          // [nr] PLOG slot or -1
+    const int *pl_ptr;                       // [npl+1] first level of each PLOG reaction
+    const double *pl_lev;                    // [levels][4] ln p, lnA, beta, E/R
     LdsImage img;
 };
 

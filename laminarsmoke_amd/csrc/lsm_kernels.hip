@@ -148,14 +148,53 @@ __device__ __forceinline__ double order_pow_der(double c, double lambda) {
     return dm * pow(a, lambda) + m * lambda * pow(a, lambda - 1.) * (1. + dm / 1.e9) - dm * csl * c + (1. - m) * csl;
 }
 
-// Net rate of reaction j [kmol/m3/s].  C, H, S return c_k, h_k/RT, s_k/R.
-template <class FC, class FH, class FS>
+// Forward constant of a PLOG reaction at pressure P [Pa]
+// (PressureLogarithmicRateExpression::KineticConstant,
+// kernel/kinetics/PressureLogarithmicRateExpression.hpp:104-119): n levels
+// lv[4i..4i+3] = ln p_i, lnA_i, beta_i, E_i/R with p_i increasing; the end level
+// outside [p_0, p_{n-1}], else ln k linear in ln P between the two levels around P.
+// Returns ln k; dlnk = d ln k / dT at fixed P.
+__device__ __forceinline__ double plog_lnk(const double *lv, int n, double P, double logT, double uT,
+                                           double &dlnk) {
+    const double lnP = log(P);
+    int i = 0;
+    double w = 0.;
+    if (lnP <= lv[0]) {
+        i = 0;
+    } else if (lnP >= lv[4 * (n - 1)]) {
+        i = n - 1;
+    } else {
+        while (i < n - 2 && !(lnP < lv[4 * (i + 1)])) i++;
+        w = (lnP - lv[4 * i]) / (lv[4 * (i + 1)] - lv[4 * i]);
+    }
+    const double *a = lv + 4 * i;
+    const double lnkA = a[1] + a[2] * logT - a[3] * uT;
+    const double dA = (a[2] + a[3] * uT) * uT;
+    if (w == 0.) { dlnk = dA; return lnkA; }
+    const double *b = a + 4;
+    const double lnkB = b[1] + b[2] * logT - b[3] * uT;
+    dlnk = dA + ((b[2] + b[3] * uT) * uT - dA) * w;
+    return lnkA + (lnkB - lnkA) * w;
+}
+
+// Net rate of reaction j [kmol/m3/s].  C, H, S return c_k, h_k/RT, s_k/R; P is the
+// pressure the kinetics see (it only enters PLOG constants).  XK: the mechanism has
+// explicitly reversible, chemically activated or PLOG reactions (without them the
+// code for those forms is not compiled in).
+template <bool XK, class FC, class FH, class FS>
 __device__ __forceinline__ double reaction_rate(const DevMech &M, int j, double T, double logT, double uT,
-                                                double logPRT, double cTot, FC C, FH H, FS S) {
-    // Arrhenius (hpp:1198-1212)
-    double kf = exp(M.rA[j] + M.rB[j] * logT - M.rE[j] * uT);
+                                                double logPRT, double P, double cTot, FC C, FH H, FS S) {
+    // Arrhenius (hpp:1198-1212); PLOG constants replace it (hpp:1342-1349)
     const int info = M.rinfo[j];
-    if (info & LSM_RXN_NEGATIVE_A) kf = -kf;
+    double kf;
+    if (XK && M.rpl[j] >= 0) {
+        const int s = M.rpl[j];
+        double d;
+        kf = exp(plog_lnk(M.pl_lev + 4 * M.pl_ptr[s], M.pl_ptr[s + 1] - M.pl_ptr[s], P, logT, uT, d));
+    } else {
+        kf = exp(M.rA[j] + M.rB[j] * logT - M.rE[j] * uT);
+        if (info & LSM_RXN_NEGATIVE_A) kf = -kf;
+    }
     double kmod = kf;
     // third body (hpp:1388-1398, 1703-1708)
     const int tb = M.rtb[j];
@@ -177,14 +216,16 @@ __device__ __forceinline__ double reaction_rate(const DevMech &M, int j, double 
             Mc = C(M.fo_single[fo]) + 1.e-16;
         }
         const double Pr = kf * Mc / kinf;
+        const int form = M.fo_type[fo] & 3;
+        const bool cabr = XK && (M.fo_type[fo] & LSM_FALLOFF_CHEMACT);
         double wF = 1.;
-        if (M.fo_type[fo] == LSM_FALLOFF_TROE) {
+        if (form == LSM_FALLOFF_TROE) {
             const double a = fp[3];
             double F = (1. - a) * exp(-T / fp[4]) + a * exp(-T / fp[5]);
             if (fp[6] != 0.) F += exp(-fp[6] / T);
             const double lf = (F < 1.e-300) ? -300. : log10(F);
             double q;
-            if (Pr > 1.e-32) {
+            if (Pr > 1.e-32 || cabr) {
                 const double nT = 0.75 - 1.27 * lf;
                 const double cT = -0.4 - 0.67 * lf;
                 const double sT = log10(Pr) + cT;
@@ -193,13 +234,15 @@ __device__ __forceinline__ double reaction_rate(const DevMech &M, int j, double 
                 q = 1. / 0.14;
             }
             wF = pow(10., lf / (1. + q * q));
-        } else if (M.fo_type[fo] == LSM_FALLOFF_SRI) {       // hpp:1283-1287, 1538-1544
+        } else if (form == LSM_FALLOFF_SRI) {       // hpp:1283-1287, 1538-1544
             const double Fs = fp[3] * exp(-fp[4] / T) + exp(-T / fp[5]);
             const double lp = log10(Pr);
             wF = pow(Fs, 1. / (1. + lp * lp)) * fp[7];
             if (fp[6] != 0.) wF *= pow(T, fp[6]);
         }
-        kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
+        // chemically activated: k0 F/(1+Pr) (hpp:1613-1659, 1721-1726)
+        if (cabr) kmod *= (1. / (1. + Pr)) * wF;
+        else kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
     }
     // products of concentrations (StoichiometricMap.hpp:512-598)
     double pf = 1.;
@@ -216,7 +259,10 @@ __device__ __forceinline__ double reaction_rate(const DevMech &M, int j, double 
             dh += nu * H(k);
             ds += nu * S(k);
         }
-        const double uKeq = exp(-ds + dh - logPRT * M.rdn[j]);
+        // explicit reverse Arrhenius: k_rev/k instead of 1/Kc (hpp:1242-1253, 1744-1750)
+        const double *rv = M.rv + 3 * j;
+        const double uKeq = (XK && (info & LSM_RXN_EXPLICIT_REV)) ? exp(rv[0] + rv[1] * logT - rv[2] * uT) / kf
+                                                                : exp(-ds + dh - logPRT * M.rdn[j]);
         double pb = 1.;
         for (int q = M.bw_ptr[j]; q < M.bw_ptr[j + 1]; q++)
             pb *= sb ? order_pow(C(M.bw_sp[q]), M.bw_nu[q]) : conc_pow(C(M.bw_sp[q]), M.bw_nu[q]);
@@ -245,17 +291,19 @@ __device__ __forceinline__ void nasa7(const DevMech &M, int k, double T, double 
 // ---------------------------------------------------------------------------
 // Typed views of the mechanism image once it sits in LDS (layout: lsm_internal.hpp).
 struct LdsMech {
-    const double *sp, *P, *nuv, *tbe, *fop, *foe;
+    const double *sp, *P, *nuv, *tbe, *fop, *foe, *rvp, *pll;
     const uint4 *W;
-    const unsigned *ent, *tbp, *tbs, *fol, *fos2;
+    const unsigned *ent, *tbp, *tbs, *fol, *fos2, *plp;
     const int *fos;
     int SP;   // species stride (ns_pad)
     __device__ LdsMech(const unsigned char *b, const LdsImage &L, int ns_pad)
         : sp((const double *)(b + L.o_sp)), P((const double *)(b + L.o_P)), nuv((const double *)(b + L.o_nuv)),
           tbe((const double *)(b + L.o_tbe)), fop((const double *)(b + L.o_fop)), foe((const double *)(b + L.o_foe)),
+          rvp((const double *)(b + L.o_rvp)), pll((const double *)(b + L.o_pll)),
           W((const uint4 *)(b + L.o_W)), ent((const unsigned *)(b + L.o_ent)), tbp((const unsigned *)(b + L.o_tbp)),
           tbs((const unsigned *)(b + L.o_tbs)), fol((const unsigned *)(b + L.o_fol)),
-          fos2((const unsigned *)(b + L.o_fos2)), fos((const int *)(b + L.o_fos)), SP(ns_pad) {}
+          fos2((const unsigned *)(b + L.o_fos2)), plp((const unsigned *)(b + L.o_plp)), fos((const int *)(b + L.o_fos)),
+          SP(ns_pad) {}
     __device__ __forceinline__ double nu(unsigned e) const {
         const unsigned c = e >> 12;
         return c < 16u ? 0.5 * (double)c : nuv[c - 16u];
@@ -267,15 +315,22 @@ struct LdsMech {
 // 1/Kc = exp(sum_k nu_k g_k - dn ln(Patm/RT)) with g = h/RT - s/R, which equals
 // the reference's exp(-dS/R + dH/RT - dn ln(Patm/RT)).
 // acc(k, v) adds v to the formation rate of species slot k.
-template <class Acc>
+template <bool XK, class Acc>
 __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, int j, double T, double logT, double uT,
-                                         double logPRT, double cTot, Acc &&acc) {
+                                         double logPRT, double Pk, double cTot, Acc &&acc) {
     const uint4 w = L.W[j];
     const double2 ab = ((const double2 *)L.P)[2 * j];
     const double2 ed = ((const double2 *)L.P)[2 * j + 1];
     const unsigned info = w.x;
-    double kf = exp(ab.x + ab.y * logT - ed.x * uT);
-    if (info & 2u) kf = -kf;
+    double kf;
+    if (XK && (info & 128u)) {    // PLOG: slot in the lnA field
+        const int s = (int)ab.x;
+        double d;
+        kf = exp(plog_lnk(L.pll + 4 * L.plp[s], (int)(L.plp[s + 1] - L.plp[s]), Pk, logT, uT, d));
+    } else {
+        kf = exp(ab.x + ab.y * logT - ed.x * uT);
+        if (info & 2u) kf = -kf;
+    }
     double kmod = kf;
     const int tb = (int)((info >> 8) & 0xffu) - 1;
     if (tb >= 0) {
@@ -295,6 +350,7 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
             Mc = scg[L.fos[fo]].x + 1.e-16;
         }
         const double Pr = kf * Mc / kinf;
+        const bool cabr = XK && (info & 64u);
         double wF = 1.;
         if (fp[7] > 0.) {    // Troe
             const double a = fp[3];
@@ -302,7 +358,7 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
             if (fp[6] != 0.) F += exp(-fp[6] / T);
             const double lf = (F < 1.e-300) ? -300. : log10(F);
             double q;
-            if (Pr > 1.e-32) {
+            if (Pr > 1.e-32 || cabr) {
                 const double nT = 0.75 - 1.27 * lf;
                 const double cT = -0.4 - 0.67 * lf;
                 const double sT = log10(Pr) + cT;
@@ -316,7 +372,8 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
             const double lp = log10(Pr);
             wF = exp(log(Fs) / (1. + lp * lp) + fp[6] * logT) * (-fp[7]);
         }
-        kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
+        if (cabr) kmod *= (1. / (1. + Pr)) * wF;    // chemically activated: k0 F/(1+Pr)
+        else kmod *= kinf * (Pr / (1. + Pr)) * wF / kf;
     }
     if (info & 4u) {
         // fixed-format record: three reactant and three product slots
@@ -365,7 +422,12 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
             dg += nu * cg.y;
             pb *= sb ? order_pow(cg.x, nu) : conc_pow(cg.x, nu);
         }
-        net -= pb * exp(dg - logPRT * ed.y);
+        if (XK && (info & 32u)) {    // explicit reverse Arrhenius (slot in the dn field): k_rev / k
+            const double *rv = L.rvp + 4 * (int)ed.y;
+            net -= pb * (exp(rv[0] + rv[1] * logT - rv[2] * uT) / kf);
+        } else {
+            net -= pb * exp(dg - logPRT * ed.y);
+        }
     }
     const double r = net * kmod;
     for (unsigned q = 0; q < nre + npr; q++) {
@@ -382,17 +444,23 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
 // dkdM its derivative with respect to the collider concentration M.
 // acc.R(k, r, drdT) adds to R_k and dR_k/dT, acc.D(i, k, v) to dR_k/dc_i; ns species,
 // `dummy` the unused slot of fixed-format records (c = 1, g = 0), sH[k] = H_k/RT.
-template <class Acc>
+template <bool XK, class Acc>
 __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, const double *sH, int ns, unsigned dummy,
-                                        int j, double T, double logT, double uT, double logPRT, double cTot,
-                                        Acc &&acc) {
-        const uint4 w = L.W[j];
+                                        int j, double T, double logT, double uT, double logPRT, double Pk,
+                                        double cTot, Acc &&acc) {
+    const uint4 w = L.W[j];
     const double2 ab = ((const double2 *)L.P)[2 * j];
     const double2 ed = ((const double2 *)L.P)[2 * j + 1];
     const unsigned info = w.x;
-    double kf = exp(ab.x + ab.y * logT - ed.x * uT);
-    if (info & 2u) kf = -kf;
-    const double dlnkf = (ab.y + ed.x * uT) * uT;
+    double kf, dlnkf;
+    if (XK && (info & 128u)) {    // PLOG
+        const int s = (int)ab.x;
+        kf = exp(plog_lnk(L.pll + 4 * L.plp[s], (int)(L.plp[s + 1] - L.plp[s]), Pk, logT, uT, dlnkf));
+    } else {
+        kf = exp(ab.x + ab.y * logT - ed.x * uT);
+        if (info & 2u) kf = -kf;
+        dlnkf = (ab.y + ed.x * uT) * uT;
+    }
     double kmod = kf, dlnkmod = dlnkf, dkdM = 0.;
     int dense = 0, single = -1;
     const int tb = (int)((info >> 8) & 0xffu) - 1;
@@ -418,6 +486,7 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
             Mc = scg[single].x + 1.e-16;
         }
         const double Pr = kf * Mc / kinf;
+        const bool cabr = XK && (info & 64u);
         double F = 1., dlgF_dlgPr = 0., dlnF_dT = 0.;
         if (fp[7] < 0.) {    // SRI: ln F = X ln Fs + e ln T + ln d, X = 1/(1 + u^2), u = log10 Pr
             const double eb = exp(-fp[4] * uT), ec = exp(-T / fp[5]);
@@ -444,7 +513,7 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
             const double Lf = (Fc < 1.e-300) ? -300. : log10(Fc);
             const double dL = (Fc < 1.e-300) ? 0. : dFc / (Fc * LN10);
             double lgF, dlgF_dL;
-            if (Pr > 1.e-32) {
+            if (Pr > 1.e-32 || cabr) {
                 const double n = 0.75 - 1.27 * Lf, cc = -0.4 - 0.67 * Lf;
                 const double sT = log10(Pr) + cc;
                 const double den = n - 0.14 * sT;
@@ -462,9 +531,15 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
             F = exp(LN10 * lgF);
             dlnF_dT = dlgF_dlgPr * (dlnkf - dlnkinf) + LN10 * dlgF_dL * dL;
         }
-        kmod = kinf * (Pr / (1. + Pr)) * F;
-        dkdM = (kf * F / (1. + Pr)) * (1. / (1. + Pr) + dlgF_dlgPr);   // d kmod / d M
-        dlnkmod = dlnkinf + (dlnkf - dlnkinf) / (1. + Pr) + dlnF_dT;
+        if (cabr) {    // k0 F/(1+Pr): d/dM = k0 F/(1+Pr) (dlog F/dlog Pr / M - (k0/k_inf)/(1+Pr))
+            kmod = kf * F / (1. + Pr);
+            dkdM = kmod * (dlgF_dlgPr / Mc - (kf / kinf) / (1. + Pr));
+            dlnkmod = dlnkf + dlnF_dT - (Pr / (1. + Pr)) * (dlnkf - dlnkinf);
+        } else {
+            kmod = kinf * (Pr / (1. + Pr)) * F;
+            dkdM = (kf * F / (1. + Pr)) * (1. / (1. + Pr) + dlgF_dlgPr);   // d kmod / d M
+            dlnkmod = dlnkinf + (dlnkf - dlnkinf) / (1. + Pr) + dlnF_dT;
+        }
     }
     if (info & 4u) {
         const unsigned sl[6] = {w.y & 0xffu, (w.y >> 8) & 0xffu, (w.y >> 16) & 0xffu,
@@ -532,7 +607,7 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
     const unsigned nord = info >> 24;
     const bool sf = info & 8u, sb = info & 16u;
     const unsigned fb = nord ? e0 + nre + npr : e0, fn = nord ? nord : nre;
-    double pf = 1., pb = 1., dg = 0., dh = 0., uK = 0.;
+    double pf = 1., pb = 1., dg = 0., dh = 0., uK = 0., dlnuK = 0.;
     for (unsigned q = 0; q < fn; q++) {
         const unsigned e = L.ent[fb + q];
         const double cq = scg[e & 0xfffu].x;
@@ -546,11 +621,19 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
             dh += nu * sH[e & 0xfffu];
             if (q >= nre) pb *= sb ? order_pow(scg[e & 0xfffu].x, nu) : conc_pow(scg[e & 0xfffu].x, nu);
         }
-        uK = exp(dg - logPRT * ed.y);
+        if (XK && (info & 32u)) {    // explicit reverse: uK = k_rev / k
+            const double *rv = L.rvp + 4 * (int)ed.y;
+            uK = exp(rv[0] + rv[1] * logT - rv[2] * uT) / kf;
+            dlnuK = (rv[1] + rv[2] * uT) * uT - dlnkf;
+        } else {
+            uK = exp(dg - logPRT * ed.y);
+            dlnuK = (ed.y - dh) * uT;
+        }
     }
     const double net = pf - pb * uK;
     const double r = kmod * net;
-    const double drdT = kmod * (dlnkmod * net - pb * uK * (ed.y - dh) * uT);
+    const double drdT = XK ? kmod * (dlnkmod * net - pb * uK * dlnuK)
+                           : kmod * (dlnkmod * net - pb * uK * (ed.y - dh) * uT);
     for (unsigned q = 0; q < nre + npr; q++) {
         const unsigned e = L.ent[e0 + q];
         const double nu = L.nu(e);
@@ -601,7 +684,7 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
     }
 }
 
-template <int NMAX>
+template <int NMAX, bool XK>
 struct TeamCtx {
     double V[NVEC][1];
     int ps;          // elimination step at which this lane's row became the pivot
@@ -646,8 +729,9 @@ struct TeamCtx {
     __device__ __forceinline__ double reduce(double v) { return wave_sum(v); }
     __device__ __forceinline__ int count(bool b) { return __popcll(__ballot(b)); }
 
-    __device__ __forceinline__ void reaction(int j, double T, double logT, double uT, double logPRT, double cTot) {
-        rxn_rate(L, scg, j, T, logT, uT, logPRT, cTot, [&](unsigned k, double v) { atomicAdd(&sR[k], v); });
+    __device__ __forceinline__ void reaction(int j, double T, double logT, double uT, double logPRT, double Pk,
+                                            double cTot) {
+        rxn_rate<XK>(L, scg, j, T, logT, uT, logPRT, Pk, cTot, [&](unsigned k, double v) { atomicAdd(&sR[k], v); });
     }
 
     __device__ __forceinline__ void rhs(int src, int dst) {
@@ -693,7 +777,10 @@ struct TeamCtx {
         LSM_TOC(7, t_rhs0);
         LSM_TIC(t_rhs1);
         const double logPRT = log(101325. / RGAS / T);
-        for (int j = lane; j < nr; j += 64) reaction(j, T, logT, uT, logPRT, cTot);
+        // the kinetics' pressure: P0, or cTot R T at constant volume
+        // (BatchReactorHomogeneousConstantVolume.H:166-176)
+        const double Pk = K.const_volume ? cTotP * RGAS * T : K.pressure;
+        for (int j = lane; j < nr; j += 64) reaction(j, T, logT, uT, logPRT, Pk, cTot);
         wave_sync();
         LSM_TOC(8, t_rhs1);
         LSM_TIC(t_rhs2);
@@ -755,7 +842,7 @@ struct TeamCtx {
         __device__ __forceinline__ void D(unsigned i, unsigned k, double v) const { atomicAdd(&Dm[i * 65 + k], v); }
     };
     __device__ __forceinline__ void reaction_jac(int j, double T, double logT, double uT, double logPRT, double cTot) {
-        rxn_jac(L, scg, sH, ns, LSM_DUMMY_SP, j, T, logT, uT, logPRT, cTot, JacAcc{sR, sDT, LU});
+        rxn_jac<XK>(L, scg, sH, ns, LSM_DUMMY_SP, j, T, logT, uT, logPRT, K.pressure, cTot, JacAcc{sR, sDT, LU});
     }
 
     // Analytic Jacobian of the constant-pressure reactor at the state in VB:
@@ -1019,7 +1106,7 @@ struct TeamCtx {
 };
 
 
-template <int NMAX>
+template <int NMAX, bool XK>
 __global__ void __launch_bounds__(256, (NMAX <= 16 ? 2 : 1)) team_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
                                                    const double *yin, double *yout, int *status, double *q,
                                                    int *counters, unsigned *queue, const int *order,
@@ -1035,7 +1122,7 @@ __global__ void __launch_bounds__(256, (NMAX <= 16 ? 2 : 1)) team_kernel(DevMech
     const int wave = threadIdx.x >> 6;
     double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * team_wave_bytes(M));
     const size_t gw = (size_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    TeamCtx<NMAX> c(M, K, lds_raw, wave_lds, jws + gw * 64 * (size_t)M.ne);
+    TeamCtx<NMAX, XK> c(M, K, lds_raw, wave_lds, jws + gw * 64 * (size_t)M.ne);
     const int lane = c.lane, ns = M.ns, ne = M.ne;
     for (;;) {
         unsigned cell_u = 0;
@@ -1111,6 +1198,7 @@ __global__ void __launch_bounds__(256, (NMAX <= 16 ? 2 : 1)) team_kernel(DevMech
 // J*v.  Formation rates are accumulated per wave and summed in wave order, so
 // the RHS is bitwise reproducible.
 // ---------------------------------------------------------------------------
+template <bool XK>
 struct WideCtx {
     double V[NVEC][1];
     const KParams &K;
@@ -1236,9 +1324,10 @@ struct WideCtx {
         double cTot = ck, cpx = isp ? cp * xk : 0.;
         wg_sum2(cTot, cpx);                        // also publishes scg and the cleared sR
         const double logT = log(T), uT = 1. / T, logPRT = log(101325. / RGAS / T);
+        const double Pk = K.const_volume ? cTotP * RGAS * T : K.pressure;   // the kinetics' pressure
         double *sRw = sR + 256 * wave;
         for (int j = tid; j < nr; j += WIDE_THREADS)
-            rxn_rate(L, scg, j, T, logT, uT, logPRT, cTot, [&](unsigned kk, double v) { atomicAdd(&sRw[kk], v); });
+            rxn_rate<XK>(L, scg, j, T, logT, uT, logPRT, Pk, cTot, [&](unsigned kk, double v) { atomicAdd(&sRw[kk], v); });
         __syncthreads();
         const double Rk = isp ? ((sR[k] + sR[256 + k]) + (sR[512 + k] + sR[768 + k])) : 0.;
         double dy = isp ? (mwk * Rk) / rho : 0.;
@@ -1357,7 +1446,7 @@ struct WideCtx {
         for (int w = 0; w < WIDE_THREADS / 64; w++) {
             if (wave == w)
                 for (int j = tid; j < nr; j += WIDE_THREADS)
-                    rxn_jac(L, scg, sH, ns, dummy, j, T, logT, uT, logPRT, cTot, acc);
+                    rxn_jac<XK>(L, scg, sH, ns, dummy, j, T, logT, uT, logPRT, K.pressure, cTot, acc);
             __syncthreads();
         }
         const double Rk = isp ? ((sR[k] + sR[256 + k]) + (sR[512 + k] + sR[768 + k])) : 0.;
@@ -1695,12 +1784,13 @@ struct WideCtx {
 
 // One cell per workgroup, persistent over an atomic queue; same harness as the
 // team kernel (chemistry_DI.H:92-328).
+template <bool XK>
 __global__ void __launch_bounds__(256) wide_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
                                                    const double *yin, double *yout, int *status, double *q,
                                                    int *counters, unsigned *queue, const int *order,
                                                    double *jws) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    WideCtx c(M, K, lds_raw, jws + (size_t)blockIdx.x * M.ne * M.ne);
+    WideCtx<XK> c(M, K, lds_raw, jws + (size_t)blockIdx.x * M.ne * M.ne);
     const int tid = threadIdx.x, ns = M.ns, ne = M.ne;
     int *cell_slot = (int *)(c.red + 16);
     for (;;) {
@@ -1767,6 +1857,7 @@ __global__ void __launch_bounds__(256) wide_kernel(DevMech M, KParams K, int64_t
 // states (GetEquations of the batch-reactor ODE system), one wave per cell, in
 // grid-stride order over the cells so that the per-wave workspace of the
 // integrator is reused.  y: SoA [ne][n]; f: SoA [ne][n]; J: [n][ne][ne] row-major.
+template <bool XK>
 __global__ void __launch_bounds__(256) equations_kernel(DevMech M, KParams K, int64_t n, const double *yin,
                                                         double *f, double *J, double *jws) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -1780,7 +1871,7 @@ __global__ void __launch_bounds__(256) equations_kernel(DevMech M, KParams K, in
     const int wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
     double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * team_wave_bytes(M));
     const size_t gw = (size_t)blockIdx.x * wpb + wave;
-    TeamCtx<64> c(M, K, lds_raw, wave_lds, jws + gw * 64 * (size_t)M.ne);
+    TeamCtx<64, XK> c(M, K, lds_raw, wave_lds, jws + gw * 64 * (size_t)M.ne);
     const int lane = c.lane, ne = M.ne;
     for (int64_t cell = (int64_t)gw; cell < n; cell += (int64_t)gridDim.x * wpb) {
         c.V[VB_][0] = lane < ne ? yin[(int64_t)lane * n + cell] : 0.;
@@ -1800,6 +1891,7 @@ __global__ void __launch_bounds__(256) equations_kernel(DevMech M, KParams K, in
 // ---------------------------------------------------------------------------
 // Plain context: one cell per thread, all state in HBM
 // ---------------------------------------------------------------------------
+template <bool XK>
 struct PlainCtx {
     double *W;        // vectors  [NVEC][ne][slots]
     double *Jm, *Gm;  // matrices [ne][ne][slots]
@@ -1860,9 +1952,10 @@ struct PlainCtx {
             cpx += cp * xk;
         }
         const double logPRT = log(101325. / RGAS / T);
+        const double Pk = K.const_volume ? cTotP * RGAS * T : K.pressure;   // the kinetics' pressure
         for (int j = 0; j < nr; j++)
-            sr[(int64_t)j * S + slot] = reaction_rate(
-                M, j, T, logT, uT, logPRT, cTot, [&](int i) { return sc[(int64_t)i * S + slot]; },
+            sr[(int64_t)j * S + slot] = reaction_rate<XK>(
+                M, j, T, logT, uT, logPRT, Pk, cTot, [&](int i) { return sc[(int64_t)i * S + slot]; },
                 [&](int i) { return sh[(int64_t)i * S + slot]; }, [&](int i) { return ss[(int64_t)i * S + slot]; });
         double hs = 0., rs = 0.;
         for (int k = 0; k < ns; k++) {
@@ -1965,12 +2058,13 @@ struct PlainCtx {
     }
 };
 
+template <bool XK>
 __global__ void __launch_bounds__(256) plain_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
                                                     const double *yin, double *yout, int *status, double *q,
                                                     int *counters, double *work, int64_t slots) {
     const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= slots) return;
-    PlainCtx c(M, K, work, slots, slot);
+    PlainCtx<XK> c(M, K, work, slots, slot);
     const int ns = M.ns, ne = M.ne;
     for (int64_t cell = slot; cell < n; cell += slots) {
         const double T0 = yin[(int64_t)ns * n + cell];
@@ -2052,19 +2146,19 @@ int team_waves_per_block(const DevMech &m, size_t lds_per_block) {
     return (int)(w < 4 ? w : 4);
 }
 
-template <int NMAX>
+template <int NMAX, bool XK>
 static int launch_team_n(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                          double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order,
                          int grid, int wpb, double *jws, hipStream_t s) {
     const size_t lds = (size_t)m.img.bytes + (size_t)wpb * team_wave_bytes(m);
     static size_t lds_set = 65536;   // largest dynamic LDS size granted so far (not re-set under graph capture)
     if (lds > lds_set) {
-        if (hipFuncSetAttribute((const void *)team_kernel<NMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        if (hipFuncSetAttribute((const void *)team_kernel<NMAX, XK>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds) != hipSuccess)
             return -1;
         lds_set = lds;
     }
-    hipLaunchKernelGGL(team_kernel<NMAX>, dim3(grid), dim3(64 * wpb), lds, s, m, k, n, dt, yin, yout, status, q,
+    hipLaunchKernelGGL((team_kernel<NMAX, XK>), dim3(grid), dim3(64 * wpb), lds, s, m, k, n, dt, yin, yout, status, q,
                        counters, queue, order, jws);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -2075,8 +2169,11 @@ int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt,
     hipStream_t s = (hipStream_t)stream;
     if (wpb < 1 || wpb > 4) return -2;
 #define LSM_TEAM_CASE(NM)                                                                                           \
-    if (m.ne <= NM) return launch_team_n<NM>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, \
-                                             jws, s);
+    if (m.ne <= NM)                                                                                                 \
+        return m.xk ? launch_team_n<NM, true>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, \
+                                              jws, s)                                                               \
+                    : launch_team_n<NM, false>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid,    \
+                                               wpb, jws, s);
     LSM_TEAM_CASE(16)
     LSM_TEAM_CASE(32)
     LSM_TEAM_CASE(48)
@@ -2148,15 +2245,19 @@ int launch_equations(const DevMech &m, const KParams &k, int64_t n, const double
     const int g = (int)(need < grid ? need : grid);
     if (g < 1) return 0;
     const size_t lds = (size_t)m.img.bytes + (size_t)wpb * team_wave_bytes(m);
-    if (lds > 65536 &&
-        hipFuncSetAttribute((const void *)equations_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-            hipSuccess)
+    const void *fn = m.xk ? (const void *)equations_kernel<true> : (const void *)equations_kernel<false>;
+    if (lds > 65536 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return -1;
-    hipLaunchKernelGGL(equations_kernel, dim3(g), dim3(64 * wpb), lds, (hipStream_t)stream, m, k, n, yin, f, J, jws);
+    if (m.xk)
+        hipLaunchKernelGGL(equations_kernel<true>, dim3(g), dim3(64 * wpb), lds, (hipStream_t)stream, m, k, n, yin, f,
+                           J, jws);
+    else
+        hipLaunchKernelGGL(equations_kernel<false>, dim3(g), dim3(64 * wpb), lds, (hipStream_t)stream, m, k, n, yin, f,
+                           J, jws);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-size_t wide_lds_bytes(const DevMech &m) { return WideCtx::lds_bytes(m.ne); }
+size_t wide_lds_bytes(const DevMech &m) { return WideCtx<false>::lds_bytes(m.ne); }
 size_t wide_jac_doubles(const DevMech &m) { return (size_t)m.ne * m.ne; }
 
 int launch_wide(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
@@ -2164,15 +2265,18 @@ int launch_wide(const DevMech &m, const KParams &k, int64_t n, const double *dt,
                 void *stream) {
     if (!m.img.wide_ok || m.ne > WIDE_MAX_NEQ) return -2;
     const size_t lds = wide_lds_bytes(m);
-    static size_t lds_set = 65536;
-    if (lds > lds_set) {
-        if (hipFuncSetAttribute((const void *)wide_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
-            hipSuccess)
-            return -1;
-        lds_set = lds;
+    static size_t lds_set[2] = {65536, 65536};
+    const void *fn = m.xk ? (const void *)wide_kernel<true> : (const void *)wide_kernel<false>;
+    if (lds > lds_set[m.xk]) {
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -1;
+        lds_set[m.xk] = lds;
     }
-    hipLaunchKernelGGL(wide_kernel, dim3(grid), dim3(WIDE_THREADS), lds, (hipStream_t)stream, m, k, n, dt, yin, yout,
-                       status, q, counters, queue, order, jws);
+    if (m.xk)
+        hipLaunchKernelGGL(wide_kernel<true>, dim3(grid), dim3(WIDE_THREADS), lds, (hipStream_t)stream, m, k, n, dt,
+                           yin, yout, status, q, counters, queue, order, jws);
+    else
+        hipLaunchKernelGGL(wide_kernel<false>, dim3(grid), dim3(WIDE_THREADS), lds, (hipStream_t)stream, m, k, n, dt,
+                           yin, yout, status, q, counters, queue, order, jws);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -2186,8 +2290,12 @@ int launch_plain(const DevMech &m, const KParams &k, int64_t n, const double *dt
                  int *status, double *q, int *counters, double *work, int64_t slots, void *stream) {
     const int bs = 256;
     const int grid = (int)((slots + bs - 1) / bs);
-    hipLaunchKernelGGL(plain_kernel, dim3(grid), dim3(bs), 0, (hipStream_t)stream, m, k, n, dt, yin, yout, status,
-                       q, counters, work, slots);
+    if (m.xk)
+        hipLaunchKernelGGL(plain_kernel<true>, dim3(grid), dim3(bs), 0, (hipStream_t)stream, m, k, n, dt, yin, yout,
+                           status, q, counters, work, slots);
+    else
+        hipLaunchKernelGGL(plain_kernel<false>, dim3(grid), dim3(bs), 0, (hipStream_t)stream, m, k, n, dt, yin, yout,
+                           status, q, counters, work, slots);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

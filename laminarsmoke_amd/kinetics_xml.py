@@ -14,9 +14,12 @@ Reader: the sections read by
     Reversible, Reversible-Thermodynamics, Reversible-Explicit, ThreeBody, FallOff
     with Lindemann/Troe/SRI, CABR, ...), KineticParameters (Direct lnA /
     negative-lnA / Beta / E_over_R; ThreeBody per-reaction species and efficiency
-    lists; FallOff k_inf lnA / Beta / E_over_R and per-reaction "lindemann" |
+    lists; FallOff and CABR k_inf lnA / Beta / E_over_R and per-reaction "lindemann" |
     "troe n a b c [d]" | "sri n a b c [d e]" followed by "species k" or "thirdbody"
-    and the species / efficiency lists);
+    and the species / efficiency lists; Reverse lnA / Beta / E_over_R of the
+    Reversible-Explicit reactions; PressureLog per-reaction "n (P A beta E)... convA
+    convE" lists, P in atm (kernel/kinetics/PressureLogarithmicRateExpression.hpp:71-102,
+    121-132));
   * StoichiometricMap::ReadFromASCIIFile (maps/StoichiometricMap.hpp:78-254): the
     species-major lists by stoichiometric class (1, 2, 3, 0.5, other) for reactants
     (numDir/jDir/valueDir5), products of all reactions (numRevTot/jRevTot/
@@ -32,8 +35,8 @@ CHEMKIN input can be stored the way laminarSMOKE cases keep them.
 
 Explicit forward reaction orders (the lambda lists after the flag) are read into
 Mechanism.ord_*; explicit reverse orders that differ from the product coefficients,
-and forms the kernels do not evaluate (explicitly reversible, CABR, Chebyshev, PLOG,
-extended fall-off, FIT1, Janev-Langer, Landau-Teller), raise KineticsXmlError.
+and forms the kernels do not evaluate (Chebyshev, extended PLOG and fall-off, FIT1,
+Janev-Langer, Landau-Teller), raise KineticsXmlError.
 """
 from __future__ import annotations
 
@@ -42,11 +45,11 @@ import xml.etree.ElementTree as ET
 import numpy as np
 
 from .chemkin import ATOMIC_WEIGHTS
-from .mechanism import FALLOFF_LINDEMANN, FALLOFF_SRI, FALLOFF_TROE, RXN_NEGATIVE_A, RXN_REVERSIBLE, Mechanism
+from .mechanism import (FALLOFF_CHEMACT, FALLOFF_LINDEMANN, FALLOFF_SRI, FALLOFF_TROE, R_J_KMOL, RXN_EXPLICIT_REV,
+                        RXN_NEGATIVE_A, RXN_REVERSIBLE, Mechanism)
 
 VERSION = "04-22-2013"
-_UNSUPPORTED_LISTS = ("Reversible-Explicit", "CABR", "Chebyshev", "PressureLog", "ExtendedPressureLog",
-                      "ExtendedFallOff", "FIT1", "JanevLanger", "LandauTeller")
+_UNSUPPORTED_LISTS = ("Chebyshev", "ExtendedPressureLog", "ExtendedFallOff", "FIT1", "JanevLanger", "LandauTeller")
 
 
 class KineticsXmlError(ValueError):
@@ -117,7 +120,10 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
     if kin.get("type") != "OpenSMOKE" or kin.get("version") != VERSION:
         raise KineticsXmlError("The current kinetic scheme is not supported (type/version)")
     nr = int(_node(kin, "NumberOfReactions").text.split()[0])
-    reversible = set(_ilist(kin, "Reversible-Thermodynamics"))
+    reversible = set(_ilist(kin, "Reversible"))
+    explicit = _ilist(kin, "Reversible-Explicit", required=False)
+    if not set(explicit) <= reversible:
+        raise KineticsXmlError("Reversible-Explicit reactions must be in the Reversible list")
     for tag in _UNSUPPORTED_LISTS:
         n = kin.find(tag)
         if n is not None and _Tokens(n.text).vec(int):
@@ -125,6 +131,9 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
     tb_idx = _ilist(kin, "ThreeBody")
     fo_node = _node(kin, "FallOff")
     fo_idx = _Tokens(fo_node.text).vec(int)
+    cabr_node = kin.find("CABR")
+    cabr_idx = _Tokens(cabr_node.text).vec(int) if cabr_node is not None else []
+    pl_idx = _ilist(kin, "PressureLog", required=False)
 
     kp = _node(kin, "KineticParameters")
     direct = _node(kp, "Direct")
@@ -140,6 +149,15 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
             flags[j - 1] |= RXN_REVERSIBLE
         if j in neg:
             flags[j - 1] |= RXN_NEGATIVE_A
+    rev_arr = np.zeros((nr, 3))
+    if explicit:                                   # KineticsMap_CHEMKIN.hpp:611-639
+        rv = _node(kp, "Reverse")
+        ra = [_Tokens(_node(rv, tag).text).vec() for tag in ("lnA", "Beta", "E_over_R")]
+        if not all(len(v) == len(explicit) for v in ra):
+            raise KineticsXmlError("Reverse parameters do not match the Reversible-Explicit list")
+        for q, j in enumerate(explicit):
+            flags[j - 1] |= RXN_EXPLICIT_REV
+            rev_arr[j - 1] = (ra[0][q], ra[1][q], ra[2][q])
 
     tb_sp, tb_eff = [], []
     if tb_idx:
@@ -154,33 +172,36 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
             tb_eff.append(ef)
 
     fo_type, fo_inf, fo_troe, fo_sri, fo_single, fo_sp, fo_eff = [], [], [], [], [], [], []
-    if fo_idx:
-        fk = _node(kp, "FallOff")
+    # fall-off entries, then the chemically activated ones (KineticsMap_CHEMKIN.hpp:786-912)
+    for tag, idx_list, chem in (("FallOff", fo_idx, 0), ("CABR", cabr_idx, FALLOFF_CHEMACT)):
+        if not idx_list:
+            continue
+        fk = _node(kp, tag)
         a = _Tokens(_node(fk, "lnA").text).vec()
         b = _Tokens(_node(fk, "Beta").text).vec()
         e = _Tokens(_node(fk, "E_over_R").text).vec()
         nodes = list(_node(fk, "Parameters").findall("reaction"))
-        if not (len(a) == len(b) == len(e) == len(nodes) == len(fo_idx)):
-            raise KineticsXmlError("FallOff parameters do not match the FallOff list")
+        if not (len(a) == len(b) == len(e) == len(nodes) == len(idx_list)):
+            raise KineticsXmlError(f"{tag} parameters do not match the {tag} list")
         for k, nd in enumerate(nodes):
             fo_inf.append((a[k], b[k], e[k]))
             t = _Tokens(nd.text)
             kind = t.word()
             troe, sri = (0.0, 1.0, 1.0, 0.0), (0.0, 0.0, 1.0, 1.0, 0.0)
             if kind == "lindemann":
-                fo_type.append(FALLOFF_LINDEMANN)
+                fo_type.append(FALLOFF_LINDEMANN | chem)
             elif kind == "troe":
                 c = t.vec()
                 if len(c) not in (3, 4):
                     raise KineticsXmlError("troe needs 3 or 4 parameters")
                 troe = tuple(c) + ((0.0,) if len(c) == 3 else ())          # KineticsMap_CHEMKIN.hpp:736-746
-                fo_type.append(FALLOFF_TROE)
+                fo_type.append(FALLOFF_TROE | chem)
             elif kind == "sri":
                 c = t.vec()
                 if len(c) not in (3, 5):
                     raise KineticsXmlError("sri needs 3 or 5 parameters")
                 sri = tuple(c) + ((1.0, 0.0) if len(c) == 3 else ())       # KineticsMap_CHEMKIN.hpp:749-767
-                fo_type.append(FALLOFF_SRI)
+                fo_type.append(FALLOFF_SRI | chem)
             else:
                 raise KineticsXmlError(f"unknown fall-off form {kind!r}")
             fo_troe.append(troe)
@@ -195,6 +216,32 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
                 fo_sp.append([k - 1 for k in t.vec(int)])
                 fo_eff.append([v - 1.0 for v in t.vec()])
 
+    # one fall-off table in reaction order (the FallOff and CABR lists are each sorted)
+    fo_all = list(fo_idx) + list(cabr_idx)
+    perm = sorted(range(len(fo_all)), key=lambda i: fo_all[i])
+    fo_all = [fo_all[i] for i in perm]
+    fo_type, fo_inf, fo_troe, fo_sri, fo_single, fo_sp, fo_eff = (
+        [v[i] for i in perm] for v in (fo_type, fo_inf, fo_troe, fo_sri, fo_single, fo_sp, fo_eff))
+
+    pl_lev = []
+    if pl_idx:                                     # KineticsMap_CHEMKIN.hpp:927-937
+        t = _Tokens(_node(kp, "PressureLog").text)
+        for _ in pl_idx:
+            c = t.vec()
+            nlev = (len(c) - 2) // 4
+            if nlev < 1 or len(c) != 4 * nlev + 2:
+                raise KineticsXmlError("PressureLog entry is not n (P A beta E)... convA convE")
+            ca, ce = c[-2], c[-1]
+            lev = []
+            for i in range(nlev):
+                P, A, bp, E = c[4 * i:4 * i + 4]
+                if A < 0.0:
+                    raise KineticsXmlError("PLOG needs non-negative frequency factors")
+                lev.append((P * 101325.0, np.log(max(A * ca, 1e-32)), bp, E * ce / R_J_KMOL))
+            if any(y[0] <= x[0] for x, y in zip(lev[:-1], lev[1:])):
+                raise KineticsXmlError("PLOG pressures must increase")
+            pl_lev.append(lev)
+
     st = _node(kin, "Stoichiometry")
     if st.get("type") != "OpenSMOKE" or st.get("version") != VERSION:
         raise KineticsXmlError("The current stoichiometric data are not supported")
@@ -205,7 +252,7 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
         ptr[1:] = np.cumsum([len(x) for x in lists])
         return ptr
 
-    nfo = len(fo_idx)
+    nfo = len(fo_idx) + len(cabr_idx)
     return Mechanism(
         species=list(species), elements=atoms, nasa=nasa,
         lnA=lnA, beta=beta, E_over_R=EoR, rflags=flags,
@@ -216,7 +263,8 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
         tb_rxn=np.array([j - 1 for j in tb_idx], dtype=np.int32), tb_ptr=csr(tb_sp),
         tb_sp=np.array([s for x in tb_sp for s in x], dtype=np.int32),
         tb_eff_m1=np.array([v for x in tb_eff for v in x], dtype=np.float64),
-        fo_rxn=np.array([j - 1 for j in fo_idx], dtype=np.int32), fo_type=np.array(fo_type, dtype=np.int32),
+        fo_rxn=np.array([j - 1 for j in fo_all], dtype=np.int32),
+        fo_type=np.array(fo_type, dtype=np.int32),
         fo_inf=np.array(fo_inf, dtype=np.float64).reshape(nfo, 3),
         fo_troe=np.array(fo_troe, dtype=np.float64).reshape(nfo, 4),
         fo_single=np.array(fo_single, dtype=np.int32), fo_ptr=csr(fo_sp),
@@ -225,6 +273,9 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
         fo_sri=np.array(fo_sri, dtype=np.float64).reshape(nfo, 5),
         ord_ptr=csr(orders), ord_sp=np.array([s for x in orders for s, _ in x], dtype=np.int32),
         ord_val=np.array([v for x in orders for _, v in x], dtype=np.float64),
+        rev_arr=rev_arr, pl_rxn=np.array([j - 1 for j in pl_idx], dtype=np.int32),
+        pl_ptr=np.concatenate([[0], np.cumsum([len(x) for x in pl_lev])]).astype(np.int32),
+        pl_par=np.array([v for x in pl_lev for v in x], dtype=np.float64).reshape(-1, 4),
         name=name, element_names=tuple(e.upper() for e in el_names), atomic_weights=aw,
     )
 
@@ -377,21 +428,30 @@ def write_kinetics_xml(m: Mechanism) -> str:
                       ' type="NASA"'))
     k = [f'<Kinetics type="OpenSMOKE" version="{VERSION}">', _tag("NumberOfReactions", str(nr))]
     k.append(_tag("Irreversible", _ivec(idx(~rev))))
+    xrev = (m.rflags & RXN_EXPLICIT_REV) != 0
     k.append(_tag("Reversible", _ivec(idx(rev))))
-    k.append(_tag("Reversible-Thermodynamics", _ivec(idx(rev))))
-    k.append(_tag("Reversible-Explicit", _ivec([])))
+    k.append(_tag("Reversible-Thermodynamics", _ivec(idx(rev & ~xrev))))
+    k.append(_tag("Reversible-Explicit", _ivec(idx(xrev))))
     k.append(_tag("ThreeBody", _ivec(m.tb_rxn + 1)))
     fo_types = m.fo_type.tolist()
-    k.append("<FallOff>\n" + _ivec(m.fo_rxn + 1) + "\n" +
-             _tag("Lindemann", _ivec([int(m.fo_rxn[i]) + 1 for i, t in enumerate(fo_types) if t == FALLOFF_LINDEMANN])) + "\n" +
-             _tag("Troe", _ivec([int(m.fo_rxn[i]) + 1 for i, t in enumerate(fo_types) if t == FALLOFF_TROE])) + "\n" +
-             _tag("SRI", _ivec([int(m.fo_rxn[i]) + 1 for i, t in enumerate(fo_types) if t == FALLOFF_SRI])) +
-             "\n</FallOff>")
-    k.append("<CABR>\n0\n" + _tag("Lindemann", "0") + "\n" + _tag("Troe", "0") + "\n" + _tag("SRI", "0") + "\n</CABR>")
+    groups = {}
+    for chem, tag in ((0, "FallOff"), (FALLOFF_CHEMACT, "CABR")):
+        sel = [i for i, t in enumerate(fo_types) if (t & FALLOFF_CHEMACT) == chem]
+        groups[tag] = sel
+        by = lambda form: _ivec([int(m.fo_rxn[i]) + 1 for i in sel if fo_types[i] & 3 == form])
+        k.append(f"<{tag}>\n" + _ivec([int(m.fo_rxn[i]) + 1 for i in sel]) + "\n" +
+                 _tag("Lindemann", by(FALLOFF_LINDEMANN)) + "\n" + _tag("Troe", by(FALLOFF_TROE)) + "\n" +
+                 _tag("SRI", by(FALLOFF_SRI)) + f"\n</{tag}>")
+    if len(m.pl_rxn):
+        k.append(_tag("PressureLog", _ivec(m.pl_rxn + 1)))
     neg = [j + 1 for j in range(nr) if m.rflags[j] & RXN_NEGATIVE_A]
     direct = "\n".join([_tag("lnA", _vec(m.lnA)), _tag("negative-lnA", _ivec(neg)),
                         _tag("Beta", _vec(m.beta)), _tag("E_over_R", _vec(m.E_over_R))])
     kp = [_tag("Direct", direct)]
+    if xrev.any():
+        rj = np.nonzero(xrev)[0]
+        kp.append(_tag("Reverse", "\n".join(_tag(t, _vec(m.rev_arr[rj, i]))
+                                            for i, t in enumerate(("lnA", "Beta", "E_over_R")))))
     if len(m.tb_rxn):
         rs = []
         for s in range(len(m.tb_rxn)):
@@ -399,10 +459,13 @@ def write_kinetics_xml(m: Mechanism) -> str:
             body = _ivec(m.tb_sp[a:b] + 1) + "\n" + _vec(m.tb_eff_m1[a:b] + 1.0, per_line=30)
             rs.append(_tag("reaction", body, f' i="{int(m.tb_rxn[s]) + 1}"'))
         kp.append(_tag("ThreeBody", "\n".join(rs)))
-    if len(m.fo_rxn):
+    for tag in ("FallOff", "CABR"):
+        sel = groups[tag]
+        if not sel:
+            continue
         ps = []
-        for s in range(len(m.fo_rxn)):
-            t = fo_types[s]
+        for s in sel:
+            t = fo_types[s] & 3
             if t == FALLOFF_LINDEMANN:
                 head = "lindemann"
             elif t == FALLOFF_TROE:
@@ -416,9 +479,17 @@ def write_kinetics_xml(m: Mechanism) -> str:
                 a, b = m.fo_ptr[s], m.fo_ptr[s + 1]
                 tail = "thirdbody\n" + _ivec(m.fo_sp[a:b] + 1) + "\n" + _vec(m.fo_eff_m1[a:b] + 1.0, per_line=30)
             ps.append(_tag("reaction", head + "\n" + tail, f' i="{int(m.fo_rxn[s]) + 1}"'))
-        kp.append(_tag("FallOff", "\n".join([_tag("lnA", _vec(m.fo_inf[:, 0])), _tag("Beta", _vec(m.fo_inf[:, 1])),
-                                             _tag("E_over_R", _vec(m.fo_inf[:, 2])),
-                                             _tag("Parameters", "\n".join(ps))])))
+        inf = m.fo_inf[sel]
+        kp.append(_tag(tag, "\n".join([_tag("lnA", _vec(inf[:, 0])), _tag("Beta", _vec(inf[:, 1])),
+                                       _tag("E_over_R", _vec(inf[:, 2])), _tag("Parameters", "\n".join(ps))])))
+    if len(m.pl_rxn):
+        # raw values with unit factors 1 (A) and R (E): P [atm], A, beta, E/R
+        es = []
+        for s in range(len(m.pl_rxn)):
+            lev = m.pl_par[m.pl_ptr[s]:m.pl_ptr[s + 1]]
+            vals = [v for P, a, b, e in lev for v in (P / 101325.0, np.exp(a), b, e)] + [1.0, R_J_KMOL]
+            es.append(f"{len(vals)}\n" + " ".join(f"{v:.17g}" for v in vals))
+        kp.append(_tag("PressureLog", "\n".join(es)))
     k.append(_tag("KineticParameters", "\n".join(kp)))
     k.append(_tag("Stoichiometry", _stoichiometry(m), f' type="OpenSMOKE" version="{VERSION}"'))
     k.append("</Kinetics>")

@@ -39,9 +39,11 @@ R_J_MOL = 8.3144621
 J_FROM_CAL = 4.18443
 RXN_REVERSIBLE = 1
 RXN_NEGATIVE_A = 2
+RXN_EXPLICIT_REV = 4      # with RXN_REVERSIBLE: reverse k from rev_arr (REV)
 FALLOFF_LINDEMANN = 0
 FALLOFF_TROE = 1
 FALLOFF_SRI = 2
+FALLOFF_CHEMACT = 8       # bit ORed onto the form: chemically activated (HIGH, CABR)
 
 
 @dataclass
@@ -78,6 +80,10 @@ class Mechanism:
     ord_ptr: object = None            # [nr+1] explicit forward orders (FORD), CSR; empty row: orders = coefficients
     ord_sp: object = None
     ord_val: object = None
+    rev_arr: object = None            # [nr, 3] explicit reverse lnA, beta, E_over_R (RXN_EXPLICIT_REV rows)
+    pl_rxn: object = None             # [npl] PLOG reactions
+    pl_ptr: object = None             # [npl+1] their pressure levels
+    pl_par: object = None             # [levels, 4] P [Pa], lnA, beta, E_over_R
     element_names: tuple = ELEMENTS    # columns of `elements`
     atomic_weights: object = None      # [n_elements] kg/kmol (default: C, H, O, N)
     _struct: object = field(default=None, repr=False)
@@ -116,6 +122,14 @@ class Mechanism:
         self.ord_ptr = np.ascontiguousarray(self.ord_ptr, dtype=np.int32)
         self.ord_sp = np.ascontiguousarray(self.ord_sp, dtype=np.int32)
         self.ord_val = np.ascontiguousarray(self.ord_val, dtype=np.float64)
+        if self.rev_arr is None:
+            self.rev_arr = np.zeros((self.nr, 3))
+        self.rev_arr = np.ascontiguousarray(np.asarray(self.rev_arr, dtype=np.float64).reshape(self.nr, 3))
+        if self.pl_rxn is None:
+            self.pl_rxn, self.pl_ptr, self.pl_par = [], [0], np.zeros((0, 4))
+        self.pl_rxn = np.ascontiguousarray(self.pl_rxn, dtype=np.int32)
+        self.pl_ptr = np.ascontiguousarray(self.pl_ptr, dtype=np.int32)
+        self.pl_par = np.ascontiguousarray(np.asarray(self.pl_par, dtype=np.float64).reshape(-1, 4))
         if self.atomic_weights is None:
             self.atomic_weights = ATOMIC_WEIGHTS
         self.atomic_weights = np.asarray(self.atomic_weights, dtype=np.float64)
@@ -137,6 +151,7 @@ class Mechanism:
         assert self.fo_ptr.shape == (len(self.fo_rxn) + 1,)
         assert self.fo_inf.shape == (len(self.fo_rxn), 3)
         assert self.fo_troe.shape == (len(self.fo_rxn), 4)
+        assert self.pl_ptr.shape == (len(self.pl_rxn) + 1,) and self.pl_ptr[-1] == self.pl_par.shape[0]
 
     def struct(self) -> LsmMechanism:
         """ctypes view of the tables (valid while this object lives)."""
@@ -160,6 +175,9 @@ class Mechanism:
             s.fo_eff_m1 = P(self.fo_eff_m1, d)
             s.fo_sri = P(self.fo_sri, d)
             s.ord_ptr, s.ord_sp, s.ord_val = P(self.ord_ptr, i), P(self.ord_sp, i), P(self.ord_val, d)
+            s.rev_arr = P(self.rev_arr, d)
+            s.npl = len(self.pl_rxn)
+            s.pl_rxn, s.pl_ptr, s.pl_par = P(self.pl_rxn, i), P(self.pl_ptr, i), P(self.pl_par, d)
             self._struct = s
         return self._struct
 
@@ -445,9 +463,12 @@ def custom_mechanism(species, nasa, reactions, elements=None, name="custom") -> 
 
     ``reactions`` is a list of dicts with keys ``reac`` / ``prod`` (lists of
     (species index, coefficient)), ``lnA``, ``beta``, ``E_over_R``, optional
-    ``reversible`` (default True), ``tb`` (list of (species, eff)) and ``falloff``
-    (dict with ``inf`` = (lnA, beta, EoR), ``troe`` = (a, T3, T1, T2) or None,
-    ``single`` = species or -1, ``eff`` = list of (species, eff)).
+    ``reversible`` (default True), ``rev`` (explicit reverse (lnA, beta, EoR)),
+    ``tb`` (list of (species, eff)), ``falloff`` (dict with ``inf`` = (lnA, beta,
+    EoR), ``troe`` = (a, T3, T1, T2) or None, ``sri`` = (a, b, c, d, e) or None,
+    ``cabr`` = True for a chemically activated reaction, ``single`` = species or
+    -1, ``eff`` = list of (species, eff)) and ``plog`` (list of (P [Pa], lnA,
+    beta, EoR) levels).
     """
     ns = len(species)
     nasa = np.asarray(nasa, dtype=np.float64).reshape(ns, 18)
@@ -456,6 +477,7 @@ def custom_mechanism(species, nasa, reactions, elements=None, name="custom") -> 
     lnA, beta, EoR, flags, reac, prod = [], [], [], [], [], []
     tb_rxn, tb_sp, tb_eff = [], [], []
     fo_rxn, fo_type, fo_inf, fo_troe, fo_single, fo_sp, fo_eff = [], [], [], [], [], [], []
+    fo_sri, rev_arr, pl_rxn, pl_lev = [], np.zeros((len(reactions), 3)), [], []
     for j, r in enumerate(reactions):
         lnA.append(r["lnA"])
         beta.append(r.get("beta", 0.0))
@@ -463,6 +485,12 @@ def custom_mechanism(species, nasa, reactions, elements=None, name="custom") -> 
         fl = RXN_REVERSIBLE if r.get("reversible", True) else 0
         if r.get("negative_A", False):
             fl |= RXN_NEGATIVE_A
+        if r.get("rev") is not None:
+            fl |= RXN_REVERSIBLE | RXN_EXPLICIT_REV
+            rev_arr[j] = r["rev"]
+        if r.get("plog") is not None:
+            pl_rxn.append(j)
+            pl_lev.append([tuple(v) for v in r["plog"]])
         flags.append(fl)
         reac.append(list(r["reac"]))
         prod.append(list(r["prod"]))
@@ -473,7 +501,10 @@ def custom_mechanism(species, nasa, reactions, elements=None, name="custom") -> 
         if "falloff" in r:
             f = r["falloff"]
             fo_rxn.append(j)
-            fo_type.append(FALLOFF_TROE if f.get("troe") is not None else FALLOFF_LINDEMANN)
+            form = (FALLOFF_TROE if f.get("troe") is not None else
+                    FALLOFF_SRI if f.get("sri") is not None else FALLOFF_LINDEMANN)
+            fo_type.append(form | (FALLOFF_CHEMACT if f.get("cabr") else 0))
+            fo_sri.append(f.get("sri") or (0.0, 0.0, 1.0, 1.0, 0.0))
             fo_inf.append(f["inf"])
             fo_troe.append(f.get("troe") or (0.0, 1.0, 1.0, 0.0))
             fo_single.append(f.get("single", -1))
@@ -503,6 +534,9 @@ def custom_mechanism(species, nasa, reactions, elements=None, name="custom") -> 
         fo_single=np.array(fo_single, dtype=np.int32), fo_ptr=csr(fo_sp),
         fo_sp=np.array([s for x in fo_sp for s in x], dtype=np.int32),
         fo_eff_m1=np.array([e for x in fo_eff for e in x], dtype=np.float64),
+        fo_sri=np.array(fo_sri, dtype=np.float64).reshape(nfo, 5),
+        rev_arr=rev_arr, pl_rxn=np.array(pl_rxn, dtype=np.int32), pl_ptr=csr(pl_lev),
+        pl_par=np.array([v for x in pl_lev for v in x], dtype=np.float64).reshape(-1, 4),
         name=name,
     )
 

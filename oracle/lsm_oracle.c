@@ -261,7 +261,28 @@ static double order_pow(double C, double lambda) {
 }
 
 /* h, s must already hold h/RT and s/R at T. */
-static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, const double *c) {
+/* PressureLogarithmicRateExpression::KineticConstant
+ * (kernel/kinetics/PressureLogarithmicRateExpression.hpp:104-119): the Arrhenius set
+ * of the first/last level outside the pressure range, else ln k interpolated
+ * linearly in ln P between the two levels around P. */
+static double plog_k(const lsm_mechanism *m, int k, double T, double P) {
+    const double *L = m->pl_par + 4 * m->pl_ptr[k];
+    const int n = m->pl_ptr[k + 1] - m->pl_ptr[k];
+    if (P <= L[0]) return exp(L[1] + L[2] * log(T) - L[3] / T);
+    if (P >= L[4 * (n - 1)]) return exp(L[4 * (n - 1) + 1] + L[4 * (n - 1) + 2] * log(T) - L[4 * (n - 1) + 3] / T);
+    int i;
+    for (i = 0; i < n - 1; i++)
+        if (P < L[4 * (i + 1)]) break;
+    const double *a = L + 4 * i, *b = L + 4 * (i + 1);
+    const double lnkA = a[1] + a[2] * log(T) - a[3] / T;
+    const double lnkB = b[1] + b[2] * log(T) - b[3] / T;
+    const double lnpA = log(a[0]), lnpB = log(b[0]);
+    return exp(lnkA + (lnkB - lnkA) * (log(P) - lnpA) / (lnpB - lnpA));
+}
+
+/* P is the pressure the kinetics map is set to (SetPressure): it only enters the
+ * PLOG constants. */
+static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, double P, const double *c) {
     const lsm_mechanism *m = M->m;
     const int ns = M->ns, nr = M->nr;
     /* SetTemperature, KineticsMap_CHEMKIN.hpp:271-287 */
@@ -291,20 +312,31 @@ static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, const double *
         w->kf[j] = exp(m->lnA[j] + m->beta[j] * logT - m->E_over_R[j] * uT);
         if (m->rflags[j] & LSM_RXN_NEGATIVE_A) w->kf[j] *= -1.;
     }
-    /* hpp:1215-1222: 1/Kc = exp(-dS/R + dH/RT - dn ln(Patm/RT)) */
-    for (int j = 0; j < nr; j++)
-        w->uKeq[j] = (m->rflags[j] & LSM_RXN_REVERSIBLE)
-                         ? exp(-w->dS[j] + w->dH[j] - log_Patm_over_RT * M->dn[j]) : 0.;
-    /* hpp:1256-1290: fall-off k_inf and Troe log10(Fcent) */
+    /* hpp:1342-1349: PLOG constants replace the Arrhenius ones */
+    for (int k = 0; k < m->npl; k++) w->kf[m->pl_rxn[k]] = plog_k(m, k, T, P);
+    /* hpp:1215-1222: 1/Kc = exp(-dS/R + dH/RT - dn ln(Patm/RT)); explicitly
+     * reversible reactions scale the reverse product by k_rev/k instead
+     * (hpp:1242-1253, 1744-1750) */
+    for (int j = 0; j < nr; j++) {
+        if (!(m->rflags[j] & LSM_RXN_REVERSIBLE)) { w->uKeq[j] = 0.; continue; }
+        if (m->rflags[j] & LSM_RXN_EXPLICIT_REV) {
+            const double *p = m->rev_arr + 3 * j;
+            w->uKeq[j] = exp(p[0] + p[1] * logT - p[2] * uT) / w->kf[j];
+        } else {
+            w->uKeq[j] = exp(-w->dS[j] + w->dH[j] - log_Patm_over_RT * M->dn[j]);
+        }
+    }
+    /* hpp:1256-1290 (fall-off) and 1292-1326 (CABR): k_inf and Troe log10(Fcent) */
     for (int k = 0; k < m->nfo; k++) {
         const double *p = m->fo_inf + 3 * k;
+        const int form = m->fo_type[k] & 3;
         w->finf[k] = exp(p[0] + p[1] * logT - p[2] * uT);
-        if (m->fo_type[k] == LSM_FALLOFF_TROE) {
+        if (form == LSM_FALLOFF_TROE) {
             const double *t = m->fo_troe + 4 * k;
             double F = (t[3] != 0.) ? (1. - t[0]) * exp(-T / t[1]) + t[0] * exp(-T / t[2]) + exp(-t[3] / T)
                                     : (1. - t[0]) * exp(-T / t[1]) + t[0] * exp(-T / t[2]);
             w->logFc[k] = (F < 1.e-300) ? -300. : log10(F);
-        } else if (m->fo_type[k] == LSM_FALLOFF_SRI) {
+        } else if (form == LSM_FALLOFF_SRI) {
             /* hpp:1283-1287: a exp(-b/T) + exp(-T/c), kept in the logFcent slot */
             const double *t = m->fo_sri + 5 * k;
             w->logFc[k] = t[0] * exp(-t[1] / T) + exp(-T / t[2]);
@@ -320,8 +352,10 @@ static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, const double *
         for (int q = m->tb_ptr[s]; q < m->tb_ptr[s + 1]; q++) Meff += c[m->tb_sp[q]] * m->tb_eff_m1[q];
         w->kmod[m->tb_rxn[s]] *= Meff;
     }
-    /* FallOffReactions hpp:1496-1550 and hpp:1710-1716 */
+    /* FallOffReactions hpp:1496-1550 and hpp:1710-1716; chemically activated
+     * reactions: ChemicallyActivatedBimolecularReactions hpp:1613-1659, 1721-1726 */
     for (int k = 0; k < m->nfo; k++) {
+        const int form = m->fo_type[k] & 3, cabr = (m->fo_type[k] & LSM_FALLOFF_CHEMACT) != 0;
         double Mc;
         if (m->fo_single[k] < 0) {
             Mc = cTot;
@@ -332,9 +366,9 @@ static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, const double *
         const int j = m->fo_rxn[k];
         const double Pr = w->kf[j] * Mc / w->finf[k];
         double wF = 1.;
-        if (m->fo_type[k] == LSM_FALLOFF_TROE) {
+        if (form == LSM_FALLOFF_TROE) {
             const double lf = w->logFc[k];
-            if (Pr > 1.e-32) {
+            if (Pr > 1.e-32 || cabr) {
                 const double nT = 0.75 - 1.27 * lf;
                 const double cT = -0.4 - 0.67 * lf;
                 const double sT = log10(Pr) + cT;
@@ -344,14 +378,15 @@ static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, const double *
                 const double q = 1. / 0.14;
                 wF = pow(10., lf / (1. + q * q));
             }
-        } else if (m->fo_type[k] == LSM_FALLOFF_SRI) {   /* hpp:1538-1544 */
+        } else if (form == LSM_FALLOFF_SRI) {   /* hpp:1538-1544, 1645-1651 */
             const double *t = m->fo_sri + 5 * k;
             const double lp = log10(Pr);
             const double xSRI = 1. / (1. + lp * lp);
             wF = pow(w->logFc[k], xSRI) * t[3];
             if (t[4] != 0.) wF *= pow(T, t[4]);
         }
-        w->kmod[j] *= w->finf[k] * (Pr / (1. + Pr)) * wF / w->kf[j];
+        if (cabr) w->kmod[j] *= (1. / (1. + Pr)) * wF;
+        else w->kmod[j] *= w->finf[k] * (Pr / (1. + Pr)) * wF / w->kf[j];
     }
     /* ProductOfConcentrations StoichiometricMap.hpp:512-598 (species order) */
     for (int j = 0; j < nr; j++) {
@@ -383,12 +418,17 @@ static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, const double *
 
 void lsm_oracle_kinetics(const lsm_mechanism *m, double T, const double *c, double *kf,
                          double *uKeq, double *kmod, double *r, double *R) {
+    lsm_oracle_kinetics_p(m, T, LSM_P_ATM, c, kf, uKeq, kmod, r, R);
+}
+
+void lsm_oracle_kinetics_p(const lsm_mechanism *m, double T, double P, const double *c, double *kf,
+                           double *uKeq, double *kmod, double *r, double *R) {
     mech_t M;
     kin_ws_t w;
     mech_init(&M, m);
     kin_ws_alloc(&w, m->ns, m->nr);
     thermo_eval(&M, T, w.cp, w.h, w.s);
-    kinetics_eval(&M, &w, T, c);
+    kinetics_eval(&M, &w, T, P, c);
     if (kf) memcpy(kf, w.kf, sizeof(double) * m->nr);
     if (uKeq) memcpy(uKeq, w.uKeq, sizeof(double) * m->nr);
     if (kmod) memcpy(kmod, w.kmod, sizeof(double) * m->nr);
@@ -430,7 +470,9 @@ static void reactor_rhs(void *vctx, const double *y, double *dy) {
     for (int i = 0; i < ns; i++) w->c[i] = cTot * w->x[i];
     const double rho = R->cv ? R->rho0 : cTot * MW;
     thermo_eval(M, T, w->cp, w->h, w->s);
-    kinetics_eval(M, w, T, w->c);
+    /* the map's pressure: P0, or cTot R T in the constant-volume reactor
+     * (BatchReactorHomogeneousConstantVolume.H:166-176) */
+    kinetics_eval(M, w, T, R->cv ? cTot * LSM_R_J_KMOL * T : R->P, w->c);
     for (int i = 0; i < ns; i++) dy[i] = M->MW[i] * w->R[i] / rho;
     dy[ns] = 0.;
     if (R->energy && R->cv) {

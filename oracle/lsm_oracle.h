@@ -68,6 +68,11 @@ void lsm_oracle_species_thermo(const lsm_mechanism *m, double T,
  * net rates r [kmol/m3/s] and formation rates R [kmol/m3/s]. Any output may be NULL. */
 void lsm_oracle_kinetics(const lsm_mechanism *m, double T, const double *c,
                          double *kf, double *uKeq, double *kmod, double *r, double *R);
+/* The same with the map set to pressure P [Pa] (it only enters PLOG constants;
+ * lsm_oracle_kinetics uses 1 atm).  For explicitly reversible reactions uKeq
+ * holds k_rev/k. */
+void lsm_oracle_kinetics_p(const lsm_mechanism *m, double T, double P, const double *c,
+                           double *kf, double *uKeq, double *kmod, double *r, double *R);
 
 /* Constant-pressure RHS (BatchReactorHomogeneousConstantPressure::Equations);
  * y = [Y_0..Y_{ns-1}, T], returns QR [J/m3/s]. */

@@ -83,12 +83,13 @@ def species_thermo(mech, T):
     return cp, h, s
 
 
-def kinetics(mech, T, c):
+def kinetics(mech, T, c, P=101325.0):
+    """Kinetics at (T, c) with the map at pressure P (it enters PLOG constants only)."""
     c = np.ascontiguousarray(c, dtype=np.float64)
     nr, ns = mech.nr, mech.ns
     kf, uK, km, r, R = np.zeros(nr), np.zeros(nr), np.zeros(nr), np.zeros(nr), np.zeros(ns)
-    lib().lsm_oracle_kinetics(ctypes.byref(mech.struct()), ctypes.c_double(T), _p(c), _p(kf), _p(uK),
-                              _p(km), _p(r), _p(R))
+    lib().lsm_oracle_kinetics_p(ctypes.byref(mech.struct()), ctypes.c_double(T), ctypes.c_double(P), _p(c),
+                                _p(kf), _p(uK), _p(km), _p(r), _p(R))
     return dict(kf=kf, uKeq=uK, kmod=km, r=r, R=R)
 
 

@@ -27,7 +27,8 @@ import numpy as np
 import pytest
 
 from laminarsmoke_amd.chemkin import ChemkinError, load_chemkin, read_chemkin, read_thermo
-from laminarsmoke_amd.mechanism import (FALLOFF_LINDEMANN, FALLOFF_SRI, FALLOFF_TROE, RXN_NEGATIVE_A,
+from laminarsmoke_amd.mechanism import (FALLOFF_CHEMACT, FALLOFF_LINDEMANN, FALLOFF_SRI, FALLOFF_TROE,
+                                        RXN_EXPLICIT_REV, RXN_NEGATIVE_A,
                                         RXN_REVERSIBLE)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -206,14 +207,44 @@ def test_coefficients_negative_A_and_collider():
     assert m.lnA[2] == pytest.approx(np.log(1e18 * 1e-6)) and m.fo_inf[0, 0] == pytest.approx(np.log(1e12 * 1e-3))
 
 
+def test_rev_high_plog_units():
+    """REV converts with the product order (+1 with M), HIGH with order - 1 (the
+    reaction line of a chemically activated reaction is k0 with the reaction
+    order), PLOG levels like the reaction line, P in atm -> Pa
+    (ReactionPolicy_CHEMKIN.hpp:1359-1376, 1396-1404, 1459-1478)."""
+    m = _small("H+O2=OH+O 1.0E13 0.5 1000.0\n REV/ 2.0E12 -0.3 500.0/\n"
+               "H+OH+M=H2O+M 1.0E22 -2.0 0.0\n REV/ 3.0E23 -1.5 120000.0/\n"
+               "H+O2(+M)=HO2(+M) 5.0E12 0.2 300.0\n HIGH/ 4.0E14 0.1 900.0/\n TROE/ 0.5 100 2000 /\n"
+               "H2+O=H+OH 1.0E99 0.0 0.0\n PLOG/ 0.1 1.0E12 0.5 2000.0/\n PLOG/ 10.0 4.0E13 0.0 3000.0/\n"
+               "H2+O2=HO2+H 1.0E10 0.0 0.0\n PLOG/ 1.0 0.0 0.0 0.0/")
+    assert m.rflags[0] == RXN_REVERSIBLE | RXN_EXPLICIT_REV
+    np.testing.assert_allclose(m.rev_arr[0], [np.log(2e12 * 1e-3), -0.3, 500.0 * CAL / R], rtol=1e-14)
+    # third body, 1 product + M: order 2
+    np.testing.assert_allclose(m.rev_arr[1], [np.log(3e23 * 1e-3), -1.5, 120000.0 * CAL / R], rtol=1e-14)
+    assert m.fo_type.tolist() == [FALLOFF_TROE | FALLOFF_CHEMACT]
+    assert m.lnA[2] == pytest.approx(np.log(5e12 * 1e-3), rel=1e-14)           # k0: order 2
+    assert m.fo_inf[0, 0] == pytest.approx(np.log(4e14), rel=1e-14)             # k_inf: order 1
+    assert m.pl_rxn.tolist() == [3, 4] and m.pl_ptr.tolist() == [0, 2, 3]
+    np.testing.assert_allclose(m.pl_par[:2], [[0.1 * 101325.0, np.log(1e12 * 1e-3), 0.5, 2000.0 * CAL / R],
+                                              [10.0 * 101325.0, np.log(4e13 * 1e-3), 0.0, 3000.0 * CAL / R]],
+                               rtol=1e-14)
+    assert m.pl_par[2, 1] == pytest.approx(np.log(1e-32))                       # A = 0 floored
+
+
 @pytest.mark.parametrize("text,msg", [
     ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0\n LOW/ 1.0E18 -1.0 0.0/\n SRI/ 0.5 100 1000 0.0 0.0/", "SRI needs d > 0"),
-    ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0\n HIGH/ 1.0E18 -1.0 0.0/", "HIGH"),
+    ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0\n HIGH/ 1.0E18 -1.0 0.0/\n LOW/ 1.0E18 -1.0 0.0/", "one of LOW and HIGH"),
+    ("H+O2=HO2 1.0E12 0.5 0.0\n HIGH/ 1.0E18 -1.0 0.0/", "without \\(\\+M\\)"),
     ("H2+O2=>H2O+O 1.0E12 0.0 0.0\n RORD /H2 1.5/", "RORD"),
-    ("H+O2=OH+O 1.0E12 0.0 0.0\n REV/ 1.0E11 0.0 0.0/", "REV"),
-    ("H+O2=OH+O 1.0E12 0.0 0.0\n PLOG/ 1.0 1.0E12 0.0 0.0/", "PLOG"),
+    ("H+O2=>OH+O 1.0E12 0.0 0.0\n REV/ 1.0E11 0.0 0.0/", "REV needs a reversible"),
+    ("H+O2=OH+O 1.0E12 0.0 0.0\n REV/ 1.0E11 0.0 0.0/\n REV/ 1.0E11 0.0 0.0/", "more than once"),
+    ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0\n LOW/ 1.0E18 -1.0 0.0/\n REV/ 1.0E11 0.0 0.0/", "REV needs"),
+    ("H+O2+M=HO2+M 1.0E12 0.0 0.0\n PLOG/ 1.0 1.0E12 0.0 0.0/", "PLOG cannot"),
+    ("H+O2=OH+O 1.0E12 0.0 0.0\n PLOG/ 10.0 1.0E12 0.0 0.0/\n PLOG/ 1.0 1.0E12 0.0 0.0/", "must increase"),
+    ("H+O2=OH+O 1.0E12 0.0 0.0\n PLOG/ 1.0 -1.0E12 0.0 0.0/", "non-negative"),
+    ("H+O2=OH+O 1.0E12 0.0 0.0\n CHEB/ 7 4/", "CHEB"),
     ("H+O2=OH+XX 1.0E12 0.0 0.0", "unknown species"),
-    ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0", "without LOW"),
+    ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0", "one of LOW and HIGH"),
 ])
 def test_unsupported_options_raise(text, msg):
     with pytest.raises(ChemkinError, match=msg):

@@ -142,9 +142,13 @@ def test_hand_written_file():
 def _same(a, b):
     for f in ("nasa", "lnA", "beta", "E_over_R", "rflags", "reac_ptr", "reac_sp", "reac_nu", "prod_ptr", "prod_sp",
               "prod_nu", "tb_rxn", "tb_ptr", "tb_sp", "tb_eff_m1", "fo_rxn", "fo_type", "fo_inf", "fo_troe",
-              "fo_single", "fo_ptr", "fo_sp", "fo_eff_m1", "fo_sri", "ord_ptr", "ord_sp", "ord_val", "elements"):
+              "fo_single", "fo_ptr", "fo_sp", "fo_eff_m1", "fo_sri", "ord_ptr", "ord_sp", "ord_val", "elements",
+              "rev_arr", "pl_rxn", "pl_ptr"):
         x, y = getattr(a, f), getattr(b, f)
         assert x.shape == y.shape and np.array_equal(x, y), f
+    # PLOG levels travel as raw CHEMKIN values (P in atm, A, E) with unit factors
+    assert a.pl_par.shape == b.pl_par.shape
+    np.testing.assert_allclose(a.pl_par, b.pl_par, rtol=1e-14, atol=0)
     assert a.species == b.species
 
 
@@ -161,9 +165,52 @@ def test_round_trip(name):
     _same(m, read_kinetics_xml(write_kinetics_xml(m)))
 
 
+REV_CABR_PLOG = """ELEMENTS H O N END
+SPECIES H2 O2 H2O H O OH HO2 N2 END
+REACTIONS
+H+O2=OH+O 1.0E13 0.5 1000.0
+ REV/ 2.0E12 -0.3 500.0/
+H+O2(+M)=HO2(+M) 5.0E12 0.2 300.0
+ HIGH/ 4.0E14 0.1 900.0/
+ TROE/ 0.5 100 2000 /
+ H2O/12.0/ N2/0.8/
+H2+O=H+OH 1.0E12 0.0 0.0
+ PLOG/ 0.1 1.0E12 0.5 2000.0/
+ PLOG/ 10.0 4.0E13 0.0 3000.0/
+H+OH+M=H2O+M 1.0E22 -2.0 0.0
+ REV/ 3.0E23 -1.5 120000.0/
+H+HO2(+N2)=H2+O2(+N2) 1.0E12 0.0 0.0
+ LOW/ 1.0E16 0.0 0.0/
+H+HO2(+N2)=2OH(+N2) 1.0E11 0.0 0.0
+ HIGH/ 1.0E9 0.0 0.0/
+ SRI/ 0.5 200 900 /
+END
+"""
+
+
+def test_round_trip_rev_cabr_plog(tmp_path):
+    """Explicit reverse, chemically activated (between two fall-off entries) and PLOG
+    reactions come back as the same tables; the XML holds the reference's lists."""
+    from laminarsmoke_amd.chemkin import read_chemkin
+    from laminarsmoke_amd.mechanism import FALLOFF_CHEMACT, RXN_EXPLICIT_REV
+    m = read_chemkin(REV_CABR_PLOG, open(os.path.join(GOLDEN, "POLIMI_H2CO_1412_thermo.CKT")).read())
+    assert m.fo_type.tolist() == [FALLOFF_TROE | FALLOFF_CHEMACT, 0, FALLOFF_SRI | FALLOFF_CHEMACT]
+    assert (m.rflags & RXN_EXPLICIT_REV).nonzero()[0].tolist() == [0, 3]
+    text = write_kinetics_xml(m)
+    for tag, body in (("Reversible-Explicit", "2\n1 4"), ("Reversible-Thermodynamics", "4\n2 3 5 6"),
+                      ("PressureLog", "1\n3")):
+        assert f"<{tag}>\n{body}\n</{tag}>" in text, tag
+    assert "<CABR>\n2\n2 6\n" in text and "<FallOff>\n1\n5\n" in text
+    _same(m, read_kinetics_xml(text))
+
+
 @pytest.mark.parametrize("old,new,msg", [
-    ("<Reversible-Explicit> 0 </Reversible-Explicit>", "<Reversible-Explicit> 1 3 </Reversible-Explicit>",
+    ("<Reversible-Explicit> 0 </Reversible-Explicit>", "<Reversible-Explicit> 1 2 </Reversible-Explicit>",
      "Reversible-Explicit"),
+    ("<Reversible-Explicit> 0 </Reversible-Explicit>", "<Reversible-Explicit> 1 3 </Reversible-Explicit>",
+     "<Reverse> not found"),
+    ("<Reversible-Explicit> 0 </Reversible-Explicit>",
+     "<Reversible-Explicit> 0 </Reversible-Explicit>\n<Chebyshev> 1 2 </Chebyshev>", "Chebyshev"),
     ("0.7\n0\n</Stoichiometry>", "0.7\n1\n</Stoichiometry>", "unexpected end"),   # flag without lambda lists
     ('version="04-22-2013">\n<NumberOfReactions>', 'version="01-01-2000">\n<NumberOfReactions>', "not supported"),
 ])

@@ -1,0 +1,174 @@
+"""Pins of the oracle's explicitly reversible (REV), chemically activated (CABR,
+HIGH) and pressure-logarithmic (PLOG) reactions (CPU only).
+
+Each check uses a fact the oracle's code does not restate: the closed form of a
+net rate with an explicit reverse constant (kf c_f - kr c_b), the identity
+between a chemically activated and a fall-off reaction with the same data
+(k_CABR = k_falloff / M), the Lindemann limits of a chemically activated
+reaction, and the PLOG interpolation properties (exact Arrhenius at every
+pressure level, clamping outside the range, ln k linear in ln P between levels).
+"""
+import numpy as np
+import pytest
+
+from laminarsmoke_amd.mechanism import constant_cp_nasa, custom_mechanism
+
+ROWS = [constant_cp_nasa(3.5, 0.0, 20.0, 20.0), constant_cp_nasa(4.0, -3000.0, 22.0, 20.0),
+        constant_cp_nasa(4.5, 1500.0, 24.0, 20.0), constant_cp_nasa(3.0, 800.0, 21.0, 20.0)]
+ATM = 101325.0
+
+
+def _mech(*rxns):
+    return custom_mechanism(["A", "B", "C", "D"], ROWS, list(rxns))
+
+
+def _arr(p, T):
+    return np.exp(p[0] + p[1] * np.log(T) - p[2] / T)
+
+
+@pytest.mark.parametrize("T", [700.0, 1500.0])
+def test_explicit_reverse_closed_form(oracle, T):
+    """A + B = C + D with REV: r = kf cA cB - kr cC cD (not k/Kc)."""
+    fwd, rev = (np.log(3e7), 0.4, 1200.0), (np.log(5e6), -0.3, 4000.0)
+    m = _mech(dict(reac=[(0, 1), (1, 1)], prod=[(2, 1), (3, 1)], lnA=fwd[0], beta=fwd[1], E_over_R=fwd[2],
+                   rev=rev))
+    c = np.array([0.02, 0.03, 0.05, 0.07])
+    k = oracle.kinetics(m, T, c)
+    kf, kr = _arr(fwd, T), _arr(rev, T)
+    assert k["r"][0] == pytest.approx(kf * 0.02 * 0.03 - kr * 0.05 * 0.07, rel=1e-13)
+    assert k["uKeq"][0] == pytest.approx(kr / kf, rel=1e-13)
+    # the same reaction without REV uses the equilibrium constant instead
+    m2 = _mech(dict(reac=[(0, 1), (1, 1)], prod=[(2, 1), (3, 1)], lnA=fwd[0], beta=fwd[1], E_over_R=fwd[2]))
+    assert oracle.kinetics(m2, T, c)["r"][0] != pytest.approx(k["r"][0], rel=1e-6)
+
+
+def test_explicit_reverse_third_body_and_negative_A(oracle):
+    """Third-body REV: r = M (kf cA cB - kr cC); a negative A keeps r = kf(<0) cA cB - kr cC."""
+    fwd, rev = (np.log(2e9), 0.0, 500.0), (np.log(7e8), 0.2, 9000.0)
+    T = 1100.0
+    c = np.array([0.01, 0.02, 0.03, 0.04])
+    m = _mech(dict(reac=[(0, 1), (1, 1)], prod=[(2, 1)], lnA=fwd[0], beta=fwd[1], E_over_R=fwd[2], rev=rev,
+                   tb=[(0, 2.0), (3, 0.5)]))
+    M = c.sum() + (2.0 - 1) * c[0] + (0.5 - 1) * c[3]
+    kf, kr = _arr(fwd, T), _arr(rev, T)
+    assert oracle.kinetics(m, T, c)["r"][0] == pytest.approx(M * (kf * 0.01 * 0.02 - kr * 0.03), rel=1e-13)
+    mn = _mech(dict(reac=[(0, 1), (1, 1)], prod=[(2, 1)], lnA=fwd[0], beta=fwd[1], E_over_R=fwd[2], rev=rev,
+                    negative_A=True))
+    assert oracle.kinetics(mn, T, c)["r"][0] == pytest.approx(-kf * 0.01 * 0.02 - kr * 0.03, rel=1e-13)
+
+
+def _pair(form, single=-1, eff=()):
+    """The same data as a fall-off and as a chemically activated reaction."""
+    f = dict(inf=(np.log(4e9), 0.3, 800.0), single=single, eff=list(eff))
+    f.update(form)
+    base = dict(reac=[(0, 1), (1, 1)], prod=[(2, 1)], lnA=np.log(6e12), beta=-0.5, E_over_R=300.0,
+                reversible=False)
+    return _mech(dict(base, falloff=dict(f))), _mech(dict(base, falloff=dict(f, cabr=True)))
+
+
+@pytest.mark.parametrize("form", [{}, dict(troe=(0.7, 150.0, 1800.0, 4500.0)), dict(troe=(0.4, 90.0, 2500.0, 0.0)),
+                                  dict(sri=(0.6, 400.0, 900.0, 1.3, 0.2)), dict(sri=(1.1, 250.0, 1200.0, 1.0, 0.0))])
+@pytest.mark.parametrize("scale", [1e-6, 1e-3, 1.0, 1e3])
+def test_cabr_is_falloff_over_M(oracle, form, scale):
+    """k0 F/(1+Pr) = (k_inf Pr/(1+Pr) F) / M for the same k0, k_inf, F and M."""
+    mf, mc = _pair(form, eff=[(0, 3.0), (2, 0.4)])
+    c = np.array([0.01, 0.02, 0.03, 0.04]) * scale
+    M = c.sum() + 2.0 * c[0] - 0.6 * c[2]
+    for T in (600.0, 1400.0, 2300.0):
+        kf = oracle.kinetics(mf, T, c)["kmod"][0]
+        kc = oracle.kinetics(mc, T, c)["kmod"][0]
+        assert kc == pytest.approx(kf / M, rel=1e-12)
+
+
+def test_cabr_lindemann_limits(oracle):
+    """Lindemann CABR: k -> k0 as M -> 0 and k -> k_inf / M as M -> infinity."""
+    _, mc = _pair({}, single=3)
+    T = 1000.0
+    k0, kinf = _arr((np.log(6e12), -0.5, 300.0), T), _arr((np.log(4e9), 0.3, 800.0), T)
+    lo = oracle.kinetics(mc, T, np.array([0.1, 0.1, 0.1, 0.0]))["kmod"][0]     # M = c_D + 1e-16
+    assert lo == pytest.approx(k0, rel=1e-6)
+    big = 1e9
+    hi = oracle.kinetics(mc, T, np.array([0.1, 0.1, 0.1, big]))["kmod"][0]
+    assert hi == pytest.approx(kinf / big, rel=1e-6)
+
+
+def test_cabr_troe_has_no_low_pr_asymptote(oracle):
+    """Below Pr = 1e-32 a fall-off Troe reaction switches to the asymptotic
+    broadening; a chemically activated one keeps the full Troe form."""
+    mf, mc = _pair(dict(troe=(0.5, 200.0, 2000.0, 0.0)))
+    T = 1500.0
+    c = np.array([1e-44, 0.0, 0.0, 0.0])
+    M = c.sum()
+    k0, kinf = _arr((np.log(6e12), -0.5, 300.0), T), _arr((np.log(4e9), 0.3, 800.0), T)
+    Pr = k0 * M / kinf
+    assert Pr < 1e-32
+    lf = np.log10(0.5 * np.exp(-T / 200.0) + 0.5 * np.exp(-T / 2000.0))
+    s = np.log10(Pr) - 0.4 - 0.67 * lf
+    F = 10 ** (lf / (1 + (s / (0.75 - 1.27 * lf - 0.14 * s)) ** 2))
+    assert oracle.kinetics(mc, T, c)["kmod"][0] == pytest.approx(k0 * F / (1 + Pr), rel=1e-10)
+    F_asym = 10 ** (lf / (1 + (1 / 0.14) ** 2))
+    assert oracle.kinetics(mf, T, c)["kmod"][0] == pytest.approx(kinf * Pr * F_asym, rel=1e-10)
+
+
+LEVELS = [(0.01 * ATM, np.log(1e9), 0.5, 2000.0), (1.0 * ATM, np.log(3e10), 0.1, 2600.0),
+          (10.0 * ATM, np.log(2e11), -0.2, 3100.0), (100.0 * ATM, np.log(9e11), -0.4, 3500.0)]
+
+
+def _plog_mech(levels=LEVELS, **extra):
+    r = dict(reac=[(0, 1), (1, 1)], prod=[(2, 1), (3, 1)], lnA=np.log(1e99), beta=0.0, E_over_R=0.0,
+             reversible=False, plog=levels)
+    r.update(extra)
+    return _mech(r)
+
+
+@pytest.mark.parametrize("T", [500.0, 1200.0, 2500.0])
+def test_plog_levels_and_clamping(oracle, T):
+    m = _plog_mech()
+    c = np.array([0.01, 0.02, 0.0, 0.0])
+    for lv in LEVELS:
+        assert oracle.kinetics(m, T, c, P=lv[0])["kf"][0] == pytest.approx(_arr(lv[1:], T), rel=1e-13)
+    assert oracle.kinetics(m, T, c, P=1e-4 * ATM)["kf"][0] == pytest.approx(_arr(LEVELS[0][1:], T), rel=1e-14)
+    assert oracle.kinetics(m, T, c, P=1e4 * ATM)["kf"][0] == pytest.approx(_arr(LEVELS[-1][1:], T), rel=1e-14)
+    # the direct parameters are ignored; r = k_plog cA cB
+    k = oracle.kinetics(m, T, c, P=3.0 * ATM)
+    assert k["r"][0] == pytest.approx(k["kf"][0] * 0.01 * 0.02, rel=1e-14)
+
+
+@pytest.mark.parametrize("T", [800.0, 1900.0])
+def test_plog_log_linear_between_levels(oracle, T):
+    m = _plog_mech()
+    c = np.array([0.01, 0.02, 0.0, 0.0])
+    for a, b in zip(LEVELS[:-1], LEVELS[1:]):
+        lnk = lambda P: np.log(oracle.kinetics(m, T, c, P=P)["kf"][0])
+        for w in (0.1, 0.5, 0.9):
+            P = np.exp((1 - w) * np.log(a[0]) + w * np.log(b[0]))
+            expect = (1 - w) * np.log(_arr(a[1:], T)) + w * np.log(_arr(b[1:], T))
+            assert lnk(P) == pytest.approx(expect, abs=1e-12 * max(1.0, abs(expect)))
+
+
+def test_plog_single_level_and_reverse(oracle):
+    """One level: k is that Arrhenius set at every pressure.  A reversible PLOG
+    reaction divides by Kc; with REV the reverse constant is k_rev."""
+    lv = [(ATM, np.log(5e10), 0.2, 1500.0)]
+    T = 1300.0
+    c = np.array([0.01, 0.02, 0.03, 0.04])
+    m = _plog_mech(lv)
+    for P in (0.1 * ATM, ATM, 50 * ATM):
+        assert oracle.kinetics(m, T, c, P=P)["kf"][0] == pytest.approx(_arr(lv[0][1:], T), rel=1e-14)
+    rev = (np.log(4e9), 0.0, 6000.0)
+    mr = _plog_mech(LEVELS, reversible=True, rev=rev)
+    k = oracle.kinetics(mr, T, c, P=5 * ATM)
+    assert k["r"][0] == pytest.approx(k["kf"][0] * 0.01 * 0.02 - _arr(rev, T) * 0.03 * 0.04, rel=1e-13)
+
+
+def test_plog_constant_volume_pressure(oracle):
+    """In the constant-volume reactor the map's pressure is cTot R T, so the RHS at
+    density rho equals the constant-pressure RHS at P = rho R T / MW."""
+    m = _plog_mech()
+    y = np.array([0.3, 0.3, 0.2, 0.2, 1400.0])
+    mw = 20.0
+    for P in (0.05 * ATM, 3.0 * ATM, 40.0 * ATM):
+        rho = P * mw / (8314.4621 * y[-1])
+        f_p, _ = oracle.rhs(m, y, P=P, energy=False)
+        f_v, _ = oracle.rhs(m, y, energy=False, rho0=rho)
+        np.testing.assert_allclose(f_v[:-1], f_p[:-1], rtol=1e-12, atol=0)
