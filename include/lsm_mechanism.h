@@ -46,6 +46,12 @@
  *                    interpolated in ln P between per-pressure Arrhenius sets and
  *                    replaces the direct parameters (hpp:1342-1349)
  *                    (kernel/kinetics/PressureLogarithmicRateExpression.hpp).
+ *   ch_*             Chebyshev reactions (CHEB): k = conv 10^(sum_nm a_nm
+ *                    phi_n(T~) phi_m(P~)), phi_n(x) = cos((n-1) acos x), T~ and P~
+ *                    the reduced 1/T and log10 P over [Tmin, Tmax] x [Pmin, Pmax];
+ *                    replaces the direct parameters (hpp:1333-1340,
+ *                    kernel/kinetics/ChebyshevPolynomialRateExpression.hpp:95-116).
+ *                    Outside those ranges acos is undefined and k is NaN, as there.
  *
  * All indices are 0-based.  The arrays are owned by the caller and are only
  * read during lsm_create()/oracle calls.
@@ -123,6 +129,13 @@ typedef struct lsm_mechanism {
     const int32_t *pl_ptr;     /* [npl+1] pressure levels per reaction (>= 1)   */
     const double *pl_par;      /* [pl_ptr[npl]*4] per level: P [Pa] (strictly
                                   increasing), lnA, beta, E_over_R               */
+    int32_t nch;               /* number of Chebyshev reactions                 */
+    const int32_t *ch_rxn;     /* [nch] reaction index                          */
+    const int32_t *ch_dims;    /* [nch*2] N (temperature), M (pressure) >= 1    */
+    const double *ch_lim;      /* [nch*5] Tmin, Tmax [K], Pmin, Pmax [Pa], conv
+                                  (k multiplier, SI units)                      */
+    const int32_t *ch_ptr;     /* [nch+1] first coefficient of each reaction    */
+    const double *ch_a;        /* [ch_ptr[nch]] a_nm, row-major (n over T)       */
 } lsm_mechanism;
 
 /* Physical constants of the reference (math/PhysicalConstants.h:24). */

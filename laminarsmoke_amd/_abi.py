@@ -17,6 +17,7 @@ class LsmMechanism(ctypes.Structure):
         ("fo_single", _i), ("fo_ptr", _i), ("fo_sp", _i), ("fo_eff_m1", _d), ("fo_sri", _d),
         ("ord_ptr", _i), ("ord_sp", _i), ("ord_val", _d), ("rev_arr", _d),
         ("npl", ctypes.c_int32), ("pl_rxn", _i), ("pl_ptr", _i), ("pl_par", _d),
+        ("nch", ctypes.c_int32), ("ch_rxn", _i), ("ch_dims", _i), ("ch_lim", _d), ("ch_ptr", _i), ("ch_a", _d),
     ]
 
 

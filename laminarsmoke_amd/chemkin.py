@@ -22,6 +22,10 @@ Supported:
 * pressure-logarithmic reactions (``PLOG /P A beta E/`` lines, P in atm,
   increasing) on reactions without ``M``: A converted like the reaction line and
   floored at 1e-32 in the log (PressureLogarithmicRateExpression.hpp:71-102);
+* Chebyshev reactions (``(+M)`` with ``CHEB/ N M a11 ... aNM /``, optional
+  ``TCHEB/ Tmin Tmax /`` and ``PCHEB/ Pmin Pmax /`` in atm, defaults 300-2500 K and
+  0.001-100 atm; k in the reaction's A units, converted with 1e3^(1 - order))
+  (ReactionPolicy_CHEMKIN.hpp:456-474, 751-770, 1072-1092, 1322-1344, 1409-1412);
 * ``DUPLICATE``; negative pre-exponential factors on elementary reactions;
 * explicit forward reaction orders (``FORD``): the orders are the reactant
   coefficients with the named species' orders replaced or added
@@ -29,8 +33,7 @@ Supported:
 * energy units CAL/MOLE (default), KCAL/MOLE, JOULES/MOLE, KJOULES/MOLE, KELVINS;
   quantity unit MOLES.
 
-Anything else the kernels do not evaluate (``CHEB``, ``RORD``, ``LT``,
-``MOLECULES``, ...) raises
+Anything else the kernels do not evaluate (``RORD``, ``LT``, ``MOLECULES``, ...) raises
 :class:`ChemkinError` naming the reaction, rather than being dropped.
 
 Unit conversions follow ReactionPolicy_CHEMKIN::ConvertUnits /
@@ -70,7 +73,7 @@ E_CONVERSION = {               # J/kmol per unit (ReactionPolicy_CHEMKIN.hpp:148
     "CAL/MOLE": 1e3 * J_FROM_CAL, "KCAL/MOLE": 1e6 * J_FROM_CAL,
     "JOULES/MOLE": 1e3, "KJOULES/MOLE": 1e6, "KELVINS": R_J_KMOL,
 }
-_UNSUPPORTED = {"CHEB", "TCHEB", "PCHEB", "RORD", "LT", "RLT",
+_UNSUPPORTED = {"RORD", "LT", "RLT",
                 "JAN", "FIT1", "TDEP", "EXCI", "MOME", "XSMI", "USRPROG", "LOWMX", "LOWSP", "TROEMX",
                 "TROESP", "SRIMX", "SRISP", "PLOGMX", "PLOGSP", "COLLEFF", "UNITS"}
 
@@ -260,7 +263,7 @@ def _parse_reaction_line(line: str, species_set: set, idx: int):
         raise ChemkinError(f"{what}: both +M and (+M)")
     return dict(eq=eq, what=what, reac=r, prod=p, rev=rev, A=A, beta=beta, E=E,
                 third=tb_l, collider=co_l, low=None, high=None, troe=None, sri=None, eff={}, ford={}, dup=False,
-                rev_arr=None, plog=[])
+                rev_arr=None, plog=[], cheb=[], tcheb=None, pcheb=None)
 
 
 def read_kinetics(text: str):
@@ -315,6 +318,12 @@ def read_kinetics(text: str):
                             if cur[key] is not None:
                                 raise ChemkinError(f"{cur['what']}: the {kw} option is used more than once")
                             cur[key] = [float(v) for v in vals]
+                        elif kw == "CHEB":
+                            cur["cheb"] += [float(v) for v in vals]
+                        elif kw in ("TCHEB", "PCHEB"):
+                            if len(vals) != 2:
+                                raise ChemkinError(f"{cur['what']}: {kw} needs 2 values")
+                            cur[kw.lower()] = [float(v) for v in vals]
                         elif kw == "PLOG":
                             if len(vals) != 4:
                                 raise ChemkinError(f"{cur['what']}: PLOG needs 4 values (P, A, beta, E)")
@@ -390,6 +399,7 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
     ord_rows = []
     rev_arr = np.zeros((len(reactions), 3))
     pl_rxn, pl_lev = [], []
+    ch_rxn, ch_dims, ch_lim, ch_a = [], [], [], []
     for j, r in enumerate(reactions):
         what = r["what"]
         lam = dict(r["reac"])
@@ -399,16 +409,24 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
         else:
             ord_rows.append([])
         order = sum(lam.values())             # sumLambdaReactants_ (ReactionPolicy_CHEMKIN.hpp:1360-1366)
-        falloff = r["collider"] is not None
+        cheb = bool(r["cheb"])
+        falloff = r["collider"] is not None and not cheb
         cabr = r["high"] is not None
         if falloff and (r["low"] is None) == (r["high"] is None):
             raise ChemkinError(f"{what}: a (+M) reaction needs exactly one of LOW and HIGH")
-        if not falloff and any(r[k] is not None for k in ("low", "high", "troe", "sri")):
+        if not falloff and not cheb and any(r[k] is not None for k in ("low", "high", "troe", "sri")):
             raise ChemkinError(f"{what}: LOW/HIGH/TROE/SRI on a reaction without (+M)")
         if r["rev_arr"] is not None and (not r["rev"] or falloff):
             raise ChemkinError(f"{what}: REV needs a reversible reaction without (+M)")
         if r["plog"] and (falloff or r["third"]):
             raise ChemkinError(f"{what}: PLOG cannot be used on third-body or fall-off reactions")
+        if (r["tcheb"] is not None or r["pcheb"] is not None) and not cheb:
+            raise ChemkinError(f"{what}: TCHEB/PCHEB without CHEB")
+        if cheb:
+            if r["collider"] != "M":
+                raise ChemkinError(f"{what}: CHEB can be used only with (+M)")
+            if any(r[k] is not None for k in ("low", "high", "troe", "sri")) or r["plog"] or r["eff"]:
+                raise ChemkinError(f"{what}: CHEB cannot be combined with LOW/HIGH/TROE/SRI/PLOG or efficiencies")
         if r["troe"] is not None and r["sri"] is not None:
             raise ChemkinError(f"{what}: both TROE and SRI")
         if r["sri"] is not None and (r["sri"][3] <= 0.0 or r["sri"][2] == 0.0):
@@ -464,6 +482,18 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
                 tb_eff.append([e for _, e in items])
             elif r["eff"]:
                 raise ChemkinError(f"{what}: efficiencies on a reaction without M")
+            if cheb:
+                cv = r["cheb"]
+                if len(cv) < 3 or int(cv[0]) * int(cv[1]) != len(cv) - 2 or cv[0] < 1 or cv[1] < 1:
+                    raise ChemkinError(f"{what}: CHEB needs N M and N x M coefficients")
+                if r["A"] < 0.0:
+                    raise ChemkinError(f"{what}: negative pre-exponential factor on a Chebyshev reaction")
+                Tl = r["tcheb"] or [300.0, 2500.0]
+                Pl = r["pcheb"] or [0.001, 100.0]
+                ch_rxn.append(j)
+                ch_dims.append((int(cv[0]), int(cv[1])))
+                ch_lim.append((Tl[0], Tl[1], Pl[0] * 101325.0, Pl[1] * 101325.0, 1e3 ** (1.0 - order)))
+                ch_a.append(cv[2:])
             if r["rev_arr"] is not None:
                 Ar, br, Er = r["rev_arr"]
                 if Ar < 0.0:
@@ -517,6 +547,9 @@ def read_chemkin(kinetics: str, thermo: str, name: str = "chemkin") -> Mechanism
         ord_val=np.array([v for x in ord_rows for _, v in x], dtype=np.float64),
         rev_arr=rev_arr, pl_rxn=np.array(pl_rxn, dtype=np.int32), pl_ptr=csr(pl_lev),
         pl_par=np.array([v for x in pl_lev for v in x], dtype=np.float64).reshape(-1, 4),
+        ch_rxn=np.array(ch_rxn, dtype=np.int32), ch_dims=np.array(ch_dims, dtype=np.int32).reshape(-1, 2),
+        ch_lim=np.array(ch_lim, dtype=np.float64).reshape(-1, 5), ch_ptr=csr(ch_a),
+        ch_a=np.array([v for x in ch_a for v in x], dtype=np.float64),
     )
 
 

@@ -393,6 +393,23 @@ static int check_mechanism(const lsm_mechanism *m) {
         // PLOG cannot be combined with LOW / HIGH (ReactionPolicy_CHEMKIN.hpp:778-785)
         for (int s = 0; s < m->nfo; s++)
             if (pl[m->fo_rxn[s]]) return fail(LSM_E_INVALID, "PLOG on a fall-off / CABR reaction");
+        // Chebyshev: not with LOW / HIGH / PLOG (ReactionPolicy_CHEMKIN.hpp:456-474)
+        if (m->nch < 0 || (m->nch > 0 && (!m->ch_rxn || !m->ch_dims || !m->ch_lim || !m->ch_ptr || !m->ch_a)))
+            return fail(LSM_E_INVALID, "Chebyshev tables");
+        for (int s = 0; s < m->nch; s++) {
+            const int j = m->ch_rxn[s];
+            if (j < 0 || j >= m->nr) return fail(LSM_E_INVALID, "Chebyshev reaction index");
+            if (pl[j]) return fail(LSM_E_INVALID, "reaction listed twice among PLOG / Chebyshev reactions");
+            pl[j] = 1;
+            const int N = m->ch_dims[2 * s], M = m->ch_dims[2 * s + 1];
+            if (N < 1 || M < 1 || N > 64 || M > 64 || m->ch_ptr[s + 1] - m->ch_ptr[s] != N * M)
+                return fail(LSM_E_INVALID, "Chebyshev coefficient table is not N x M (1 <= N, M <= 64)");
+            const double *L = m->ch_lim + 5 * s;
+            if (!(L[0] > 0. && L[1] > L[0] && L[2] > 0. && L[3] > L[2]))
+                return fail(LSM_E_INVALID, "Chebyshev limits need 0 < Tmin < Tmax and 0 < Pmin < Pmax");
+        }
+        for (int s = 0; s < m->nfo; s++)
+            if (pl[m->fo_rxn[s]]) return fail(LSM_E_INVALID, "Chebyshev on a fall-off / CABR reaction");
     }
     if (m->ord_ptr) {
         if (m->ord_ptr[0] != 0) return fail(LSM_E_INVALID, "ord_ptr[0] must be 0");
@@ -447,6 +464,18 @@ static int plog_slot_of(const lsm_mechanism *m, int j) {
     for (int s = 0; s < m->npl; s++) if (m->pl_rxn[s] == j) return s;
     return -1;
 }
+static int cheb_slot_of(const lsm_mechanism *m, int j) {
+    for (int s = 0; s < m->nch; s++) if (m->ch_rxn[s] == j) return s;
+    return -1;
+}
+// Chebyshev block of slot s: N, M, Tmin, Tmax, log10 Pmin, log10 Pmax, conv, a[N*M]
+static void cheb_block(const lsm_mechanism *m, int s, std::vector<double> &out) {
+    const int N = m->ch_dims[2 * s], M = m->ch_dims[2 * s + 1];
+    const double *L = m->ch_lim + 5 * s;
+    out.push_back(N); out.push_back(M); out.push_back(L[0]); out.push_back(L[1]);
+    out.push_back(std::log10(L[2])); out.push_back(std::log10(L[3])); out.push_back(L[4]);
+    for (int q = m->ch_ptr[s]; q < m->ch_ptr[s] + N * M; q++) out.push_back(m->ch_a[q]);
+}
 static bool smooth_reverse(const lsm_mechanism *m, int j) {
     if (!(m->rflags[j] & LSM_RXN_REVERSIBLE)) return false;
     for (int q = m->prod_ptr[j]; q < m->prod_ptr[j + 1]; q++) if (m->prod_nu[q] < 1.) return true;
@@ -477,6 +506,10 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
         }
     }
     plp.push_back((unsigned)(pll.size() / 4));
+    std::vector<double> chd;
+    std::vector<unsigned> chp;
+    for (int s = 0; s < m->nch; s++) { chp.push_back((unsigned)chd.size()); cheb_block(m, s, chd); }
+    chp.push_back((unsigned)chd.size());
     auto code_of = [&](double nu) -> unsigned {
         const double t = 2. * nu;
         if (t == std::floor(t) && t >= 1. && t <= 15.) return (unsigned)t;
@@ -492,8 +525,10 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
         for (int q = m->reac_ptr[j]; q < m->reac_ptr[j + 1]; q++) { re.push_back({m->reac_sp[q], m->reac_nu[q]}); d -= m->reac_nu[q]; }
         for (int q = m->prod_ptr[j]; q < m->prod_ptr[j + 1]; q++) { pr.push_back({m->prod_sp[q], m->prod_nu[q]}); d += m->prod_nu[q]; }
         P[4 * jn + 3] = d;
-        const int pls = plog_slot_of(m, j);
-        if (pls >= 0) P[4 * jn + 0] = (double)pls;
+        const int pls = plog_slot_of(m, j), chs = cheb_slot_of(m, j);
+        if (pls >= 0) { P[4 * jn + 0] = (double)pls; P[4 * jn + 1] = 0.; }
+        if (chs >= 0) { P[4 * jn + 0] = (double)chs; P[4 * jn + 1] = 1.; }
+        const bool tab = pls >= 0 || chs >= 0;
         const bool xrev = (m->rflags[j] & LSM_RXN_EXPLICIT_REV) != 0;
         if (xrev) {
             P[4 * jn + 3] = (double)(rvp.size() / 4);
@@ -528,12 +563,13 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
         W[4 * jn + 1] = wr;
         W[4 * jn + 2] = wp;
         // bits: 0 reversible, 1 negative A, 2 fixed format, 3/4 non-elementary forward/reverse
-        // side, 5 explicit reverse, 6 chemically activated, 7 PLOG, 8-15 third-body slot + 1,
-        // 16-23 fall-off slot + 1, 24-31 explicit orders.  A PLOG constant replaces the
-        // sign-flipped Arrhenius one (KineticsMap_CHEMKIN.hpp:1208-1210, 1342-1349).
+        // side, 5 explicit reverse, 6 chemically activated, 7 PLOG / Chebyshev, 8-15 third-body
+        // slot + 1, 16-23 fall-off slot + 1, 24-31 explicit orders.  A PLOG or Chebyshev
+        // constant replaces the sign-flipped Arrhenius one (KineticsMap_CHEMKIN.hpp:1208-1210,
+        // 1333-1349).
         const bool cabr = fo_of[j] >= 0 && (m->fo_type[fo_of[j]] & LSM_FALLOFF_CHEMACT);
-        W[4 * jn + 0] = (unsigned)(m->rflags[j] & (pls >= 0 ? 1 : 3)) | (fast ? 4u : 0u) | (sf ? 8u : 0u) |
-                        (sb ? 16u : 0u) | (xrev ? 32u : 0u) | (cabr ? 64u : 0u) | (pls >= 0 ? 128u : 0u) |
+        W[4 * jn + 0] = (unsigned)(m->rflags[j] & (tab ? 1 : 3)) | (fast ? 4u : 0u) | (sf ? 8u : 0u) |
+                        (sb ? 16u : 0u) | (xrev ? 32u : 0u) | (cabr ? 64u : 0u) | (tab ? 128u : 0u) |
                         ((unsigned)(tb_of[j] + 1) << 8) | ((unsigned)(fo_of[j] + 1) << 16) |
                         ((unsigned)od.size() << 24);
     }
@@ -573,6 +609,7 @@ static bool build_image(const lsm_mechanism *m, const std::vector<int> &perm, co
     L.o_tbp = im.put(tbp); L.o_tbs = im.put(tbs); L.o_tbe = im.put(tbe);
     L.o_fop = im.put(fop); L.o_fos = im.put(fos); L.o_fol = im.put(fol); L.o_fos2 = im.put(fos2); L.o_foe = im.put(foe);
     L.o_rvp = im.put(rvp); L.o_plp = im.put(plp); L.o_pll = im.put(pll);
+    L.o_chp = im.put(chp); L.o_chd = im.put(chd);
     im.b.resize((im.b.size() + 15) & ~size_t(15), 0);
     L.bytes = (int)im.b.size();
     L.team_ok = ns <= 63 ? 1 : 0;
@@ -586,7 +623,7 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     const int ns = m->ns, nr = m->nr;
     DevMech &d = h->dm;
     d.ns = ns; d.nr = nr; d.ne = ns + 1; d.ntb = m->ntb; d.nfo = m->nfo;
-    d.xk = m->npl > 0 ? 1 : 0;
+    d.xk = (m->npl > 0 || m->nch > 0) ? 1 : 0;
     for (int j = 0; j < nr; j++) if (m->rflags[j] & LSM_RXN_EXPLICIT_REV) d.xk = 1;
     for (int s = 0; s < m->nfo; s++) if (m->fo_type[s] & LSM_FALLOFF_CHEMACT) d.xk = 1;
     d.ns_pad = (ns + 63) / 64 * 64;
@@ -638,7 +675,10 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     for (int jn = 0; jn < nr; jn++) inv[perm[jn]] = jn;
     std::vector<double> rA(nr), rB(nr), rE(nr), rdn(nr), fw_nu, bw_nu, rx_nu;
     std::vector<int> rinfo(nr), rtb(nr), rfo(nr), fw_ptr(nr + 1), bw_ptr(nr + 1), rx_ptr(nr + 1), fw_sp, bw_sp, rx_sp;
-    std::vector<int> rpl(nr), pl_ptr(m->npl + 1);
+    std::vector<int> rpl(nr), rch(nr), pl_ptr(m->npl + 1), ch_ptr(m->nch + 1);
+    std::vector<double> ch_dat;
+    for (int s = 0; s < m->nch; s++) { ch_ptr[s] = (int)ch_dat.size(); cheb_block(m, s, ch_dat); }
+    ch_ptr[m->nch] = (int)ch_dat.size();
     std::vector<double> rv((size_t)3 * nr, 0.), pl_lev;
     for (int s = 0; s < m->npl; s++) {
         pl_ptr[s] = (int)(pl_lev.size() / 4);
@@ -653,6 +693,7 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
         rA[jn] = m->lnA[j]; rB[jn] = m->beta[j]; rE[jn] = m->E_over_R[j];
         rinfo[jn] = (m->rflags[j] & 7) | (smooth_forward(m, j) ? 8 : 0) | (smooth_reverse(m, j) ? 16 : 0);
         rpl[jn] = plog_slot_of(m, j);
+        rch[jn] = cheb_slot_of(m, j);
         if (m->rflags[j] & LSM_RXN_EXPLICIT_REV)
             for (int i = 0; i < 3; i++) rv[3 * jn + i] = m->rev_arr[3 * j + i];
         rtb[jn] = tb_of[j]; rfo[jn] = fo_of[j];
@@ -741,6 +782,7 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     size_t o_fop = b.add(fo_ptr), o_fos = b.add(fo_sp), o_foe = b.add(fo_eff);
     size_t o_fp = b.add(f_ptr), o_frx = b.add(f_rx), o_fnu = b.add(f_nu);
     size_t o_rv = b.add(rv), o_rpl = b.add(rpl), o_plptr = b.add(pl_ptr), o_pllev = b.add(pl_lev);
+    size_t o_rch = b.add(rch), o_chptr = b.add(ch_ptr), o_chdat = b.add(ch_dat);
     std::vector<char> image;
     const bool img_ok = build_image(m, perm, tb_of, fo_of, d.ns_pad, d.img, image);
     size_t o_img = img_ok ? b.add(image) : 0;
@@ -760,6 +802,7 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     d.fo_ptr = I(o_fop); d.fo_sp = I(o_fos); d.fo_eff = D(o_foe);
     d.f_ptr = I(o_fp); d.f_rx = I(o_frx); d.f_nu = D(o_fnu);
     d.rv = D(o_rv); d.rpl = I(o_rpl); d.pl_ptr = I(o_plptr); d.pl_lev = D(o_pllev);
+    d.rch = I(o_rch); d.ch_ptr = I(o_chptr); d.ch_dat = D(o_chdat);
     d.img.src = img_ok ? (const void *)(base + o_img) : nullptr;
     return LSM_OK;
 }

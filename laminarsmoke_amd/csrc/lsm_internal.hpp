@@ -27,7 +27,8 @@ enum SpRow {
 //   W    uint   [nr][4]             info, reactant bytes, product bytes, general list:
 //        info: bit0 reversible, bit1 negative A, bit2 fixed-format record, bits 3/4
 //              non-elementary forward/reverse side, bit5 explicit reverse Arrhenius,
-//              bit6 chemically activated (CABR) fall-off entry, bit7 PLOG, bits 8-15
+//              bit6 chemically activated (CABR) fall-off entry, bit7 PLOG (beta field 0)
+//              or Chebyshev (beta field 1) with its slot in the lnA field, bits 8-15
 //              third-body slot+1, bits 16-23 fall-off slot+1, bits 24-31 number of
 //              explicit forward orders (their entries follow the products);
 //        fixed-format record: up to 3 reactant and 3 product species, integer orders as
@@ -42,6 +43,8 @@ enum SpRow {
 //   fos  int [nfo] single collider or -1; fol uint [nfo+1]; fos2 uint []; foe double []
 //   rvp  double [nrev][4]           explicit reverse lnA, beta, E/R, 0
 //   plp  uint [npl+1], pll double [levels][4]   PLOG levels: ln p, lnA, beta, E/R
+//   chp  uint [nch+1], chd double []            Chebyshev: N, M, Tmin, Tmax, log10 Pmin,
+//                                               log10 Pmax, conv, a[N*M]
 enum { LSM_DUMMY_SP = 63 };
 enum { LSP_ROWS = 16 };
 struct LdsImage {
@@ -51,6 +54,8 @@ struct LdsImage {
     int o_rvp;     // [nrev][4] explicit reverse lnA, beta, E/R, 0
     int o_plp;     // [npl+1] first level of each PLOG reaction
     int o_pll;     // [levels][4] ln p, lnA, beta, E/R
+    int o_chp;     // [nch+1] first double of each Chebyshev reaction in chd
+    int o_chd;     // per Chebyshev reaction: N, M, Tmin, Tmax, log10 Pmin, log10 Pmax, conv, a[N*M]
     int team_ok;   // the team kernel can use the image (ns <= 63)
     int wide_ok;   // the wide kernel can use the image (ns <= 254)
     int dummy;     // unused slot of fixed-format records: 63 if ns <= 63, else 255
@@ -85,6 +90,9 @@ struct DevMech {
     const int *rpl;                          // [nr] PLOG slot or -1
     const int *pl_ptr;                       // [npl+1] first level of each PLOG reaction
     const double *pl_lev;                    // [levels][4] ln p, lnA, beta, E/R
+    const int *rch;                          // [nr] Chebyshev slot or -1
+    const int *ch_ptr;                       // [nch+1] first double of each Chebyshev reaction
+    const double *ch_dat;                    // N, M, Tmin, Tmax, log10 Pmin, log10 Pmax, conv, a[N*M]
     LdsImage img;
 };
 

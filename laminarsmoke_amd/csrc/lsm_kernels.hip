@@ -177,6 +177,40 @@ __device__ __forceinline__ double plog_lnk(const double *lv, int n, double P, do
     return lnkA + (lnkB - lnkA) * w;
 }
 
+// Forward constant of a Chebyshev reaction at (T, P)
+// (ChebyshevPolynomialRateExpression::KineticConstant,
+// kernel/kinetics/ChebyshevPolynomialRateExpression.hpp:95-116): d = [N, M, Tmin, Tmax,
+// log10 Pmin, log10 Pmax, conv, a_00 .. a_(N-1)(M-1)], k = conv 10^(sum a_nm phi_n(T~)
+// phi_m(P~)).  phi_n(x) = cos(n acos x) is evaluated by the recurrence
+// phi_(n+1) = 2 x phi_n - phi_(n-1) (the same polynomial, T_n(cos t) = cos(n t)); outside
+// [-1, 1] the reference's acos is undefined, and k is NaN here as well.
+// dlnk = d ln k / dT at fixed P (T_n' from its own recurrence).
+__device__ __forceinline__ double cheb_k(const double *d, double uT, double P, double &dlnk) {
+    const int N = (int)d[0], M = (int)d[1];
+    const double Tt = (2. * uT - 1. / d[2] - 1. / d[3]) / (1. / d[3] - 1. / d[2]);
+    const double Pt = (2. * log10(P) - d[4] - d[5]) / (d[5] - d[4]);
+    if (!(fabs(Tt) <= 1.) || !(fabs(Pt) <= 1.)) { dlnk = NAN; return NAN; }
+    const double *a = d + 7;
+    double sum = 0., dsum = 0.;
+    double t0 = 1., t1 = Tt, dt0 = 0., dt1 = 1.;
+    for (int n = 0; n < N; n++) {
+        const double tn = n == 0 ? t0 : t1, dtn = n == 0 ? dt0 : dt1;
+        double p0 = 1., p1 = Pt;
+        for (int m = 0; m < M; m++) {
+            const double pm = m == 0 ? p0 : p1;
+            sum += a[n * M + m] * tn * pm;
+            dsum += a[n * M + m] * dtn * pm;
+            if (m > 0) { const double p2 = 2. * Pt * p1 - p0; p0 = p1; p1 = p2; }
+        }
+        if (n > 0) {
+            const double t2 = 2. * Tt * t1 - t0, dt2 = 2. * t1 + 2. * Tt * dt1 - dt0;
+            t0 = t1; t1 = t2; dt0 = dt1; dt1 = dt2;
+        }
+    }
+    dlnk = LN10 * dsum * (-2. * uT * uT / (1. / d[3] - 1. / d[2]));
+    return exp(LN10 * sum) * d[6];
+}
+
 // Net rate of reaction j [kmol/m3/s].  C, H, S return c_k, h_k/RT, s_k/R; P is the
 // pressure the kinetics see (it only enters PLOG constants).  XK: the mechanism has
 // explicitly reversible, chemically activated or PLOG reactions (without them the
@@ -191,6 +225,9 @@ __device__ __forceinline__ double reaction_rate(const DevMech &M, int j, double 
         const int s = M.rpl[j];
         double d;
         kf = exp(plog_lnk(M.pl_lev + 4 * M.pl_ptr[s], M.pl_ptr[s + 1] - M.pl_ptr[s], P, logT, uT, d));
+    } else if (XK && M.rch[j] >= 0) {    // Chebyshev (hpp:1333-1340)
+        double d;
+        kf = cheb_k(M.ch_dat + M.ch_ptr[M.rch[j]], uT, P, d);
     } else {
         kf = exp(M.rA[j] + M.rB[j] * logT - M.rE[j] * uT);
         if (info & LSM_RXN_NEGATIVE_A) kf = -kf;
@@ -291,18 +328,19 @@ __device__ __forceinline__ void nasa7(const DevMech &M, int k, double T, double 
 // ---------------------------------------------------------------------------
 // Typed views of the mechanism image once it sits in LDS (layout: lsm_internal.hpp).
 struct LdsMech {
-    const double *sp, *P, *nuv, *tbe, *fop, *foe, *rvp, *pll;
+    const double *sp, *P, *nuv, *tbe, *fop, *foe, *rvp, *pll, *chd;
     const uint4 *W;
-    const unsigned *ent, *tbp, *tbs, *fol, *fos2, *plp;
+    const unsigned *ent, *tbp, *tbs, *fol, *fos2, *plp, *chp;
     const int *fos;
     int SP;   // species stride (ns_pad)
     __device__ LdsMech(const unsigned char *b, const LdsImage &L, int ns_pad)
         : sp((const double *)(b + L.o_sp)), P((const double *)(b + L.o_P)), nuv((const double *)(b + L.o_nuv)),
           tbe((const double *)(b + L.o_tbe)), fop((const double *)(b + L.o_fop)), foe((const double *)(b + L.o_foe)),
-          rvp((const double *)(b + L.o_rvp)), pll((const double *)(b + L.o_pll)),
+          rvp((const double *)(b + L.o_rvp)), pll((const double *)(b + L.o_pll)), chd((const double *)(b + L.o_chd)),
           W((const uint4 *)(b + L.o_W)), ent((const unsigned *)(b + L.o_ent)), tbp((const unsigned *)(b + L.o_tbp)),
           tbs((const unsigned *)(b + L.o_tbs)), fol((const unsigned *)(b + L.o_fol)),
-          fos2((const unsigned *)(b + L.o_fos2)), plp((const unsigned *)(b + L.o_plp)), fos((const int *)(b + L.o_fos)),
+          fos2((const unsigned *)(b + L.o_fos2)), plp((const unsigned *)(b + L.o_plp)),
+          chp((const unsigned *)(b + L.o_chp)), fos((const int *)(b + L.o_fos)),
           SP(ns_pad) {}
     __device__ __forceinline__ double nu(unsigned e) const {
         const unsigned c = e >> 12;
@@ -323,10 +361,11 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
     const double2 ed = ((const double2 *)L.P)[2 * j + 1];
     const unsigned info = w.x;
     double kf;
-    if (XK && (info & 128u)) {    // PLOG: slot in the lnA field
+    if (XK && (info & 128u)) {    // PLOG (beta field 0) or Chebyshev (1): slot in the lnA field
         const int s = (int)ab.x;
         double d;
-        kf = exp(plog_lnk(L.pll + 4 * L.plp[s], (int)(L.plp[s + 1] - L.plp[s]), Pk, logT, uT, d));
+        kf = ab.y == 0. ? exp(plog_lnk(L.pll + 4 * L.plp[s], (int)(L.plp[s + 1] - L.plp[s]), Pk, logT, uT, d))
+                        : cheb_k(L.chd + L.chp[s], uT, Pk, d);
     } else {
         kf = exp(ab.x + ab.y * logT - ed.x * uT);
         if (info & 2u) kf = -kf;
@@ -453,9 +492,10 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
     const double2 ed = ((const double2 *)L.P)[2 * j + 1];
     const unsigned info = w.x;
     double kf, dlnkf;
-    if (XK && (info & 128u)) {    // PLOG
+    if (XK && (info & 128u)) {    // PLOG or Chebyshev
         const int s = (int)ab.x;
-        kf = exp(plog_lnk(L.pll + 4 * L.plp[s], (int)(L.plp[s + 1] - L.plp[s]), Pk, logT, uT, dlnkf));
+        kf = ab.y == 0. ? exp(plog_lnk(L.pll + 4 * L.plp[s], (int)(L.plp[s + 1] - L.plp[s]), Pk, logT, uT, dlnkf))
+                        : cheb_k(L.chd + L.chp[s], uT, Pk, dlnkf);
     } else {
         kf = exp(ab.x + ab.y * logT - ed.x * uT);
         if (info & 2u) kf = -kf;

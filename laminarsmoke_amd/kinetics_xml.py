@@ -19,7 +19,8 @@ Reader: the sections read by
     and the species / efficiency lists; Reverse lnA / Beta / E_over_R of the
     Reversible-Explicit reactions; PressureLog per-reaction "n (P A beta E)... convA
     convE" lists, P in atm (kernel/kinetics/PressureLogarithmicRateExpression.hpp:71-102,
-    121-132));
+    121-132); Chebyshev per-reaction "n N M a11 .. aNM conv", "Pmin Pmax" [atm],
+    "Tmin Tmax" (kernel/kinetics/ChebyshevPolynomialRateExpression.hpp:71-93, 130-149));
   * StoichiometricMap::ReadFromASCIIFile (maps/StoichiometricMap.hpp:78-254): the
     species-major lists by stoichiometric class (1, 2, 3, 0.5, other) for reactants
     (numDir/jDir/valueDir5), products of all reactions (numRevTot/jRevTot/
@@ -35,8 +36,8 @@ CHEMKIN input can be stored the way laminarSMOKE cases keep them.
 
 Explicit forward reaction orders (the lambda lists after the flag) are read into
 Mechanism.ord_*; explicit reverse orders that differ from the product coefficients,
-and forms the kernels do not evaluate (Chebyshev, extended PLOG and fall-off, FIT1,
-Janev-Langer, Landau-Teller), raise KineticsXmlError.
+and forms the kernels do not evaluate (extended PLOG and fall-off, FIT1, Janev-Langer,
+Landau-Teller), raise KineticsXmlError.
 """
 from __future__ import annotations
 
@@ -49,7 +50,7 @@ from .mechanism import (FALLOFF_CHEMACT, FALLOFF_LINDEMANN, FALLOFF_SRI, FALLOFF
                         RXN_NEGATIVE_A, RXN_REVERSIBLE, Mechanism)
 
 VERSION = "04-22-2013"
-_UNSUPPORTED_LISTS = ("Chebyshev", "ExtendedPressureLog", "ExtendedFallOff", "FIT1", "JanevLanger", "LandauTeller")
+_UNSUPPORTED_LISTS = ("ExtendedPressureLog", "ExtendedFallOff", "FIT1", "JanevLanger", "LandauTeller")
 
 
 class KineticsXmlError(ValueError):
@@ -134,6 +135,7 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
     cabr_node = kin.find("CABR")
     cabr_idx = _Tokens(cabr_node.text).vec(int) if cabr_node is not None else []
     pl_idx = _ilist(kin, "PressureLog", required=False)
+    ch_idx = _ilist(kin, "Chebyshev", required=False)
 
     kp = _node(kin, "KineticParameters")
     direct = _node(kp, "Direct")
@@ -242,6 +244,18 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
                 raise KineticsXmlError("PLOG pressures must increase")
             pl_lev.append(lev)
 
+    ch_dims, ch_lim, ch_a = [], [], []
+    if ch_idx:                                     # KineticsMap_CHEMKIN.hpp:915-925
+        t = _Tokens(_node(kp, "Chebyshev").text)
+        for _ in ch_idx:
+            c = t.vec()
+            if len(c) < 3 or int(c[0]) < 1 or int(c[1]) < 1 or int(c[0]) * int(c[1]) != len(c) - 3:
+                raise KineticsXmlError("Chebyshev entry is not n N M a11 .. aNM conv")
+            pmin, pmax, tmin, tmax = (float(t.word()) for _ in range(4))
+            ch_dims.append((int(c[0]), int(c[1])))
+            ch_lim.append((tmin, tmax, pmin * 101325.0, pmax * 101325.0, c[-1]))
+            ch_a.append(c[2:-1])
+
     st = _node(kin, "Stoichiometry")
     if st.get("type") != "OpenSMOKE" or st.get("version") != VERSION:
         raise KineticsXmlError("The current stoichiometric data are not supported")
@@ -276,6 +290,10 @@ def read_kinetics_xml(text: str, name: str = "kinetics.xml") -> Mechanism:
         rev_arr=rev_arr, pl_rxn=np.array([j - 1 for j in pl_idx], dtype=np.int32),
         pl_ptr=np.concatenate([[0], np.cumsum([len(x) for x in pl_lev])]).astype(np.int32),
         pl_par=np.array([v for x in pl_lev for v in x], dtype=np.float64).reshape(-1, 4),
+        ch_rxn=np.array([j - 1 for j in ch_idx], dtype=np.int32),
+        ch_dims=np.array(ch_dims, dtype=np.int32).reshape(-1, 2),
+        ch_lim=np.array(ch_lim, dtype=np.float64).reshape(-1, 5), ch_ptr=csr(ch_a),
+        ch_a=np.array([v for x in ch_a for v in x], dtype=np.float64),
         name=name, element_names=tuple(e.upper() for e in el_names), atomic_weights=aw,
     )
 
@@ -442,6 +460,8 @@ def write_kinetics_xml(m: Mechanism) -> str:
         k.append(f"<{tag}>\n" + _ivec([int(m.fo_rxn[i]) + 1 for i in sel]) + "\n" +
                  _tag("Lindemann", by(FALLOFF_LINDEMANN)) + "\n" + _tag("Troe", by(FALLOFF_TROE)) + "\n" +
                  _tag("SRI", by(FALLOFF_SRI)) + f"\n</{tag}>")
+    if len(m.ch_rxn):
+        k.append(_tag("Chebyshev", _ivec(m.ch_rxn + 1)))
     if len(m.pl_rxn):
         k.append(_tag("PressureLog", _ivec(m.pl_rxn + 1)))
     neg = [j + 1 for j in range(nr) if m.rflags[j] & RXN_NEGATIVE_A]
@@ -482,6 +502,15 @@ def write_kinetics_xml(m: Mechanism) -> str:
         inf = m.fo_inf[sel]
         kp.append(_tag(tag, "\n".join([_tag("lnA", _vec(inf[:, 0])), _tag("Beta", _vec(inf[:, 1])),
                                        _tag("E_over_R", _vec(inf[:, 2])), _tag("Parameters", "\n".join(ps))])))
+    if len(m.ch_rxn):
+        es = []
+        for s in range(len(m.ch_rxn)):
+            N, M = (int(v) for v in m.ch_dims[s])
+            vals = [N, M] + m.ch_a[m.ch_ptr[s]:m.ch_ptr[s + 1]].tolist() + [m.ch_lim[s, 4]]
+            Tl, Tu, Pl, Pu = m.ch_lim[s, :4]
+            es.append(f"{len(vals)}\n" + " ".join(f"{v:.17g}" for v in vals) +
+                      f"\n{Pl / 101325.0:.17g} {Pu / 101325.0:.17g}\n{Tl:.17g} {Tu:.17g}")
+        kp.append(_tag("Chebyshev", "\n".join(es)))
     if len(m.pl_rxn):
         # raw values with unit factors 1 (A) and R (E): P [atm], A, beta, E/R
         es = []

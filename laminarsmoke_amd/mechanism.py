@@ -84,6 +84,11 @@ class Mechanism:
     pl_rxn: object = None             # [npl] PLOG reactions
     pl_ptr: object = None             # [npl+1] their pressure levels
     pl_par: object = None             # [levels, 4] P [Pa], lnA, beta, E_over_R
+    ch_rxn: object = None             # [nch] Chebyshev reactions
+    ch_dims: object = None            # [nch, 2] N (temperature), M (pressure)
+    ch_lim: object = None             # [nch, 5] Tmin, Tmax [K], Pmin, Pmax [Pa], conv
+    ch_ptr: object = None             # [nch+1] first coefficient of each
+    ch_a: object = None               # coefficients a_nm, row-major
     element_names: tuple = ELEMENTS    # columns of `elements`
     atomic_weights: object = None      # [n_elements] kg/kmol (default: C, H, O, N)
     _struct: object = field(default=None, repr=False)
@@ -130,6 +135,13 @@ class Mechanism:
         self.pl_rxn = np.ascontiguousarray(self.pl_rxn, dtype=np.int32)
         self.pl_ptr = np.ascontiguousarray(self.pl_ptr, dtype=np.int32)
         self.pl_par = np.ascontiguousarray(np.asarray(self.pl_par, dtype=np.float64).reshape(-1, 4))
+        if self.ch_rxn is None:
+            self.ch_rxn, self.ch_dims, self.ch_lim, self.ch_ptr, self.ch_a = [], [], [], [0], []
+        self.ch_rxn = np.ascontiguousarray(self.ch_rxn, dtype=np.int32)
+        self.ch_dims = np.ascontiguousarray(np.asarray(self.ch_dims, dtype=np.int32).reshape(-1, 2))
+        self.ch_lim = np.ascontiguousarray(np.asarray(self.ch_lim, dtype=np.float64).reshape(-1, 5))
+        self.ch_ptr = np.ascontiguousarray(self.ch_ptr, dtype=np.int32)
+        self.ch_a = np.ascontiguousarray(self.ch_a, dtype=np.float64)
         if self.atomic_weights is None:
             self.atomic_weights = ATOMIC_WEIGHTS
         self.atomic_weights = np.asarray(self.atomic_weights, dtype=np.float64)
@@ -152,6 +164,9 @@ class Mechanism:
         assert self.fo_inf.shape == (len(self.fo_rxn), 3)
         assert self.fo_troe.shape == (len(self.fo_rxn), 4)
         assert self.pl_ptr.shape == (len(self.pl_rxn) + 1,) and self.pl_ptr[-1] == self.pl_par.shape[0]
+        nch = len(self.ch_rxn)
+        assert self.ch_dims.shape == (nch, 2) and self.ch_lim.shape == (nch, 5)
+        assert self.ch_ptr.shape == (nch + 1,) and self.ch_ptr[-1] == self.ch_a.shape[0]
 
     def struct(self) -> LsmMechanism:
         """ctypes view of the tables (valid while this object lives)."""
@@ -178,6 +193,9 @@ class Mechanism:
             s.rev_arr = P(self.rev_arr, d)
             s.npl = len(self.pl_rxn)
             s.pl_rxn, s.pl_ptr, s.pl_par = P(self.pl_rxn, i), P(self.pl_ptr, i), P(self.pl_par, d)
+            s.nch = len(self.ch_rxn)
+            s.ch_rxn, s.ch_dims, s.ch_lim = P(self.ch_rxn, i), P(self.ch_dims, i), P(self.ch_lim, d)
+            s.ch_ptr, s.ch_a = P(self.ch_ptr, i), P(self.ch_a, d)
             self._struct = s
         return self._struct
 
@@ -467,8 +485,9 @@ def custom_mechanism(species, nasa, reactions, elements=None, name="custom") -> 
     ``tb`` (list of (species, eff)), ``falloff`` (dict with ``inf`` = (lnA, beta,
     EoR), ``troe`` = (a, T3, T1, T2) or None, ``sri`` = (a, b, c, d, e) or None,
     ``cabr`` = True for a chemically activated reaction, ``single`` = species or
-    -1, ``eff`` = list of (species, eff)) and ``plog`` (list of (P [Pa], lnA,
-    beta, EoR) levels).
+    -1, ``eff`` = list of (species, eff)), ``plog`` (list of (P [Pa], lnA,
+    beta, EoR) levels) and ``cheb`` (dict with ``a`` = N x M coefficients, ``T`` =
+    (Tmin, Tmax), ``P`` = (Pmin, Pmax) [Pa], ``conv`` = multiplier).
     """
     ns = len(species)
     nasa = np.asarray(nasa, dtype=np.float64).reshape(ns, 18)
@@ -478,6 +497,7 @@ def custom_mechanism(species, nasa, reactions, elements=None, name="custom") -> 
     tb_rxn, tb_sp, tb_eff = [], [], []
     fo_rxn, fo_type, fo_inf, fo_troe, fo_single, fo_sp, fo_eff = [], [], [], [], [], [], []
     fo_sri, rev_arr, pl_rxn, pl_lev = [], np.zeros((len(reactions), 3)), [], []
+    ch_rxn, ch_dims, ch_lim, ch_a = [], [], [], []
     for j, r in enumerate(reactions):
         lnA.append(r["lnA"])
         beta.append(r.get("beta", 0.0))
@@ -491,6 +511,13 @@ def custom_mechanism(species, nasa, reactions, elements=None, name="custom") -> 
         if r.get("plog") is not None:
             pl_rxn.append(j)
             pl_lev.append([tuple(v) for v in r["plog"]])
+        if r.get("cheb") is not None:
+            cb = r["cheb"]
+            a = np.atleast_2d(np.asarray(cb["a"], dtype=np.float64))
+            ch_rxn.append(j)
+            ch_dims.append(a.shape)
+            ch_lim.append((*cb["T"], *cb["P"], cb.get("conv", 1.0)))
+            ch_a.append(a.ravel().tolist())
         flags.append(fl)
         reac.append(list(r["reac"]))
         prod.append(list(r["prod"]))
@@ -537,6 +564,9 @@ def custom_mechanism(species, nasa, reactions, elements=None, name="custom") -> 
         fo_sri=np.array(fo_sri, dtype=np.float64).reshape(nfo, 5),
         rev_arr=rev_arr, pl_rxn=np.array(pl_rxn, dtype=np.int32), pl_ptr=csr(pl_lev),
         pl_par=np.array([v for x in pl_lev for v in x], dtype=np.float64).reshape(-1, 4),
+        ch_rxn=np.array(ch_rxn, dtype=np.int32), ch_dims=np.array(ch_dims, dtype=np.int32).reshape(-1, 2),
+        ch_lim=np.array(ch_lim, dtype=np.float64).reshape(-1, 5), ch_ptr=csr(ch_a),
+        ch_a=np.array([v for x in ch_a for v in x], dtype=np.float64),
         name=name,
     )
 

@@ -280,8 +280,25 @@ static double plog_k(const lsm_mechanism *m, int k, double T, double P) {
     return exp(lnkA + (lnkB - lnkA) * (log(P) - lnpA) / (lnpB - lnpA));
 }
 
+/* ChebyshevPolynomialRateExpression::KineticConstant
+ * (kernel/kinetics/ChebyshevPolynomialRateExpression.hpp:95-116, Phi at :187-190). */
+static double cheb_k(const lsm_mechanism *m, int k, double T, double P) {
+    const int N = m->ch_dims[2 * k], Mm = m->ch_dims[2 * k + 1];
+    const double *L = m->ch_lim + 5 * k, *a = m->ch_a + m->ch_ptr[k];
+    const double Tmin = L[0], Tmax = L[1];
+    const double lPmin = log10(L[2]), lPmax = log10(L[3]);
+    const double Tt = (2. / T - 1. / Tmin - 1. / Tmax) / (1. / Tmax - 1. / Tmin);
+    const double Pt = (2. * log10(P) - lPmin - lPmax) / (lPmax - lPmin);
+    double sum = 0.;
+    for (int n = 0; n < N; n++) {
+        const double pn = cos((double)n * acos(Tt));
+        for (int q = 0; q < Mm; q++) sum += a[n * Mm + q] * pn * cos((double)q * acos(Pt));
+    }
+    return pow(10., sum) * L[4];
+}
+
 /* P is the pressure the kinetics map is set to (SetPressure): it only enters the
- * PLOG constants. */
+ * PLOG and Chebyshev constants. */
 static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, double P, const double *c) {
     const lsm_mechanism *m = M->m;
     const int ns = M->ns, nr = M->nr;
@@ -312,7 +329,8 @@ static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, double P, cons
         w->kf[j] = exp(m->lnA[j] + m->beta[j] * logT - m->E_over_R[j] * uT);
         if (m->rflags[j] & LSM_RXN_NEGATIVE_A) w->kf[j] *= -1.;
     }
-    /* hpp:1342-1349: PLOG constants replace the Arrhenius ones */
+    /* hpp:1333-1349: Chebyshev and PLOG constants replace the Arrhenius ones */
+    for (int k = 0; k < m->nch; k++) w->kf[m->ch_rxn[k]] = cheb_k(m, k, T, P);
     for (int k = 0; k < m->npl; k++) w->kf[m->pl_rxn[k]] = plog_k(m, k, T, P);
     /* hpp:1215-1222: 1/Kc = exp(-dS/R + dH/RT - dn ln(Patm/RT)); explicitly
      * reversible reactions scale the reverse product by k_rev/k instead

@@ -242,7 +242,10 @@ def test_rev_high_plog_units():
     ("H+O2+M=HO2+M 1.0E12 0.0 0.0\n PLOG/ 1.0 1.0E12 0.0 0.0/", "PLOG cannot"),
     ("H+O2=OH+O 1.0E12 0.0 0.0\n PLOG/ 10.0 1.0E12 0.0 0.0/\n PLOG/ 1.0 1.0E12 0.0 0.0/", "must increase"),
     ("H+O2=OH+O 1.0E12 0.0 0.0\n PLOG/ 1.0 -1.0E12 0.0 0.0/", "non-negative"),
-    ("H+O2=OH+O 1.0E12 0.0 0.0\n CHEB/ 7 4/", "CHEB"),
+    ("H+O2=OH+O 1.0E12 0.0 0.0\n CHEB/ 1 1 3.0/", "CHEB can be used only with"),
+    ("H+O2(+M)=HO2(+M) 1.0E12 0.0 0.0\n CHEB/ 2 2 3.0 1.0 0.5/", "N x M coefficients"),
+    ("H+O2(+M)=HO2(+M) 1.0E12 0.0 0.0\n LOW/ 1.0E18 -1.0 0.0/\n CHEB/ 1 1 3.0/", "CHEB cannot"),
+    ("H+O2=OH+O 1.0E12 0.0 0.0\n TCHEB/ 300 2500/", "without CHEB"),
     ("H+O2=OH+XX 1.0E12 0.0 0.0", "unknown species"),
     ("H+O2(+M)=HO2(+M) 1.0E12 0.5 0.0", "one of LOW and HIGH"),
 ])

@@ -1,5 +1,5 @@
-"""GPU: explicitly reversible (REV), chemically activated (CABR) and pressure-logarithmic
-(PLOG) reactions through every kernel, against the oracle, at pressures on, between
+"""GPU: explicitly reversible (REV), chemically activated (CABR), pressure-logarithmic
+(PLOG) and Chebyshev reactions through every kernel, against the oracle, at pressures on, between
 and outside the PLOG levels, and in the constant-volume reactor (whose kinetics see
 the pressure cTot R T of the current state).
 
@@ -11,7 +11,10 @@ The mechanism is the reference's H2/CO mechanism (POLIMI_H2CO_1412) with
   activated ones (Troe, and SRI with d and e), with k0 = the fall-off k_inf and
   k_inf = k_inf^2/k0 so that Pr is the fall-off's;
 * PLOG tables of four levels (0.1-100 atm) on four reactions (reversible,
-  irreversible, and with REV).
+  irreversible, and with REV);
+* Chebyshev tables (6 x 4 over 250-4000 K and 0.005-1000 atm) on two reactions
+  (reversible and irreversible): least-squares fits of their Arrhenius law times a
+  pressure factor.
 """
 import dataclasses
 import os
@@ -81,11 +84,26 @@ def forms_mechanism(oracle):
                              (100.0, 2.5, -0.1, 1.1)):
             lev.append((P * ATM, lnA[j] + np.log(f), beta[j] + db, EoR[j] * fe))
         levels.append(lev)
+    ch = [_find(base, ["H", "HO2"], ["H2", "O2"]), _find(base, ["HO2", "H2O"], ["H2O2", "OH"])]
+    Tl, Pl = (250.0, 4000.0), (0.005 * ATM, 1000.0 * ATM)
+    xs = np.cos(np.pi * (np.arange(24) + 0.5) / 24)                 # Chebyshev nodes in [-1, 1]
+    Tn = 2.0 / ((xs + 1.0) * (1.0 / Tl[1] - 1.0 / Tl[0]) + 2.0 / Tl[0])
+    Pn = 10 ** ((xs[:12] * (np.log10(Pl[1]) - np.log10(Pl[0])) + np.log10(Pl[1]) + np.log10(Pl[0])) / 2.0)
+    xx, yy = np.meshgrid(xs, xs[:12], indexing="ij")
+    TT, PP = np.meshgrid(Tn, Pn, indexing="ij")
+    V = np.polynomial.chebyshev.chebvander2d(xx.ravel(), yy.ravel(), [5, 3])
+    ch_a = []
+    for j in ch:
+        target = (lnA[j] + beta[j] * np.log(TT) - EoR[j] / TT) / np.log(10.0) + \
+            0.15 * np.log10(PP / ATM) * (1.0 - 300.0 / TT)
+        ch_a.append(np.linalg.lstsq(V, target.ravel(), rcond=None)[0])
     m = dataclasses.replace(base, rflags=rflags, rev_arr=rev_arr, lnA=lnA, beta=beta, E_over_R=EoR,
                             fo_type=fo_type, fo_inf=fo_inf, fo_sri=fo_sri, pl_rxn=np.array(pl, dtype=np.int32),
                             pl_ptr=np.array([0, 4, 8, 12, 16], dtype=np.int32),
-                            pl_par=np.array([v for x in levels for v in x]), name="H2CO_rev_cabr_plog",
-                            _struct=None)
+                            pl_par=np.array([v for x in levels for v in x]),
+                            ch_rxn=np.array(ch, dtype=np.int32), ch_dims=np.array([[6, 4], [6, 4]], dtype=np.int32),
+                            ch_lim=np.array([[*Tl, *Pl, 1.0]] * 2), ch_ptr=np.array([0, 24, 48], dtype=np.int32),
+                            ch_a=np.concatenate(ch_a), name="H2CO_rev_cabr_plog_cheb", _struct=None)
     return m
 
 

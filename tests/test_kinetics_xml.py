@@ -143,12 +143,14 @@ def _same(a, b):
     for f in ("nasa", "lnA", "beta", "E_over_R", "rflags", "reac_ptr", "reac_sp", "reac_nu", "prod_ptr", "prod_sp",
               "prod_nu", "tb_rxn", "tb_ptr", "tb_sp", "tb_eff_m1", "fo_rxn", "fo_type", "fo_inf", "fo_troe",
               "fo_single", "fo_ptr", "fo_sp", "fo_eff_m1", "fo_sri", "ord_ptr", "ord_sp", "ord_val", "elements",
-              "rev_arr", "pl_rxn", "pl_ptr"):
+              "rev_arr", "pl_rxn", "pl_ptr", "ch_rxn", "ch_dims", "ch_ptr", "ch_a"):
         x, y = getattr(a, f), getattr(b, f)
         assert x.shape == y.shape and np.array_equal(x, y), f
     # PLOG levels travel as raw CHEMKIN values (P in atm, A, E) with unit factors
     assert a.pl_par.shape == b.pl_par.shape
     np.testing.assert_allclose(a.pl_par, b.pl_par, rtol=1e-14, atol=0)
+    assert a.ch_lim.shape == b.ch_lim.shape                       # Chebyshev pressures travel in atm
+    np.testing.assert_allclose(a.ch_lim, b.ch_lim, rtol=1e-15, atol=0)
     assert a.species == b.species
 
 
@@ -166,7 +168,7 @@ def test_round_trip(name):
 
 
 REV_CABR_PLOG = """ELEMENTS H O N END
-SPECIES H2 O2 H2O H O OH HO2 N2 END
+SPECIES H2 O2 H2O H O OH HO2 N2 H2O2 END
 REACTIONS
 H+O2=OH+O 1.0E13 0.5 1000.0
  REV/ 2.0E12 -0.3 500.0/
@@ -184,23 +186,28 @@ H+HO2(+N2)=H2+O2(+N2) 1.0E12 0.0 0.0
 H+HO2(+N2)=2OH(+N2) 1.0E11 0.0 0.0
  HIGH/ 1.0E9 0.0 0.0/
  SRI/ 0.5 200 900 /
+H2O2(+M)=2OH(+M) 1.0 0.0 0.0
+ TCHEB/ 290.0 3000.0 / PCHEB/ 0.01 98.7 /
+ CHEB/ 3 2 8.0 0.5 -1.2 0.1 0.05 -0.02 /
 END
 """
 
 
 def test_round_trip_rev_cabr_plog(tmp_path):
-    """Explicit reverse, chemically activated (between two fall-off entries) and PLOG
-    reactions come back as the same tables; the XML holds the reference's lists."""
+    """Explicit reverse, chemically activated (between two fall-off entries), PLOG and
+    Chebyshev reactions come back as the same tables; the XML holds the reference's lists."""
     from laminarsmoke_amd.chemkin import read_chemkin
     from laminarsmoke_amd.mechanism import FALLOFF_CHEMACT, RXN_EXPLICIT_REV
     m = read_chemkin(REV_CABR_PLOG, open(os.path.join(GOLDEN, "POLIMI_H2CO_1412_thermo.CKT")).read())
     assert m.fo_type.tolist() == [FALLOFF_TROE | FALLOFF_CHEMACT, 0, FALLOFF_SRI | FALLOFF_CHEMACT]
     assert (m.rflags & RXN_EXPLICIT_REV).nonzero()[0].tolist() == [0, 3]
     text = write_kinetics_xml(m)
-    for tag, body in (("Reversible-Explicit", "2\n1 4"), ("Reversible-Thermodynamics", "4\n2 3 5 6"),
-                      ("PressureLog", "1\n3")):
+    for tag, body in (("Reversible-Explicit", "2\n1 4"), ("Reversible-Thermodynamics", "5\n2 3 5 6 7"),
+                      ("PressureLog", "1\n3"), ("Chebyshev", "1\n7")):
         assert f"<{tag}>\n{body}\n</{tag}>" in text, tag
     assert "<CABR>\n2\n2 6\n" in text and "<FallOff>\n1\n5\n" in text
+    assert m.ch_dims.tolist() == [[3, 2]] and m.ch_lim[0, 4] == 1.0        # unimolecular: conv 1e3^0
+    np.testing.assert_array_equal(m.ch_a, [8.0, 0.5, -1.2, 0.1, 0.05, -0.02])
     _same(m, read_kinetics_xml(text))
 
 

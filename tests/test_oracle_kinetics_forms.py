@@ -1,12 +1,13 @@
 """Pins of the oracle's explicitly reversible (REV), chemically activated (CABR,
-HIGH) and pressure-logarithmic (PLOG) reactions (CPU only).
+HIGH), pressure-logarithmic (PLOG) and Chebyshev (CHEB) reactions (CPU only).
 
 Each check uses a fact the oracle's code does not restate: the closed form of a
 net rate with an explicit reverse constant (kf c_f - kr c_b), the identity
 between a chemically activated and a fall-off reaction with the same data
 (k_CABR = k_falloff / M), the Lindemann limits of a chemically activated
 reaction, and the PLOG interpolation properties (exact Arrhenius at every
-pressure level, clamping outside the range, ln k linear in ln P between levels).
+pressure level, clamping outside the range, ln k linear in ln P between levels),
+and numpy's own 2-D Chebyshev series for CHEB.
 """
 import numpy as np
 import pytest
@@ -172,3 +173,53 @@ def test_plog_constant_volume_pressure(oracle):
         f_p, _ = oracle.rhs(m, y, P=P, energy=False)
         f_v, _ = oracle.rhs(m, y, energy=False, rho0=rho)
         np.testing.assert_allclose(f_v[:-1], f_p[:-1], rtol=1e-12, atol=0)
+
+
+CHEB = dict(T=(300.0, 2800.0), P=(0.01 * ATM, 100.0 * ATM), conv=1e-3)
+
+
+def _cheb_mech(a, **extra):
+    r = dict(reac=[(0, 1), (1, 1)], prod=[(2, 1), (3, 1)], lnA=np.log(1e99), beta=0.0, E_over_R=0.0,
+             reversible=False, cheb=dict(CHEB, a=a))
+    r.update(extra)
+    return _mech(r)
+
+
+def _reduced(T, P):
+    T1, T2 = CHEB["T"]
+    l1, l2 = np.log10(CHEB["P"][0]), np.log10(CHEB["P"][1])
+    return (2 / T - 1 / T1 - 1 / T2) / (1 / T2 - 1 / T1), (2 * np.log10(P) - l1 - l2) / (l2 - l1)
+
+
+def test_chebyshev_matches_numpy_chebval2d(oracle):
+    """log10(k/conv) is the 2-D Chebyshev series of numpy.polynomial.chebyshev at the
+    reduced temperature and pressure (T~ from 1/T, P~ from log10 P)."""
+    a = np.random.default_rng(7).normal(scale=[[4.0], [1.0], [0.3], [0.1], [0.05], [0.02]], size=(6, 4))
+    a[0, 0] += 10.0
+    m = _cheb_mech(a)
+    c = np.array([0.01, 0.02, 0.0, 0.0])
+    for T in (300.5, 451.0, 1000.0, 1999.0, 2799.0):
+        for P in (0.0101 * ATM, 0.3 * ATM, ATM, 42.0 * ATM, 99.0 * ATM):
+            x, y = _reduced(T, P)
+            expect = np.polynomial.chebyshev.chebval2d(x, y, a)
+            k = oracle.kinetics(m, T, c, P=P)["kf"][0]
+            assert np.log10(k / 1e-3) == pytest.approx(expect, abs=1e-12 * max(1.0, abs(expect)))
+
+
+def test_chebyshev_closed_forms(oracle):
+    """N = 2, M = 1: log10 k = a0 + a1 T~, pressure independent; T~ -> -1 at Tmin and +1 at
+    Tmax; outside [Tmin, Tmax] acos is undefined and k is NaN, as in the reference (at
+    the ends themselves the rounding of T~ decides)."""
+    m = _cheb_mech([[11.0], [-2.0]])
+    c = np.array([0.01, 0.02, 0.0, 0.0])
+    kf = lambda T, P=ATM: oracle.kinetics(m, T, c, P=P)["kf"][0]
+    assert kf(300.0 * (1 + 1e-12)) == pytest.approx(1e-3 * 10 ** 13.0, rel=1e-9)
+    assert kf(2800.0 * (1 - 1e-12)) == pytest.approx(1e-3 * 10 ** 9.0, rel=1e-9)
+    for P in (0.0101 * ATM, ATM, 99 * ATM):     # (phi_0 = cos(0 acos P~) is NaN outside the range too)
+        assert kf(1500.0, P) == pytest.approx(1e-3 * 10 ** (11.0 - 2.0 * _reduced(1500.0, P)[0]), rel=1e-13)
+    assert np.isnan(kf(290.0)) and np.isnan(kf(3000.0))
+    m2 = _cheb_mech([[11.0, 0.5]])                     # N = 1, M = 2: log10 k = 11 + 0.5 P~
+    assert oracle.kinetics(m2, 1200.0, c, P=0.01 * ATM * (1 + 1e-12))["kf"][0] == pytest.approx(
+        1e-3 * 10 ** 10.5, rel=1e-9)
+    assert oracle.kinetics(m2, 1200.0, c, P=100 * ATM * (1 - 1e-12))["kf"][0] == pytest.approx(
+        1e-3 * 10 ** 11.5, rel=1e-9)
