@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 #include "lsm_mechanism.h"
+#include "lsm_transport.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -256,7 +257,7 @@ int lsm_equations_device(lsm_handle *h, int64_t n_cells, const double *d_y, doub
  * step in every PIMPLE iteration (solvers/laminarSMOKE/properties.H:41-131): mole
  * fractions X, MWmix, cTot = p/(RT), psi = cTot MWmix/p, h, cp, cv and the species
  * cp (CpSpecies, with massDiffusionInEnergyEquation), per cell.  The transport part of
- * that loop (mu, lambda, Dmix) is not provided.
+ * that loop is lsm_transport_properties below.
  *   y: SoA [ns+1][n_cells] (Y, then T);  p: [n_cells] pressure, or NULL for the
  *   handle's reactor pressure;  props: [LSM_NPROPS][n_cells];  x, cp_species:
  *   [ns][n_cells] or NULL.  Host variant synchronous; _device takes device pointers
@@ -274,6 +275,24 @@ int lsm_properties(lsm_handle *h, int64_t n_cells, const double *y, const double
                    double *cp_species);
 int lsm_properties_device(lsm_handle *h, int64_t n_cells, const double *d_y, const double *d_p, double *d_props,
                           double *d_x, double *d_cp_species, void *stream);
+
+/* Transport part of the same loop (properties.H:101-147): the mixture viscosity
+ * (TransportPropertiesMap::DynamicViscosity), thermal conductivity
+ * (ThermalConductivity), mixture-averaged diffusion coefficients
+ * (MassDiffusionCoefficients) and thermal diffusion ratios (ThermalDiffusionRatios),
+ * from the fits of include/lsm_transport.h
+ * (maps/TransportPropertiesMap.hpp:41-71, maps/TransportPropertiesMap_CHEMKIN.hpp:690-1117).
+ * lsm_set_transport copies the fits to the device (tr->ns must equal the mechanism's);
+ * it may be called again to replace them.
+ *   y: SoA [ns+1][n_cells] (Y, then T);  p: [n_cells] or NULL (the handle's pressure);
+ *   mu [kg/m/s], lambda [W/m/K]: [n_cells];  dmix [m2/s], teta [-]: [ns][n_cells] or
+ *   NULL (teta is 0 for species without thermal diffusion ratios).  Host variant
+ *   synchronous; _device takes device pointers and a hipStream_t, no synchronisation. */
+int lsm_set_transport(lsm_handle *h, const lsm_transport *tr);
+int lsm_transport_properties(lsm_handle *h, int64_t n_cells, const double *y, const double *p, double *mu,
+                             double *lambda, double *dmix, double *teta);
+int lsm_transport_properties_device(lsm_handle *h, int64_t n_cells, const double *d_y, const double *d_p,
+                                    double *d_mu, double *d_lambda, double *d_dmix, double *d_teta, void *stream);
 
 /* Text of the last error of this thread ("" if none). */
 const char *lsm_last_error(void);

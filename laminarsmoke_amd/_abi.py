@@ -21,6 +21,14 @@ class LsmMechanism(ctypes.Structure):
     ]
 
 
+class LsmTransport(ctypes.Structure):
+    """include/lsm_transport.h: struct lsm_transport."""
+    _fields_ = [
+        ("ns", ctypes.c_int32), ("mw", _d), ("lambda_fit", _d), ("eta_fit", _d), ("gamma_fit", _d),
+        ("nteta", ctypes.c_int32), ("teta_sp", _i), ("teta_fit", _d), ("viscosity_model", ctypes.c_int32),
+    ]
+
+
 class LsmOdeParameters(ctypes.Structure):
     """include/laminarsmoke_amd.h: struct lsm_ode_parameters."""
     _fields_ = [

@@ -63,6 +63,9 @@ def lib():
         L.lsm_integrate_batch_rho.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_properties.argtypes = [vp, i64, vp, vp, vp, vp, vp]
         L.lsm_properties_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
+        L.lsm_set_transport.argtypes = [vp, vp]
+        L.lsm_transport_properties.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
+        L.lsm_transport_properties_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_integrate_batch_device_rho.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_cost_order.argtypes = [vp, i64, vp, vp, vp]
         L.lsm_autotune.argtypes = [vp, i64, vp, vp, i64, vp, vp]
@@ -226,6 +229,31 @@ class ChemistryStep:
 
     def properties_device(self, n, y_ptr, props_ptr, p_ptr=None, x_ptr=None, cpk_ptr=None, stream=None):
         _check(lib().lsm_properties_device(self.h, int(n), y_ptr, p_ptr, props_ptr, x_ptr, cpk_ptr, stream))
+
+    def set_transport(self, tr):
+        """Copy the transport fits (laminarsmoke_amd.transport.Transport) to the device."""
+        self._transport = tr
+        _check(lib().lsm_set_transport(self.h, ctypes.byref(tr.struct())))
+
+    def transport(self, y, p=None, species=True):
+        """Host arrays: y[ns+1, n] -> (mu[n], lambda[n], dmix[ns, n], teta[ns, n]) (the last two None
+        unless `species`)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        neq, n = y.shape
+        if neq != self.mech.neq:
+            raise ValueError(f"y has {neq} rows, mechanism needs {self.mech.neq}")
+        mu, lam = np.empty(n), np.empty(n)
+        dmix = np.empty((self.mech.ns, n)) if species else None
+        teta = np.empty((self.mech.ns, n)) if species else None
+        if p is not None:
+            p = np.ascontiguousarray(np.broadcast_to(np.asarray(p, dtype=np.float64), (n,)))
+        q = lambda a: a.ctypes.data if a is not None else None
+        _check(lib().lsm_transport_properties(self.h, n, q(y), q(p), q(mu), q(lam), q(dmix), q(teta)))
+        return mu, lam, dmix, teta
+
+    def transport_device(self, n, y_ptr, mu_ptr, lambda_ptr, p_ptr=None, dmix_ptr=None, teta_ptr=None, stream=None):
+        _check(lib().lsm_transport_properties_device(self.h, int(n), y_ptr, p_ptr, mu_ptr, lambda_ptr, dmix_ptr,
+                                                     teta_ptr, stream))
 
     def integrate_device(self, n, dt_ptr, y_in_ptr, y_out_ptr, status_ptr=None, q_ptr=None, counters_ptr=None,
                          stream=None, order_ptr=None, rho_ptr=None):

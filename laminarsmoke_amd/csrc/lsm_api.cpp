@@ -236,6 +236,9 @@ struct lsm_handle {
     void *d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int64_t sort_cap = 0;
+    // transport fits (lsm_set_transport)
+    DevTransport tr{};
+    void *d_transport = nullptr;
 };
 
 int lsm_fail(int code, const char *msg) { return fail(code, msg); }
@@ -933,6 +936,7 @@ int lsm_destroy(lsm_handle *h) {
     if (h->d_q) (void)hipFree(h->d_q);
     if (h->d_status) (void)hipFree(h->d_status);
     if (h->d_cnt) (void)hipFree(h->d_cnt);
+    if (h->d_transport) (void)hipFree(h->d_transport);
     delete h;
     return LSM_OK;
 }
@@ -1101,6 +1105,114 @@ int lsm_properties(lsm_handle *h, int64_t n, const double *y, const double *p, d
         rc = fail(LSM_E_HIP, "copy x");
     if (!rc && cpk && hipMemcpy(cpk, dc, sizeof(double) * ns * n, hipMemcpyDeviceToHost) != hipSuccess)
         rc = fail(LSM_E_HIP, "copy cp_species");
+    release();
+    return rc;
+}
+
+int lsm_set_transport(lsm_handle *h, const lsm_transport *t) {
+    if (!h || !t) return fail(LSM_E_INVALID, "null handle / transport");
+    const int ns = h->dm.ns;
+    if (t->ns != ns) return fail(LSM_E_INVALID, "transport ns differs from the mechanism's");
+    if (!t->mw || !t->lambda_fit || !t->eta_fit || (ns > 1 && !t->gamma_fit))
+        return fail(LSM_E_INVALID, "transport table pointer is null");
+    if (t->nteta < 0 || t->nteta > ns || (t->nteta > 0 && (!t->teta_sp || !t->teta_fit)))
+        return fail(LSM_E_INVALID, "thermal diffusion ratio tables");
+    if (t->viscosity_model != LSM_VISCOSITY_WILKE && t->viscosity_model != LSM_VISCOSITY_HERNING &&
+        t->viscosity_model != LSM_VISCOSITY_MATHUR_SAXENA)
+        return fail(LSM_E_INVALID, "viscosity model");
+    if ((size_t)3 * ns * sizeof(double) * 4 > 65536) return fail(LSM_E_UNSUPPORTED, "transport needs ns <= 682");
+    for (int k = 0; k < ns; k++)
+        if (!(t->mw[k] > 0.)) return fail(LSM_E_INVALID, "transport molecular weights must be positive");
+    std::vector<int> slot(ns, -1);
+    for (int q = 0; q < t->nteta; q++) {
+        const int k = t->teta_sp[q];
+        if (k < 0 || k >= ns || slot[k] >= 0) return fail(LSM_E_INVALID, "thermal diffusion species index");
+        slot[k] = q;
+    }
+    const size_t np = (size_t)ns * (ns - 1) / 2;
+    // Wilke constants (TransportPropertiesMap_CHEMKIN::CompleteInitialization, :655-670)
+    std::vector<double> sup(np), inf(np), r14(np);
+    {
+        const double *M = t->mw;
+        size_t i = 0;
+        for (int k = 0; k < ns; k++)
+            for (int j = k + 1; j < ns; j++, i++) {
+                sup[i] = 1. / std::sqrt(8. * (1. + M[k] / M[j]));
+                inf[i] = 1. / std::sqrt(8. * (1. + M[j] / M[k]));
+                r14[i] = std::sqrt(std::sqrt(M[j] / M[k]));
+            }
+    }
+    Blob b;
+    const size_t o_mw = b.add(std::vector<double>(t->mw, t->mw + ns));
+    const size_t o_l = b.add(std::vector<double>(t->lambda_fit, t->lambda_fit + 4 * ns));
+    const size_t o_e = b.add(std::vector<double>(t->eta_fit, t->eta_fit + 4 * ns));
+    const size_t o_g = b.add(np ? std::vector<double>(t->gamma_fit, t->gamma_fit + 4 * np) : std::vector<double>());
+    const size_t o_s = b.add(sup), o_i = b.add(inf), o_r = b.add(r14), o_sl = b.add(slot);
+    const size_t o_t = b.add(t->nteta ? std::vector<double>(t->teta_fit, t->teta_fit + (size_t)4 * t->nteta * ns)
+                                      : std::vector<double>());
+    void *d = nullptr;
+    if (hipMalloc(&d, b.bytes.size()) != hipSuccess) return fail(LSM_E_NOMEM, "transport tables");
+    if (hipMemcpy(d, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return fail(LSM_E_HIP, "copy transport tables");
+    }
+    if (h->d_transport) (void)hipFree(h->d_transport);
+    h->d_transport = d;
+    char *base = (char *)d;
+    DevTransport &T = h->tr;
+    T.ns = ns; T.nteta = t->nteta; T.model = t->viscosity_model;
+    T.mw = (const double *)(base + o_mw); T.lam = (const double *)(base + o_l); T.eta = (const double *)(base + o_e);
+    T.gam = (const double *)(base + o_g);
+    T.wsup = (const double *)(base + o_s); T.winf = (const double *)(base + o_i); T.wr14 = (const double *)(base + o_r);
+    T.teta_slot = (const int *)(base + o_sl); T.teta = (const double *)(base + o_t);
+    return LSM_OK;
+}
+
+int lsm_transport_properties_device(lsm_handle *h, int64_t n, const double *d_y, const double *d_p, double *d_mu,
+                                    double *d_lambda, double *d_dmix, double *d_teta, void *stream) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (!h->d_transport) return fail(LSM_E_INVALID, "no transport fits (lsm_set_transport)");
+    if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
+    if (n == 0) return LSM_OK;
+    if (!d_y || !d_mu || !d_lambda) return fail(LSM_E_INVALID, "null y / mu / lambda pointer");
+    if (launch_transport(h->dm, h->tr, h->kp.pressure, n, d_y, d_p, d_mu, d_lambda, d_dmix, d_teta, h->num_cu,
+                         stream))
+        return fail(LSM_E_HIP, std::string("transport kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
+    return LSM_OK;
+}
+
+int lsm_transport_properties(lsm_handle *h, int64_t n, const double *y, const double *p, double *mu, double *lambda,
+                             double *dmix, double *teta) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (!h->d_transport) return fail(LSM_E_INVALID, "no transport fits (lsm_set_transport)");
+    if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
+    if (n == 0) return LSM_OK;
+    if (!y || !mu || !lambda) return fail(LSM_E_INVALID, "null y / mu / lambda pointer");
+    const size_t ns = (size_t)h->dm.ns;
+    double *dy = nullptr, *dp = nullptr, *dmu = nullptr, *dl = nullptr, *dd = nullptr, *dt = nullptr;
+    auto release = [&]() {
+        for (double *q : {dy, dp, dmu, dl, dd, dt}) if (q) (void)hipFree(q);
+    };
+    int rc = LSM_OK;
+    if (hipMalloc(&dy, sizeof(double) * (ns + 1) * n) != hipSuccess || hipMalloc(&dmu, sizeof(double) * n) != hipSuccess ||
+        hipMalloc(&dl, sizeof(double) * n) != hipSuccess || (p && hipMalloc(&dp, sizeof(double) * n) != hipSuccess) ||
+        (dmix && hipMalloc(&dd, sizeof(double) * ns * n) != hipSuccess) ||
+        (teta && hipMalloc(&dt, sizeof(double) * ns * n) != hipSuccess)) {
+        release();
+        return fail(LSM_E_NOMEM, "transport buffers");
+    }
+    if (hipMemcpy(dy, y, sizeof(double) * (ns + 1) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        (p && hipMemcpy(dp, p, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(LSM_E_HIP, "copy inputs");
+    if (!rc) rc = lsm_transport_properties_device(h, n, dy, dp, dmu, dl, dd, dt, nullptr);
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = fail(LSM_E_HIP, "transport kernel failed");
+    if (!rc && (hipMemcpy(mu, dmu, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(lambda, dl, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = fail(LSM_E_HIP, "copy mu / lambda");
+    if (!rc && dmix && hipMemcpy(dmix, dd, sizeof(double) * ns * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(LSM_E_HIP, "copy dmix");
+    if (!rc && teta && hipMemcpy(teta, dt, sizeof(double) * ns * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(LSM_E_HIP, "copy teta");
     release();
     return rc;
 }

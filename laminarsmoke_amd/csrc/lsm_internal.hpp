@@ -96,6 +96,17 @@ struct DevMech {
     LdsImage img;
 };
 
+// Transport fits and mixture-rule constants (TransportPropertiesMap_CHEMKIN) on the device.
+struct DevTransport {
+    int ns, nteta, model;                    // model: LSM_VISCOSITY_*
+    const double *mw;                        // [ns] kg/kmol
+    const double *lam, *eta;                 // [ns][4] ln-T fits
+    const double *gam;                       // [ns(ns-1)/2][4] ln-T fits of ln D_jk at 1 bar, pairs j < k
+    const double *wsup, *winf, *wr14;        // [ns(ns-1)/2] Wilke pair constants (CompleteInitialization)
+    const int *teta_slot;                    // [ns] row of teta, or -1
+    const double *teta;                      // [nteta][ns][4] T fits of the thermal diffusion ratios
+};
+
 struct KParams {
     BdfConst bk;
     unsigned long long *phase_cycles;   // [12] per-phase cycle totals (builds with LSM_PHASE_TIMERS only)
@@ -131,6 +142,10 @@ size_t plain_work_doubles(const DevMech &m, int64_t slots);
 // the states yin; either output may be null.  Uses the team workspace jws.
 int launch_properties(const DevMech &m, double p_default, int64_t n, const double *y, const double *p, double *out,
                       double *x, double *cpk, int num_cu, void *stream);
+// Transport part of the property refresh: mu, lambda [n], dmix, teta [ns][n] (the last
+// two may be null).  One wave per cell, lane k on species k (k + 64 ...).
+int launch_transport(const DevMech &m, const DevTransport &t, double p_default, int64_t n, const double *y,
+                     const double *p, double *mu, double *lam, double *dmix, double *teta, int num_cu, void *stream);
 int launch_equations(const DevMech &m, const KParams &k, int64_t n, const double *yin, double *f, double *J,
                      int grid, int waves_per_block, double *jws, void *stream);
 int launch_reset_queue(unsigned *queue, void *stream);

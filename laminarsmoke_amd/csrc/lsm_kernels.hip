@@ -15,6 +15,7 @@
 #include <math.h>
 
 #include "../../include/lsm_mechanism.h"
+#include "../../include/lsm_transport.h"
 #include "lsm_internal.hpp"
 
 namespace lsm {
@@ -2266,6 +2267,140 @@ __global__ void __launch_bounds__(256) properties_kernel(DevMech M, double p_def
         out[4 * n + c] = (cpx * RGAS) / MW;
         out[5 * n + c] = (cpx * RGAS - RGAS) / MW;
     }
+}
+
+// Transport part of the property refresh (solvers/laminarSMOKE/properties.H:101-147):
+// DynamicViscosity, ThermalConductivity, MassDiffusionCoefficients and
+// ThermalDiffusionRatios of TransportPropertiesMap_CHEMKIN (maps/TransportPropertiesMap_CHEMKIN.hpp:
+// 690-1117) per cell.  One wave per cell; lane k handles species k, k + 64, ...; the
+// wave's mole fractions and sqrt(eta) sit in LDS.  Each species sums over its partners
+// in increasing index order, the order in which the reference's pair loops add to it;
+// the sums over species are wave reductions.
+__device__ __forceinline__ int pair_index(int a, int b, int ns) {   // a < b
+    return a * ns - a * (a + 1) / 2 + (b - a - 1);
+}
+
+__global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport Tr, double p_default, int64_t n,
+                                                        const double *__restrict__ y, const double *__restrict__ p,
+                                                        double *__restrict__ mu, double *__restrict__ lam,
+                                                        double *__restrict__ dmix, double *__restrict__ teta) {
+    extern __shared__ double tsh[];
+    const int ns = M.ns, P = M.ns_pad, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double *sx = tsh + (size_t)wv * 3 * ns, *sq = sx + ns, *s3 = sq + ns;   // x, sqrt(eta), eta / corrected x
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv; c < n; c += nw) {
+        const double T = y[(int64_t)ns * n + c];
+        const double pc = p ? p[c] : p_default;
+        const double logT = log(T), logT2 = logT * logT, logT3 = logT * logT2;
+        // mole fractions (ThermodynamicsMap_CHEMKIN::MoleFractions_From_MassFractions)
+        double u = 0.;
+        for (int k = lane; k < ns; k += 64) u += y[(int64_t)k * n + c] / M.sp[SP_MW * P + k];
+        const double MW = 1. / wave_sum(u);
+        double l1 = 0., l2 = 0.;
+        for (int k = lane; k < ns; k += 64) {
+            const double xk = (MW * y[(int64_t)k * n + c]) / M.sp[SP_MW * P + k];
+            const double *a = Tr.lam + 4 * k, *e = Tr.eta + 4 * k;
+            double sl = a[0];
+            sl += a[1] * logT; sl += a[2] * logT2; sl += a[3] * logT3;
+            double se = e[0];
+            se += e[1] * logT; se += e[2] * logT2; se += e[3] * logT3;
+            const double lk = exp(sl), ek = exp(se);
+            sx[k] = xk;
+            sq[k] = sqrt(ek);
+            s3[k] = ek;
+            l1 += xk * lk;
+            l2 += xk / lk;
+        }
+        wave_sync();
+        const double L1 = wave_sum(l1), L2 = wave_sum(l2);
+        if (lane == 0) lam[c] = 0.50 * (L1 + 1. / L2);                 // lambdaMix (:947-957)
+        // etaMix (:959-1030)
+        double m1 = 0., m2 = 0.;
+        for (int k = lane; k < ns; k += 64) {
+            const double xk = sx[k], ek = s3[k];
+            if (Tr.model == LSM_VISCOSITY_WILKE) {
+                const double sqk = sq[k], usqk = 1. / sqk;
+                double sumK = xk;
+                for (int q = 0; q < ns; q++) {
+                    if (q == k) continue;
+                    if (q < k) {
+                        const int i = pair_index(q, k, ns);
+                        const double d = sq[q] * usqk * Tr.wr14[i];
+                        sumK += sx[q] * Tr.winf[i] * (1. + 1. / d) * (1. + 1. / d);
+                    } else {
+                        const int i = pair_index(k, q, ns);
+                        const double d = sqk * (1. / sq[q]) * Tr.wr14[i];
+                        sumK += sx[q] * Tr.wsup[i] * (1. + d) * (1. + d);
+                    }
+                }
+                m1 += xk * ek / sumK;
+            } else if (Tr.model == LSM_VISCOSITY_HERNING) {
+                const double sm = sqrt(Tr.mw[k]);
+                m1 += xk * ek * sm;
+                m2 += xk * sm;
+            } else {
+                m1 += xk * ek;
+                m2 += xk / ek;
+            }
+        }
+        const double M1 = wave_sum(m1), M2 = wave_sum(m2);
+        if (lane == 0)
+            mu[c] = Tr.model == LSM_VISCOSITY_WILKE ? M1 : (Tr.model == LSM_VISCOSITY_HERNING ? M1 / M2
+                                                                                             : 0.50 * (M1 + 1. / M2));
+        // tetaMix (:768-796, 1103-1117): theta_kj x_k x_j for the light species
+        if (teta)
+            for (int k = lane; k < ns; k += 64) {
+                const int sl = Tr.teta_slot[k];
+                double tm = 0.;
+                if (sl >= 0) {
+                    const double T2 = T * T, T3 = T * T2, xk = sx[k];
+                    const double *f = Tr.teta + (size_t)4 * sl * ns;
+                    for (int j = 0; j < ns; j++) {
+                        double sum = f[4 * j];
+                        sum += f[4 * j + 1] * T; sum += f[4 * j + 2] * T2; sum += f[4 * j + 3] * T3;
+                        tm += sum * xk * sx[j];
+                    }
+                }
+                teta[(int64_t)k * n + c] = tm;
+            }
+        // gamma() and gammaMix (:798-859, 1032-1059)
+        if (dmix) {
+            wave_sync();
+            const double thr = 1.e-14, sum_thr = 1. + thr * (double)ns, P_bar = pc / 100000.;
+            double mc = 0.;
+            for (int k = lane; k < ns; k += 64) {
+                const double xc = (sx[k] + thr) / sum_thr;
+                s3[k] = xc;
+                mc += xc * Tr.mw[k];
+            }
+            wave_sync();
+            const double MWc = wave_sum(mc);
+            for (int k = lane; k < ns; k += 64) {
+                double sdc = 0.;
+                for (int q = 0; q < ns; q++) {
+                    if (q == k) continue;
+                    const double *f = Tr.gam + 4 * pair_index(q < k ? q : k, q < k ? k : q, ns);
+                    double sum = f[0];
+                    sum += f[1] * logT; sum += f[2] * logT2; sum += f[3] * logT3;
+                    sdc += s3[q] * (P_bar / exp(sum));      // x_q / D_kq
+                }
+                dmix[(int64_t)k * n + c] = (MWc - s3[k] * Tr.mw[k]) / (MWc * sdc);
+            }
+        }
+        wave_sync();
+    }
+}
+
+int launch_transport(const DevMech &m, const DevTransport &t, double p_default, int64_t n, const double *y,
+                     const double *p, double *mu, double *lam, double *dmix, double *teta, int num_cu, void *stream) {
+    if (n <= 0) return 0;
+    const size_t lds = (size_t)4 * 3 * m.ns * sizeof(double);
+    if (lds > 65536) return -2;
+    const int64_t blocks = (n + 3) / 4;
+    const int g = (int)(blocks < (int64_t)num_cu * 16 ? blocks : (int64_t)num_cu * 16);
+    hipLaunchKernelGGL(transport_kernel, dim3(g), dim3(256), lds, (hipStream_t)stream, m, t, p_default, n, y, p, mu,
+                       lam, dmix, teta);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_properties(const DevMech &m, double p_default, int64_t n, const double *y, const double *p, double *out,

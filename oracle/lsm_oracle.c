@@ -1374,3 +1374,98 @@ void lsm_oracle_chemistry_step(const lsm_mechanism *m, const lsm_oracle_options 
     }
     pthread_mutex_destroy(&P.lock);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Transport: TransportPropertiesMap_CHEMKIN.hpp                              */
+/* ------------------------------------------------------------------------- */
+void lsm_oracle_transport(const lsm_transport *tr, double T, double P, const double *x, double *mu,
+                          double *lambda, double *dmix, double *teta) {
+    const int ns = tr->ns;
+    const double logT = log(T), logT2 = logT * logT, logT3 = logT * logT2;
+    double *lam = malloc(sizeof(double) * ns), *eta = malloc(sizeof(double) * ns);
+    double *sumK = malloc(sizeof(double) * ns), *xc = malloc(sizeof(double) * ns);
+    double *sdc = malloc(sizeof(double) * ns);
+    /* lambda(), eta(): :690-766 */
+    for (int j = 0; j < ns; j++) {
+        const double *k = tr->lambda_fit + 4 * j, *e = tr->eta_fit + 4 * j;
+        double s1 = k[0];
+        s1 += k[1] * logT; s1 += k[2] * logT2; s1 += k[3] * logT3;
+        lam[j] = exp(s1);
+        double s2 = e[0];
+        s2 += e[1] * logT; s2 += e[2] * logT2; s2 += e[3] * logT3;
+        eta[j] = exp(s2);
+    }
+    /* lambdaMix, :947-957 (Mathur, Todor, Saxena) */
+    if (lambda) {
+        double sum1 = 0., sum2 = 0.;
+        for (int i = 0; i < ns; i++) { sum1 += x[i] * lam[i]; sum2 += x[i] / lam[i]; }
+        *lambda = 0.50 * (sum1 + 1. / sum2);
+    }
+    /* etaMix, :959-1030 */
+    if (mu) {
+        const double *M = tr->mw;
+        if (tr->viscosity_model == LSM_VISCOSITY_WILKE) {
+            for (int k = 0; k < ns; k++) sumK[k] = x[k];
+            double *sq = malloc(sizeof(double) * ns), *usq = malloc(sizeof(double) * ns);
+            for (int k = 0; k < ns; k++) { sq[k] = sqrt(eta[k]); usq[k] = 1. / sq[k]; }
+            for (int k = 0; k < ns; k++)
+                for (int j = k + 1; j < ns; j++) {
+                    /* CompleteInitialization, :655-670 */
+                    const double sup = 1. / sqrt(8. * (1. + M[k] / M[j]));
+                    const double inf = 1. / sqrt(8. * (1. + M[j] / M[k]));
+                    const double r14 = sqrt(sqrt(M[j] / M[k]));
+                    const double d = sq[k] * usq[j] * r14;
+                    sumK[k] += x[j] * sup * (1. + d) * (1. + d);
+                    sumK[j] += x[k] * inf * (1. + 1. / d) * (1. + 1. / d);
+                }
+            double m = 0.;
+            for (int k = 0; k < ns; k++) m += x[k] * eta[k] / sumK[k];
+            *mu = m;
+            free(sq); free(usq);
+        } else if (tr->viscosity_model == LSM_VISCOSITY_HERNING) {
+            double sum = 0., m = 0.;
+            for (int k = 0; k < ns; k++) sum += x[k] * sqrt(M[k]);
+            for (int k = 0; k < ns; k++) m += x[k] * eta[k] * sqrt(M[k]);
+            *mu = m / sum;
+        } else {
+            double sum1 = 0., sum2 = 0.;
+            for (int i = 0; i < ns; i++) { sum1 += x[i] * eta[i]; sum2 += x[i] / eta[i]; }
+            *mu = 0.50 * (sum1 + 1. / sum2);
+        }
+    }
+    /* gamma() and gammaMix, :798-859, 1032-1059 */
+    if (dmix) {
+        const double threshold = 1.e-14, sum_threshold = 1. + threshold * (double)ns;
+        const double P_bar = P / 100000.;
+        for (int k = 0; k < ns; k++) sdc[k] = 0.;
+        for (int i = 0; i < ns; i++) xc[i] = (x[i] + threshold) / sum_threshold;
+        double MWmix = 0.;
+        for (int i = 0; i < ns; i++) MWmix += xc[i] * tr->mw[i];
+        const double *d = tr->gamma_fit;
+        for (int k = 0; k < ns; k++)
+            for (int j = k + 1; j < ns; j++) {
+                double sum = d[0];
+                sum += d[1] * logT; sum += d[2] * logT2; sum += d[3] * logT3;
+                const double D = P_bar / exp(sum);
+                sdc[j] += xc[k] * D;
+                sdc[k] += xc[j] * D;
+                d += 4;
+            }
+        for (int k = 0; k < ns; k++) dmix[k] = (MWmix - xc[k] * tr->mw[k]) / (MWmix * sdc[k]);
+    }
+    /* teta() and tetaMix, :768-796, 1103-1117 */
+    if (teta) {
+        const double T2 = T * T, T3 = T * T2;
+        for (int j = 0; j < ns; j++) teta[j] = 0.;
+        for (int q = 0; q < tr->nteta; q++) {
+            const int i = tr->teta_sp[q];
+            const double *f = tr->teta_fit + 4 * (size_t)q * ns;
+            for (int j = 0; j < ns; j++) {
+                double sum = f[4 * j];
+                sum += f[4 * j + 1] * T; sum += f[4 * j + 2] * T2; sum += f[4 * j + 3] * T3;
+                teta[i] += sum * x[i] * x[j];
+            }
+        }
+    }
+    free(lam); free(eta); free(sumK); free(xc); free(sdc);
+}

@@ -22,6 +22,7 @@
 
 #include <stdint.h>
 #include "../include/lsm_mechanism.h"
+#include "../include/lsm_transport.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -73,6 +74,14 @@ void lsm_oracle_kinetics(const lsm_mechanism *m, double T, const double *c,
  * holds k_rev/k. */
 void lsm_oracle_kinetics_p(const lsm_mechanism *m, double T, double P, const double *c,
                            double *kf, double *uKeq, double *kmod, double *r, double *R);
+
+/* Transport properties of one cell at (T, P) and mole fractions x[ns]: the map's
+ * species fits and mixture rules as properties.H:101-147 calls them
+ * (TransportPropertiesMap_CHEMKIN.hpp:690-1117).  mu [kg/m/s], lambda [W/m/K],
+ * dmix[ns] mixture-averaged diffusion coefficients [m2/s], teta[ns] thermal
+ * diffusion ratios (0 for species without them); any output may be NULL. */
+void lsm_oracle_transport(const lsm_transport *tr, double T, double P, const double *x, double *mu,
+                          double *lambda, double *dmix, double *teta);
 
 /* Constant-pressure RHS (BatchReactorHomogeneousConstantPressure::Equations);
  * y = [Y_0..Y_{ns-1}, T], returns QR [J/m3/s]. */

@@ -93,6 +93,17 @@ def kinetics(mech, T, c, P=101325.0):
     return dict(kf=kf, uKeq=uK, kmod=km, r=r, R=R)
 
 
+def transport(tr, T, P, x):
+    """mu, lambda, dmix[ns], teta[ns] of one cell (x: mole fractions)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    ns = tr.ns
+    mu, lam = ctypes.c_double(), ctypes.c_double()
+    dmix, teta = np.zeros(ns), np.zeros(ns)
+    lib().lsm_oracle_transport(ctypes.byref(tr.struct()), ctypes.c_double(T), ctypes.c_double(P), _p(x),
+                               ctypes.byref(mu), ctypes.byref(lam), _p(dmix), _p(teta))
+    return mu.value, lam.value, dmix, teta
+
+
 def rhs(mech, y, P=101325.0, energy=True, rho0=None):
     """Constant-pressure RHS, or the constant-volume RHS at density rho0 when given."""
     y = np.ascontiguousarray(y, dtype=np.float64)

@@ -1,0 +1,125 @@
+"""GPU: the transport part of the per-cell property refresh (lsm_transport_properties)
+against the oracle on the 14-, 53- and 160-species workload fields and the
+reference's CH4 skeletal mechanism, with every viscosity model; the pure-species
+limit; and the bandwidth of the kernel on a 262,144-cell field.
+
+Tolerances: mu and lambda to 1e-12 relative, the diffusion coefficients and thermal
+diffusion ratios to 1e-12 relative (against the largest of the cell): the kernel sums
+over species in a different order from the oracle (wave reductions), so results agree
+to rounding, not bit for bit.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from laminarsmoke_amd import api
+from laminarsmoke_amd.chemkin import load_chemkin
+from laminarsmoke_amd.fields import workload_cells
+from laminarsmoke_amd.mechanism import workload_mechanism
+from laminarsmoke_amd.transport import (VISCOSITY_HERNING, VISCOSITY_MATHUR_SAXENA, VISCOSITY_WILKE, Transport,
+                                        synthetic_transport)
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _oracle(oracle, mech, tr, y, p):
+    n = y.shape[1]
+    ns = mech.ns
+    mu, lam = np.empty(n), np.empty(n)
+    dmix, teta = np.empty((ns, n)), np.empty((ns, n))
+    mw = mech.mw
+    for c in range(n):
+        u = np.sum(y[:ns, c] / mw)
+        x = ((1.0 / u) * y[:ns, c]) / mw    # x_k = MW Y_k / MW_k, MW = 1 / sum(Y/MW)
+        mu[c], lam[c], dmix[:, c], teta[:, c] = oracle.transport(tr, y[ns, c], p[c], x)
+    return mu, lam, dmix, teta
+
+
+def _check(mech, tr, y, p, oracle, what):
+    step = api.ChemistryStep(mech)
+    try:
+        step.set_transport(tr)
+        mu, lam, dmix, teta = step.transport(y, p)
+    finally:
+        step.close()
+    mo, lo, do, to = _oracle(oracle, mech, tr, y, p)
+    np.testing.assert_allclose(mu, mo, rtol=1e-12, err_msg=f"{what}: mu")
+    np.testing.assert_allclose(lam, lo, rtol=1e-12, err_msg=f"{what}: lambda")
+    np.testing.assert_allclose(dmix, do, rtol=1e-12, err_msg=f"{what}: dmix")
+    scale = np.abs(to).max(axis=0, keepdims=True) + 1e-300
+    assert np.all(np.abs(teta - to) <= 1e-12 * scale), f"{what}: teta"
+    print(f"{what}: mu {mu.min():.3g}-{mu.max():.3g} kg/m/s, lambda {lam.min():.3g}-{lam.max():.3g} W/m/K, "
+          f"max rel dev mu {np.max(np.abs(mu / mo - 1)):.2g}, dmix {np.max(np.abs(dmix / do - 1)):.2g}", flush=True)
+
+
+@pytest.mark.parametrize("cfg", [1, 2, 4])
+@pytest.mark.parametrize("model", [VISCOSITY_WILKE, VISCOSITY_HERNING, VISCOSITY_MATHUR_SAXENA])
+def test_workload_fields(hip_device, oracle, cfg, model):
+    mech, y = workload_cells(cfg, n=96)
+    tr = synthetic_transport(mech, seed=cfg, model=model)
+    p = np.random.default_rng(cfg).uniform(2e4, 2e6, y.shape[1])
+    print(f"transport cfg{cfg} model {model}: 96 cells, {mech.ns} species", flush=True)
+    _check(mech, tr, y, p, oracle, f"cfg{cfg}/model{model}")
+
+
+def test_ch4_skeletal(hip_device, oracle):
+    mech = load_chemkin(os.path.join(GOLDEN, "POLIMI_CH4_SKELETAL_1412.CKI"),
+                        os.path.join(GOLDEN, "POLIMI_CH4_SKELETAL_1412_thermo.CKT"))
+    from laminarsmoke_amd.fields import premixed_cells
+    y = premixed_cells(mech, 64, seed=4)
+    tr = synthetic_transport(mech, seed=11)
+    print("transport CH4 skeletal: 64 cells", flush=True)
+    _check(mech, tr, y, np.full(64, 101325.0), oracle, "ch4")
+
+
+def test_pure_species(hip_device):
+    """A cell of one species returns that species' fitted mu and lambda."""
+    mech = workload_mechanism(14)
+    tr = synthetic_transport(mech, seed=2)
+    n = mech.ns
+    y = np.zeros((mech.ns + 1, n))
+    y[np.arange(n), np.arange(n)] = 1.0
+    y[mech.ns] = 1234.0
+    step = api.ChemistryStep(mech)
+    try:
+        step.set_transport(tr)
+        mu, lam, _, _ = step.transport(y, species=False)
+    finally:
+        step.close()
+    lt = np.log(1234.0)
+    v = np.array([1.0, lt, lt * lt, lt ** 3])
+    np.testing.assert_allclose(mu, np.exp(tr.eta_fit @ v), rtol=1e-13)
+    np.testing.assert_allclose(lam, np.exp(tr.lambda_fit @ v), rtol=1e-13)
+
+
+def test_bandwidth(hip_device):
+    """262,144 cells of the 53-species field, device pointers: time per field refresh."""
+    import torch
+    mech, y = workload_cells(2, n=262144)
+    tr = synthetic_transport(mech, seed=2)
+    dev = torch.device("cuda:0")
+    n = y.shape[1]
+    yd = torch.from_numpy(y).to(dev)
+    mu, lam = torch.empty(n, dtype=torch.float64, device=dev), torch.empty(n, dtype=torch.float64, device=dev)
+    dmix = torch.empty((mech.ns, n), dtype=torch.float64, device=dev)
+    step = api.ChemistryStep(mech)
+    try:
+        step.set_transport(tr)
+        for _ in range(2):
+            step.transport_device(n, yd.data_ptr(), mu.data_ptr(), lam.data_ptr(), dmix_ptr=dmix.data_ptr())
+        torch.cuda.synchronize()
+        t = time.time()
+        reps = 5
+        for _ in range(reps):
+            step.transport_device(n, yd.data_ptr(), mu.data_ptr(), lam.data_ptr(), dmix_ptr=dmix.data_ptr())
+        torch.cuda.synchronize()
+        dt = (time.time() - t) / reps
+    finally:
+        step.close()
+    assert torch.isfinite(mu).all() and torch.isfinite(dmix).all()
+    nbytes = 8.0 * n * ((mech.ns + 1) + 2 + mech.ns)
+    print(f"transport 262144 cells, {mech.ns} species: {dt * 1e3:.3f} ms per field, {n / dt:.3g} cells/s, "
+          f"{nbytes / dt / 1e9:.0f} GB/s of state in + mu, lambda, dmix out", flush=True)
