@@ -57,20 +57,44 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline(mech, y, n_sample, threads, dt, rtol, atol, full_pivoting=False):
+def cpu_baseline(mech, y, n_sample, threads, dt, rtol, atol, full_pivoting=False, y_gpu=None, st_gpu=None):
+    """Oracle timing on a seeded sample of the field; with the GPU's outputs of the same
+    cells, also their deviation from the oracle in units of rtol |y| + atol."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     pyoracle.build()
     idx = np.sort(np.random.default_rng(2024).choice(y.shape[1], n_sample, replace=False))
     ys = np.ascontiguousarray(y[:, idx])
     t0 = time.perf_counter()
-    _, st, _, cnt = pyoracle.chemistry_step(mech, ys, dt, rtol=rtol, atol=atol, threads=threads,
-                                            full_pivoting=full_pivoting)
+    yo, st, _, cnt = pyoracle.chemistry_step(mech, ys, dt, rtol=rtol, atol=atol, threads=threads,
+                                             full_pivoting=full_pivoting)
     el = time.perf_counter() - t0
-    return dict(value=n_sample / el, unit="cell-integrations/s", cores=threads, kind="port",
-                sample=f"{n_sample} cells drawn at random (seed 2024) from the same field, "
-                       f"{el:.1f} s, oracle C restatement, {threads} thread(s), host {platform.processor() or platform.machine()} "
-                       f"with {os.cpu_count()} logical CPUs")
+    out = dict(value=n_sample / el, unit="cell-integrations/s", cores=threads, kind="port",
+               sample=f"{n_sample} cells drawn at random (seed 2024) from the same field, "
+                      f"{el:.1f} s, oracle C restatement, {threads} thread(s), host {platform.processor() or platform.machine()} "
+                      f"with {os.cpu_count()} logical CPUs")
+    err = None
+    if y_gpu is not None:
+        dev = lambda a: (np.abs(a - yo) / (rtol * np.abs(yo) + atol)).max(axis=0)
+        r = dev(y_gpu[:, idx])
+        # the oracle's own spread under 1e-15 relative input noise (3 seeded draws)
+        rng = np.random.default_rng(12345)
+        env = np.zeros(n_sample)
+        for _ in range(3):
+            yp = ys * (1.0 + 1e-15 * rng.standard_normal(ys.shape))
+            y2, *_ = pyoracle.chemistry_step(mech, yp, dt, rtol=rtol, atol=atol, threads=threads,
+                                             full_pivoting=full_pivoting)
+            env = np.maximum(env, dev(y2))
+        bound = np.maximum(10.0, 3.0 * env)
+        err = {"cells": int(n_sample), "max": float(r.max()), "p50": float(np.median(r)),
+               "within_10x": float(np.mean(r <= 10.0)),
+               "oracle_self_envelope_max": float(env.max()),
+               "within_max_10_3env": float(np.mean(r <= bound)),
+               "max_over_bound": float((r / bound).max()),
+               "statuses_equal": bool(np.array_equal(st_gpu[idx], st)),
+               "unit": "max_k |y_gpu - y_oracle| / (rtol |y_oracle| + atol) per cell; envelope: the oracle "
+                       "against itself under 1e-15 input noise"}
+    return out, err
 
 
 def measured_traffic(workload_name, method, jacobian):
@@ -243,7 +267,12 @@ def main():
     }
     if not args.no_cpu:
         ns = args.cpu_sample if args.cpu_sample is not None else {4: 8, "ch4": 4096}.get(args.workload, 96)
-        out["cpu_baseline"] = cpu_baseline(mech, y, min(ns, n), args.cpu_threads, dt, rtol, atol, full_pivoting)
+        y_gpu = y_out.cpu().numpy()
+        out["cpu_baseline"], err = cpu_baseline(mech, y, min(ns, n), args.cpu_threads, dt, rtol, atol, full_pivoting,
+                                                y_gpu=y_gpu, st_gpu=status)
+        # the same cells through the oracle: per-cell deviation of the GPU result (the analytic
+        # Jacobian follows its own Newton path; the parity tests judge it against a tight solution)
+        out["extra"]["error_vs_oracle"] = err
     step.close()
     print(json.dumps(out), flush=True)
 
