@@ -273,6 +273,9 @@ def main():
         # the same cells through the oracle: per-cell deviation of the GPU result (the analytic
         # Jacobian follows its own Newton path; the parity tests judge it against a tight solution)
         out["extra"]["error_vs_oracle"] = err
+        # the oracle on all of the box's CPU share (16 threads, dynamic chunks), on a 16x larger sample
+        par, _ = cpu_baseline(mech, y, min(16 * ns, n), 16, dt, rtol, atol, full_pivoting)
+        out["extra"]["cpu_baseline_16_threads"] = par
     step.close()
     print(json.dumps(out), flush=True)
 
