@@ -123,3 +123,33 @@ def test_bandwidth(hip_device):
     nbytes = 8.0 * n * ((mech.ns + 1) + 2 + mech.ns)
     print(f"transport 262144 cells, {mech.ns} species: {dt * 1e3:.3f} ms per field, {n / dt:.3g} cells/s, "
           f"{nbytes / dt / 1e9:.0f} GB/s of state in + mu, lambda, dmix out", flush=True)
+
+
+def test_edge_cases(hip_device):
+    """Empty batch; p = None is the handle's pressure; errors before lsm_set_transport and
+    for fits of another species count; replacing the fits takes effect."""
+    mech, y = workload_cells(1, n=8)
+    tr = synthetic_transport(mech, seed=5)
+    step = api.ChemistryStep(mech, pressure=2.5e5)
+    try:
+        with pytest.raises(RuntimeError, match="lsm_set_transport"):
+            step.transport(y)
+        other = synthetic_transport(workload_mechanism(53), seed=1)
+        with pytest.raises(RuntimeError, match="ns differs"):
+            step.set_transport(other)
+        step.set_transport(tr)
+        mu0, lam0, d0, t0 = step.transport(y[:, :0])
+        assert mu0.size == 0 and d0.shape == (mech.ns, 0)
+        mu1, lam1, d1, _ = step.transport(y)
+        mu2, lam2, d2, _ = step.transport(y, np.full(8, 2.5e5))
+        assert np.array_equal(mu1, mu2) and np.array_equal(d1, d2)
+        mu3, _, d3, _ = step.transport(y, np.full(8, 5e5))
+        np.testing.assert_allclose(d3 * 2.0, d1, rtol=1e-13)           # D ~ 1/P
+        assert np.array_equal(mu3, mu1)                                 # mu does not depend on P
+        tr2 = Transport(tr.mw, tr.lambda_fit, tr.eta_fit + np.array([np.log(2.0), 0, 0, 0]), tr.gamma_fit,
+                        tr.teta_sp, tr.teta_fit)
+        step.set_transport(tr2)
+        mu4, _, _, _ = step.transport(y)
+        np.testing.assert_allclose(mu4, 2.0 * mu1, rtol=1e-13)        # every eta doubled
+    finally:
+        step.close()
