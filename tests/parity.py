@@ -1,6 +1,6 @@
 """Shared parity criteria for the chemistry step (GPU vs the CPU oracle).
 
-final T and Y_k within 10*(rtol*|y| + atol) of the oracle.
+Target bound: final T and Y_k within 10*(rtol*|y| + atol) of the oracle.
 
 The reference integrator is not well conditioned on every cell: on the flame cells
 of these mechanisms a 1e-15 relative perturbation of its own input moves its
@@ -12,7 +12,7 @@ envelope, so the per-cell bound used here is
 
 where envelope(cell) is the largest deviation of the oracle from itself over
 three runs on inputs perturbed by 1e-15 (relative).  On cells where the oracle
-is stable (envelope < 10/3) this is exactly the north-star 10x bound.
+is stable (envelope < 10/3) this is exactly the 10x target bound.
 """
 import numpy as np
 
