@@ -2275,7 +2275,9 @@ __global__ void __launch_bounds__(256) properties_kernel(DevMech M, double p_def
 // 690-1117) per cell.  One wave per cell; lane k handles species k, k + 64, ...; the
 // wave's mole fractions and sqrt(eta) sit in LDS.  Each species sums over its partners
 // in increasing index order, the order in which the reference's pair loops add to it;
-// the sums over species are wave reductions.
+// the sums over species are wave reductions.  With share_pairs, each binary 1/D_jk is
+// evaluated once (pairs dealt round-robin to the lanes) into a per-wave LDS table that
+// both partners read; otherwise each partner evaluates it (the same value either way).
 __device__ __forceinline__ int pair_index(int a, int b, int ns) {   // a < b
     return a * ns - a * (a + 1) / 2 + (b - a - 1);
 }
@@ -2283,10 +2285,14 @@ __device__ __forceinline__ int pair_index(int a, int b, int ns) {   // a < b
 __global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport Tr, double p_default, int64_t n,
                                                         const double *__restrict__ y, const double *__restrict__ p,
                                                         double *__restrict__ mu, double *__restrict__ lam,
-                                                        double *__restrict__ dmix, double *__restrict__ teta) {
+                                                        double *__restrict__ dmix, double *__restrict__ teta,
+                                                        int share_pairs) {
     extern __shared__ double tsh[];
     const int ns = M.ns, P = M.ns_pad, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    double *sx = tsh + (size_t)wv * 3 * ns, *sq = sx + ns, *s3 = sq + ns;   // x, sqrt(eta), eta / corrected x
+    const int np = ns * (ns - 1) / 2;
+    const size_t per_wave = (size_t)3 * ns + (share_pairs ? (size_t)np : 0);
+    double *sx = tsh + (size_t)wv * per_wave, *sq = sx + ns, *s3 = sq + ns;   // x, sqrt(eta), eta / corrected x
+    double *sp = s3 + ns;                                                     // 1/D of each pair (share_pairs)
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv; c < n; c += nw) {
         const double T = y[(int64_t)ns * n + c];
@@ -2373,16 +2379,30 @@ __global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport 
                 s3[k] = xc;
                 mc += xc * Tr.mw[k];
             }
+            if (share_pairs)
+                for (int q = lane; q < np; q += 64) {
+                    const double *f = Tr.gam + 4 * q;
+                    double sum = f[0];
+                    sum += f[1] * logT; sum += f[2] * logT2; sum += f[3] * logT3;
+                    sp[q] = P_bar / exp(sum);
+                }
             wave_sync();
             const double MWc = wave_sum(mc);
             for (int k = lane; k < ns; k += 64) {
                 double sdc = 0.;
                 for (int q = 0; q < ns; q++) {
                     if (q == k) continue;
-                    const double *f = Tr.gam + 4 * pair_index(q < k ? q : k, q < k ? k : q, ns);
-                    double sum = f[0];
-                    sum += f[1] * logT; sum += f[2] * logT2; sum += f[3] * logT3;
-                    sdc += s3[q] * (P_bar / exp(sum));      // x_q / D_kq
+                    const int i = pair_index(q < k ? q : k, q < k ? k : q, ns);
+                    double dinv;
+                    if (share_pairs) {
+                        dinv = sp[i];
+                    } else {
+                        const double *f = Tr.gam + 4 * i;
+                        double sum = f[0];
+                        sum += f[1] * logT; sum += f[2] * logT2; sum += f[3] * logT3;
+                        dinv = P_bar / exp(sum);
+                    }
+                    sdc += s3[q] * dinv;      // x_q / D_kq
                 }
                 dmix[(int64_t)k * n + c] = (MWc - s3[k] * Tr.mw[k]) / (MWc * sdc);
             }
@@ -2394,12 +2414,14 @@ __global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport 
 int launch_transport(const DevMech &m, const DevTransport &t, double p_default, int64_t n, const double *y,
                      const double *p, double *mu, double *lam, double *dmix, double *teta, int num_cu, void *stream) {
     if (n <= 0) return 0;
-    const size_t lds = (size_t)4 * 3 * m.ns * sizeof(double);
+    const size_t base = (size_t)3 * m.ns, np = (size_t)m.ns * (m.ns - 1) / 2;
+    const int share = (base + np) * sizeof(double) * 4 <= 65536 ? 1 : 0;
+    const size_t lds = (base + (share ? np : 0)) * sizeof(double) * 4;
     if (lds > 65536) return -2;
     const int64_t blocks = (n + 3) / 4;
     const int g = (int)(blocks < (int64_t)num_cu * 16 ? blocks : (int64_t)num_cu * 16);
     hipLaunchKernelGGL(transport_kernel, dim3(g), dim3(256), lds, (hipStream_t)stream, m, t, p_default, n, y, p, mu,
-                       lam, dmix, teta);
+                       lam, dmix, teta, share);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
