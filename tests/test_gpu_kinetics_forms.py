@@ -187,7 +187,7 @@ def test_step(hip_device, oracle, mech, P, method, jac):
     print(f"P = {P / ATM:g} atm {method}/{jac}: max ratio {cell.max():.3g}; {ignited} of 48 ignite", flush=True)
 
 
-@pytest.mark.parametrize("method", ["team", "plain"])
+@pytest.mark.parametrize("method", ["team", "wide", "plain"])
 def test_constant_volume(hip_device, oracle, mech, method):
     """Constant volume: the PLOG constants follow the pressure of the state."""
     y = premixed_cells(mech, 32, seed=9)
@@ -207,3 +207,95 @@ def test_constant_volume(hip_device, oracle, mech, method):
     assert np.array_equal(st_g, st_o)
     cell, _ = assert_parity(yg, yo, RTOL, ATOL, env, what=f"cv/{method}")
     print(f"constant volume {method}: max ratio {cell.max():.3g}", flush=True)
+
+
+def _device_batch(y, dt):
+    import torch
+    dev = torch.device("cuda:0")
+    n = y.shape[1]
+    b = dict(n=n, y=torch.from_numpy(y).to(dev), dt=torch.full((n,), dt, dtype=torch.float64, device=dev))
+    b["out"] = torch.empty_like(b["y"])
+    b["st"] = torch.zeros(n, dtype=torch.int32, device=dev)
+    b["q"] = torch.zeros(n, dtype=torch.float64, device=dev)
+    b["cnt"] = torch.zeros((n, 12), dtype=torch.int32, device=dev)
+    return b
+
+
+def _run(step, b, stream, order=None):
+    step.integrate_device(b["n"], b["dt"].data_ptr(), b["y"].data_ptr(), b["out"].data_ptr(), b["st"].data_ptr(),
+                          b["q"].data_ptr(), b["cnt"].data_ptr(), stream.cuda_stream,
+                          order.data_ptr() if order is not None else None)
+
+
+def _result(b):
+    import torch
+    torch.cuda.synchronize()
+    return b["out"].cpu().numpy().copy(), b["st"].cpu().numpy().copy(), b["cnt"].cpu().numpy().copy()
+
+
+@pytest.mark.parametrize("method,jac", [("team", "numerical"), ("team", "analytic"), ("wide", "analytic")])
+def test_order_does_not_change_results(hip_device, mech, method, jac):
+    """Cost order and its reverse give the unordered call's results bit for bit."""
+    import torch
+    y = premixed_cells(mech, 256, seed=21)
+    print(f"order test {method}/{jac}, forms mechanism, 256 cells", flush=True)
+    ode = api.ODE_Parameters()
+    ode.SetRelativeTolerance(RTOL)
+    ode.SetAbsoluteTolerance(ATOL)
+    step = api.ChemistryStep(mech, ode, method=method, jacobian=jac, pressure=3.7 * ATM)
+    b = _device_batch(y, DT)
+    s = torch.cuda.current_stream()
+    try:
+        _run(step, b, s)
+        y0, st0, c0 = _result(b)
+        order = torch.empty(b["n"], dtype=torch.int32, device="cuda:0")
+        step.cost_order(b["n"], b["cnt"].data_ptr(), order.data_ptr(), s.cuda_stream)
+        torch.cuda.synchronize()
+        assert np.array_equal(np.sort(order.cpu().numpy()), np.arange(b["n"]))
+        _run(step, b, s, order)
+        y1, st1, c1 = _result(b)
+        _run(step, b, s, torch.flip(order, [0]).contiguous())
+        y2, st2, c2 = _result(b)
+    finally:
+        step.close()
+    for ya, sa, ca in ((y1, st1, c1), (y2, st2, c2)):
+        assert np.array_equal(ya, y0) and np.array_equal(sa, st0) and np.array_equal(ca, c0)
+    assert np.all(st0 >= 0)
+    print(f"{method}/{jac}: ordered results identical; steps {c0[:, 0].min()}-{c0[:, 0].max()}", flush=True)
+
+
+def test_graph_replay_matches_eager(hip_device, mech):
+    """A team step of the forms mechanism captured in a HIP graph: three replays, with
+    eager steps between them, each equal to the eager result."""
+    import torch
+    y = premixed_cells(mech, 512, seed=23)
+    print("graph test, forms mechanism, 512 cells", flush=True)
+    ode = api.ODE_Parameters()
+    ode.SetRelativeTolerance(RTOL)
+    ode.SetAbsoluteTolerance(ATOL)
+    step = api.ChemistryStep(mech, ode, method="team", pressure=0.02 * ATM)
+    b = _device_batch(y, DT)
+    s = torch.cuda.current_stream()
+    try:
+        _run(step, b, s)
+        y0, st0, c0 = _result(b)
+        side = torch.cuda.Stream()
+        side.wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(g, stream=side):
+                _run(step, b, side)
+        s.wait_stream(side)
+        torch.cuda.synchronize()
+        for rep in range(3):
+            b["out"].zero_()
+            b["st"].fill_(-99)
+            g.replay()
+            y1, st1, c1 = _result(b)
+            assert np.array_equal(y1, y0) and np.array_equal(st1, st0) and np.array_equal(c1, c0), rep
+            _run(step, b, s)
+            y2, _, _ = _result(b)
+            assert np.array_equal(y2, y0)
+    finally:
+        step.close()
+    print("graph: 3 replays identical to eager", flush=True)
