@@ -289,6 +289,14 @@ int lsm_properties_device(lsm_handle *h, int64_t n_cells, const double *d_y, con
  *   NULL (teta is 0 for species without thermal diffusion ratios).  Host variant
  *   synchronous; _device takes device pointers and a hipStream_t, no synchronisation. */
 int lsm_set_transport(lsm_handle *h, const lsm_transport *tr);
+
+/* The physical-model options of the same refresh (readOptions.H:94-181, 283-360;
+ * properties.H:109-192): Lewis-number diffusivities, simplified transport properties,
+ * Soret diffusion coefficients and soot-diffusivity corrections; include/lsm_transport.h
+ * gives their meaning.  NULL restores the defaults (mixture-averaged diffusion, the
+ * fitted mu and lambda, theta output, no corrections).  Takes effect for the following
+ * lsm_transport_properties calls; may be called before or after lsm_set_transport. */
+int lsm_set_transport_options(lsm_handle *h, const lsm_transport_options *o);
 int lsm_transport_properties(lsm_handle *h, int64_t n_cells, const double *y, const double *p, double *mu,
                              double *lambda, double *dmix, double *teta);
 int lsm_transport_properties_device(lsm_handle *h, int64_t n_cells, const double *d_y, const double *d_p,

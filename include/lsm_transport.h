@@ -47,6 +47,39 @@ typedef struct lsm_transport {
     int32_t viscosity_model;     /* LSM_VISCOSITY_*                                  */
 } lsm_transport;
 
+/*
+ * Options of laminarSMOKE's per-cell property refresh (solvers/laminarSMOKE/readOptions.H
+ * and properties.H), applied on top of the fits above:
+ *
+ *   diffusivity_model   LSM_DIFFUSIVITY_* (readOptions.H:161-181).  Lewis numbers:
+ *                       D_k = lambda / (p psi) / cp / Le_k with psi = cTot MW / p
+ *                       (properties.H:153-164); lewis[ns] holds Le_k.
+ *   simplified          simplifiedTransportProperties (readOptions.H:129,
+ *                       properties.H:130-139): mu = 1.8405e-5 (T / 300)^0.6759 kg/m/s,
+ *                       lambda = mu cp / 0.7, before the diffusion coefficients.
+ *   soret               soretEffect (properties.H:180-192): the thermal-diffusion output
+ *                       holds Dsoret_k = D_k theta_k MW_k / MW [m2/s] for the species with
+ *                       thermal diffusion ratios (0 for the others) instead of theta_k.
+ *   soot_reference      physicalSootDiffusivity (readOptions.H:283-360, properties.H:166-177):
+ *   n_soot, soot_sp,    -1 for none; else D_{soot_sp[i]} = D_{soot_reference} soot_corr[i],
+ *   soot_corr           with D_{soot_reference} taken before any correction.
+ */
+enum {
+    LSM_DIFFUSIVITY_MULTICOMPONENT = 0,  /* mixture-averaged (MassDiffusionCoefficients) */
+    LSM_DIFFUSIVITY_LEWIS_NUMBERS = 1    /* fixed Lewis numbers                          */
+};
+
+typedef struct lsm_transport_options {
+    int32_t diffusivity_model;   /* LSM_DIFFUSIVITY_*                               */
+    const double *lewis;         /* [ns] Lewis numbers (LSM_DIFFUSIVITY_LEWIS_NUMBERS) */
+    int32_t simplified;          /* 0/1                                             */
+    int32_t soret;               /* 0/1                                             */
+    int32_t soot_reference;      /* species index, or -1                            */
+    int32_t n_soot;              /* number of corrected species                     */
+    const int32_t *soot_sp;      /* [n_soot] distinct species indices               */
+    const double *soot_corr;     /* [n_soot] correction factors                     */
+} lsm_transport_options;
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,15 @@ class LsmTransport(ctypes.Structure):
     ]
 
 
+class LsmTransportOptions(ctypes.Structure):
+    """include/lsm_transport.h: struct lsm_transport_options."""
+    _fields_ = [
+        ("diffusivity_model", ctypes.c_int32), ("lewis", _d), ("simplified", ctypes.c_int32),
+        ("soret", ctypes.c_int32), ("soot_reference", ctypes.c_int32), ("n_soot", ctypes.c_int32),
+        ("soot_sp", _i), ("soot_corr", _d),
+    ]
+
+
 class LsmOdeParameters(ctypes.Structure):
     """include/laminarsmoke_amd.h: struct lsm_ode_parameters."""
     _fields_ = [

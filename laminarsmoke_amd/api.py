@@ -64,6 +64,7 @@ def lib():
         L.lsm_properties.argtypes = [vp, i64, vp, vp, vp, vp, vp]
         L.lsm_properties_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.lsm_set_transport.argtypes = [vp, vp]
+        L.lsm_set_transport_options.argtypes = [vp, vp]
         L.lsm_transport_properties.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.lsm_transport_properties_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_integrate_batch_device_rho.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
@@ -234,6 +235,12 @@ class ChemistryStep:
         """Copy the transport fits (laminarsmoke_amd.transport.Transport) to the device."""
         self._transport = tr
         _check(lib().lsm_set_transport(self.h, ctypes.byref(tr.struct())))
+
+    def set_transport_options(self, opts=None):
+        """The physical-model options of the refresh (laminarsmoke_amd.transport.TransportOptions);
+        None restores the defaults."""
+        self._transport_opts = opts
+        _check(lib().lsm_set_transport_options(self.h, ctypes.byref(opts.struct()) if opts is not None else None))
 
     def transport(self, y, p=None, species=True):
         """Host arrays: y[ns+1, n] -> (mu[n], lambda[n], dmix[ns, n], teta[ns, n]) (the last two None

@@ -239,6 +239,7 @@ struct lsm_handle {
     // transport fits (lsm_set_transport)
     DevTransport tr{};
     void *d_transport = nullptr;
+    void *d_transport_opts = nullptr;        // Lewis numbers and soot corrections (lsm_set_transport_options)
 };
 
 int lsm_fail(int code, const char *msg) { return fail(code, msg); }
@@ -937,6 +938,7 @@ int lsm_destroy(lsm_handle *h) {
     if (h->d_status) (void)hipFree(h->d_status);
     if (h->d_cnt) (void)hipFree(h->d_cnt);
     if (h->d_transport) (void)hipFree(h->d_transport);
+    if (h->d_transport_opts) (void)hipFree(h->d_transport_opts);
     delete h;
     return LSM_OK;
 }
@@ -1165,6 +1167,67 @@ int lsm_set_transport(lsm_handle *h, const lsm_transport *t) {
     T.gam = (const double *)(base + o_g);
     T.wsup = (const double *)(base + o_s); T.winf = (const double *)(base + o_i); T.wr14 = (const double *)(base + o_r);
     T.teta_slot = (const int *)(base + o_sl); T.teta = (const double *)(base + o_t);
+    return LSM_OK;
+}
+
+int lsm_set_transport_options(lsm_handle *h, const lsm_transport_options *o) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    DevTransport &T = h->tr;
+    if (!o) {
+        if (h->d_transport_opts) (void)hipFree(h->d_transport_opts);
+        h->d_transport_opts = nullptr;
+        T.dmodel = LSM_DIFFUSIVITY_MULTICOMPONENT; T.simplified = 0; T.soret = 0; T.soot_ref = -1;
+        T.lewis = nullptr; T.soot_slot = nullptr; T.soot_corr = nullptr;
+        return LSM_OK;
+    }
+    const int ns = h->dm.ns;
+    if (o->diffusivity_model != LSM_DIFFUSIVITY_MULTICOMPONENT && o->diffusivity_model != LSM_DIFFUSIVITY_LEWIS_NUMBERS)
+        return fail(LSM_E_INVALID, "diffusivity model");
+    const bool lewis = o->diffusivity_model == LSM_DIFFUSIVITY_LEWIS_NUMBERS;
+    if (lewis) {
+        if (!o->lewis) return fail(LSM_E_INVALID, "Lewis numbers pointer is null");
+        for (int k = 0; k < ns; k++)
+            if (!(o->lewis[k] > 0.) || !std::isfinite(o->lewis[k]))
+                return fail(LSM_E_INVALID, "Lewis numbers must be positive and finite");
+    }
+    std::vector<int> slot(ns, -1);
+    const bool soot = o->soot_reference >= 0;
+    if (soot) {
+        if (o->soot_reference >= ns) return fail(LSM_E_INVALID, "soot reference species index");
+        if (o->n_soot < 0 || o->n_soot > ns || (o->n_soot > 0 && (!o->soot_sp || !o->soot_corr)))
+            return fail(LSM_E_INVALID, "soot correction tables");
+        for (int i = 0; i < o->n_soot; i++) {
+            const int k = o->soot_sp[i];
+            if (k < 0 || k >= ns || slot[k] >= 0) return fail(LSM_E_INVALID, "soot species index");
+            if (!std::isfinite(o->soot_corr[i])) return fail(LSM_E_INVALID, "soot correction factor");
+            slot[k] = i;
+        }
+    } else if (o->soot_reference != -1 || o->n_soot != 0) {
+        return fail(LSM_E_INVALID, "soot corrections need a reference species");
+    }
+    Blob b;
+    const size_t o_l = b.add(lewis ? std::vector<double>(o->lewis, o->lewis + ns) : std::vector<double>());
+    const size_t o_s = b.add(soot ? slot : std::vector<int>());
+    const size_t o_c = b.add(soot && o->n_soot ? std::vector<double>(o->soot_corr, o->soot_corr + o->n_soot)
+                                               : std::vector<double>());
+    void *d = nullptr;
+    if (b.bytes.size()) {
+        if (hipMalloc(&d, b.bytes.size()) != hipSuccess) return fail(LSM_E_NOMEM, "transport option tables");
+        if (hipMemcpy(d, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(d);
+            return fail(LSM_E_HIP, "copy transport option tables");
+        }
+    }
+    if (h->d_transport_opts) (void)hipFree(h->d_transport_opts);
+    h->d_transport_opts = d;
+    char *base = (char *)d;
+    T.dmodel = o->diffusivity_model;
+    T.simplified = o->simplified ? 1 : 0;
+    T.soret = o->soret ? 1 : 0;
+    T.soot_ref = soot ? o->soot_reference : -1;
+    T.lewis = lewis ? (const double *)(base + o_l) : nullptr;
+    T.soot_slot = soot ? (const int *)(base + o_s) : nullptr;
+    T.soot_corr = soot && o->n_soot ? (const double *)(base + o_c) : nullptr;
     return LSM_OK;
 }
 

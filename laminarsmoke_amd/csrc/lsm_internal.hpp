@@ -105,6 +105,13 @@ struct DevTransport {
     const double *wsup, *winf, *wr14;        // [ns(ns-1)/2] Wilke pair constants (CompleteInitialization)
     const int *teta_slot;                    // [ns] row of teta, or -1
     const double *teta;                      // [nteta][ns][4] T fits of the thermal diffusion ratios
+    // physical-model options (lsm_set_transport_options)
+    int dmodel;                              // LSM_DIFFUSIVITY_*
+    int simplified, soret;                   // 0/1
+    int soot_ref = -1;                       // reference species of the soot corrections, or -1
+    const double *lewis;                     // [ns] Lewis numbers (LSM_DIFFUSIVITY_LEWIS_NUMBERS)
+    const int *soot_slot;                    // [ns] row of soot_corr, or -1 (null when soot_ref < 0)
+    const double *soot_corr;                 // [n_soot]
 };
 
 struct KParams {

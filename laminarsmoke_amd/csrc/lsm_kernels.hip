@@ -2269,10 +2269,11 @@ __global__ void __launch_bounds__(256) properties_kernel(DevMech M, double p_def
     }
 }
 
-// Transport part of the property refresh (solvers/laminarSMOKE/properties.H:101-147):
+// Transport part of the property refresh (solvers/laminarSMOKE/properties.H:101-192):
 // DynamicViscosity, ThermalConductivity, MassDiffusionCoefficients and
 // ThermalDiffusionRatios of TransportPropertiesMap_CHEMKIN (maps/TransportPropertiesMap_CHEMKIN.hpp:
-// 690-1117) per cell.  One wave per cell; lane k handles species k, k + 64, ...; the
+// 690-1117) per cell, with the solver's options: simplified mu and lambda, Lewis-number
+// diffusivities, soot-diffusivity corrections and Soret coefficients.  One wave per cell; lane k handles species k, k + 64, ...; the
 // wave's mole fractions and sqrt(eta) sit in LDS.  Each species sums over its partners
 // in increasing index order, the order in which the reference's pair loops add to it;
 // the sums over species are wave reductions.  With share_pairs, each binary 1/D_jk is
@@ -2291,8 +2292,11 @@ __global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport 
     const int ns = M.ns, P = M.ns_pad, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int np = ns * (ns - 1) / 2;
     const size_t per_wave = (size_t)3 * ns + (share_pairs ? (size_t)np : 0);
-    double *sx = tsh + (size_t)wv * per_wave, *sq = sx + ns, *s3 = sq + ns;   // x, sqrt(eta), eta / corrected x
+    double *sx = tsh + (size_t)wv * per_wave, *sq = sx + ns, *s3 = sq + ns;   // x, sqrt(eta) then D, eta / corrected x
     double *sp = s3 + ns;                                                     // 1/D of each pair (share_pairs)
+    const bool lewis = Tr.dmodel == LSM_DIFFUSIVITY_LEWIS_NUMBERS;
+    const bool need_cp = lewis || Tr.simplified;               // cpCells (properties.H:109-113)
+    const bool need_d = dmix || (teta && Tr.soret);
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t c = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv; c < n; c += nw) {
         const double T = y[(int64_t)ns * n + c];
@@ -2302,7 +2306,7 @@ __global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport 
         double u = 0.;
         for (int k = lane; k < ns; k += 64) u += y[(int64_t)k * n + c] / M.sp[SP_MW * P + k];
         const double MW = 1. / wave_sum(u);
-        double l1 = 0., l2 = 0.;
+        double l1 = 0., l2 = 0., cpx = 0.;
         for (int k = lane; k < ns; k += 64) {
             const double xk = (MW * y[(int64_t)k * n + c]) / M.sp[SP_MW * P + k];
             const double *a = Tr.lam + 4 * k, *e = Tr.eta + 4 * k;
@@ -2316,46 +2320,119 @@ __global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport 
             s3[k] = ek;
             l1 += xk * lk;
             l2 += xk / lk;
+            if (need_cp) {                                     // cp_k / R (ThermodynamicsMap_CHEMKIN::cp_over_R)
+                const bool hi = T > M.sp[SP_TMID * P + k];
+                const int rc = hi ? SP_CP_HT : SP_CP_LT;
+                const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+                cpx += xk * (M.sp[rc * P + k] + T * M.sp[(rc + 1) * P + k] + T2 * M.sp[(rc + 2) * P + k] +
+                             T3 * M.sp[(rc + 3) * P + k] + T4 * M.sp[(rc + 4) * P + k]);
+            }
         }
         wave_sync();
         const double L1 = wave_sum(l1), L2 = wave_sum(l2);
-        if (lane == 0) lam[c] = 0.50 * (L1 + 1. / L2);                 // lambdaMix (:947-957)
-        // etaMix (:959-1030)
-        double m1 = 0., m2 = 0.;
-        for (int k = lane; k < ns; k += 64) {
-            const double xk = sx[k], ek = s3[k];
-            if (Tr.model == LSM_VISCOSITY_WILKE) {
-                const double sqk = sq[k], usqk = 1. / sqk;
-                double sumK = xk;
-                for (int q = 0; q < ns; q++) {
-                    if (q == k) continue;
-                    if (q < k) {
-                        const int i = pair_index(q, k, ns);
-                        const double d = sq[q] * usqk * Tr.wr14[i];
-                        sumK += sx[q] * Tr.winf[i] * (1. + 1. / d) * (1. + 1. / d);
-                    } else {
-                        const int i = pair_index(k, q, ns);
-                        const double d = sqk * (1. / sq[q]) * Tr.wr14[i];
-                        sumK += sx[q] * Tr.wsup[i] * (1. + d) * (1. + d);
-                    }
-                }
-                m1 += xk * ek / sumK;
-            } else if (Tr.model == LSM_VISCOSITY_HERNING) {
-                const double sm = sqrt(Tr.mw[k]);
-                m1 += xk * ek * sm;
-                m2 += xk * sm;
-            } else {
-                m1 += xk * ek;
-                m2 += xk / ek;
-            }
-        }
-        const double M1 = wave_sum(m1), M2 = wave_sum(m2);
-        if (lane == 0)
-            mu[c] = Tr.model == LSM_VISCOSITY_WILKE ? M1 : (Tr.model == LSM_VISCOSITY_HERNING ? M1 / M2
-                                                                                             : 0.50 * (M1 + 1. / M2));
-        // tetaMix (:768-796, 1103-1117): theta_kj x_k x_j for the light species
-        if (teta)
+        const double cp = need_cp ? (wave_sum(cpx) * RGAS) / MW : 0.;    // J/kg/K
+        double lam_c = 0.50 * (L1 + 1. / L2);                          // lambdaMix (:947-957)
+        double mu_c;
+        if (Tr.simplified) {                                           // properties.H:130-139
+            mu_c = 1.8405e-5 * pow(T / 300., 0.6759);
+            lam_c = mu_c * cp / 0.7;
+        } else {
+            // etaMix (:959-1030)
+            double m1 = 0., m2 = 0.;
             for (int k = lane; k < ns; k += 64) {
+                const double xk = sx[k], ek = s3[k];
+                if (Tr.model == LSM_VISCOSITY_WILKE) {
+                    const double sqk = sq[k], usqk = 1. / sqk;
+                    double sumK = xk;
+                    for (int q = 0; q < ns; q++) {
+                        if (q == k) continue;
+                        if (q < k) {
+                            const int i = pair_index(q, k, ns);
+                            const double d = sq[q] * usqk * Tr.wr14[i];
+                            sumK += sx[q] * Tr.winf[i] * (1. + 1. / d) * (1. + 1. / d);
+                        } else {
+                            const int i = pair_index(k, q, ns);
+                            const double d = sqk * (1. / sq[q]) * Tr.wr14[i];
+                            sumK += sx[q] * Tr.wsup[i] * (1. + d) * (1. + d);
+                        }
+                    }
+                    m1 += xk * ek / sumK;
+                } else if (Tr.model == LSM_VISCOSITY_HERNING) {
+                    const double sm = sqrt(Tr.mw[k]);
+                    m1 += xk * ek * sm;
+                    m2 += xk * sm;
+                } else {
+                    m1 += xk * ek;
+                    m2 += xk / ek;
+                }
+            }
+            const double M1 = wave_sum(m1), M2 = wave_sum(m2);
+            mu_c = Tr.model == LSM_VISCOSITY_WILKE ? M1
+                                                   : (Tr.model == LSM_VISCOSITY_HERNING ? M1 / M2 : 0.50 * (M1 + 1. / M2));
+        }
+        if (lane == 0) {
+            mu[c] = mu_c;
+            lam[c] = lam_c;
+        }
+        // Diffusion coefficients into sq (sqrt(eta) is no longer read): gamma() and gammaMix
+        // (:798-859, 1032-1059), or lambda / (p psi) / cp / Le_k (properties.H:153-164)
+        if (need_d) {
+            wave_sync();
+            if (lewis) {
+                const double cTot = pc / RGAS / T, psi = cTot * MW / pc;
+                const double coef = lam_c / (pc * psi) / cp;
+                for (int k = lane; k < ns; k += 64) sq[k] = coef / Tr.lewis[k];
+            } else {
+                const double thr = 1.e-14, sum_thr = 1. + thr * (double)ns, P_bar = pc / 100000.;
+                double mc = 0.;
+                for (int k = lane; k < ns; k += 64) {
+                    const double xc = (sx[k] + thr) / sum_thr;
+                    s3[k] = xc;
+                    mc += xc * Tr.mw[k];
+                }
+                if (share_pairs)
+                    for (int q = lane; q < np; q += 64) {
+                        const double *f = Tr.gam + 4 * q;
+                        double sum = f[0];
+                        sum += f[1] * logT; sum += f[2] * logT2; sum += f[3] * logT3;
+                        sp[q] = P_bar / exp(sum);
+                    }
+                wave_sync();
+                const double MWc = wave_sum(mc);
+                for (int k = lane; k < ns; k += 64) {
+                    double sdc = 0.;
+                    for (int q = 0; q < ns; q++) {
+                        if (q == k) continue;
+                        const int i = pair_index(q < k ? q : k, q < k ? k : q, ns);
+                        double dinv;
+                        if (share_pairs) {
+                            dinv = sp[i];
+                        } else {
+                            const double *f = Tr.gam + 4 * i;
+                            double sum = f[0];
+                            sum += f[1] * logT; sum += f[2] * logT2; sum += f[3] * logT3;
+                            dinv = P_bar / exp(sum);
+                        }
+                        sdc += s3[q] * dinv;      // x_q / D_kq
+                    }
+                    sq[k] = (MWc - s3[k] * Tr.mw[k]) / (MWc * sdc);
+                }
+            }
+            wave_sync();
+        }
+        // soot corrections (properties.H:166-177), then the outputs; tetaMix (:768-796,
+        // 1103-1117): theta_kj x_k x_j for the light species, or Dsoret (properties.H:180-192)
+        for (int k = lane; k < ns; k += 64) {
+            double D = 0.;
+            if (need_d) {
+                D = sq[k];
+                if (Tr.soot_ref >= 0) {
+                    const int si = Tr.soot_slot[k];
+                    if (si >= 0) D = sq[Tr.soot_ref] * Tr.soot_corr[si];
+                }
+                if (dmix) dmix[(int64_t)k * n + c] = D;
+            }
+            if (teta) {
                 const int sl = Tr.teta_slot[k];
                 double tm = 0.;
                 if (sl >= 0) {
@@ -2366,45 +2443,9 @@ __global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport 
                         sum += f[4 * j + 1] * T; sum += f[4 * j + 2] * T2; sum += f[4 * j + 3] * T3;
                         tm += sum * xk * sx[j];
                     }
+                    if (Tr.soret) tm = D * tm * M.sp[SP_MW * P + k] / MW;
                 }
                 teta[(int64_t)k * n + c] = tm;
-            }
-        // gamma() and gammaMix (:798-859, 1032-1059)
-        if (dmix) {
-            wave_sync();
-            const double thr = 1.e-14, sum_thr = 1. + thr * (double)ns, P_bar = pc / 100000.;
-            double mc = 0.;
-            for (int k = lane; k < ns; k += 64) {
-                const double xc = (sx[k] + thr) / sum_thr;
-                s3[k] = xc;
-                mc += xc * Tr.mw[k];
-            }
-            if (share_pairs)
-                for (int q = lane; q < np; q += 64) {
-                    const double *f = Tr.gam + 4 * q;
-                    double sum = f[0];
-                    sum += f[1] * logT; sum += f[2] * logT2; sum += f[3] * logT3;
-                    sp[q] = P_bar / exp(sum);
-                }
-            wave_sync();
-            const double MWc = wave_sum(mc);
-            for (int k = lane; k < ns; k += 64) {
-                double sdc = 0.;
-                for (int q = 0; q < ns; q++) {
-                    if (q == k) continue;
-                    const int i = pair_index(q < k ? q : k, q < k ? k : q, ns);
-                    double dinv;
-                    if (share_pairs) {
-                        dinv = sp[i];
-                    } else {
-                        const double *f = Tr.gam + 4 * i;
-                        double sum = f[0];
-                        sum += f[1] * logT; sum += f[2] * logT2; sum += f[3] * logT3;
-                        dinv = P_bar / exp(sum);
-                    }
-                    sdc += s3[q] * dinv;      // x_q / D_kq
-                }
-                dmix[(int64_t)k * n + c] = (MWc - s3[k] * Tr.mw[k]) / (MWc * sdc);
             }
         }
         wave_sync();

@@ -11,6 +11,10 @@ theta_kj.
   kinetics.xml (ImportCoefficientsFromXMLFile / ImportCoefficientsFromASCIIFile,
   :340-403, 420-465) and the optional ``<ViscosityModel>`` (:634-651);
   :func:`write_transport_block` writes that block.
+* :class:`TransportOptions` holds the solver's physical-model options of the property
+  refresh (readOptions.H:94-181, 283-360): Lewis-number diffusivities, simplified
+  transport properties, Soret coefficients and the soot-diffusivity corrections that
+  :func:`soot_diffusivity_corrections` derives from the BIN species.
 * :func:`synthetic_transport` makes physically shaped fits for any mechanism
   (Chapman-Enskog viscosity and diffusion with Neufeld's collision-integral
   correlations, an Eucken-type conductivity) for tests and benchmarks; it is not the
@@ -23,11 +27,14 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._abi import LsmTransport
+from ._abi import LsmTransport, LsmTransportOptions
 
 VISCOSITY_WILKE = 0
 VISCOSITY_HERNING = 1
 VISCOSITY_MATHUR_SAXENA = 2
+DIFFUSIVITY_MULTICOMPONENT = 0
+DIFFUSIVITY_LEWIS_NUMBERS = 1
+_DIFFUSIVITY = {"multi-component": DIFFUSIVITY_MULTICOMPONENT, "lewis-numbers": DIFFUSIVITY_LEWIS_NUMBERS}
 MAX_MW_THERMALDIFFUSION_RATIOS = 20.0        # math/PhysicalConstants.h:44
 _MODELS = {"Wilke": VISCOSITY_WILKE, "Herning": VISCOSITY_HERNING, "MathurSaxena": VISCOSITY_MATHUR_SAXENA}
 
@@ -76,6 +83,64 @@ class Transport:
             s.viscosity_model = int(self.viscosity_model)
             self._struct = s
         return self._struct
+
+
+@dataclass
+class TransportOptions:
+    """Options of the property refresh (include/lsm_transport.h, lsm_transport_options).
+
+    diffusivity: "multi-component" or "lewis-numbers" (readOptions.H:161-181), with
+    `lewis` [ns]; simplified: simplifiedTransportProperties; soret: soretEffect (the
+    thermal-diffusion output then holds Dsoret); soot_reference / soot_species /
+    soot_correction: physicalSootDiffusivity (see soot_diffusivity_corrections)."""
+
+    diffusivity: str = "multi-component"
+    lewis: np.ndarray | None = None
+    simplified: bool = False
+    soret: bool = False
+    soot_reference: int = -1
+    soot_species: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    soot_correction: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    _struct: object = field(default=None, repr=False)
+
+    def __post_init__(self):
+        if self.diffusivity not in _DIFFUSIVITY:
+            raise ValueError(f"diffusivity must be one of {sorted(_DIFFUSIVITY)}")
+        if self.lewis is not None:
+            self.lewis = np.ascontiguousarray(self.lewis, dtype=np.float64)
+        self.soot_species = np.ascontiguousarray(self.soot_species, dtype=np.int32)
+        self.soot_correction = np.ascontiguousarray(self.soot_correction, dtype=np.float64)
+
+    def struct(self) -> LsmTransportOptions:
+        if self._struct is None:
+            P = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t)) if a is not None and a.size else None
+            s = LsmTransportOptions()
+            s.diffusivity_model = _DIFFUSIVITY[self.diffusivity]
+            s.lewis = P(self.lewis, ctypes.c_double)
+            s.simplified, s.soret = int(bool(self.simplified)), int(bool(self.soret))
+            s.soot_reference = int(self.soot_reference)
+            s.n_soot = int(self.soot_species.size)
+            s.soot_sp, s.soot_corr = P(self.soot_species, ctypes.c_int32), P(self.soot_correction, ctypes.c_double)
+            self._struct = s
+        return self._struct
+
+
+def soot_diffusivity_corrections(species, mw, index: int, exponent: float = -0.681, additional: float = 1.0):
+    """(reference, species, corrections) of physicalSootDiffusivity (readOptions.H:318-360):
+    the reference is BIN1A; every species named BIN* gets
+    min(MW_i / MW_ref, 2^index)^exponent, times `additional` when MW_i / MW_ref >= 10."""
+    names = list(species)
+    if "BIN1A" not in names:
+        raise ValueError("Species BIN1A is not included in the kinetic mechanism")
+    ref = names.index("BIN1A")
+    sp, corr = [], []
+    for i, name in enumerate(names):
+        if name[:3] == "BIN":
+            r = mw[i] / mw[ref]
+            c = min(r, 2.0 ** float(index)) ** exponent
+            corr.append(c if r < 10.0 else c * additional)
+            sp.append(i)
+    return ref, np.array(sp, dtype=np.int32), np.array(corr)
 
 
 def read_transport_xml(text: str, ns: int | None = None) -> Transport:

@@ -1469,3 +1469,50 @@ void lsm_oracle_transport(const lsm_transport *tr, double T, double P, const dou
     }
     free(lam); free(eta); free(sumK); free(xc); free(sdc);
 }
+
+/* properties.H:79-192, one cell */
+void lsm_oracle_refresh_cell(const lsm_mechanism *m, const lsm_transport *tr, const lsm_transport_options *o,
+                             double P, const double *y, double *mu, double *lambda, double *dmix, double *teta) {
+    const int ns = m->ns;
+    const double T = y[ns];
+    double *x = malloc(sizeof(double) * ns), *cpR = malloc(sizeof(double) * ns);
+    double *hRT = malloc(sizeof(double) * ns), *sR = malloc(sizeof(double) * ns);
+    double *D = malloc(sizeof(double) * ns), *th = malloc(sizeof(double) * ns);
+    /* MoleFractions_From_MassFractions (ThermodynamicsMap_CHEMKIN.hpp:429-434) */
+    double u = 0.;
+    for (int k = 0; k < ns; k++) u += y[k] / m->nasa[18 * k + 17];
+    const double MW = 1. / u;
+    for (int k = 0; k < ns; k++) x[k] = MW * y[k] / m->nasa[18 * k + 17];
+    const double cTot = P / LSM_R_J_KMOL / T, psi = cTot * MW / P;          /* :97-98 */
+    const int lewis = o && o->diffusivity_model == LSM_DIFFUSIVITY_LEWIS_NUMBERS;
+    const int soret = o && o->soret;
+    double mu_c, lam_c;
+    lsm_oracle_transport(tr, T, P, x, &mu_c, &lam_c, lewis ? NULL : D, th);
+    /* cpMolar_Mixture_From_MoleFractions / MW (:109-113) */
+    lsm_oracle_species_thermo(m, T, cpR, hRT, sR);
+    double cp = 0.;
+    for (int k = 0; k < ns; k++) cp += cpR[k] * x[k];
+    cp = cp * LSM_R_J_KMOL / MW;
+    if (o && o->simplified) {                                              /* :130-139 */
+        mu_c = 1.8405e-5 * pow(T / 300., 0.6759);
+        lam_c = mu_c * cp / 0.7;
+    }
+    if (lewis) {                                                           /* :153-164 */
+        const double coefficient = lam_c / (P * psi) / cp;
+        for (int k = 0; k < ns; k++) D[k] = coefficient / o->lewis[k];
+    }
+    if (o && o->soot_reference >= 0) {                                     /* :166-177 */
+        const double Dref = D[o->soot_reference];
+        for (int i = 0; i < o->n_soot; i++) D[o->soot_sp[i]] = Dref * o->soot_corr[i];
+    }
+    if (soret)                                                             /* :180-192 */
+        for (int q = 0; q < tr->nteta; q++) {
+            const int k = tr->teta_sp[q];
+            th[k] = D[k] * th[k] * m->nasa[18 * k + 17] / MW;
+        }
+    if (mu) *mu = mu_c;
+    if (lambda) *lambda = lam_c;
+    if (dmix) memcpy(dmix, D, sizeof(double) * ns);
+    if (teta) memcpy(teta, th, sizeof(double) * ns);
+    free(x); free(cpR); free(hRT); free(sR); free(D); free(th);
+}

@@ -83,6 +83,14 @@ void lsm_oracle_kinetics_p(const lsm_mechanism *m, double T, double P, const dou
 void lsm_oracle_transport(const lsm_transport *tr, double T, double P, const double *x, double *mu,
                           double *lambda, double *dmix, double *teta);
 
+/* One cell of laminarSMOKE's property refresh with its physical-model options
+ * (solvers/laminarSMOKE/properties.H:79-192): mixture mu and lambda (fitted, or the
+ * simplified power law), diffusion coefficients (mixture-averaged or from Lewis numbers,
+ * then the soot corrections), and theta or, with o->soret, Dsoret.  y: [ns+1] (Y, T).
+ * o may be NULL (the defaults).  dmix / teta may be NULL. */
+void lsm_oracle_refresh_cell(const lsm_mechanism *m, const lsm_transport *tr, const lsm_transport_options *o,
+                             double P, const double *y, double *mu, double *lambda, double *dmix, double *teta);
+
 /* Constant-pressure RHS (BatchReactorHomogeneousConstantPressure::Equations);
  * y = [Y_0..Y_{ns-1}, T], returns QR [J/m3/s]. */
 double lsm_oracle_rhs(const lsm_mechanism *m, double P, int energy, const double *y, double *f);

@@ -104,6 +104,18 @@ def transport(tr, T, P, x):
     return mu.value, lam.value, dmix, teta
 
 
+def refresh_cell(mech, tr, opts, P, y):
+    """mu, lambda, dmix[ns], teta-or-Dsoret[ns] of one cell y[ns+1] with the refresh options."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    ns = tr.ns
+    mu, lam = ctypes.c_double(), ctypes.c_double()
+    dmix, teta = np.zeros(ns), np.zeros(ns)
+    lib().lsm_oracle_refresh_cell(ctypes.byref(mech.struct()), ctypes.byref(tr.struct()),
+                                  ctypes.byref(opts.struct()) if opts is not None else None, ctypes.c_double(P),
+                                  _p(y), ctypes.byref(mu), ctypes.byref(lam), _p(dmix), _p(teta))
+    return mu.value, lam.value, dmix, teta
+
+
 def rhs(mech, y, P=101325.0, energy=True, rho0=None):
     """Constant-pressure RHS, or the constant-volume RHS at density rho0 when given."""
     y = np.ascontiguousarray(y, dtype=np.float64)

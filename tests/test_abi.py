@@ -8,7 +8,7 @@ import re
 import pytest
 
 from laminarsmoke_amd import api
-from laminarsmoke_amd._abi import LsmOdeParameters, LsmReactorOptions, LsmMechanism, LsmTransport
+from laminarsmoke_amd._abi import LsmOdeParameters, LsmReactorOptions, LsmMechanism, LsmTransport, LsmTransportOptions
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -37,10 +37,11 @@ def test_struct_layouts_match_header():
 #include <stddef.h>
 #include "laminarsmoke_amd.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lsm_mechanism), sizeof(lsm_ode_parameters),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(lsm_mechanism), sizeof(lsm_ode_parameters),
          sizeof(lsm_reactor_options), offsetof(lsm_mechanism, fo_eff_m1), offsetof(lsm_ode_parameters, linear_algebra),
          offsetof(lsm_mechanism, rev_arr), offsetof(lsm_mechanism, npl), offsetof(lsm_mechanism, ch_a),
-         sizeof(lsm_transport), offsetof(lsm_transport, viscosity_model));
+         sizeof(lsm_transport), offsetof(lsm_transport, viscosity_model), sizeof(lsm_transport_options),
+         offsetof(lsm_transport_options, soot_corr));
   return 0; }
 '''
     import subprocess
@@ -55,7 +56,8 @@ int main(void) {
     assert got == [ctypes.sizeof(LsmMechanism), ctypes.sizeof(LsmOdeParameters), ctypes.sizeof(LsmReactorOptions),
                    LsmMechanism.fo_eff_m1.offset, LsmOdeParameters.linear_algebra.offset,
                    LsmMechanism.rev_arr.offset, LsmMechanism.npl.offset, LsmMechanism.ch_a.offset,
-                   ctypes.sizeof(LsmTransport), LsmTransport.viscosity_model.offset]
+                   ctypes.sizeof(LsmTransport), LsmTransport.viscosity_model.offset,
+                   ctypes.sizeof(LsmTransportOptions), LsmTransportOptions.soot_corr.offset]
 
 
 def test_defaults_mirror_ode_parameters():
