@@ -1144,12 +1144,27 @@ int lsm_set_transport(lsm_handle *h, const lsm_transport *t) {
                 r14[i] = std::sqrt(std::sqrt(M[j] / M[k]));
             }
     }
+    // The same constants as full partner-major matrices for the kernel's branch-free loop:
+    // species k sums x_q ww[q][k] (1 + e)^2 with e = sqrt(eta_k / eta_q) wr[q][k], which is
+    // d = sqrt(eta_k/eta_j) r14 for a later partner and 1/d for an earlier one (:988-993).
+    std::vector<double> ww((size_t)ns * ns, 0.), wr((size_t)ns * ns, 0.);
+    {
+        size_t i = 0;
+        for (int k = 0; k < ns; k++)
+            for (int j = k + 1; j < ns; j++, i++) {
+                ww[(size_t)j * ns + k] = sup[i];          // species k, later partner j
+                wr[(size_t)j * ns + k] = r14[i];
+                ww[(size_t)k * ns + j] = inf[i];          // species j, earlier partner k
+                wr[(size_t)k * ns + j] = 1. / r14[i];
+            }
+    }
     Blob b;
     const size_t o_mw = b.add(std::vector<double>(t->mw, t->mw + ns));
     const size_t o_l = b.add(std::vector<double>(t->lambda_fit, t->lambda_fit + 4 * ns));
     const size_t o_e = b.add(std::vector<double>(t->eta_fit, t->eta_fit + 4 * ns));
     const size_t o_g = b.add(np ? std::vector<double>(t->gamma_fit, t->gamma_fit + 4 * np) : std::vector<double>());
     const size_t o_s = b.add(sup), o_i = b.add(inf), o_r = b.add(r14), o_sl = b.add(slot);
+    const size_t o_ww = b.add(ww), o_wr = b.add(wr);
     const size_t o_t = b.add(t->nteta ? std::vector<double>(t->teta_fit, t->teta_fit + (size_t)4 * t->nteta * ns)
                                       : std::vector<double>());
     void *d = nullptr;
@@ -1167,6 +1182,7 @@ int lsm_set_transport(lsm_handle *h, const lsm_transport *t) {
     T.gam = (const double *)(base + o_g);
     T.wsup = (const double *)(base + o_s); T.winf = (const double *)(base + o_i); T.wr14 = (const double *)(base + o_r);
     T.teta_slot = (const int *)(base + o_sl); T.teta = (const double *)(base + o_t);
+    T.ww = (const double *)(base + o_ww); T.wr = (const double *)(base + o_wr);
     return LSM_OK;
 }
 

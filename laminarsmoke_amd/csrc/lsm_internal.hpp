@@ -103,6 +103,8 @@ struct DevTransport {
     const double *lam, *eta;                 // [ns][4] ln-T fits
     const double *gam;                       // [ns(ns-1)/2][4] ln-T fits of ln D_jk at 1 bar, pairs j < k
     const double *wsup, *winf, *wr14;        // [ns(ns-1)/2] Wilke pair constants (CompleteInitialization)
+    const double *ww, *wr;                   // [ns][ns], partner-major: Wilke weight and sqrt-ratio factor of
+                                             // species k with partner q at [q ns + k] (0 on the diagonal)
     const int *teta_slot;                    // [ns] row of teta, or -1
     const double *teta;                      // [nteta][ns][4] T fits of the thermal diffusion ratios
     // physical-model options (lsm_set_transport_options)

@@ -2273,10 +2273,12 @@ __global__ void __launch_bounds__(256) properties_kernel(DevMech M, double p_def
 // DynamicViscosity, ThermalConductivity, MassDiffusionCoefficients and
 // ThermalDiffusionRatios of TransportPropertiesMap_CHEMKIN (maps/TransportPropertiesMap_CHEMKIN.hpp:
 // 690-1117) per cell, with the solver's options: simplified mu and lambda, Lewis-number
-// diffusivities, soot-diffusivity corrections and Soret coefficients.  One wave per cell; lane k handles species k, k + 64, ...; the
-// wave's mole fractions and sqrt(eta) sit in LDS.  Each species sums over its partners
-// in increasing index order, the order in which the reference's pair loops add to it;
-// the sums over species are wave reductions.  With share_pairs, each binary 1/D_jk is
+// diffusivities, soot-diffusivity corrections and Soret coefficients.  One wave per cell;
+// lane k handles species k, k + 64, ...; the wave's mole fractions and 1/sqrt(eta) sit in
+// LDS.  Each species sums over its partners in increasing index order, the order in which
+// the reference's pair loops add to it; the sums over species are wave reductions.  The
+// Wilke factor of an earlier partner is formed as sqrt(eta_k/eta_q) / r14 rather than the
+// reference's 1/(sqrt(eta_q/eta_k) r14): equal up to rounding, and without a division.  With share_pairs, each binary 1/D_jk is
 // evaluated once (pairs dealt round-robin to the lanes) into a per-wave LDS table that
 // both partners read; otherwise each partner evaluates it (the same value either way).
 __device__ __forceinline__ int pair_index(int a, int b, int ns) {   // a < b
@@ -2292,7 +2294,7 @@ __global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport 
     const int ns = M.ns, P = M.ns_pad, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int np = ns * (ns - 1) / 2;
     const size_t per_wave = (size_t)3 * ns + (share_pairs ? (size_t)np : 0);
-    double *sx = tsh + (size_t)wv * per_wave, *sq = sx + ns, *s3 = sq + ns;   // x, sqrt(eta) then D, eta / corrected x
+    double *sx = tsh + (size_t)wv * per_wave, *sq = sx + ns, *s3 = sq + ns;   // x, 1/sqrt(eta) then D, eta / corrected x
     double *sp = s3 + ns;                                                     // 1/D of each pair (share_pairs)
     const bool lewis = Tr.dmodel == LSM_DIFFUSIVITY_LEWIS_NUMBERS;
     const bool need_cp = lewis || Tr.simplified;               // cpCells (properties.H:109-113)
@@ -2316,7 +2318,7 @@ __global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport 
             se += e[1] * logT; se += e[2] * logT2; se += e[3] * logT3;
             const double lk = exp(sl), ek = exp(se);
             sx[k] = xk;
-            sq[k] = sqrt(ek);
+            sq[k] = 1. / sqrt(ek);
             s3[k] = ek;
             l1 += xk * lk;
             l2 += xk / lk;
@@ -2342,19 +2344,14 @@ __global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport 
             for (int k = lane; k < ns; k += 64) {
                 const double xk = sx[k], ek = s3[k];
                 if (Tr.model == LSM_VISCOSITY_WILKE) {
-                    const double sqk = sq[k], usqk = 1. / sqk;
+                    // partner-major matrices: lane k reads [q ns + k], contiguous across the
+                    // wave; the diagonal weight is 0, so q = k adds nothing and needs no branch
+                    const double sqk = sqrt(ek);
+                    const double *wwk = Tr.ww + k, *wrk = Tr.wr + k;
                     double sumK = xk;
                     for (int q = 0; q < ns; q++) {
-                        if (q == k) continue;
-                        if (q < k) {
-                            const int i = pair_index(q, k, ns);
-                            const double d = sq[q] * usqk * Tr.wr14[i];
-                            sumK += sx[q] * Tr.winf[i] * (1. + 1. / d) * (1. + 1. / d);
-                        } else {
-                            const int i = pair_index(k, q, ns);
-                            const double d = sqk * (1. / sq[q]) * Tr.wr14[i];
-                            sumK += sx[q] * Tr.wsup[i] * (1. + d) * (1. + d);
-                        }
+                        const double e = sqk * sq[q] * wrk[(size_t)q * ns];
+                        sumK += sx[q] * wwk[(size_t)q * ns] * (1. + e) * (1. + e);
                     }
                     m1 += xk * ek / sumK;
                 } else if (Tr.model == LSM_VISCOSITY_HERNING) {
@@ -2374,7 +2371,7 @@ __global__ void __launch_bounds__(256) transport_kernel(DevMech M, DevTransport 
             mu[c] = mu_c;
             lam[c] = lam_c;
         }
-        // Diffusion coefficients into sq (sqrt(eta) is no longer read): gamma() and gammaMix
+        // Diffusion coefficients into sq (1/sqrt(eta) is no longer read): gamma() and gammaMix
         // (:798-859, 1032-1059), or lambda / (p psi) / cp / Le_k (properties.H:153-164)
         if (need_d) {
             wave_sync();
