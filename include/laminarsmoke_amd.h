@@ -81,14 +81,14 @@ int lsm_ode_parameters_parse(const char *text, lsm_ode_parameters *p);
  * Reactor / cell-loop options (the variables chemistry_DI.H reads).
  * ------------------------------------------------------------------------- */
 typedef struct lsm_reactor_options {
-    double pressure;            /* thermodynamicPressure [Pa] (chemistry_DI.H:134)            */
+    double pressure;            /* default thermodynamicPressure [Pa] of the property refresh
+                                   (p NULL) and the serving layer; the chemistry step takes its
+                                   pressure per call (chemistry_DI.H:134)                      */
     double min_temperature;     /* minTemperature4Chemistry: cells with T <= it are not
                                    integrated (chemistry_DI.H:97)                              */
     int32_t energy_equation;    /* energyEquation (chemistry_DI.H:135, 231-233)               */
     double mass_fractions_tol;  /* massFractionsTol: |sum Y - 1| warning threshold
                                    (utilities.H:159)                                          */
-    double t0;                  /* start time of the chemistry step, t0 = t - dt of the CFD
-                                   step (Policy_MomentumTransportReaction.H:62)               */
     int32_t constant_volume;    /* !constPressureBatchReactor: constant-volume reactor
                                    (BatchReactorHomogeneousConstantVolume.H:150-199, selected
                                    at chemistry_DI.H:171-208); numerical Jacobian only        */
@@ -186,17 +186,26 @@ int lsm_set_jacobian(lsm_handle *h, int32_t type);
 int lsm_set_refinement(lsm_handle *h, int32_t sweeps);
 
 /* Chemistry step over n_cells cells, host pointers (the whole forAll loop of
- * chemistry_DI.H:92-328).  y_in/y_out: SoA [ns+1][n_cells] (species rows, then T).
- * dt: per-cell integration time (DeltaTCells).  status (may be NULL): OdeStatus per
- * cell.  q (may be NULL): heat release QR [W/m3] of integrated cells; entries of
- * skipped cells are left untouched (QCells keeps its value).  counters (may be
- * NULL): [n_cells][LSM_NCOUNTERS].  Synchronous. */
-int lsm_integrate_batch(lsm_handle *h, int64_t n_cells, const double *dt, const double *y_in,
-                        double *y_out, int32_t *status, double *q, int32_t *counters);
+ * chemistry_DI.H:92-328).
+ *   t0: start time of the step, the t0 = t - dt of the CFD time step
+ *       (Policy_MomentumTransportReaction.H:62) that the loop passes to
+ *       SetInitialConditions(t0, y0) and Solve(t0 + DeltaTCells[celli])
+ *       (chemistry_DI.H:138,157).  It is not a mere offset: it sets the bounds of the
+ *       first step size (MultiValueSolver.hpp:649-685) and the step-too-small test
+ *       t + h == t (:812-820), so stiff cells late in a run can end with status -6.
+ *   pressure: thermodynamicPressure [Pa] of the reactors (SetReactor(P),
+ *       chemistry_DI.H:134), > 0.
+ *   y_in/y_out: SoA [ns+1][n_cells] (species rows, then T).  dt: per-cell integration
+ *   time (DeltaTCells).  status (may be NULL): OdeStatus per cell.  q (may be NULL):
+ *   heat release QR [W/m3] of integrated cells; entries of skipped cells are left
+ *   untouched (QCells keeps its value).  counters (may be NULL):
+ *   [n_cells][LSM_NCOUNTERS].  Synchronous. */
+int lsm_integrate_batch(lsm_handle *h, int64_t n_cells, double t0, double pressure, const double *dt,
+                        const double *y_in, double *y_out, int32_t *status, double *q, int32_t *counters);
 
 /* Same contract on device pointers, enqueued on `stream` (a hipStream_t; NULL =
  * default stream).  y_in and y_out may alias.  Not synchronising. */
-int lsm_integrate_batch_device(lsm_handle *h, int64_t n_cells, const double *d_dt,
+int lsm_integrate_batch_device(lsm_handle *h, int64_t n_cells, double t0, double pressure, const double *d_dt,
                                const double *d_y_in, double *d_y_out, int32_t *d_status,
                                double *d_q, int32_t *d_counters, void *stream);
 
@@ -206,21 +215,23 @@ int lsm_integrate_batch_device(lsm_handle *h, int64_t n_cells, const double *d_d
  * order; the makespan does: expensive cells first shortens the tail.  The plain
  * kernel ignores the order.  No counterpart in the reference, whose cell loop is
  * serial (chemistry_DI.H:92). */
-int lsm_integrate_batch_device_ordered(lsm_handle *h, int64_t n_cells, const double *d_dt,
-                                       const double *d_y_in, double *d_y_out, int32_t *d_status,
-                                       double *d_q, int32_t *d_counters, const int32_t *d_order,
-                                       void *stream);
+int lsm_integrate_batch_device_ordered(lsm_handle *h, int64_t n_cells, double t0, double pressure,
+                                       const double *d_dt, const double *d_y_in, double *d_y_out,
+                                       int32_t *d_status, double *d_q, int32_t *d_counters,
+                                       const int32_t *d_order, void *stream);
 
 /* Constant-volume variant: as lsm_integrate_batch / _device_ordered (d_order may be
  * NULL), with the per-cell density rho0 [kg/m3] of the constant-volume reactor,
  * rhoCells in BatchReactorHomogeneousConstantVolume::SetReactor(V, P, rho)
  * (chemistry_DI.H:173).  rho NULL: the ideal-gas density of each cell's initial state
- * at the reactor pressure.  Constant-pressure handles ignore rho. */
-int lsm_integrate_batch_rho(lsm_handle *h, int64_t n_cells, const double *dt, const double *rho,
-                            const double *y_in, double *y_out, int32_t *status, double *q, int32_t *counters);
-int lsm_integrate_batch_device_rho(lsm_handle *h, int64_t n_cells, const double *d_dt, const double *d_rho,
-                                   const double *d_y_in, double *d_y_out, int32_t *d_status, double *d_q,
-                                   int32_t *d_counters, const int32_t *d_order, void *stream);
+ * at `pressure`.  Constant-pressure handles ignore rho. */
+int lsm_integrate_batch_rho(lsm_handle *h, int64_t n_cells, double t0, double pressure, const double *dt,
+                            const double *rho, const double *y_in, double *y_out, int32_t *status, double *q,
+                            int32_t *counters);
+int lsm_integrate_batch_device_rho(lsm_handle *h, int64_t n_cells, double t0, double pressure,
+                                   const double *d_dt, const double *d_rho, const double *d_y_in,
+                                   double *d_y_out, int32_t *d_status, double *d_q, int32_t *d_counters,
+                                   const int32_t *d_order, void *stream);
 
 /* Visiting order from the per-cell counters of a previous call on the same cells
  * (the cpuChemistry field of chemistry_DI.H:170 plays this role in the reference):
@@ -232,10 +243,11 @@ int lsm_cost_order(lsm_handle *h, int64_t n_cells, const int32_t *d_counters, in
 /* Launch-shape autotuner for the team kernel: times waves per workgroup (the
  * workgroup shares one LDS copy of the mechanism) x workgroups per CU on a strided
  * sample of `sample` cells (0: 65536) of the given device batch, keeps the fastest
- * in the handle.  result (may be NULL): {waves per workgroup, grid, best time in
- * microseconds}.  Synchronous; allocates temporaries.  Other methods: no-op. */
-int lsm_autotune(lsm_handle *h, int64_t n_cells, const double *d_dt, const double *d_y,
-                 int64_t sample, void *stream, int32_t *result);
+ * in the handle.  t0 and pressure: those of the steps to come (lsm_integrate_batch).
+ * result (may be NULL): {waves per workgroup, grid, best time in microseconds}.
+ * Synchronous; allocates temporaries.  Other methods: no-op. */
+int lsm_autotune(lsm_handle *h, int64_t n_cells, double t0, double pressure, const double *d_dt,
+                 const double *d_y, int64_t sample, void *stream, int32_t *result);
 
 /* Launch configuration in use: {method, waves per workgroup, grid, Jacobian type,
  * refinement sweeps}. */

@@ -43,11 +43,13 @@ int lsm_server_destroy(lsm_server *s);
  * case's state alone exceeds the budget. */
 int lsm_server_add_case(lsm_server *s, lsm_handle *h, int64_t n_cells, const double *y0, int32_t *case_id);
 
-/* Queues one chemistry step of every cell of a case over dt; returns its ticket.
+/* Queues one chemistry step of every cell of a case from t0 (the case's time at the
+ * start of the step, t - dt of its CFD step) over dt, at the pressure of the case's
+ * handle (lsm_reactor_options::pressure); returns its ticket.
  * y (may be NULL): new fields for the case (host, SoA [ns+1][n_cells], copied at
  * once), e.g. the state after the CFD solver's transport step; NULL continues
  * from the case's current state. */
-int lsm_server_submit(lsm_server *s, int32_t case_id, double dt, const double *y, uint64_t *ticket);
+int lsm_server_submit(lsm_server *s, int32_t case_id, double t0, double dt, const double *y, uint64_t *ticket);
 
 /* Non-blocking progress: retires finished steps, admits queued ones.  Writes up to
  * `capacity` finished tickets with their latency (submission to observed

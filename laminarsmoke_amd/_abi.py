@@ -61,7 +61,6 @@ class LsmReactorOptions(ctypes.Structure):
         ("min_temperature", ctypes.c_double),
         ("energy_equation", ctypes.c_int32),
         ("mass_fractions_tol", ctypes.c_double),
-        ("t0", ctypes.c_double),
         ("constant_volume", ctypes.c_int32),
         ("strang_compact", ctypes.c_int32),
     ]

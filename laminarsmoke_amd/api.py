@@ -57,24 +57,25 @@ def lib():
         L.lsm_equations.argtypes = [vp, i64, vp, vp, vp]
         L.lsm_equations_device.argtypes = [vp, i64, vp, vp, vp, vp]
         L.lsm_destroy.argtypes = [vp]
-        L.lsm_integrate_batch.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
-        L.lsm_integrate_batch_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
-        L.lsm_integrate_batch_device_ordered.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp]
-        L.lsm_integrate_batch_rho.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        f64 = ctypes.c_double
+        L.lsm_integrate_batch.argtypes = [vp, i64, f64, f64, vp, vp, vp, vp, vp, vp]
+        L.lsm_integrate_batch_device.argtypes = [vp, i64, f64, f64, vp, vp, vp, vp, vp, vp, vp]
+        L.lsm_integrate_batch_device_ordered.argtypes = [vp, i64, f64, f64, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lsm_integrate_batch_rho.argtypes = [vp, i64, f64, f64, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_properties.argtypes = [vp, i64, vp, vp, vp, vp, vp]
         L.lsm_properties_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.lsm_set_transport.argtypes = [vp, vp]
         L.lsm_set_transport_options.argtypes = [vp, vp]
         L.lsm_transport_properties.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.lsm_transport_properties_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
-        L.lsm_integrate_batch_device_rho.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lsm_integrate_batch_device_rho.argtypes = [vp, i64, f64, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_cost_order.argtypes = [vp, i64, vp, vp, vp]
-        L.lsm_autotune.argtypes = [vp, i64, vp, vp, i64, vp, vp]
+        L.lsm_autotune.argtypes = [vp, i64, f64, f64, vp, vp, i64, vp, vp]
         L.lsm_get_launch.argtypes = [vp, vp]
         L.lsm_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int32, ctypes.POINTER(vp)]
         L.lsm_server_destroy.argtypes = [vp]
         L.lsm_server_add_case.argtypes = [vp, vp, i64, vp, ctypes.POINTER(ctypes.c_int32)]
-        L.lsm_server_submit.argtypes = [vp, ctypes.c_int32, ctypes.c_double, vp, ctypes.POINTER(ctypes.c_uint64)]
+        L.lsm_server_submit.argtypes = [vp, ctypes.c_int32, f64, f64, vp, ctypes.POINTER(ctypes.c_uint64)]
         L.lsm_server_poll.argtypes = [vp, vp, vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
         L.lsm_server_drain.argtypes = [vp]
         L.lsm_server_read_state.argtypes = [vp, ctypes.c_int32, vp]
@@ -132,14 +133,15 @@ class ODE_Parameters:
         return self.c.maximum_order
 
 
-def reactor_options(pressure=101325.0, min_temperature=0.0, energy=True, mass_fractions_tol=1e-4, t0=0.0,
+def reactor_options(pressure=101325.0, min_temperature=0.0, energy=True, mass_fractions_tol=1e-4,
                     constant_volume=False, compact=False):
-    """lsm_reactor_options; constant_volume: constant-volume reactor; compact: Strang "compact"
-    output (y_out holds the sources RR and RT instead of the final state)."""
+    """lsm_reactor_options; pressure: the handle's default pressure (chemistry steps take theirs per
+    call); constant_volume: constant-volume reactor; compact: Strang "compact" output (y_out holds the
+    sources RR and RT instead of the final state)."""
     r = LsmReactorOptions()
     _check(lib().lsm_reactor_options_default(ctypes.byref(r)))
     r.pressure, r.min_temperature, r.energy_equation = pressure, min_temperature, int(bool(energy))
-    r.mass_fractions_tol, r.t0 = mass_fractions_tol, t0
+    r.mass_fractions_tol = mass_fractions_tol
     r.constant_volume, r.strang_compact = int(bool(constant_volume)), int(bool(compact))
     return r
 
@@ -193,9 +195,13 @@ class ChemistryStep:
         except Exception:
             pass
 
-    def integrate(self, y, dt, q=None, with_counters=True, rho=None):
-        """Host arrays: y[ns+1, n] -> (y_out, status, q, counters[n, 12]).  rho: per-cell densities of a
-        constant-volume reactor (default: ideal-gas density of the input state)."""
+    def _p(self, pressure):
+        return float(self.reactor.pressure if pressure is None else pressure)
+
+    def integrate(self, y, dt, q=None, with_counters=True, rho=None, t0=0.0, pressure=None):
+        """Host arrays: y[ns+1, n] -> (y_out, status, q, counters[n, 12]).  t0: start time of the step
+        (t - dt of the CFD step); pressure: reactor pressure [Pa] (default: the handle's); rho: per-cell
+        densities of a constant-volume reactor (default: ideal-gas density of the input state)."""
         y = np.ascontiguousarray(y, dtype=np.float64)
         neq, n = y.shape
         if neq != self.mech.neq:
@@ -208,7 +214,8 @@ class ChemistryStep:
         p = lambda a: a.ctypes.data if a is not None else None
         if rho is not None:
             rho = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, dtype=np.float64), (n,)))
-        _check(lib().lsm_integrate_batch_rho(self.h, n, p(dt), p(rho), p(y), p(yout), p(st), p(qa), p(cnt)))
+        _check(lib().lsm_integrate_batch_rho(self.h, n, float(t0), self._p(pressure), p(dt), p(rho), p(y), p(yout),
+                                             p(st), p(qa), p(cnt)))
         return yout, st, qa, cnt
 
     PROPS = ("MW", "cTot", "psi", "h", "cp", "cv")
@@ -263,25 +270,27 @@ class ChemistryStep:
                                                      teta_ptr, stream))
 
     def integrate_device(self, n, dt_ptr, y_in_ptr, y_out_ptr, status_ptr=None, q_ptr=None, counters_ptr=None,
-                         stream=None, order_ptr=None, rho_ptr=None):
+                         stream=None, order_ptr=None, rho_ptr=None, t0=0.0, pressure=None):
         """Device pointers (ints), enqueued on `stream` (int handle or None); `order_ptr`: visiting order;
-        `rho_ptr`: per-cell densities of a constant-volume reactor."""
+        `rho_ptr`: per-cell densities of a constant-volume reactor; t0, pressure as in integrate()."""
+        t0, P = float(t0), self._p(pressure)
         if rho_ptr:
-            _check(lib().lsm_integrate_batch_device_rho(self.h, int(n), dt_ptr, rho_ptr, y_in_ptr, y_out_ptr,
+            _check(lib().lsm_integrate_batch_device_rho(self.h, int(n), t0, P, dt_ptr, rho_ptr, y_in_ptr, y_out_ptr,
                                                         status_ptr, q_ptr, counters_ptr, order_ptr, stream))
         elif order_ptr:
-            _check(lib().lsm_integrate_batch_device_ordered(self.h, int(n), dt_ptr, y_in_ptr, y_out_ptr, status_ptr,
-                                                            q_ptr, counters_ptr, order_ptr, stream))
+            _check(lib().lsm_integrate_batch_device_ordered(self.h, int(n), t0, P, dt_ptr, y_in_ptr, y_out_ptr,
+                                                            status_ptr, q_ptr, counters_ptr, order_ptr, stream))
         else:
-            _check(lib().lsm_integrate_batch_device(self.h, int(n), dt_ptr, y_in_ptr, y_out_ptr, status_ptr, q_ptr,
-                                                    counters_ptr, stream))
+            _check(lib().lsm_integrate_batch_device(self.h, int(n), t0, P, dt_ptr, y_in_ptr, y_out_ptr, status_ptr,
+                                                    q_ptr, counters_ptr, stream))
 
     def cost_order(self, n, counters_ptr, order_ptr, stream=None):
         _check(lib().lsm_cost_order(self.h, int(n), counters_ptr, order_ptr, stream))
 
-    def autotune(self, n, dt_ptr, y_ptr, sample=0, stream=None):
+    def autotune(self, n, dt_ptr, y_ptr, sample=0, stream=None, t0=0.0, pressure=None):
         out = (ctypes.c_int32 * 3)()
-        _check(lib().lsm_autotune(self.h, int(n), dt_ptr, y_ptr, int(sample), stream, out))
+        _check(lib().lsm_autotune(self.h, int(n), float(t0), self._p(pressure), dt_ptr, y_ptr, int(sample), stream,
+                                  out))
         return dict(waves_per_block=out[0], grid=out[1], best_us=out[2])
 
     def launch(self):
@@ -315,14 +324,16 @@ class Server:
         self.neq[cid.value] = y0.shape
         return cid.value
 
-    def submit(self, case_id, dt, y=None):
-        """Queues a chemistry step; y (optional): new fields for the case (host array [ns+1, n])."""
+    def submit(self, case_id, dt, y=None, t0=0.0):
+        """Queues a chemistry step from t0 over dt; y (optional): new fields for the case (host array
+        [ns+1, n])."""
         t = ctypes.c_uint64()
         if y is not None:
             y = np.ascontiguousarray(y, dtype=np.float64)
             if y.shape != self.neq[case_id]:
                 raise ValueError(f"y has shape {y.shape}, case has {self.neq[case_id]}")
-        _check(lib().lsm_server_submit(self.s, int(case_id), float(dt), y.ctypes.data if y is not None else None,
+        _check(lib().lsm_server_submit(self.s, int(case_id), float(t0), float(dt),
+                                       y.ctypes.data if y is not None else None,
                                        ctypes.byref(t)))
         return t.value
 

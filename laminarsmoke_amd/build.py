@@ -15,7 +15,9 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "liblsm_hip.so")
 ARCH = os.environ.get("LSM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["lsm_kernels.hip", "lsm_api.cpp", "lsm_serve.cpp"]
+# (source, extra defines): lsm_kernels.hip is compiled as six units (LSM_PART 1-6), the team
+# kernel's register-array sizes apart from the rest, so that they build in parallel.
+SOURCES = [("lsm_kernels.hip", [f"-DLSM_PART={p}"]) for p in range(1, 7)] + [("lsm_api.cpp", []), ("lsm_serve.cpp", [])]
 HEADERS = ["lsm_bdf.hpp", "lsm_internal.hpp"]
 
 # Packed-FP32 instructions are switched off for every HIP translation unit.
@@ -35,16 +37,19 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False, extra=(), out: str = LIB) -> str:
-    """Compile every source (always; the build check must compile) into `out`."""
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(os.path.dirname(out), os.path.basename(out) + "." + src + ".o")
-        lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
-        cmd = [hipcc(), *HIPCC_FLAGS, *extra, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
+    """Compile every source (always; the build check must compile) into `out`, the translation
+    units in parallel."""
+    objs, procs = [], []
+    for src, defs in SOURCES:
+        obj = os.path.join(os.path.dirname(out), os.path.basename(out) + "." + src + "".join(defs) + ".o")
+        cmd = [hipcc(), *HIPCC_FLAGS, *defs, *extra, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
+        procs.append((subprocess.Popen(cmd), cmd))
         objs.append(obj)
+    failed = [cmd for p, cmd in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = out + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

@@ -253,6 +253,7 @@ size_t lsm_handle_device_bytes(const lsm_handle *h) {
 }
 
 int lsm_handle_neq(const lsm_handle *h) { return h->dm.ne; }
+double lsm_handle_pressure(const lsm_handle *h) { return h->reactor.pressure; }
 
 extern "C" {
 
@@ -318,7 +319,6 @@ int lsm_reactor_options_default(lsm_reactor_options *r) {
     r->min_temperature = 0.;
     r->energy_equation = 1;
     r->mass_fractions_tol = 1.e-4;
-    r->t0 = 0.;
     r->constant_volume = 0;
     r->strang_compact = 0;
     return LSM_OK;
@@ -846,7 +846,7 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
     gear_r(k.bk.r);
     k.bk.abs_tol = p.absolute_tolerance;
     k.bk.rel_tol = p.relative_tolerance;
-    k.bk.t0 = r.t0;
+    k.bk.t0 = 0.;                                // set per call (lsm_integrate_batch*)
     k.bk.max_steps = p.maximum_number_of_steps > 0 ? (unsigned)p.maximum_number_of_steps : 500000u;
     k.pressure = r.pressure;
     k.min_temperature = r.min_temperature;
@@ -949,38 +949,52 @@ static int pick_method(const lsm_handle *h) {
     return h->wide_grid > 0 ? LSM_METHOD_WIDE : LSM_METHOD_PLAIN;
 }
 
-static int integrate_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_rho, const double *d_y_in,
-                            double *d_y_out, int32_t *d_status, double *d_q, int32_t *d_counters,
-                            const int32_t *d_order, void *stream);
+static int integrate_device(lsm_handle *h, int64_t n, double t0, double pressure, const double *d_dt,
+                            const double *d_rho, const double *d_y_in, double *d_y_out, int32_t *d_status,
+                            double *d_q, int32_t *d_counters, const int32_t *d_order, void *stream);
 
-int lsm_integrate_batch_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in, double *d_y_out,
-                               int32_t *d_status, double *d_q, int32_t *d_counters, void *stream) {
-    return integrate_device(h, n, d_dt, nullptr, d_y_in, d_y_out, d_status, d_q, d_counters, nullptr, stream);
+int lsm_integrate_batch_device(lsm_handle *h, int64_t n, double t0, double pressure, const double *d_dt,
+                               const double *d_y_in, double *d_y_out, int32_t *d_status, double *d_q,
+                               int32_t *d_counters, void *stream) {
+    return integrate_device(h, n, t0, pressure, d_dt, nullptr, d_y_in, d_y_out, d_status, d_q, d_counters, nullptr,
+                            stream);
 }
 
-int lsm_integrate_batch_device_ordered(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y_in,
-                                       double *d_y_out, int32_t *d_status, double *d_q, int32_t *d_counters,
-                                       const int32_t *d_order, void *stream) {
+int lsm_integrate_batch_device_ordered(lsm_handle *h, int64_t n, double t0, double pressure, const double *d_dt,
+                                       const double *d_y_in, double *d_y_out, int32_t *d_status, double *d_q,
+                                       int32_t *d_counters, const int32_t *d_order, void *stream) {
     if (!d_order) return fail(LSM_E_INVALID, "null order");
-    return integrate_device(h, n, d_dt, nullptr, d_y_in, d_y_out, d_status, d_q, d_counters, d_order, stream);
+    return integrate_device(h, n, t0, pressure, d_dt, nullptr, d_y_in, d_y_out, d_status, d_q, d_counters, d_order,
+                            stream);
 }
 
-int lsm_integrate_batch_device_rho(lsm_handle *h, int64_t n, const double *d_dt, const double *d_rho,
-                                   const double *d_y_in, double *d_y_out, int32_t *d_status, double *d_q,
-                                   int32_t *d_counters, const int32_t *d_order, void *stream) {
-    return integrate_device(h, n, d_dt, d_rho, d_y_in, d_y_out, d_status, d_q, d_counters, d_order, stream);
+int lsm_integrate_batch_device_rho(lsm_handle *h, int64_t n, double t0, double pressure, const double *d_dt,
+                                   const double *d_rho, const double *d_y_in, double *d_y_out, int32_t *d_status,
+                                   double *d_q, int32_t *d_counters, const int32_t *d_order, void *stream) {
+    return integrate_device(h, n, t0, pressure, d_dt, d_rho, d_y_in, d_y_out, d_status, d_q, d_counters, d_order,
+                            stream);
 }
 
-static int integrate_device(lsm_handle *h, int64_t n, const double *d_dt, const double *d_rho, const double *d_y_in,
-                            double *d_y_out, int32_t *d_status, double *d_q, int32_t *d_counters,
-                            const int32_t *d_order, void *stream) {
+// Per-call start time and pressure of a chemistry step (chemistry_DI.H:134,138,157).
+static int check_step_args(double t0, double pressure) {
+    if (!std::isfinite(t0)) return fail(LSM_E_INVALID, "t0 must be finite");
+    if (!(pressure > 0.) || !std::isfinite(pressure)) return fail(LSM_E_INVALID, "pressure must be positive");
+    return LSM_OK;
+}
+
+static int integrate_device(lsm_handle *h, int64_t n, double t0, double pressure, const double *d_dt,
+                            const double *d_rho, const double *d_y_in, double *d_y_out, int32_t *d_status,
+                            double *d_q, int32_t *d_counters, const int32_t *d_order, void *stream) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
+    if (int rc = check_step_args(t0, pressure)) return rc;
     if (n == 0) return LSM_OK;
     if (!d_dt || !d_y_in || !d_y_out) return fail(LSM_E_INVALID, "null dt / y pointer");
     if (n > (int64_t)0x7fffffff) return fail(LSM_E_INVALID, "more than 2^31-1 cells in one call");
     const int method = pick_method(h);
-    KParams kp = h->kp;                          // per-call: the constant-volume densities
+    KParams kp = h->kp;                          // per call: start time, pressure, constant-volume densities
+    kp.bk.t0 = t0;
+    kp.pressure = pressure;
     kp.rho = h->kp.const_volume ? d_rho : nullptr;
     if (method == LSM_METHOD_TEAM) {
         if (launch_reset_queue(h->d_queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
@@ -1014,15 +1028,16 @@ static int integrate_device(lsm_handle *h, int64_t n, const double *d_dt, const 
     return LSM_OK;
 }
 
-int lsm_integrate_batch(lsm_handle *h, int64_t n, const double *dt, const double *y_in, double *y_out,
-                        int32_t *status, double *q, int32_t *counters) {
-    return lsm_integrate_batch_rho(h, n, dt, nullptr, y_in, y_out, status, q, counters);
+int lsm_integrate_batch(lsm_handle *h, int64_t n, double t0, double pressure, const double *dt, const double *y_in,
+                        double *y_out, int32_t *status, double *q, int32_t *counters) {
+    return lsm_integrate_batch_rho(h, n, t0, pressure, dt, nullptr, y_in, y_out, status, q, counters);
 }
 
-int lsm_integrate_batch_rho(lsm_handle *h, int64_t n, const double *dt, const double *rho, const double *y_in,
-                            double *y_out, int32_t *status, double *q, int32_t *counters) {
+int lsm_integrate_batch_rho(lsm_handle *h, int64_t n, double t0, double pressure, const double *dt, const double *rho,
+                            const double *y_in, double *y_out, int32_t *status, double *q, int32_t *counters) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
+    if (int rc = check_step_args(t0, pressure)) return rc;
     if (n == 0) return LSM_OK;
     if (!dt || !y_in || !y_out) return fail(LSM_E_INVALID, "null dt / y pointer");
     const int ne = h->dm.ne;
@@ -1053,7 +1068,7 @@ int lsm_integrate_batch_rho(lsm_handle *h, int64_t n, const double *dt, const do
             return fail(LSM_E_HIP, "copy rho");
         }
     }
-    int rc = integrate_device(h, n, h->d_dt, d_rho, h->d_y, h->d_y, status ? h->d_status : nullptr,
+    int rc = integrate_device(h, n, t0, pressure, h->d_dt, d_rho, h->d_y, h->d_y, status ? h->d_status : nullptr,
                               q ? h->d_q : nullptr, counters ? h->d_cnt : nullptr, nullptr, nullptr);
     if (rc) { if (d_rho) (void)hipFree(d_rho); return rc; }
     const hipError_t sync = hipDeviceSynchronize();
@@ -1369,10 +1384,14 @@ int lsm_cost_order(lsm_handle *h, int64_t n, const int32_t *d_counters, int32_t 
     return LSM_OK;
 }
 
-int lsm_autotune(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y, int64_t sample, void *stream,
-                 int32_t *result) {
+int lsm_autotune(lsm_handle *h, int64_t n, double t0, double pressure, const double *d_dt, const double *d_y,
+                 int64_t sample, void *stream, int32_t *result) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (n <= 0 || !d_dt || !d_y) return fail(LSM_E_INVALID, "autotune needs cells");
+    if (int rc = check_step_args(t0, pressure)) return rc;
+    KParams kp = h->kp;
+    kp.bk.t0 = t0;
+    kp.pressure = pressure;
     if (pick_method(h) != LSM_METHOD_TEAM) {
         if (result) { result[0] = h->team_wpb; result[1] = h->team_grid; result[2] = -1; }
         return LSM_OK;                                     // only the team kernel has launch shapes to choose
@@ -1413,7 +1432,7 @@ int lsm_autotune(lsm_handle *h, int64_t n, const double *d_dt, const double *d_y
             for (int rep = warmed ? 0 : -1; rep < 1 && !rc; rep++) {   // the first candidate runs twice (warm-up)
                 if (launch_reset_queue(h->d_queue, stream)) { rc = -1; break; }
                 if (hipEventRecord(e0, s) != hipSuccess) { rc = -1; break; }
-                if (launch_team(h->dm, h->kp, m, dts, ys, yo, nullptr, nullptr, nullptr, h->d_queue, nullptr, grid, wpb,
+                if (launch_team(h->dm, kp, m, dts, ys, yo, nullptr, nullptr, nullptr, h->d_queue, nullptr, grid, wpb,
                                 h->d_jws, stream)) { rc = -1; break; }
                 if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = -1; break; }
                 float ms = 0.f;

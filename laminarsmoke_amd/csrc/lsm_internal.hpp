@@ -136,7 +136,7 @@ int team_max_neq();
 // LDS layout of a team workgroup: the mechanism image, then per wave the species
 // scratch (4 x 64 doubles).  Each wave's Jacobian (ne x 64 doubles) lives in an
 // HBM workspace indexed by the global wave number.
-__host__ __device__ size_t team_wave_bytes(const DevMech &m);
+__host__ __device__ inline size_t team_wave_bytes(const DevMech &m) { return (size_t)8 * (320 + 65 * (size_t)m.ne); }
 size_t team_jac_doubles(const DevMech &m);
 int team_waves_per_block(const DevMech &m, size_t lds_per_block);
 // order (may be null): the cells in the order the queue hands them out.
@@ -179,5 +179,7 @@ struct lsm_handle;
 // Device bytes a handle holds (tables, workspaces), for the serving budget.
 size_t lsm_handle_device_bytes(const lsm_handle *h);
 int lsm_handle_neq(const lsm_handle *h);
+// Default reactor pressure of the handle (lsm_reactor_options::pressure).
+double lsm_handle_pressure(const lsm_handle *h);
 // Sets the thread's last-error text and returns code.
 int lsm_fail(int code, const char *msg);

@@ -18,6 +18,15 @@
 #include "../../include/lsm_transport.h"
 #include "lsm_internal.hpp"
 
+// The file is compiled as several translation units in parallel (laminarsmoke_amd/build.py):
+// LSM_PART 1-5 each instantiate the team kernel for one register-array size (NMAX 16, 32,
+// 48, 56, 64), LSM_PART 6 holds every other kernel and the host launchers; LSM_PART 0 (the
+// default) is all of it in one unit.
+#ifndef LSM_PART
+#define LSM_PART 0
+#endif
+#define LSM_IN_PART(p) (LSM_PART == 0 || LSM_PART == (p))
+
 namespace lsm {
 
 constexpr int LSM_NCOUNTERS_DEV = 12;
@@ -2168,14 +2177,13 @@ __global__ void __launch_bounds__(256) plain_kernel(DevMech M, KParams K, int64_
     }
 }
 
+#if LSM_IN_PART(6)
 __global__ void reset_queue_kernel(unsigned *queue) { *queue = 0u; }
 
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 int team_max_neq() { return 64; }
-
-__host__ __device__ size_t team_wave_bytes(const DevMech &m) { return (size_t)8 * (320 + 65 * (size_t)m.ne); }
 
 size_t team_jac_doubles(const DevMech &m) { return (size_t)64 * m.ne; }
 
@@ -2186,6 +2194,8 @@ int team_waves_per_block(const DevMech &m, size_t lds_per_block) {
     const size_t w = (lds_per_block - (size_t)m.img.bytes) / per;
     return (int)(w < 4 ? w : 4);
 }
+
+#endif  // LSM_IN_PART(6)
 
 template <int NMAX, bool XK>
 static int launch_team_n(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
@@ -2204,23 +2214,50 @@ static int launch_team_n(const DevMech &m, const KParams &k, int64_t n, const do
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// One launcher per register-array size, each in its own translation unit.
+#define LSM_TEAM_LAUNCHER(NM)                                                                                       \
+    int launch_team_##NM(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,        \
+                         double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order,    \
+                         int grid, int wpb, double *jws, hipStream_t s)
+#define LSM_TEAM_LAUNCHER_BODY(NM)                                                                                  \
+    LSM_TEAM_LAUNCHER(NM) {                                                                                         \
+        return m.xk ? launch_team_n<NM, true>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, \
+                                              jws, s)                                                               \
+                    : launch_team_n<NM, false>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid,    \
+                                               wpb, jws, s);                                                        \
+    }
+LSM_TEAM_LAUNCHER(16);
+LSM_TEAM_LAUNCHER(32);
+LSM_TEAM_LAUNCHER(48);
+LSM_TEAM_LAUNCHER(56);
+LSM_TEAM_LAUNCHER(64);
+#if LSM_IN_PART(1)
+LSM_TEAM_LAUNCHER_BODY(16)
+#endif
+#if LSM_IN_PART(2)
+LSM_TEAM_LAUNCHER_BODY(32)
+#endif
+#if LSM_IN_PART(3)
+LSM_TEAM_LAUNCHER_BODY(48)
+#endif
+#if LSM_IN_PART(4)
+LSM_TEAM_LAUNCHER_BODY(56)
+#endif
+#if LSM_IN_PART(5)
+LSM_TEAM_LAUNCHER_BODY(64)
+#endif
+
+#if LSM_IN_PART(6)
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
                 int *status, double *q, int *counters, unsigned *queue, const int *order, int grid, int wpb,
                 double *jws, void *stream) {
     hipStream_t s = (hipStream_t)stream;
     if (wpb < 1 || wpb > 4) return -2;
-#define LSM_TEAM_CASE(NM)                                                                                           \
-    if (m.ne <= NM)                                                                                                 \
-        return m.xk ? launch_team_n<NM, true>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, \
-                                              jws, s)                                                               \
-                    : launch_team_n<NM, false>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid,    \
-                                               wpb, jws, s);
-    LSM_TEAM_CASE(16)
-    LSM_TEAM_CASE(32)
-    LSM_TEAM_CASE(48)
-    LSM_TEAM_CASE(56)
-    LSM_TEAM_CASE(64)
-#undef LSM_TEAM_CASE
+    if (m.ne <= 16) return launch_team_16(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, jws, s);
+    if (m.ne <= 32) return launch_team_32(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, jws, s);
+    if (m.ne <= 48) return launch_team_48(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, jws, s);
+    if (m.ne <= 56) return launch_team_56(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, jws, s);
+    if (m.ne <= 64) return launch_team_64(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, jws, s);
     return -2;
 }
 
@@ -2584,5 +2621,6 @@ int launch_reset_queue(unsigned *queue, void *stream) {
     hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(1), 0, (hipStream_t)stream, queue);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+#endif  // LSM_IN_PART(6)
 
 }  // namespace lsm

@@ -18,7 +18,7 @@ double now_s() {
 
 struct Pending {
     uint64_t ticket;
-    double dt;
+    double t0, dt;
     double t_submit;
     std::vector<double> y;       // new fields shipped with the request (empty: continue from the state)
 };
@@ -136,10 +136,12 @@ int launch_step(lsm_server *s, Case &c) {
     }
     if (lsm::launch_fill(c.d_dt, c.n, c.running.dt, c.stream)) return lsm_fail(LSM_E_HIP, "dt fill failed");
     if (hipEventRecord(c.ev0, c.stream) != hipSuccess) return lsm_fail(LSM_E_HIP, "event record failed");
+    const double P = lsm_handle_pressure(c.h);
     int rc = c.have_order
-                 ? lsm_integrate_batch_device_ordered(c.h, c.n, c.d_dt, c.d_y, c.d_y, c.d_status, nullptr, c.d_cnt,
-                                                      c.d_order, c.stream)
-                 : lsm_integrate_batch_device(c.h, c.n, c.d_dt, c.d_y, c.d_y, c.d_status, nullptr, c.d_cnt, c.stream);
+                 ? lsm_integrate_batch_device_ordered(c.h, c.n, c.running.t0, P, c.d_dt, c.d_y, c.d_y, c.d_status,
+                                                      nullptr, c.d_cnt, c.d_order, c.stream)
+                 : lsm_integrate_batch_device(c.h, c.n, c.running.t0, P, c.d_dt, c.d_y, c.d_y, c.d_status, nullptr,
+                                              c.d_cnt, c.stream);
     if (rc) return rc;
     // visiting order of the next step: this step's per-cell cost
     rc = lsm_cost_order(c.h, c.n, c.d_cnt, c.d_order, c.stream);
@@ -249,13 +251,14 @@ int lsm_server_add_case(lsm_server *s, lsm_handle *h, int64_t n_cells, const dou
     return LSM_OK;
 }
 
-int lsm_server_submit(lsm_server *s, int32_t case_id, double dt, const double *y, uint64_t *ticket) {
+int lsm_server_submit(lsm_server *s, int32_t case_id, double t0, double dt, const double *y, uint64_t *ticket) {
     if (!s || !ticket) return lsm_fail(LSM_E_INVALID, "null argument");
     if (case_id < 0 || case_id >= (int32_t)s->cases.size()) return lsm_fail(LSM_E_INVALID, "unknown case");
     if (!(dt > 0.)) return lsm_fail(LSM_E_INVALID, "dt must be positive");
+    if (!(t0 == t0) || t0 - t0 != 0.) return lsm_fail(LSM_E_INVALID, "t0 must be finite");
     Case &c = s->cases[case_id];
     *ticket = s->next_ticket++;
-    Pending p{*ticket, dt, now_s(), {}};
+    Pending p{*ticket, t0, dt, now_s(), {}};
     if (y) p.y.assign(y, y + (size_t)c.ne * c.n);
     c.queue.push_back(std::move(p));
     return admit(s);

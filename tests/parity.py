@@ -11,10 +11,22 @@ envelope, so the per-cell bound used here is
     max_k |y_gpu - y_ref| / (rtol*|y_ref| + atol)  <=  max(10, 3 * envelope(cell))
 
 where envelope(cell) is the largest deviation of the oracle from itself over
-three runs on inputs perturbed by 1e-15 (relative).  On cells where the oracle
-is stable (envelope < 10/3) this is exactly the 10x target bound.
+five runs on inputs perturbed by 1e-15 (relative; three draws under-estimate it on
+chaotic cells: 2.3 against 16.4 times the tolerance on one 14-species cell).  On
+cells where the oracle is stable (envelope < 10/3) this is exactly the 10x target
+bound.
+
+Element mass: the BDF iteration conserves linear invariants exactly, but the
+reference's yMin = 0 clamps of predicted and Newton states add mass at the atol
+scale.  Cells without clamp events are held to 1e-12; cells with clamp events to
+1e-12 x (1 + events): on 64 seeded 53-species flame cells, all of which clamp, the
+oracle itself drifts by up to 8.9e-11 (3.0e-13 per event).
 """
+import os
+
 import numpy as np
+
+ORACLE_THREADS = min(16, os.cpu_count() or 1)
 
 FACTOR = 10.0
 ENVELOPE_SLACK = 3.0
@@ -24,8 +36,9 @@ def ratio(y, y_ref, rtol, atol):
     return np.abs(y - y_ref) / (rtol * np.abs(y_ref) + atol)
 
 
-def oracle_envelope(oracle, mech, y, dt, rtol, atol, y_ref, samples=3, eps=1e-15, threads=8, **kw):
+def oracle_envelope(oracle, mech, y, dt, rtol, atol, y_ref, samples=5, eps=1e-15, threads=None, **kw):
     """Per-cell max deviation of the oracle from itself under eps-relative input noise."""
+    threads = threads or ORACLE_THREADS
     rng = np.random.default_rng(12345)
     env = np.zeros(y.shape[1])
     for _ in range(samples):

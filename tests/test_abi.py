@@ -121,7 +121,7 @@ def test_parse_errors_are_reported_not_fatal(bad):
 def test_null_arguments_rejected():
     L = api.lib()
     assert L.lsm_ode_parameters_default(None) == -1
-    assert L.lsm_integrate_batch(None, 1, None, None, None, None, None, None) == -1
+    assert L.lsm_integrate_batch(None, 1, 0.0, 101325.0, None, None, None, None, None, None) == -1
     assert L.lsm_destroy(None) == 0
     out = ctypes.c_void_p()
     assert L.lsm_create(None, None, None, ctypes.byref(out)) == -1

@@ -1,0 +1,195 @@
+"""GPU: the semantics of the per-cell loop of chemistry_DI.H:92-328 through the C-ABI.
+
+* the start time t0 of the step (t0 = t - dt of the CFD step,
+  Policy_MomentumTransportReaction.H:62), passed per call as the loop passes it to
+  SetInitialConditions(t0, y0) / Solve(t0 + DeltaTCells[celli]) (chemistry_DI.H:138,157).
+  It bounds the first step size (MultiValueSolver.hpp:649-685) and decides the
+  step-too-small status -6 (t + h == t, :812-820): on the seeded 53-species flame
+  cells the oracle returns -6 on 22 of 48 cells at t0 = 1e-9 s and on all of them
+  from t0 = 1e-6 s on, while the 14-species cells integrate at every t0;
+* the temperature gate (chemistry_DI.H:97): cells at or below minTemperature4Chemistry
+  keep their (renormalised) input and their Q;
+* the failure statuses of OdeSolverUtilities.h:83-99 that a caller can force: -9 (an
+  initial state outside yMin/yMax, MethodGear.hpp:539-569), -1 (the step limit), -2
+  (tolerances so tight that EPS * ||y||_wrms > 1, MultiValueSolver.hpp:545-550); a
+  failed cell returns its clipped, renormalised input and leaves Q untouched;
+* the heat release Q (HeatRelease, KineticsMap_CHEMKIN.hpp:1961-1964) of every
+  integrated cell equals the reactor's QR at that cell's own final state.
+
+Each case runs on the team, plain and wide kernels against the oracle on the same cells;
+statuses must be identical and final states within tests/parity.py's bound.
+"""
+import numpy as np
+import pytest
+
+from laminarsmoke_amd import api
+from laminarsmoke_amd.fields import workload_cells, coflow_field
+from laminarsmoke_amd.mechanism import workload_mechanism
+from parity import assert_parity, assert_element_conservation, oracle_envelope
+
+pytestmark = pytest.mark.gpu
+
+RTOL, ATOL, DT = 1e-7, 1e-12, 1e-5
+
+
+def make_step(mech, method, rtol=RTOL, atol=ATOL, max_steps=None, **kw):
+    ode = api.ODE_Parameters()
+    ode.SetRelativeTolerance(rtol)
+    ode.SetAbsoluteTolerance(atol)
+    if max_steps:
+        ode.SetMaximumNumberOfSteps(max_steps)
+    jac = "numerical"
+    if method.endswith("-analytic"):
+        method, jac = method[:-len("-analytic")], "analytic"
+    return api.ChemistryStep(mech, ode, method=method, jacobian=jac, **kw)
+
+
+def run_gpu(mech, y, method, dt=DT, t0=0.0, q=None, **kw):
+    step = make_step(mech, method, **kw)
+    try:
+        return step.integrate(y, dt, q=q, t0=t0)
+    finally:
+        step.close()
+
+
+def check_q(oracle, mech, yg, st, q_g, q_in):
+    """Q of integrated cells = QR at the cell's final state; Q of the other cells untouched.  The
+    kernel takes QR before the renormalisation, which moves Y by a few ulp: where forward and
+    reverse rates nearly cancel that moves QR by up to ~5e-8 relative (the oracle against itself
+    on the 53-species cells), hence 1e-6."""
+    for c in range(yg.shape[1]):
+        if st[c] > 0:
+            _, qr = oracle.rhs(mech, yg[:, c])
+            assert abs(q_g[c] - qr) <= 1e-6 * abs(qr) + 1e-300, (c, q_g[c], qr)
+        else:
+            assert q_g[c] == q_in[c], (c, q_g[c], q_in[c])
+
+
+def parity_at(oracle, mech, y, method, t0, what):
+    n = y.shape[1]
+    q_in = np.full(n, -12345.0)
+    yo, st_o, _, c_o = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=8, t0=t0)
+    env = oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, yo, samples=5, t0=t0)
+    yg, st_g, q_g, c_g = run_gpu(mech, y, method, t0=t0, q=q_in)
+    assert np.array_equal(st_g, st_o), (what, st_g, st_o)
+    cell, frac = assert_parity(yg, yo, RTOL, ATOL, env, what=what)
+    assert_element_conservation(mech, y, yg, c_g[:, 10], what=what)
+    np.testing.assert_allclose(yg[:mech.ns].sum(0), 1.0, atol=1e-14)
+    check_q(oracle, mech, yg, st_g, q_g, q_in)
+    fail = st_o <= 0
+    if fail.any():   # failed cells: the clipped, renormalised input, as the oracle returns it
+        np.testing.assert_allclose(yg[:, fail], yo[:, fail], rtol=1e-14, atol=1e-300)
+    print(f"{what}: statuses {dict(zip(*np.unique(st_g, return_counts=True)))}; max ratio {cell.max():.3g} "
+          f"(envelope max {env.max():.3g}); steps gpu/oracle median {np.median(c_g[:, 0])}/{np.median(c_o[:, 0])}",
+          flush=True)
+
+
+METHODS = ["team", "plain", "wide", "team-analytic", "wide-analytic"]
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("t0", [1e-3, 0.1])
+def test_cfg1_start_time(hip_device, oracle, method, t0):
+    mech, y = workload_cells(1, n=64)
+    print(f"cfg1 t0={t0} method={method} cells={y.shape[1]}", flush=True)
+    parity_at(oracle, mech, y, method, t0, f"cfg1/{method}/t0={t0}")
+
+
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic"])
+@pytest.mark.parametrize("t0", [1e-9, 1e-3])
+def test_cfg2_start_time(hip_device, oracle, method, t0):
+    """Flame cells fail with -6 once t0 is large against the steps they need; cold cells do not."""
+    mech = workload_mechanism(53)
+    yfull, flame = coflow_field(mech, 512, 512, 22)
+    rng = np.random.default_rng(7)
+    idx = np.sort(np.concatenate([rng.choice(np.where(flame)[0], 24, replace=False),
+                                  rng.choice(np.where(~flame)[0], 8, replace=False)]))
+    y = np.ascontiguousarray(yfull[:, idx])
+    print(f"cfg2 t0={t0} method={method} cells={y.shape[1]}", flush=True)
+    parity_at(oracle, mech, y, method, t0, f"cfg2/{method}/t0={t0}")
+
+
+def gate_cells():
+    """14-species cells: 8 below the gate, 4 at T = 7000 K (> yMax 6000 K: -9), 4 with
+    negative mass fractions (clipped), the rest ordinary."""
+    mech, y = workload_cells(1, n=40)
+    y = y.copy()
+    y[-1, 0:8] = np.linspace(700.0, 1000.0, 8)          # T <= 1000: skipped
+    y[-1, 8:12] = 7000.0
+    y[0, 12:16] = -1e-6
+    y[3, 12:16] = -2e-9
+    return mech, y
+
+
+@pytest.mark.parametrize("method", ["team", "plain", "wide"])
+def test_gate_skip_and_illegal_constraints(hip_device, oracle, method):
+    mech, y = gate_cells()
+    n = y.shape[1]
+    q_in = np.linspace(1.0, 2.0, n)
+    print(f"gate/-9 method={method} cells={n}", flush=True)
+    yo, st_o, q_o, _ = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=8, min_temperature=1000.0,
+                                             q=q_in)
+    yg, st_g, q_g, c_g = run_gpu(mech, y, method, q=q_in, min_temperature=1000.0)
+    assert (st_o[:8] == 0).all() and (st_o[8:12] == -9).all() and (st_o[12:16] == 2).all(), st_o
+    assert np.array_equal(st_g, st_o), (st_g, st_o)
+    # skipped cells: input renormalised (normalizeMassFractions), T kept, no counters, Q kept
+    np.testing.assert_allclose(yg[:, :12], yo[:, :12], rtol=1e-14, atol=1e-300)
+    assert (c_g[:8] == 0).all()
+    check_q(oracle, mech, yg, st_g, q_g, q_in)
+    env = oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, yo, samples=5, min_temperature=1000.0)
+    assert_parity(yg, yo, RTOL, ATOL, env, what=f"gate/{method}")
+    assert (yg[:mech.ns] >= 0).all()
+
+
+@pytest.mark.parametrize("method", ["team", "plain", "wide"])
+def test_step_limit_status(hip_device, oracle, method):
+    """@MaximumNumberOfSteps 5: cells that need more steps stop with -1 and return their input."""
+    mech, y = workload_cells(1, n=32)
+    print(f"-1 method={method}", flush=True)
+    yo, st_o, _, c_o = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=8, max_steps=5)
+    yg, st_g, q_g, c_g = run_gpu(mech, y, method, max_steps=5, q=np.zeros(32))
+    assert (st_o == -1).any(), st_o
+    assert np.array_equal(st_g, st_o), (st_g, st_o)
+    f = st_o == -1
+    np.testing.assert_allclose(yg[:, f], yo[:, f], rtol=1e-14, atol=1e-300)
+    assert (c_g[f, 0] == 5).all() and (c_o[f, 0] == 5).all()
+
+
+@pytest.mark.parametrize("method", ["team", "plain", "wide"])
+def test_too_strict_tolerance_status(hip_device, oracle, method):
+    """rtol 1e-17: EPS * ||y||_wrms > 1 at the first step, status -2, input returned."""
+    mech, y = workload_cells(1, n=16)
+    print(f"-2 method={method}", flush=True)
+    yo, st_o, _, _ = oracle.chemistry_step(mech, y, DT, rtol=1e-17, atol=1e-20, threads=8)
+    yg, st_g, _, _ = run_gpu(mech, y, method, rtol=1e-17, atol=1e-20)
+    assert (st_o == -2).all(), st_o
+    assert np.array_equal(st_g, st_o)
+    np.testing.assert_allclose(yg, yo, rtol=1e-14, atol=1e-300)
+
+
+def test_step_arguments_rejected(hip_device):
+    mech, y = workload_cells(1, n=4)
+    step = make_step(mech, "team")
+    try:
+        for kw in (dict(pressure=0.0), dict(pressure=-1.0), dict(t0=float("nan")), dict(t0=float("inf"))):
+            with pytest.raises(api.LsmError):
+                step.integrate(y, DT, **kw)
+    finally:
+        step.close()
+
+
+def test_pressure_per_call(hip_device, oracle):
+    """The pressure of the call, not the handle's default, is the reactor pressure (chemistry_DI.H:134)."""
+    mech, y = workload_cells(1, n=32)
+    print("pressure per call", flush=True)
+    yo, st_o, _, _ = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=8, pressure=5 * 101325.0)
+    step = make_step(mech, "team")
+    try:
+        yg, st_g, _, _ = step.integrate(y, DT, pressure=5 * 101325.0)
+        y1, *_ = step.integrate(y, DT)
+    finally:
+        step.close()
+    assert np.array_equal(st_g, st_o)
+    env = oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, yo, samples=5, pressure=5 * 101325.0)
+    assert_parity(yg, yo, RTOL, ATOL, env, what="pressure 5 atm")
+    assert not np.array_equal(y1, yg)

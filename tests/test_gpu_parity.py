@@ -10,7 +10,7 @@ import pytest
 from laminarsmoke_amd import api
 from laminarsmoke_amd.fields import workload_cells, coflow_field
 from laminarsmoke_amd.mechanism import workload_mechanism
-from parity import assert_parity, assert_element_conservation, oracle_envelope
+from parity import assert_parity, assert_element_conservation, oracle_envelope, ORACLE_THREADS
 
 pytestmark = pytest.mark.gpu
 
@@ -31,9 +31,12 @@ def gpu_step(mech, y, dt, method, rtol=RTOL, atol=ATOL, **kw):
         step.close()
 
 
-def _check(oracle, mech, y, method, what):
-    yo, st_o, q_o, c_o = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=8)
-    env = oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, yo)
+def _check(oracle, mech, y, method, what, ref=None, env=None):
+    if ref is None:
+        ref = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=ORACLE_THREADS)
+    yo, st_o, q_o, c_o = ref
+    if env is None:
+        env = oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, yo)
     yg, st_g, q_g, c_g = gpu_step(mech, y, DT, method)
     assert np.array_equal(st_g, st_o), (st_g, st_o)
     cell, frac = assert_parity(yg, yo, RTOL, ATOL, env, what=what)
@@ -45,22 +48,44 @@ def _check(oracle, mech, y, method, what):
     return yg, yo, q_g, q_o
 
 
-# The wide kernel with the analytic Jacobian (FP32 dR/dc and FP32 Newton LU) is held
-# to the tight-solution criterion instead (tests/test_gpu_jacobian.py).
-@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic"])
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic", "wide-analytic"])
 def test_cfg1_parity(hip_device, oracle, method):
     mech, y = workload_cells(1, n=64)
     print(f"cfg1 parity method={method} cells={y.shape[1]}", flush=True)
     yg, yo, q_g, q_o = _check(oracle, mech, y, method, f"cfg1/{method}")
-    # heat release of the final state, same cells
-    np.testing.assert_allclose(q_g, q_o, rtol=1e-3)
+    # heat release: QR at each cell's own final state (the states differ within the bound above)
+    for c in range(y.shape[1]):
+        assert abs(q_g[c] - oracle.rhs(mech, yg[:, c])[1]) <= 1e-6 * abs(q_g[c])
 
 
-@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic"])
+_CFG2 = {}
+
+
+def cfg2_sample(oracle):
+    """64 flame cells and 16 cold cells drawn at random (seed 5) from the 512 x 512 field, with the
+    oracle's result and 5-draw self-envelope (computed once per session)."""
+    if not _CFG2:
+        mech = workload_mechanism(53)
+        yfull, flame = coflow_field(mech, 512, 512, 22)
+        rng = np.random.default_rng(5)
+        idx = np.sort(np.concatenate([rng.choice(np.where(flame)[0], 64, replace=False),
+                                      rng.choice(np.where(~flame)[0], 16, replace=False)]))
+        y = np.ascontiguousarray(yfull[:, idx])
+        yo, st_o, q_o, c_o = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=ORACLE_THREADS)
+        env = oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, yo)
+        _CFG2.update(mech=mech, y=y, ref=(yo, st_o, q_o, c_o), env=env)
+    return _CFG2
+
+
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic", "wide-analytic"])
 def test_cfg2_sample_parity(hip_device, oracle, method):
-    mech = workload_mechanism(53)
-    yfull, flame = coflow_field(mech, 512, 512, 22)
-    idx = np.concatenate([np.where(flame)[0][:6], np.where(~flame)[0][:10]])
-    y = np.ascontiguousarray(yfull[:, idx])
+    d = cfg2_sample(oracle)
+    y, ref, env = d["y"], d["ref"], d["env"]
+    if method == "plain":
+        # one thread per cell with its state in HBM: a few cells take minutes, so the plain
+        # kernel gets the cells the oracle integrates in at most 500 steps
+        sel = np.where(ref[3][:, 0] <= 500)[0][:24]
+        yo, st_o, q_o, c_o = ref
+        y, ref, env = np.ascontiguousarray(y[:, sel]), (yo[:, sel], st_o[sel], q_o[sel], c_o[sel]), env[sel]
     print(f"cfg2 parity method={method} cells={y.shape[1]}", flush=True)
-    _check(oracle, mech, y, method, f"cfg2/{method}")
+    _check(oracle, d["mech"], y, method, f"cfg2/{method}", ref=ref, env=env)
