@@ -180,10 +180,18 @@ typedef enum lsm_jacobian_type {
 } lsm_jacobian_type;
 int lsm_set_jacobian(lsm_handle *h, int32_t type);
 
-/* Iterative-refinement sweeps after each FP32 Newton solve of the wide kernel
- * (0-4, default 1): x += LU^-1 (b - G x) with the residual in FP64.  The team and
- * plain kernels factorise in FP64 and ignore it. */
+/* Most iterative-refinement sweeps after each FP32 Newton solve of the wide kernel
+ * (0-8, default 4): x += LU^-1 (b - G x) with the residual in FP64 from the FP64
+ * Jacobian, until the correction's WRMS norm in the integration weights is below 1e-5
+ * (of the tolerance).  A factorisation whose solves do not get there (a sweep that
+ * does not halve the correction, or no sweeps left) is redone in FP64 for this and the
+ * following solves.  0: the FP32 solve alone, unchecked.  The team and plain kernels
+ * factorise in FP64 and ignore it. */
 int lsm_set_refinement(lsm_handle *h, int32_t sweeps);
+
+/* Number of wide-kernel factorisations redone in FP64 since lsm_create (synchronises the
+ * device).  No counterpart in the reference, which factorises in FP64 only. */
+int lsm_fp64_fallbacks(lsm_handle *h, uint64_t *count);
 
 /* Chemistry step over n_cells cells, host pointers (the whole forAll loop of
  * chemistry_DI.H:92-328).

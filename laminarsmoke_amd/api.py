@@ -54,6 +54,7 @@ def lib():
         L.lsm_set_method.argtypes = [vp, ctypes.c_int32]
         L.lsm_set_jacobian.argtypes = [vp, ctypes.c_int32]
         L.lsm_set_refinement.argtypes = [vp, ctypes.c_int32]
+        L.lsm_fp64_fallbacks.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
         L.lsm_equations.argtypes = [vp, i64, vp, vp, vp]
         L.lsm_equations_device.argtypes = [vp, i64, vp, vp, vp, vp]
         L.lsm_destroy.argtypes = [vp]
@@ -169,6 +170,12 @@ class ChemistryStep:
 
     def set_refinement(self, sweeps):
         _check(lib().lsm_set_refinement(self.h, int(sweeps)))
+
+    def fp64_fallbacks(self):
+        """Wide-kernel factorisations redone in FP64 since creation."""
+        v = ctypes.c_uint64()
+        _check(lib().lsm_fp64_fallbacks(self.h, ctypes.byref(v)))
+        return v.value
 
     def set_jacobian(self, kind):
         _check(lib().lsm_set_jacobian(self.h, JACOBIANS[kind] if isinstance(kind, str) else int(kind)))

@@ -225,10 +225,11 @@ struct lsm_handle {
     int64_t cap = 0;
     int team_grid = 0;
     int team_wpb = 0;
-    double *d_jws = nullptr;
+    size_t lds_cu = 0;          // LDS bytes per CU
     int wide_grid = 0;
-    double *d_wjws = nullptr;   // wide kernel Jacobians, [wide_grid][ne][ne]
-    int team_max_waves = 0;     // waves the team Jacobian workspace is sized for
+    double *d_wjws = nullptr;   // wide kernel workspaces, [wide_grid][2][ne][ne]: Jacobian, FP64 LU
+    unsigned long long *d_fallbacks = nullptr;   // wide kernel: factorisations redone in FP64
+    int team_max_waves = 0;     // waves per CU x CUs the team kernel's register cap allows
     size_t tables_bytes = 0;
     // cost-order sort buffers
     unsigned *d_keys = nullptr, *d_keys_out = nullptr;
@@ -244,9 +245,26 @@ struct lsm_handle {
 
 int lsm_fail(int code, const char *msg) { return fail(code, msg); }
 
+// Default team launch shape: the most waves per CU within the register cap and the LDS, in
+// the fewest workgroups (each copies the mechanism image); false when the team kernel cannot
+// run this mechanism.
+static bool team_shape_default(lsm_handle *h) {
+    const int cap = team_waves_per_cu(h->dm);
+    int best_w = 0, best_wpb = 0, best_bpc = 0;
+    for (int wpb = 4; wpb >= 1; wpb--)
+        for (int bpc = 1; wpb * bpc <= cap; bpc++) {
+            if (!team_shape_fits(h->dm, wpb, bpc, h->lds_cu)) continue;
+            if (wpb * bpc > best_w) { best_w = wpb * bpc; best_wpb = wpb; best_bpc = bpc; }
+        }
+    if (best_w == 0) return false;
+    h->team_wpb = best_wpb;
+    h->team_grid = best_bpc * h->num_cu;
+    h->team_max_waves = cap * h->num_cu;
+    return true;
+}
+
 size_t lsm_handle_device_bytes(const lsm_handle *h) {
     size_t b = h->tables_bytes + 256;
-    b += sizeof(double) * team_jac_doubles(h->dm) * (size_t)h->team_max_waves;
     b += sizeof(double) * wide_jac_doubles(h->dm) * (size_t)h->wide_grid;
     b += h->d_work ? (size_t)h->work_slots * plain_work_doubles(h->dm, 1) * sizeof(double) : 0;
     return b;
@@ -480,6 +498,27 @@ static void cheb_block(const lsm_mechanism *m, int s, std::vector<double> &out) 
     out.push_back(std::log10(L[2])); out.push_back(std::log10(L[3])); out.push_back(L[4]);
     for (int q = m->ch_ptr[s]; q < m->ch_ptr[s] + N * M; q++) out.push_back(m->ch_a[q]);
 }
+// A reaction the team kernel's paired loop can take: no collider, no tabulated or explicit reverse
+// rate, no explicit orders, integer coefficients 1-3 with at most three species (counted with
+// multiplicity) on each side.  It is then a fixed-format record of the image (build_image).
+static bool smooth_reverse(const lsm_mechanism *m, int j);
+static bool simple_record(const lsm_mechanism *m, int j, const std::vector<int> &tb_of, const std::vector<int> &fo_of) {
+    if (tb_of[j] >= 0 || fo_of[j] >= 0 || plog_slot_of(m, j) >= 0 || cheb_slot_of(m, j) >= 0) return false;
+    if ((m->rflags[j] & LSM_RXN_EXPLICIT_REV) || explicit_orders(m, j) || smooth_forward(m, j) || smooth_reverse(m, j))
+        return false;
+    for (int side = 0; side < 2; side++) {
+        const int *ptr = side ? m->prod_ptr : m->reac_ptr;
+        const double *nu = side ? m->prod_nu : m->reac_nu;
+        double mult = 0.;
+        for (int q = ptr[j]; q < ptr[j + 1]; q++) {
+            if (nu[q] != std::floor(nu[q]) || nu[q] < 1. || nu[q] > 3.) return false;
+            mult += nu[q];
+        }
+        if (mult > 3.) return false;
+    }
+    return true;
+}
+
 static bool smooth_reverse(const lsm_mechanism *m, int j) {
     if (!(m->rflags[j] & LSM_RXN_REVERSIBLE)) return false;
     for (int q = m->prod_ptr[j]; q < m->prod_ptr[j + 1]; q++) if (m->prod_nu[q] < 1.) return true;
@@ -660,12 +699,18 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     for (int s = 0; s < m->ntb; s++) tb_of[m->tb_rxn[s]] = s;
     for (int s = 0; s < m->nfo; s++) fo_of[m->fo_rxn[s]] = s;
     std::vector<int> by_class, perm, inv(nr);
-    for (int pass = 2; pass >= 0; pass--)
+    // The team layout (ns + 1 <= 64) puts the simple records (simple_record) last: its kernel
+    // takes them two per lane in a loop of their own.
+    const bool team_layout = ns + 1 <= 64;
+    int n_simple = 0;
+    for (int pass = 3; pass >= 0; pass--)
         for (int j = 0; j < nr; j++) {
-            const int cls = fo_of[j] >= 0 ? 2 : (tb_of[j] >= 0 ? 1 : 0);
+            const bool simple = team_layout && simple_record(m, j, tb_of, fo_of);
+            const int cls = fo_of[j] >= 0 ? 3 : (tb_of[j] >= 0 ? 2 : (simple ? 0 : 1));
             if (cls == pass) by_class.push_back(j);
+            if (cls == pass && pass == 0) n_simple++;
         }
-    if (ns + 1 <= 64) {
+    if (team_layout) {
         perm = by_class;
     } else {
         std::vector<std::pair<int64_t, int>> slot(nr);   // (loop position, reaction)
@@ -789,6 +834,7 @@ static int upload_tables(lsm_handle *h, const lsm_mechanism *m) {
     size_t o_rch = b.add(rch), o_chptr = b.add(ch_ptr), o_chdat = b.add(ch_dat);
     std::vector<char> image;
     const bool img_ok = build_image(m, perm, tb_of, fo_of, d.ns_pad, d.img, image);
+    d.img.simple0 = nr - n_simple;
     size_t o_img = img_ok ? b.add(image) : 0;
     HIP_OK(hipMalloc(&h->d_tables, b.bytes.size()));
     h->tables_bytes = b.bytes.size();
@@ -853,11 +899,12 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
     k.energy = r.energy_equation ? 1 : 0;
     k.full_pivoting = p.full_pivoting ? 1 : 0;
     k.jac_mode = LSM_JACOBIAN_NUMERICAL;
-    k.refine = 1;
+    k.refine = 4;
     k.const_volume = r.constant_volume ? 1 : 0;
     k.compact = r.strang_compact ? 1 : 0;
     k.rho = nullptr;
     k.phase_cycles = nullptr;
+    k.fp64_fallbacks = nullptr;
 #ifdef LSM_PHASE_TIMERS
     if (hipMalloc(&k.phase_cycles, 12 * sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(k.phase_cycles, 0, 12 * sizeof(unsigned long long)) != hipSuccess) {
@@ -865,28 +912,23 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode, const l
         return fail(LSM_E_HIP, "phase timer buffer");
     }
 #endif
-    // persistent team grid: one cell per wave, up to 4 waves per block sharing the
-    // LDS copy of the mechanism; enough blocks to fill every CU (one wave per SIMD
-    // at this kernel's register footprint), capped by LDS capacity.
+    // persistent team grid: one cell per wave, the waves of a workgroup sharing the LDS copy
+    // of the mechanism; as many waves per CU as the kernel's register cap allows (its LU rows
+    // live in registers), in the workgroup shape that leaves the widest Jacobian tile.
     const size_t lds_block = std::max<size_t>(prop.sharedMemPerBlock, 65536);
     const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? prop.maxSharedMemoryPerMultiProcessor : lds_block;
-    h->team_wpb = team_waves_per_block(h->dm, std::min(lds_block, lds_cu));
-    if (h->team_wpb > 0) {
-        const size_t lds = (size_t)h->dm.img.bytes + (size_t)h->team_wpb * team_wave_bytes(h->dm);
-        int blocks = (int)std::max<size_t>(1, lds_cu / lds);
-        blocks = std::min(blocks, std::max(1, 8 / h->team_wpb));    // <= 2 waves per SIMD
-        h->team_grid = blocks * h->num_cu;
-        h->team_max_waves = 8 * h->num_cu;                            // 2 waves per SIMD (autotuner's range)
-        const size_t jbytes = sizeof(double) * team_jac_doubles(h->dm) * (size_t)h->team_max_waves;
-        if (hipMalloc(&h->d_jws, jbytes) != hipSuccess) { lsm_destroy(h); return fail(LSM_E_NOMEM, "Jacobian workspace"); }
-    }
+    h->lds_cu = std::min(lds_block, lds_cu);
+    team_shape_default(h);
     // wide grid: one 256-thread workgroup per CU (its FP32 LU takes most of the LDS)
     if (h->dm.img.wide_ok && h->dm.ne <= WIDE_MAX_NEQ && wide_lds_bytes(h->dm) <= std::min(lds_block, lds_cu)) {
         h->wide_grid = h->num_cu;
-        if (hipMalloc(&h->d_wjws, sizeof(double) * wide_jac_doubles(h->dm) * (size_t)h->wide_grid) != hipSuccess) {
+        if (hipMalloc(&h->d_wjws, sizeof(double) * wide_jac_doubles(h->dm) * (size_t)h->wide_grid) != hipSuccess ||
+            hipMalloc(&h->d_fallbacks, sizeof(unsigned long long)) != hipSuccess ||
+            hipMemset(h->d_fallbacks, 0, sizeof(unsigned long long)) != hipSuccess) {
             lsm_destroy(h);
             return fail(LSM_E_NOMEM, "wide Jacobian workspace");
         }
+        k.fp64_fallbacks = h->d_fallbacks;
     }
     *out = h;
     return LSM_OK;
@@ -915,7 +957,7 @@ int lsm_set_jacobian(lsm_handle *h, int32_t type) {
 
 int lsm_set_refinement(lsm_handle *h, int32_t sweeps) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
-    if (sweeps < 0 || sweeps > 4) return fail(LSM_E_INVALID, "refinement sweeps must be 0-4");
+    if (sweeps < 0 || sweeps > 8) return fail(LSM_E_INVALID, "refinement sweeps must be 0-8");
     h->kp.refine = sweeps;
     return LSM_OK;
 }
@@ -927,10 +969,10 @@ int lsm_destroy(lsm_handle *h) {
     if (h->d_vals) (void)hipFree(h->d_vals);
     if (h->d_sort_tmp) (void)hipFree(h->d_sort_tmp);
     if (h->d_wjws) (void)hipFree(h->d_wjws);
+    if (h->d_fallbacks) (void)hipFree(h->d_fallbacks);
     if (h->d_tables) (void)hipFree(h->d_tables);
     if (h->d_queue) (void)hipFree(h->d_queue);
     if (h->d_work) (void)hipFree(h->d_work);
-    if (h->d_jws) (void)hipFree(h->d_jws);
     if (h->kp.phase_cycles) (void)hipFree(h->kp.phase_cycles);
     if (h->d_y) (void)hipFree(h->d_y);
     if (h->d_dt) (void)hipFree(h->d_dt);
@@ -999,7 +1041,7 @@ static int integrate_device(lsm_handle *h, int64_t n, double t0, double pressure
     if (method == LSM_METHOD_TEAM) {
         if (launch_reset_queue(h->d_queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
         int rc = launch_team(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue, d_order,
-                             h->team_grid, h->team_wpb, h->d_jws, stream);
+                             h->team_grid, h->team_wpb, stream);
         if (rc) return fail(LSM_E_HIP, std::string("team kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
         return LSM_OK;
     }
@@ -1319,7 +1361,7 @@ int lsm_equations_device(lsm_handle *h, int64_t n, const double *d_y, double *d_
     if (h->team_wpb < 1) return fail(LSM_E_UNSUPPORTED, "equations need ns+1 <= 64 and a mechanism image that fits in LDS");
     if (d_jac && h->kp.const_volume)
         return fail(LSM_E_UNSUPPORTED, "the analytic Jacobian is built for the constant-pressure reactor");
-    if (launch_equations(h->dm, h->kp, n, d_y, d_f, d_jac, h->team_grid, h->team_wpb, h->d_jws, stream))
+    if (launch_equations(h->dm, h->kp, n, d_y, d_f, d_jac, h->team_grid, h->team_wpb, stream))
         return fail(LSM_E_HIP, std::string("equations kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
     return LSM_OK;
 }
@@ -1416,24 +1458,19 @@ int lsm_autotune(lsm_handle *h, int64_t n, double t0, double pressure, const dou
         return fail(LSM_E_NOMEM, "autotune buffers");
     }
     int rc = launch_gather_cells(d_y, d_dt, n, ne, m, stride, ys, dts, stream);
-    hipDeviceProp_t prop;
-    if (!rc && hipGetDeviceProperties(&prop, h->device) != hipSuccess) rc = -1;
-    const size_t lds_block = std::max<size_t>(prop.sharedMemPerBlock, 65536);
-    const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? prop.maxSharedMemoryPerMultiProcessor : lds_block;
-    const int max_wpb = team_waves_per_block(h->dm, std::min(lds_block, lds_cu));
+    const int cap = team_waves_per_cu(h->dm);
     int best_wpb = h->team_wpb, best_grid = h->team_grid;
     float best_ms = 3.4e38f;
     bool warmed = false;
-    for (int wpb = 1; wpb <= max_wpb && !rc; wpb++) {
-        const size_t lds = (size_t)h->dm.img.bytes + (size_t)wpb * team_wave_bytes(h->dm);
-        for (int bpc = 1; wpb * bpc <= 8 && bpc * lds <= lds_cu; bpc++) {
+    for (int wpb = 1; wpb <= 4 && !rc; wpb++) {
+        for (int bpc = 1; wpb * bpc <= cap; bpc++) {
             const int grid = bpc * h->num_cu;
-            if (grid * wpb > h->team_max_waves) break;
+            if (!team_shape_fits(h->dm, wpb, bpc, h->lds_cu)) continue;
             for (int rep = warmed ? 0 : -1; rep < 1 && !rc; rep++) {   // the first candidate runs twice (warm-up)
                 if (launch_reset_queue(h->d_queue, stream)) { rc = -1; break; }
                 if (hipEventRecord(e0, s) != hipSuccess) { rc = -1; break; }
                 if (launch_team(h->dm, kp, m, dts, ys, yo, nullptr, nullptr, nullptr, h->d_queue, nullptr, grid, wpb,
-                                h->d_jws, stream)) { rc = -1; break; }
+                                stream)) { rc = -1; break; }
                 if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = -1; break; }
                 float ms = 0.f;
                 if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = -1; break; }
@@ -1447,6 +1484,17 @@ int lsm_autotune(lsm_handle *h, int64_t n, double t0, double pressure, const dou
     h->team_wpb = best_wpb;
     h->team_grid = best_grid;
     if (result) { result[0] = best_wpb; result[1] = best_grid; result[2] = (int32_t)(best_ms * 1000.f); }
+    return LSM_OK;
+}
+
+int lsm_fp64_fallbacks(lsm_handle *h, uint64_t *count) {
+    if (!h || !count) return fail(LSM_E_INVALID, "null argument");
+    *count = 0;
+    if (!h->d_fallbacks) return LSM_OK;
+    unsigned long long v = 0;
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(&v, h->d_fallbacks, sizeof(v), hipMemcpyDeviceToHost));
+    *count = v;
     return LSM_OK;
 }
 

@@ -535,6 +535,14 @@ __device__ __forceinline__ int bdf_integrate(C &c, const BdfConst &K, double tOu
     s.alfa2_5 = 0.8;
     s.maximum_order = MAX_ORDER;
     n = BdfCounters{0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0};
+    // Fresh solver per cell: the Nordsieck and work arrays start at zero, as in the oracle, so that
+    // nothing of the previous cell or of uninitialised registers reaches this one.  (The reference
+    // sizes them once, MethodGear.hpp:180-192, and keeps them from cell to cell.)
+    c.each([&](int e) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) { c.x(Z0 + i, e) = 0.; c.x(V0 + i, e) = 0.; }
+        c.x(VB_, e) = 0.; c.x(VA_, e) = 0.; c.x(DB_, e) = 0.; c.x(B_, e) = 0.; c.x(AUX_, e) = 0.; c.x(Y_, e) = 0.;
+    });
     s.tOut = K.t0; s.t = K.t0; s.t0 = K.t0; s.tInMeshPoint = K.t0; s.tStabilize = K.t0;
     c.rhs(Y0_, F_);
     n.rhs_total++;

@@ -59,6 +59,9 @@ struct LdsImage {
     int team_ok;   // the team kernel can use the image (ns <= 63)
     int wide_ok;   // the wide kernel can use the image (ns <= 254)
     int dummy;     // unused slot of fixed-format records: 63 if ns <= 63, else 255
+    int simple0;   // first of the reactions, all at the end, that are fixed-format records with no
+                   // collider, tabulated rate or explicit reverse (the team kernel's paired loop);
+                   // nr when there are none or the layout is the wide kernel's
 };
 
 // Reactions are renumbered on upload: fall-off first, then third-body, then
@@ -124,7 +127,8 @@ struct KParams {
     int energy;
     int full_pivoting;
     int jac_mode;                       // 0: forward differences (KernelDense), 1: analytic
-    int refine;                         // wide kernel: iterative-refinement sweeps after each FP32 solve
+    int refine;                         // wide kernel: most refinement sweeps after an FP32 solve
+    unsigned long long *fp64_fallbacks; // wide kernel: count of factorisations redone in FP64 (may be null)
     int const_volume;                   // constant-volume reactor (BatchReactorHomogeneousConstantVolume)
     int compact;                        // Strang "compact" output: y_out holds RR [kg/m3/s] and RT [W/m3]
     const double *rho;                  // constant volume: per-cell density [kg/m3] of this call (may be null:
@@ -133,22 +137,33 @@ struct KParams {
 
 // Host-side launchers (lsm_kernels.hip).
 int team_max_neq();
-// LDS layout of a team workgroup: the mechanism image, then per wave the species
-// scratch (4 x 64 doubles).  Each wave's Jacobian (ne x 64 doubles) lives in an
-// HBM workspace indexed by the global wave number.
-__host__ __device__ inline size_t team_wave_bytes(const DevMech &m) { return (size_t)8 * (320 + 65 * (size_t)m.ne); }
-size_t team_jac_doubles(const DevMech &m);
-int team_waves_per_block(const DevMech &m, size_t lds_per_block);
+// LDS layout of a team workgroup: the mechanism image, then per wave team_wave_bytes(ne).
+// Per-wave LDS of the team kernel: species scratch (c and g pairs, R, dR/dT, h, collider
+// terms: TEAM_WAVE_FIXED doubles), then the Jacobian, ne columns of 65 doubles (lane k's row at
+// offset k of each column; the 65th slot staggers the banks), rounded up to 16 bytes so that
+// every wave's (c, g) pairs stay 16-byte aligned.
+enum { TEAM_WAVE_FIXED = 384 };
+__host__ __device__ inline size_t team_wave_bytes(int ne) {
+    return ((size_t)8 * (TEAM_WAVE_FIXED + 65 * (size_t)ne) + 15) / 16 * 16;
+}
+// Waves per SIMD the team kernel's registers are capped for.  Every register-array size needs
+// more than 256 registers (its LU rows live in registers): a tighter cap only moves values to
+// scratch memory (776 bytes per lane at three waves per SIMD for the 16-wide array).
+#define TEAM_WAVES_PER_SIMD(NMAX) 1
+int team_waves_per_cu(const DevMech &m);
+// Whether blocks_per_cu workgroups of wpb waves (each with its own copy of the mechanism image)
+// fit in lds_cu bytes of LDS.
+bool team_shape_fits(const DevMech &m, int wpb, int blocks_per_cu, size_t lds_cu);
 // order (may be null): the cells in the order the queue hands them out.
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                 double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order,
-                int grid, int waves_per_block, double *jws, void *stream);
+                int grid, int waves_per_block, void *stream);
 int launch_plain(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                  double *yout, int *status, double *q, int *counters, double *work, int64_t slots,
                  void *stream);
 size_t plain_work_doubles(const DevMech &m, int64_t slots);
 // Reactor right-hand side f (SoA [ne][n]) and analytic Jacobian J ([n][ne][ne]) at
-// the states yin; either output may be null.  Uses the team workspace jws.
+// the states yin; either output may be null.
 int launch_properties(const DevMech &m, double p_default, int64_t n, const double *y, const double *p, double *out,
                       double *x, double *cpk, int num_cu, void *stream);
 // Transport part of the property refresh: mu, lambda [n], dmix, teta [ns][n] (the last
@@ -156,7 +171,7 @@ int launch_properties(const DevMech &m, double p_default, int64_t n, const doubl
 int launch_transport(const DevMech &m, const DevTransport &t, double p_default, int64_t n, const double *y,
                      const double *p, double *mu, double *lam, double *dmix, double *teta, int num_cu, void *stream);
 int launch_equations(const DevMech &m, const KParams &k, int64_t n, const double *yin, double *f, double *J,
-                     int grid, int waves_per_block, double *jws, void *stream);
+                     int grid, int waves_per_block, void *stream);
 int launch_reset_queue(unsigned *queue, void *stream);
 int launch_order_keys(const int *counters, int64_t n, unsigned *keys, int *vals, void *stream);
 int launch_fill(double *p, int64_t n, double v, void *stream);

@@ -117,6 +117,17 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x)
     return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
 }
 
+// Wavefront maximum of non-negative doubles (lanes without a DPP source read 0), broadcast.
+__device__ __forceinline__ double wave_max_nonneg(double x) {
+    x = fmax(x, dpp0_d<0x111, 0xf>(x));
+    x = fmax(x, dpp0_d<0x112, 0xf>(x));
+    x = fmax(x, dpp0_d<0x114, 0xf>(x));
+    x = fmax(x, dpp0_d<0x118, 0xf>(x));
+    x = fmax(x, dpp0_d<0x142, 0xa>(x));
+    x = fmax(x, dpp0_d<0x143, 0xc>(x));
+    return readlane_d(x, 63);
+}
+
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -487,11 +498,12 @@ __device__ __forceinline__ void rxn_rate(const LdsMech &L, const double2 *scg, i
 }
 
 // ---- analytic Jacobian ------------------------------------------------------
-// One reaction's contributions to R_k (sR), dR_k/dT at fixed c (sDT) and
-// dR_k/dc_i (LU region, [i*65 + k]).  kmod is the effective rate constant
+// One reaction's contributions to R_k, dR_k/dT at fixed c and dR_k/dc_i.  kmod is the effective rate constant
 // (Arrhenius x M for third-body reactions, k_inf Pr/(1+Pr) F for fall-off),
 // dkdM its derivative with respect to the collider concentration M.
-// acc.R(k, r, drdT) adds to R_k and dR_k/dT, acc.D(i, k, v) to dR_k/dc_i; ns species,
+// acc.R(k, r, drdT) adds to R_k and dR_k/dT, acc.D(i, k, v) to dR_k/dc_i, acc.U(k, v) to
+// dR_k/dc_i for every species i (the part of a collider derivative dM/dc_i = 1 common to all
+// species; the efficiencies eff_i - 1 go through acc.D); ns species,
 // `dummy` the unused slot of fixed-format records (c = 1, g = 0), sH[k] = H_k/RT.
 template <bool XK, class Acc>
 __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, const double *sH, int ns, unsigned dummy,
@@ -630,10 +642,9 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
         }
         if (dense) {
             const double qv = dkdM * net;
-            for (int i = 0; i < ns; i++)
 #pragma unroll
-                for (int o = 0; o < 6; o++)
-                    if (sl[o] != dummy) acc.D(i, sl[o], o < 3 ? -qv : qv);
+            for (int o = 0; o < 6; o++)
+                if (sl[o] != dummy) acc.U(sl[o], o < 3 ? -qv : qv);
             const unsigned q0 = dense == 1 ? L.tbp[tb] : L.fol[fo], q1 = dense == 1 ? L.tbp[tb + 1] : L.fol[fo + 1];
             for (unsigned q = q0; q < q1; q++) {
                 const unsigned i = dense == 1 ? L.tbs[q] : L.fos2[q];
@@ -724,7 +735,7 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
                 acc.D(single, ko, sg * qv);
                 continue;
             }
-            for (int i = 0; i < ns; i++) acc.D(i, ko, sg * qv);
+            acc.U(ko, sg * qv);
             const unsigned q0 = dense == 1 ? L.tbp[tb] : L.fol[fo], q1 = dense == 1 ? L.tbp[tb + 1] : L.fol[fo + 1];
             for (unsigned q = q0; q < q1; q++) {
                 const unsigned i = dense == 1 ? L.tbs[q] : L.fos2[q];
@@ -737,18 +748,23 @@ __device__ __forceinline__ void rxn_jac(const LdsMech &L, const double2 *scg, co
 template <int NMAX, bool XK>
 struct TeamCtx {
     double V[NVEC][1];
+    double G[NMAX];  // this lane's row of the LU factors of the Newton matrix: multipliers in the
+                     // columns eliminated before the row became the pivot (k < ps), U from ps on.
+                     // Kept in registers from one factorisation to the next.
     int ps;          // elimination step at which this lane's row became the pivot
     int plv;         // lane k holds the row (lane) chosen as pivot at step k
     double dinv;     // 1 / pivot of this lane's row
-    double *Jl;                       // Jacobian [ne][64] of this wave (HBM/L2 workspace)
-    double *LU;                       // per-wave LDS: LU factors [ne][64], lane i holds row i
+    double *Jl;                       // per-wave LDS: Jacobian [ne][65], lane k holds row k (df_k/dy_j at
+                                      // [j*65 + k]); during an analytic Jacobian first dR_k/dc_j
     double2 *scg;                     // per-wave LDS: (c_k, g_k = h_k/RT - s_k/R) per species slot
     double *sR;                       // per-wave LDS: formation-rate accumulators
     double *sDT;                      // per-wave LDS: dR_k/dT accumulators (analytic Jacobian)
     double *sH;                       // per-wave LDS: h_k = H_k/RT (analytic Jacobian)
+    double *sU;                       // per-wave LDS: dR_k/dc_i common to every i (collider terms)
     LdsMech L;
     const KParams &K;
     int lane, ne, ns, nr;
+    int s0;                           // first reaction of the paired loop (LdsImage::simple0)
     double qr;
     double rho0;                      // constant-volume reactor: density of the current cell
 #ifdef LSM_PHASE_TIMERS
@@ -757,14 +773,15 @@ struct TeamCtx {
     int in_jac = 0;
 #endif
 
-    __device__ TeamCtx(const DevMech &m, const KParams &k, const unsigned char *img, double *wave_lds, double *jac)
-        : L(img, m.img, m.ns_pad), K(k), lane(threadIdx.x & 63), ne(m.ne), ns(m.ns), nr(m.nr), qr(0.), rho0(0.) {
-        Jl = jac;
+    __device__ TeamCtx(const DevMech &m, const KParams &k, const unsigned char *img, double *wave_lds)
+        : L(img, m.img, m.ns_pad), K(k), lane(threadIdx.x & 63), ne(m.ne), ns(m.ns), nr(m.nr), s0(m.img.simple0),
+          qr(0.), rho0(0.) {
         scg = (double2 *)wave_lds;
         sR = wave_lds + 128;
         sDT = wave_lds + 192;
         sH = wave_lds + 256;
-        LU = wave_lds + 320;
+        sU = wave_lds + 320;
+        Jl = wave_lds + TEAM_WAVE_FIXED;
     }
 
     template <class F>
@@ -781,7 +798,49 @@ struct TeamCtx {
 
     __device__ __forceinline__ void reaction(int j, double T, double logT, double uT, double logPRT, double Pk,
                                             double cTot) {
-        rxn_rate<XK>(L, scg, j, T, logT, uT, logPRT, Pk, cTot, [&](unsigned k, double v) { atomicAdd(&sR[k], v); });
+        rxn_rate<XK>(L, scg, j, T, logT, uT, logPRT, Pk, cTot, [&](unsigned k, double v) { atomicAdd(racc(k), v); });
+    }
+
+    // Accumulator of species slot k in the reaction loop.  The unused slots of fixed-format records
+    // all name the dummy species; their (zero-rate-weighted) adds go to a slot of each lane's own
+    // (sU, free during an RHS) so that they do not serialise on one LDS address.
+    __device__ __forceinline__ double *racc(unsigned k) const {
+        return k == (unsigned)LSM_DUMMY_SP ? sU + lane : sR + k;
+    }
+
+    // Two fixed-format records without collider (reactions j and j + 64, the second absent past nr),
+    // their loads, exponentials and products interleaved: the same arithmetic as rxn_rate's
+    // fixed-format path (KineticsMap_CHEMKIN.hpp:1198-1222, 1734-1767).
+    __device__ __forceinline__ void simple_pair(int j, double logT, double uT, double logPRT) {
+        const int j2 = j + 64;
+        const bool has2 = j2 < nr;
+        const int jb = has2 ? j2 : j;
+        const uint4 w1 = L.W[j], w2 = L.W[jb];
+        const double2 ab1 = ((const double2 *)L.P)[2 * j], ed1 = ((const double2 *)L.P)[2 * j + 1];
+        const double2 ab2 = ((const double2 *)L.P)[2 * jb], ed2 = ((const double2 *)L.P)[2 * jb + 1];
+        const unsigned r10 = w1.y & 0xffu, r11 = (w1.y >> 8) & 0xffu, r12 = (w1.y >> 16) & 0xffu;
+        const unsigned p10 = w1.z & 0xffu, p11 = (w1.z >> 8) & 0xffu, p12 = (w1.z >> 16) & 0xffu;
+        const unsigned r20 = w2.y & 0xffu, r21 = (w2.y >> 8) & 0xffu, r22 = (w2.y >> 16) & 0xffu;
+        const unsigned p20 = w2.z & 0xffu, p21 = (w2.z >> 8) & 0xffu, p22 = (w2.z >> 16) & 0xffu;
+        const double2 a10 = scg[r10], a11 = scg[r11], a12 = scg[r12], b10 = scg[p10], b11 = scg[p11], b12 = scg[p12];
+        const double2 a20 = scg[r20], a21 = scg[r21], a22 = scg[r22], b20 = scg[p20], b21 = scg[p21], b22 = scg[p22];
+        double kf1 = exp(ab1.x + ab1.y * logT - ed1.x * uT);
+        double kf2 = exp(ab2.x + ab2.y * logT - ed2.x * uT);
+        kf1 = (w1.x & 2u) ? -kf1 : kf1;
+        kf2 = (w2.x & 2u) ? -kf2 : kf2;
+        const double dg1 = (b10.y + b11.y + b12.y) - (a10.y + a11.y + a12.y);
+        const double dg2 = (b20.y + b21.y + b22.y) - (a20.y + a21.y + a22.y);
+        const double uk1 = exp(dg1 - logPRT * ed1.y), uk2 = exp(dg2 - logPRT * ed2.y);
+        double net1 = (a10.x * a11.x) * a12.x, net2 = (a20.x * a21.x) * a22.x;
+        const double rb1 = ((b10.x * b11.x) * b12.x) * uk1, rb2 = ((b20.x * b21.x) * b22.x) * uk2;
+        net1 = (w1.x & 1u) ? net1 - rb1 : net1;
+        net2 = (w2.x & 1u) ? net2 - rb2 : net2;
+        const double r1 = net1 * kf1;
+        const double r2 = has2 ? net2 * kf2 : 0.;
+        atomicAdd(racc(r10), -r1); atomicAdd(racc(r11), -r1); atomicAdd(racc(r12), -r1);
+        atomicAdd(racc(p10), r1);  atomicAdd(racc(p11), r1);  atomicAdd(racc(p12), r1);
+        atomicAdd(racc(r20), -r2); atomicAdd(racc(r21), -r2); atomicAdd(racc(r22), -r2);
+        atomicAdd(racc(p20), r2);  atomicAdd(racc(p21), r2);  atomicAdd(racc(p22), r2);
     }
 
     __device__ __forceinline__ void rhs(int src, int dst) {
@@ -830,7 +889,8 @@ struct TeamCtx {
         // the kinetics' pressure: P0, or cTot R T at constant volume
         // (BatchReactorHomogeneousConstantVolume.H:166-176)
         const double Pk = K.const_volume ? cTotP * RGAS * T : K.pressure;
-        for (int j = lane; j < nr; j += 64) reaction(j, T, logT, uT, logPRT, Pk, cTot);
+        for (int j = lane; j < s0; j += 64) reaction(j, T, logT, uT, logPRT, Pk, cTot);
+        for (int j = s0 + lane; j < nr; j += 128) simple_pair(j, logT, uT, logPRT);
         wave_sync();
         LSM_TOC(8, t_rhs1);
         LSM_TIC(t_rhs2);
@@ -883,28 +943,31 @@ struct TeamCtx {
         return (ok && K.energy) ? rho * (cy * RGAS) * (Tf - Tin) / dtc : 0.;
     }
 
+    // Accumulators of the analytic Jacobian: R and dR/dT, dR_k/dc_i into the Jacobian's own
+    // LDS ([i*65 + k]), the collider terms common to every column into sU.
     struct JacAcc {
-        double *sR, *sDT, *Dm;
+        double *sR, *sDT, *Dm, *sU;
         __device__ __forceinline__ void R(unsigned k, double r, double drdT) const {
-            atomicAdd(&sR[k], r);
-            atomicAdd(&sDT[k], drdT);
+            if (k != (unsigned)LSM_DUMMY_SP) {     // the dummy species of fixed-format records
+                atomicAdd(&sR[k], r);
+                atomicAdd(&sDT[k], drdT);
+            }
         }
         __device__ __forceinline__ void D(unsigned i, unsigned k, double v) const { atomicAdd(&Dm[i * 65 + k], v); }
+        __device__ __forceinline__ void U(unsigned k, double v) const { atomicAdd(&sU[k], v); }
     };
-    __device__ __forceinline__ void reaction_jac(int j, double T, double logT, double uT, double logPRT, double cTot) {
-        rxn_jac<XK>(L, scg, sH, ns, LSM_DUMMY_SP, j, T, logT, uT, logPRT, K.pressure, cTot, JacAcc{sR, sDT, LU});
-    }
 
     // Analytic Jacobian of the constant-pressure reactor at the state in VB:
     //   f_k = MW_k R_k / rho,  f_T = -sum_k H_k R_k / (rho cp)
     // with D = dR/dc (T fixed), S_k = sum_i D_ki c_i:
-    //   df_k/dY_j = MW_k / (rho MW_j) (rho D_kj - MW S_k) + f_k MW / MW_j
+    //   df_k/dY_j = (MW_k / MW_j) D_kj + (MW / MW_j) (f_k - MW_k S_k / rho)
     //   df_k/dT   = MW_k / rho (dR_k/dT - S_k / T) + f_k / T
     //   df_T/dY_j = [-(RT / MW_j)(rho sum_k h_k D_kj - MW sum_k h_k S_k) - f_T (rho / MW_j)(Cp_j - MW cp)] / (rho cp)
     //   df_T/dT   = [-sum_k (Cp_k R_k + H_k (dR_k/dT - S_k / T)) - f_T (-rho cp / T + rho sum_k Y_k Cp'_k / MW_k)] / (rho cp)
-    // (MW mixture molecular weight, cp mass heat capacity, h_k = H_k / RT).  Written to Jl.
+    // (MW mixture molecular weight, cp mass heat capacity, h_k = H_k / RT).  D is gathered in
+    // the Jacobian's LDS (the collider terms common to every column apart, in sU), its sums S and
+    // sum_k h_k D_kj are taken, and each lane then turns its own row of D into its row of J.
     __device__ __forceinline__ void jacobian_analytic_impl() {
-        double *D = LU;
         const double T = readlane_d(V[VB_][0], ns);
         const int k = lane;
         const bool isp = k < ns;
@@ -932,27 +995,35 @@ struct TeamCtx {
         if (k == LSM_DUMMY_SP) { scg[k] = make_double2(1., 0.); sH[k] = 0.; }
         sR[k] = 0.;
         sDT[k] = 0.;
-        for (int i = 0; i < ns; i++) D[i * 65 + k] = 0.;
+        sU[k] = 0.;
+        for (int i = 0; i < ne; i++) Jl[i * 65 + k] = 0.;
         double cTot = ck, cpx = isp ? cp * xk : 0.;
         wave_sum2(cTot, cpx);
         wave_sync();
         const double logPRT = log(101325. / RGAS / T);
-        for (int j = lane; j < nr; j += 64) reaction_jac(j, T, logT, uT, logPRT, cTot);
+        const JacAcc acc{sR, sDT, Jl, sU};
+        for (int j = lane; j < nr; j += 64)
+            rxn_jac<XK>(L, scg, sH, ns, LSM_DUMMY_SP, j, T, logT, uT, logPRT, K.pressure, cTot, acc);
         wave_sync();
         const double Rk = isp ? sR[k] : 0.;
         const double dRdT = isp ? sDT[k] : 0.;
-        double S = 0.;
-        if (isp)
-            for (int i = 0; i < ns; i++) S += D[i * 65 + k] * scg[i].x;
+        const double Uk = isp ? sU[k] : 0.;
+        double S = 0., A = 0.;
+        if (isp) {
+            for (int i = 0; i < ns; i++) S += (Jl[i * 65 + k] + Uk) * scg[i].x;       // own row of D
+            for (int i = 0; i < ns; i++) A += sH[i] * (Jl[k * 65 + i] + sU[i]);      // sum_i h_i dR_i/dc_k
+        }
+        wave_sync();                                                                  // D read before rewritten
         const double fk = isp ? (mwk * Rk) / rho : 0.;
         if (isp) {
+            const double g = fk - mwk * S / rho;
             for (int j = 0; j < ns; j++) {
                 const double mwj = L.sp[15 * P + j];
-                Jl[j * 64 + k] = (mwk / (rho * mwj)) * (rho * D[j * 65 + k] - MW * S) + fk * MW / mwj;
+                Jl[j * 65 + k] = (mwk / mwj) * (Jl[j * 65 + k] + Uk) + (MW / mwj) * g;
             }
-            Jl[ns * 64 + k] = (mwk / rho) * (dRdT - S * uT) + fk * uT;
+            Jl[ns * 65 + k] = (mwk / rho) * (dRdT - S * uT) + fk * uT;
         } else if (k > ns) {
-            for (int j = 0; j < ne; j++) Jl[j * 64 + k] = 0.;
+            for (int j = 0; j < ne; j++) Jl[j * 65 + k] = 0.;
         }
         // temperature row (zero without the energy equation)
         const double cpm = (cpx * RGAS) / MW;            // mass cp
@@ -961,9 +1032,6 @@ struct TeamCtx {
         wave_sum2(hs, hS);
         const double QR = -hs * RGAS * T;
         const double fT = K.energy ? QR / Phi : 0.;
-        double A = 0.;
-        if (isp)
-            for (int i = 0; i < ns; i++) A += sH[i] * D[k * 65 + i];
         double dQ = isp ? -(cp * RGAS * Rk + h * RGAS * T * (dRdT - S * uT)) : 0.;
         double dP = isp ? Yk * dcp * RGAS / mwk : 0.;
         wave_sum2(dQ, dP);
@@ -971,12 +1039,12 @@ struct TeamCtx {
             if (isp) {
                 const double dQdY = -(RGAS * T / mwk) * (rho * A - MW * hS);
                 const double dPdY = (rho / mwk) * (cp * RGAS - MW * cpm);
-                Jl[k * 64 + ns] = (dQdY - fT * dPdY) / Phi;
+                Jl[k * 65 + ns] = (dQdY - fT * dPdY) / Phi;
             }
-            if (k == ns) Jl[ns * 64 + ns] = (dQ - fT * (-Phi * uT + rho * dP)) / Phi;
+            if (k == ns) Jl[ns * 65 + ns] = (dQ - fT * (-Phi * uT + rho * dP)) / Phi;
         } else {
-            if (isp) Jl[k * 64 + ns] = 0.;
-            if (k == ns) Jl[ns * 64 + ns] = 0.;
+            if (isp) Jl[k * 65 + ns] = 0.;
+            if (k == ns) Jl[ns * 65 + ns] = 0.;
         }
         wave_sync();
     }
@@ -1000,13 +1068,22 @@ struct TeamCtx {
             rhs(VB_, AUX_);
             if (lane == j) V[VB_][0] = yh;
             const double hInv = 1. / hJ;
-            Jl[j * 64 + lane] = hInv * (V[AUX_][0] - V[F_][0]);
+            Jl[j * 65 + lane] = hInv * (V[AUX_][0] - V[F_][0]);
         }
+    }
+
+    // Row of the largest |g| among the eligible lanes (the lowest lane on a tie; the first
+    // eligible lane when every eligible |g| is 0), as a uniform value.
+    __device__ __forceinline__ int pivot_lane(double g, bool elig) {
+        const double a = elig ? fabs(g) : 0.;
+        const double m = wave_max_nonneg(a);
+        const unsigned long long hit = __ballot(elig && a == m);
+        return __builtin_amdgcn_readfirstlane(__ffsll(hit) - 1);
     }
 
     // G = I - hr0 J and row-pivoted LU (KernelDense.hpp:196-221), factorised in
     // registers (lane i holds row i, the pivot row is broadcast with v_readlane)
-    // and stored to LDS for the solves.  Rows are not moved: lane i records the
+    // and kept there for the solves.  Rows are not moved: lane i records the
     // step ps at which it was chosen; the largest |G(i,k)| among rows not yet
     // chosen wins (ties and the 6 lowest mantissa bits go to the lower lane).
     __device__ __forceinline__ void factor_impl(double hr0) {
@@ -1016,25 +1093,21 @@ struct TeamCtx {
         int lane = this->lane, ne = this->ne;
         asm volatile("" : "+v"(lane));
         asm volatile("" : "+s"(ne));
-        double G[NMAX];
 #pragma unroll
         for (int j = 0; j < NMAX; j++) {
-            const double jij = (j < ne) ? Jl[j * 64 + lane] : 0.;
+            const double jij = (j < ne) ? Jl[j * 65 + lane] : 0.;
             G[j] = jij * -hr0 + ((lane == j) ? 1. : 0.);
             if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
         }
         ps = (lane < ne) ? -1 : 1 << 20;
         dinv = 1.;
+        // The pivot of step k is the largest |G(i,k)| among rows not yet chosen, the lowest lane
+        // on a tie (or the first eligible row when the column is zero).  Each step's search runs
+        // as soon as its column has been updated, overlapping the rest of the previous update.
+        int p = pivot_lane(G[0], ps < 0);
 #pragma unroll
         for (int k = 0; k < NMAX; k++) {
             if (k < ne) {
-                const bool elig = ps < 0;
-                const unsigned long long ab = (unsigned long long)__double_as_longlong(fabs(G[k]));
-                unsigned long long key = elig ? ((ab & ~63ull) | (unsigned long long)(63 - lane)) : 0ull;
-                key = wave_max_u64(key);
-                int p = 63 - (int)(key & 63ull);
-                if (key == 0ull) p = __ffsll((unsigned long long)__ballot(elig)) - 1;
-                p = __builtin_amdgcn_readfirstlane(p);
                 const double piv = readlane_d(G[k], p);
                 const double inv = 1. / piv;
                 if (lane == p) { ps = k; dinv = inv; }
@@ -1043,14 +1116,17 @@ struct TeamCtx {
                     l = G[k] * inv;
                     G[k] = l;
                 }
+                int pn = 0;
+                if (k + 1 < NMAX) {
+                    G[k + 1] = fma(-l, readlane_d(G[k + 1], p), G[k + 1]);
+                    pn = pivot_lane(G[k + 1], ps < 0);
+                }
 #pragma unroll
-                for (int j = k + 1; j < NMAX; j++) {
+                for (int j = k + 2; j < NMAX; j++) {
                     const double u = readlane_d(G[j], p);
                     G[j] = fma(-l, u, G[j]);
-                    if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
                 }
-                LU[k * 64 + lane] = G[k];
-                __builtin_amdgcn_sched_barrier(0);
+                p = pn;
             }
         }
         // pivot order: lane k learns which row was chosen at step k (sR is free here)
@@ -1061,74 +1137,74 @@ struct TeamCtx {
         wave_sync();
     }
 
-    // Forward / backward substitution in pivot order, factors read from LDS.  The
-    // factors and pivot rows of 8 steps are fetched ahead of the dependent chain
-    // b -> readlane -> fma, and rows that take no update get a zero coefficient.
+    // Forward / backward substitution in pivot order with the factors in registers, in groups of
+    // 8 steps with no branch inside a group: the group's masked factors and pivot lanes are formed
+    // first, off the dependent chain b -> broadcast -> fma.  Steps past ne inside the last group
+    // change nothing: rows of the system have zero factors there, and the padding rows (lanes
+    // >= ne) are never read.  Masks are products, not selects, which the compiler would sink into
+    // branches.
     __device__ __forceinline__ void solve_impl(int vid) {
+        // Opaque per-solve copies of the pivot data: otherwise the compiler hoists the masked
+        // factors, pivot lanes and inverse pivots of every step out of the Newton and step loops
+        // (they only change at a factorisation) and keeps them all live, far beyond the
+        // register budget.
+        int ne = this->ne, ps_l = ps, plv_l = plv;
+        double dinv_l = dinv;
+        asm volatile("" : "+s"(ne));
+        asm volatile("" : "+v"(ps_l), "+v"(plv_l), "+v"(dinv_l));
         double b = V[vid][0];
-        int k = 0;
-        for (; k + 8 <= ne; k += 8) {
-            double lk[8];
-            int pk[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const double v = LU[(k + u) * 64 + lane];   // unconditional load, masked by a product
-                lk[u] = v * ((ps > k + u) ? 1. : 0.);       // (a select would be sunk into a branch)
-                pk[u] = __builtin_amdgcn_readlane(plv, k + u);
+        for (int k0 = 0; k0 < NMAX; k0 += 8) {
+            if (k0 < ne) {
+                double lk[8];
+                int pk[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    lk[u] = G[k0 + u] * ((ps_l > k0 + u) ? 1. : 0.);
+                    pk[u] = __builtin_amdgcn_readlane(plv_l, k0 + u);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) b = fma(-lk[u], readlane_d(b, pk[u]), b);
             }
-#pragma unroll
-            for (int u = 0; u < 8; u++) b = fma(-lk[u], readlane_d(b, pk[u]), b);
-        }
-        for (; k < ne; k++) {
-            const double v = LU[k * 64 + lane];
-            const double lk = v * ((ps > k) ? 1. : 0.);
-            b = fma(-lk, readlane_d(b, __builtin_amdgcn_readlane(plv, k)), b);
         }
         double xr = 0.;
-        k = ne - 1;
-        for (; k >= 7; k -= 8) {
-            double uk[8], dk[8];
-            int pk[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const double v = LU[(k - u) * 64 + lane];
-                uk[u] = v * ((ps < k - u) ? 1. : 0.);
-                pk[u] = __builtin_amdgcn_readlane(plv, k - u);
-                dk[u] = readlane_d(dinv, pk[u]);
-            }
+        for (int k0 = NMAX - 8; k0 >= 0; k0 -= 8) {
+            if (k0 < ne) {
+                double uk[8], dk[8];
+                int pk[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const double xk = readlane_d(b, pk[u]) * dk[u];
-                b = fma(-uk[u], xk, b);
-                if (lane == k - u) xr = xk;
+                for (int u = 7; u >= 0; u--) {
+                    uk[u] = G[k0 + u] * ((ps_l < k0 + u) ? 1. : 0.);
+                    pk[u] = __builtin_amdgcn_readlane(plv_l, k0 + u);
+                    dk[u] = readlane_d(dinv_l, pk[u]);
+                }
+#pragma unroll
+                for (int u = 7; u >= 0; u--) {
+                    const double xk = readlane_d(b, pk[u]) * dk[u];
+                    b = fma(-uk[u], xk, b);
+                    if (lane == k0 + u) xr = xk;
+                }
             }
         }
-        for (; k >= 0; k--) {
-            const double v = LU[k * 64 + lane];
-            const double uk = v * ((ps < k) ? 1. : 0.);
-            const int p = __builtin_amdgcn_readlane(plv, k);
-            const double xk = readlane_d(b, p) * readlane_d(dinv, p);
-            b = fma(-uk, xk, b);
-            if (lane == k) xr = xk;
-        }
-        V[vid][0] = xr;
+        V[vid][0] = (lane < ne) ? xr : 0.;
     }
 
-    // J v with the Jacobian columns streamed from the workspace, four independent
-    // partial sums so that the column loads overlap.
+    // J v with the Jacobian columns read from LDS, four independent partial sums so that
+    // the loads overlap.
     __device__ __forceinline__ void jac_times_impl(int src, int dst) {
         const double v = V[src][0];
         double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
         int j = 0;
         for (; j + 4 <= ne; j += 4) {
-            const double c0 = Jl[j * 64 + lane], c1 = Jl[(j + 1) * 64 + lane];
-            const double c2 = Jl[(j + 2) * 64 + lane], c3 = Jl[(j + 3) * 64 + lane];
+            const double c0 = Jl[j * 65 + lane], c1 = Jl[(j + 1) * 65 + lane];
+            const double c2 = Jl[(j + 2) * 65 + lane], c3 = Jl[(j + 3) * 65 + lane];
             a0 = fma(c0, readlane_d(v, j), a0);
             a1 = fma(c1, readlane_d(v, j + 1), a1);
             a2 = fma(c2, readlane_d(v, j + 2), a2);
             a3 = fma(c3, readlane_d(v, j + 3), a3);
         }
-        for (; j < ne; j++) a0 = fma(Jl[j * 64 + lane], readlane_d(v, j), a0);
+        for (; j < ne; j++) a0 = fma(Jl[j * 65 + lane], readlane_d(v, j), a0);
         V[dst][0] = (a0 + a1) + (a2 + a3);
     }
     // Returns the number of RHS evaluations the Jacobian took.
@@ -1156,11 +1232,11 @@ struct TeamCtx {
 };
 
 
+// Up to 4 waves per workgroup; registers capped for TEAM_WAVES_PER_SIMD(NMAX) waves per SIMD.
 template <int NMAX, bool XK>
-__global__ void __launch_bounds__(256, (NMAX <= 16 ? 2 : 1)) team_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt,
-                                                   const double *yin, double *yout, int *status, double *q,
-                                                   int *counters, unsigned *queue, const int *order,
-                                                   double *jws) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEAM_WAVES_PER_SIMD(NMAX))))
+team_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt, const double *yin, double *yout,
+            int *status, double *q, int *counters, unsigned *queue, const int *order) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     {   // stage the mechanism image (once per workgroup)
         const int4 *src = (const int4 *)M.img.src;
@@ -1170,9 +1246,8 @@ __global__ void __launch_bounds__(256, (NMAX <= 16 ? 2 : 1)) team_kernel(DevMech
         __syncthreads();
     }
     const int wave = threadIdx.x >> 6;
-    double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * team_wave_bytes(M));
-    const size_t gw = (size_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    TeamCtx<NMAX, XK> c(M, K, lds_raw, wave_lds, jws + gw * 64 * (size_t)M.ne);
+    double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * team_wave_bytes(M.ne));
+    TeamCtx<NMAX, XK> c(M, K, lds_raw, wave_lds);
     const int lane = c.lane, ns = M.ns, ne = M.ne;
     for (;;) {
         unsigned cell_u = 0;
@@ -1185,10 +1260,10 @@ __global__ void __launch_bounds__(256, (NMAX <= 16 ? 2 : 1)) team_kernel(DevMech
         if (cell < 0 || cell >= n) continue;
         {   // opaque per-cell copies: keep lane/size-derived constants from being hoisted
             // out of the cell loop (they would stay live in registers for the whole kernel)
-            int l = threadIdx.x & 63, a = M.ne, b = M.ns, r = M.nr;
+            int l = threadIdx.x & 63, a = M.ne, b = M.ns, r = M.nr, s0 = M.img.simple0;
             asm volatile("" : "+v"(l));
-            asm volatile("" : "+s"(a), "+s"(b), "+s"(r));
-            c.lane = l; c.ne = a; c.ns = b; c.nr = r;
+            asm volatile("" : "+s"(a), "+s"(b), "+s"(r), "+s"(s0));
+            c.lane = l; c.ne = a; c.ns = b; c.nr = r; c.s0 = s0;
         }
         LSM_TIC(tcell);
         const double yi = lane < ne ? yin[(int64_t)lane * n + cell] : 0.;
@@ -1271,6 +1346,9 @@ struct WideCtx {
     float *dsc;               // LDS [256]  1 / pivot of step k (U rows are stored scaled by it)
     float *A;                 // LDS [ne + 1][lda] LU factors (spare last row); dR/dc in an analytic Jacobian
     double *J;                // HBM [ne][ne] column-major, J[j*ne + i] = d f_i / d y_j
+    double *G64;              // HBM [ne][ne] column-major: FP64 LU of G when refinement does not converge
+    int *p64;                 // LDS [256]  row interchanges of the FP64 LU
+    bool fp64 = false;        // the current factorisation is the FP64 one
 #ifdef LSM_PHASE_TIMERS
     unsigned long long tp[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     __device__ __forceinline__ void add_time(int i, unsigned long long d) { tp[i] += d; }
@@ -1280,7 +1358,7 @@ struct WideCtx {
     __device__ WideCtx(const DevMech &m, const KParams &k, unsigned char *lds, double *jac)
         : K(k), L((const unsigned char *)m.img.src, m.img, m.ns_pad), tid(threadIdx.x), wave(threadIdx.x >> 6),
           lane(threadIdx.x & 63), ne(m.ne), ns(m.ns), nr(m.nr), lda(m.ne | 1), dummy((unsigned)m.img.dummy), qr(0.),
-          rho0(0.), hr0(0.), rp(0), J(jac) {
+          rho0(0.), hr0(0.), rp(0), J(jac), G64(jac + (size_t)m.ne * m.ne) {
         double *d = (double *)lds;
         scg = (double2 *)d;
         sR = d + 512;
@@ -1293,9 +1371,10 @@ struct WideCtx {
         perm = (int *)(d + 3354);
         fsc = (float *)(d + 3482);  // [1536]: [256,512) 1/pivot by step
         dsc = fsc + 256;
-        A = (float *)(d + 4250);
+        p64 = (int *)(d + 4250);
+        A = (float *)(d + 4378);
     }
-    __host__ __device__ static size_t lds_bytes(int ne) { return 8 * 4250 + 4 * (size_t)(ne + 1) * (size_t)(ne | 1); }
+    __host__ __device__ static size_t lds_bytes(int ne) { return 8 * 4378 + 4 * (size_t)(ne + 1) * (size_t)(ne | 1); }
 
     template <class F>
     __device__ __forceinline__ void each(F f) { f(0); }
@@ -1457,6 +1536,10 @@ struct WideCtx {
         __device__ __forceinline__ void D(unsigned i, unsigned k, double v) const {
             atomicAdd(&Dm[i * lda + k], (float)v);
         }
+        int ns;
+        __device__ __forceinline__ void U(unsigned k, double v) const {
+            for (int i = 0; i < ns; i++) D(i, k, v);
+        }
     };
 
     // Analytic Jacobian (same closed forms as the team kernel).  dR/dc is gathered
@@ -1488,7 +1571,7 @@ struct WideCtx {
         double cTot = ck, cpx = isp ? cp * xk : 0.;
         wg_sum2(cTot, cpx);
         const double logPRT = log(101325. / RGAS / T);
-        const WideJacAcc acc{sR + 256 * wave, sDT + 256 * wave, A, lda};
+        const WideJacAcc acc{sR + 256 * wave, sDT + 256 * wave, A, lda, ns};
         // The waves scatter into the shared dR/dc one after another: FP32 sums are
         // order-dependent, and concurrent waves would make the Jacobian (and so the
         // step sequence) vary from run to run.  Within a wave the LDS atomics are
@@ -1590,6 +1673,7 @@ struct WideCtx {
     }
     __device__ __forceinline__ void factor_impl(double hr) {
         hr0 = hr;
+        fp64 = false;
         const int n = ne;
         int *pslot = (int *)pkey;                // [2] pivot of the next step (parity)
         if (tid < n) {
@@ -1797,14 +1881,92 @@ struct WideCtx {
         return (a0 + a1) + (a2 + a3);
     }
 
-    // G^-1 b with K.refine sweeps of iterative refinement (residual in FP64 with the
-    // FP64 Jacobian: r = b - (x - hr0 J x)).
+    // FP64 LU of G = I - hr0 J with partial pivoting and physical row interchanges (the textbook
+    // form of KernelDense.hpp:196-221), column-major in the HBM workspace G64; thread i owns row
+    // i.  The fallback of a factorisation whose FP32 solves do not refine to the FP64 accuracy.
+    __device__ void factor64() {
+        const int n = ne;
+        for (int e = tid; e < n * n; e += WIDE_THREADS) {
+            const int i = e % n, j = e / n;
+            G64[e] = (i == j ? 1. : 0.) - hr0 * J[e];
+        }
+        __syncthreads();
+        for (int k = 0; k < n; k++) {
+            const double a = (tid < n && tid >= k) ? fabs(G64[(size_t)k * n + tid]) : -1.;
+            // largest |a|, lowest row on a tie
+            unsigned long long key = a >= 0. ? (((unsigned long long)__double_as_longlong(a) & ~0xffull) |
+                                                (unsigned long long)(255 - tid)) : 0ull;
+            key = wave_max_u64(key);
+            unsigned long long *kb = (unsigned long long *)(red + 8 * rp);
+            rp ^= 1;
+            if (lane == 0) kb[wave] = key;
+            __syncthreads();
+            key = umax64(umax64(kb[0], kb[1]), umax64(kb[2], kb[3]));
+            const int p = key == 0ull ? k : 255 - (int)(key & 0xffull);
+            if (tid == 0) p64[k] = p;
+            if (p != k)
+                for (int j = tid; j < n; j += WIDE_THREADS) {
+                    const double t = G64[(size_t)j * n + k];
+                    G64[(size_t)j * n + k] = G64[(size_t)j * n + p];
+                    G64[(size_t)j * n + p] = t;
+                }
+            __syncthreads();
+            const double piv = G64[(size_t)k * n + k];
+            double l = 0.;
+            if (tid > k && tid < n && piv != 0.) {
+                l = G64[(size_t)k * n + tid] / piv;
+                G64[(size_t)k * n + tid] = l;
+            }
+            if (tid > k && tid < n)
+                for (int j = k + 1; j < n; j++) G64[(size_t)j * n + tid] = fma(-l, G64[(size_t)j * n + k], G64[(size_t)j * n + tid]);
+            __syncthreads();
+        }
+        fp64 = true;
+    }
+    // bs <- G^-1 bs with the FP64 factors (row interchanges, then L and U by columns).
+    __device__ void solve64_bs() {
+        const int n = ne;
+        if (tid == 0)
+            for (int k = 0; k < n; k++) {
+                const int p = p64[k];
+                if (p != k) { const double t = bs[k]; bs[k] = bs[p]; bs[p] = t; }
+            }
+        __syncthreads();
+        for (int k = 0; k < n; k++) {
+            const double xk = bs[k];
+            if (tid > k && tid < n) bs[tid] = fma(-G64[(size_t)k * n + tid], xk, bs[tid]);
+            __syncthreads();
+        }
+        for (int k = n - 1; k >= 0; k--) {
+            if (tid == k) bs[k] = bs[k] / G64[(size_t)k * n + k];
+            __syncthreads();
+            const double xk = bs[k];
+            if (tid < k) bs[tid] = fma(-G64[(size_t)k * n + tid], xk, bs[tid]);
+            __syncthreads();
+        }
+    }
+
+    // G^-1 b: the FP32 solve, then up to K.refine sweeps of iterative refinement with the residual
+    // in FP64 from the FP64 Jacobian, r = b - (x - hr0 J x), until the correction's WRMS norm in
+    // the integration's own weights is below REFINE_TOL (1e-5 of the tolerance, far below the Newton
+    // test l0 ||dx|| < 1).  When a sweep fails to halve the correction, or the sweeps run out, the
+    // factorisation switches to FP64 (factor64) and this and the following solves use it.
+    // K.refine = 0: the FP32 solve alone, unchecked.
+    static constexpr double REFINE_TOL = 1.e-5;
     __device__ __forceinline__ void solve_impl(int vid) {
         const double b = V[vid][0];
         if (tid < ne) bs[tid] = b;
         __syncthreads();
+        if (fp64) {
+            solve64_bs();
+            V[vid][0] = tid < ne ? bs[tid] : 0.;
+            __syncthreads();
+            return;
+        }
         lu_solve_bs();
         double x = tid < ne ? bs[tid] : 0.;
+        bool ok = K.refine == 0;
+        double prev = BIG_D;
         for (int it = 0; it < K.refine; it++) {
             const double jx = jrow_dot();
             const double r = b - (x - hr0 * jx);
@@ -1812,9 +1974,23 @@ struct WideCtx {
             if (tid < ne) bs[tid] = r;
             __syncthreads();
             lu_solve_bs();
-            if (tid < ne) x += bs[tid];
-            __syncthreads();
+            const double d = tid < ne ? bs[tid] : 0.;
+            x += d;
+            const double dw = d * V[OOE_][0];
+            const double nd = sqrt(wg_sum(dw * dw) / (double)ne);
             if (tid < ne) bs[tid] = x;
+            __syncthreads();
+            if (nd <= REFINE_TOL) { ok = true; break; }
+            if (!(nd <= 0.5 * prev)) break;
+            prev = nd;
+        }
+        if (!ok) {
+            if (tid == 0 && K.fp64_fallbacks) atomicAdd(K.fp64_fallbacks, 1ull);
+            factor64();
+            if (tid < ne) bs[tid] = b;
+            __syncthreads();
+            solve64_bs();
+            x = tid < ne ? bs[tid] : 0.;
             __syncthreads();
         }
         V[vid][0] = tid < ne ? x : 0.;
@@ -1840,7 +2016,7 @@ __global__ void __launch_bounds__(256) wide_kernel(DevMech M, KParams K, int64_t
                                                    int *counters, unsigned *queue, const int *order,
                                                    double *jws) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    WideCtx<XK> c(M, K, lds_raw, jws + (size_t)blockIdx.x * M.ne * M.ne);
+    WideCtx<XK> c(M, K, lds_raw, jws + (size_t)blockIdx.x * 2 * M.ne * M.ne);
     const int tid = threadIdx.x, ns = M.ns, ne = M.ne;
     int *cell_slot = (int *)(c.red + 16);
     for (;;) {
@@ -1909,7 +2085,7 @@ __global__ void __launch_bounds__(256) wide_kernel(DevMech M, KParams K, int64_t
 // integrator is reused.  y: SoA [ne][n]; f: SoA [ne][n]; J: [n][ne][ne] row-major.
 template <bool XK>
 __global__ void __launch_bounds__(256) equations_kernel(DevMech M, KParams K, int64_t n, const double *yin,
-                                                        double *f, double *J, double *jws) {
+                                                        double *f, double *J) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     {
         const int4 *src = (const int4 *)M.img.src;
@@ -1919,9 +2095,9 @@ __global__ void __launch_bounds__(256) equations_kernel(DevMech M, KParams K, in
         __syncthreads();
     }
     const int wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
-    double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * team_wave_bytes(M));
+    double *wave_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * team_wave_bytes(M.ne));
     const size_t gw = (size_t)blockIdx.x * wpb + wave;
-    TeamCtx<64, XK> c(M, K, lds_raw, wave_lds, jws + gw * 64 * (size_t)M.ne);
+    TeamCtx<64, XK> c(M, K, lds_raw, wave_lds);
     const int lane = c.lane, ne = M.ne;
     for (int64_t cell = (int64_t)gw; cell < n; cell += (int64_t)gridDim.x * wpb) {
         c.V[VB_][0] = lane < ne ? yin[(int64_t)lane * n + cell] : 0.;
@@ -1932,7 +2108,7 @@ __global__ void __launch_bounds__(256) equations_kernel(DevMech M, KParams K, in
             c.jacobian_analytic_impl();
             double *Jc = J + cell * (int64_t)ne * ne;
             if (lane < ne)
-                for (int j = 0; j < ne; j++) Jc[(int64_t)lane * ne + j] = c.Jl[j * 64 + lane];
+                for (int j = 0; j < ne; j++) Jc[(int64_t)lane * ne + j] = c.Jl[j * 65 + lane];
         }
         wave_sync();
     }
@@ -2185,14 +2361,11 @@ __global__ void reset_queue_kernel(unsigned *queue) { *queue = 0u; }
 // ---------------------------------------------------------------------------
 int team_max_neq() { return 64; }
 
-size_t team_jac_doubles(const DevMech &m) { return (size_t)64 * m.ne; }
+int team_waves_per_cu(const DevMech &m) { return 4 * (m.ne <= 16 ? TEAM_WAVES_PER_SIMD(16) : TEAM_WAVES_PER_SIMD(64)); }
 
-int team_waves_per_block(const DevMech &m, size_t lds_per_block) {
-    if (!m.img.team_ok || m.ne > 64) return 0;
-    const size_t per = team_wave_bytes(m);
-    if ((size_t)m.img.bytes + per > lds_per_block) return 0;
-    const size_t w = (lds_per_block - (size_t)m.img.bytes) / per;
-    return (int)(w < 4 ? w : 4);
+bool team_shape_fits(const DevMech &m, int wpb, int blocks_per_cu, size_t lds_cu) {
+    if (!m.img.team_ok || m.ne > 64 || wpb < 1 || wpb > 4 || blocks_per_cu < 1) return false;
+    return (size_t)blocks_per_cu * ((size_t)m.img.bytes + (size_t)wpb * team_wave_bytes(m.ne)) <= lds_cu;
 }
 
 #endif  // LSM_IN_PART(6)
@@ -2200,8 +2373,8 @@ int team_waves_per_block(const DevMech &m, size_t lds_per_block) {
 template <int NMAX, bool XK>
 static int launch_team_n(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                          double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order,
-                         int grid, int wpb, double *jws, hipStream_t s) {
-    const size_t lds = (size_t)m.img.bytes + (size_t)wpb * team_wave_bytes(m);
+                         int grid, int wpb, hipStream_t s) {
+    const size_t lds = (size_t)m.img.bytes + (size_t)wpb * team_wave_bytes(m.ne);
     static size_t lds_set = 65536;   // largest dynamic LDS size granted so far (not re-set under graph capture)
     if (lds > lds_set) {
         if (hipFuncSetAttribute((const void *)team_kernel<NMAX, XK>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2210,7 +2383,7 @@ static int launch_team_n(const DevMech &m, const KParams &k, int64_t n, const do
         lds_set = lds;
     }
     hipLaunchKernelGGL((team_kernel<NMAX, XK>), dim3(grid), dim3(64 * wpb), lds, s, m, k, n, dt, yin, yout, status, q,
-                       counters, queue, order, jws);
+                       counters, queue, order);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -2218,13 +2391,13 @@ static int launch_team_n(const DevMech &m, const KParams &k, int64_t n, const do
 #define LSM_TEAM_LAUNCHER(NM)                                                                                       \
     int launch_team_##NM(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,        \
                          double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order,    \
-                         int grid, int wpb, double *jws, hipStream_t s)
+                         int grid, int wpb, hipStream_t s)
 #define LSM_TEAM_LAUNCHER_BODY(NM)                                                                                  \
     LSM_TEAM_LAUNCHER(NM) {                                                                                         \
         return m.xk ? launch_team_n<NM, true>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, \
-                                              jws, s)                                                               \
+                                              s)                                                                    \
                     : launch_team_n<NM, false>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid,    \
-                                               wpb, jws, s);                                                        \
+                                               wpb, s);                                                             \
     }
 LSM_TEAM_LAUNCHER(16);
 LSM_TEAM_LAUNCHER(32);
@@ -2250,14 +2423,14 @@ LSM_TEAM_LAUNCHER_BODY(64)
 #if LSM_IN_PART(6)
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
                 int *status, double *q, int *counters, unsigned *queue, const int *order, int grid, int wpb,
-                double *jws, void *stream) {
+                void *stream) {
     hipStream_t s = (hipStream_t)stream;
     if (wpb < 1 || wpb > 4) return -2;
-    if (m.ne <= 16) return launch_team_16(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, jws, s);
-    if (m.ne <= 32) return launch_team_32(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, jws, s);
-    if (m.ne <= 48) return launch_team_48(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, jws, s);
-    if (m.ne <= 56) return launch_team_56(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, jws, s);
-    if (m.ne <= 64) return launch_team_64(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, jws, s);
+    if (m.ne <= 16) return launch_team_16(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, s);
+    if (m.ne <= 32) return launch_team_32(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, s);
+    if (m.ne <= 48) return launch_team_48(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, s);
+    if (m.ne <= 56) return launch_team_56(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, s);
+    if (m.ne <= 64) return launch_team_64(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, s);
     return -2;
 }
 
@@ -2511,26 +2684,26 @@ int launch_properties(const DevMech &m, double p_default, int64_t n, const doubl
 }
 
 int launch_equations(const DevMech &m, const KParams &k, int64_t n, const double *yin, double *f, double *J,
-                     int grid, int wpb, double *jws, void *stream) {
+                     int grid, int wpb, void *stream) {
     if (wpb < 1 || wpb > 4 || m.ne > 64) return -2;
     const int64_t need = (n + wpb - 1) / wpb;
     const int g = (int)(need < grid ? need : grid);
     if (g < 1) return 0;
-    const size_t lds = (size_t)m.img.bytes + (size_t)wpb * team_wave_bytes(m);
+    const size_t lds = (size_t)m.img.bytes + (size_t)wpb * team_wave_bytes(m.ne);
     const void *fn = m.xk ? (const void *)equations_kernel<true> : (const void *)equations_kernel<false>;
     if (lds > 65536 && hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return -1;
     if (m.xk)
         hipLaunchKernelGGL(equations_kernel<true>, dim3(g), dim3(64 * wpb), lds, (hipStream_t)stream, m, k, n, yin, f,
-                           J, jws);
+                           J);
     else
         hipLaunchKernelGGL(equations_kernel<false>, dim3(g), dim3(64 * wpb), lds, (hipStream_t)stream, m, k, n, yin, f,
-                           J, jws);
+                           J);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 size_t wide_lds_bytes(const DevMech &m) { return WideCtx<false>::lds_bytes(m.ne); }
-size_t wide_jac_doubles(const DevMech &m) { return (size_t)m.ne * m.ne; }
+size_t wide_jac_doubles(const DevMech &m) { return (size_t)2 * m.ne * m.ne; }   // J, then the FP64 LU
 
 int launch_wide(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
                 int *status, double *q, int *counters, unsigned *queue, const int *order, int grid, double *jws,

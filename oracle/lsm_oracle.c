@@ -633,6 +633,7 @@ typedef struct {
     unsigned numberOfErrorCheckFailure, numberOfErrorCheckSuccess;
     unsigned iterConstSteps, iterMinimumOrder, maxOrderUsed;
     int fixedOrderMin;
+    double jac_scale;          /* factor on the FD increments (1: the reference's), test option */
     long clamps, newton_iters;
 } bdf_t;
 
@@ -658,6 +659,7 @@ static void bdf_fresh(bdf_t *s) {
     s->max_step_size = BIG_D;
     s->max_steps = 500000;
     s->clamps = 0; s->newton_iters = 0;
+    s->jac_scale = 1.;
 }
 
 static void bdf_alloc(bdf_t *s, int ne) {
@@ -733,6 +735,7 @@ static void numerical_jacobian(bdf_t *s, double *y, const double *f, double h, c
             const double hJf = hf / e[j];
             hJ = fmax(hJ, hJf);
             hJ = fmax(hJ, ZERO_DER);
+            hJ *= s->jac_scale;
             if (yh + hJ > s->ymax[j]) hJ = -hJ;
         }
         y[j] += hJ;
@@ -1287,6 +1290,7 @@ static int integrate_cell(cell_ctx_t *c, const lsm_oracle_options *o, double dt,
         c->s.rhs = reactor_rhs; c->s.ctx = &c->R;
         c->s.rel_tol = o->rel_tol; c->s.abs_tol = o->abs_tol; c->s.full_pivoting = o->full_pivoting;
         if (o->max_steps > 0) c->s.max_steps = (unsigned)o->max_steps;
+        if (o->jac_increment_scale > 0.) c->s.jac_scale = o->jac_increment_scale;
         c->s.min_c = c->s.max_c = 1; c->s.ymin = c->ymin; c->s.ymax = c->ymax;
         c->R.nrhs = 0;
         set_initial_conditions(&c->s, o->t0, c->y0);            /* :138 */

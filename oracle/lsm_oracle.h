@@ -48,6 +48,12 @@ typedef struct lsm_oracle_options {
     const double *rho;         /* constant volume: per-cell density rho0 [kg/m3]
                                   (rhoCells); NULL: ideal-gas density of the
                                   initial state at `pressure`                  */
+    double jac_increment_scale; /* test option: factor on the forward-difference
+                                  increments of the numerical Jacobian (<= 0 or 1:
+                                  the reference's increments).  Other values give
+                                  an equally valid Jacobian approximation, which
+                                  measures how the reference's result depends on
+                                  its Newton matrix.                          */
 } lsm_oracle_options;
 
 typedef void (*lsm_oracle_rhs_fn)(void *ctx, const double *y, double *f);

@@ -5,8 +5,9 @@ Six cells of the seeded 160-species / 1000-reaction workload (the four cheapest
 and two median-cost cells of a 32-cell sample of the field), integrated over
 dt = 1e-5 s by the oracle at rtol 1e-6 / atol 1e-12 and rtol 1e-8 / atol 1e-14,
 with the oracle's self-envelope at each (largest deviation, in units of
-rtol |y| + atol, of three runs on inputs perturbed by 1e-15 relative), and at
-rtol 1e-10 / atol 1e-16 as the tight solution.  The GPU tests of the wide
+rtol |y| + atol, of five runs on inputs perturbed by 1e-15 relative), its
+Jacobian envelope (largest deviation when its finite-difference increments are
+scaled by 0.5 and 2), and at rtol 1e-10 / atol 1e-16 as the tight solution.  The GPU tests of the wide
 (FP32-LU) integrator compare against these without re-running the oracle, whose
 cost on this mechanism is seconds per cell.
 """
@@ -23,7 +24,7 @@ import pyoracle  # noqa: E402
 from laminarsmoke_amd.fields import workload_cells  # noqa: E402
 
 sys.path.insert(0, os.path.dirname(HERE))
-from parity import oracle_envelope  # noqa: E402
+from parity import oracle_envelope, oracle_jacobian_envelope  # noqa: E402
 
 
 def main():
@@ -39,7 +40,8 @@ def main():
         out[f"y_{tag}"], out[f"status_{tag}"], out[f"steps_{tag}"] = yo, sto, co[:, 0]
         if tag != "tight":
             out[f"env_{tag}"] = oracle_envelope(pyoracle, mech, ys, 1e-5, rt, at, yo)
-            print("envelope", tag, out[f"env_{tag}"], flush=True)
+            out[f"jenv_{tag}"] = oracle_jacobian_envelope(pyoracle, mech, ys, 1e-5, rt, at, yo)
+            print("envelope", tag, out[f"env_{tag}"], "jacobian envelope", out[f"jenv_{tag}"], flush=True)
         print(tag, sto, co[:, 0], flush=True)
     np.savez(os.path.join(HERE, "cfg4_oracle.npz"), **out)
 

@@ -16,6 +16,11 @@ chaotic cells: 2.3 against 16.4 times the tolerance on one 14-species cell).  On
 cells where the oracle is stable (envelope < 10/3) this is exactly the 10x target
 bound.
 
+Variants that change the Newton matrix on purpose (the analytic Jacobian, the FP32 LU with
+refinement) use the same bound with the envelope widened by the oracle's own deviation when its
+finite-difference increments are scaled by 0.5 and 2 (oracle_jacobian_envelope): another valid
+Newton matrix of the reference algorithm moves its result by as much as these variants do.
+
 Element mass: the BDF iteration conserves linear invariants exactly, but the
 reference's yMin = 0 clamps of predicted and Newton states add mass at the atol
 scale.  Cells without clamp events are held to 1e-12; cells with clamp events to
@@ -46,6 +51,28 @@ def oracle_envelope(oracle, mech, y, dt, rtol, atol, y_ref, samples=5, eps=1e-15
         y2, *_ = oracle.chemistry_step(mech, yp, dt, rtol=rtol, atol=atol, threads=threads, **kw)
         env = np.maximum(env, ratio(y2, y_ref, rtol, atol).max(axis=0))
     return env
+
+
+def oracle_jacobian_envelope(oracle, mech, y, dt, rtol, atol, y_ref, scales=(0.5, 2.0), threads=None, **kw):
+    """Per-cell max deviation of the oracle from itself when its finite-difference Jacobian is taken
+    with its increments scaled by 0.5 and 2: equally valid Newton matrices of the reference
+    algorithm.  Variants whose Newton matrix differs on purpose (the analytic Jacobian, the FP32
+    factorisation with refinement) move the step sequence by as much; on the 14-species cells the
+    oracle moves by up to 27 x the tolerance this way."""
+    threads = threads or ORACLE_THREADS
+    env = np.zeros(y.shape[1])
+    for sc in scales:
+        y2, *_ = oracle.chemistry_step(mech, y, dt, rtol=rtol, atol=atol, threads=threads, jac_increment_scale=sc,
+                                       **kw)
+        env = np.maximum(env, ratio(y2, y_ref, rtol, atol).max(axis=0))
+    return env
+
+
+def newton_matrix_changed(method):
+    """Variants whose Newton matrix differs from the reference's on purpose: the analytic
+    Jacobian, and the wide integrator's FP32 factorisation (its refined solves carry errors up to
+    1e-5 of the tolerance, where the FP64 kernels carry rounding only)."""
+    return method.endswith("-analytic") or method.startswith("wide")
 
 
 def assert_parity(y_gpu, y_ref, rtol, atol, envelope=None, what=""):

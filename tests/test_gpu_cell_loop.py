@@ -25,7 +25,8 @@ import pytest
 from laminarsmoke_amd import api
 from laminarsmoke_amd.fields import workload_cells, coflow_field
 from laminarsmoke_amd.mechanism import workload_mechanism
-from parity import assert_parity, assert_element_conservation, oracle_envelope
+from parity import (assert_parity, assert_element_conservation, oracle_envelope, oracle_jacobian_envelope,
+                    newton_matrix_changed)
 
 pytestmark = pytest.mark.gpu
 
@@ -52,15 +53,26 @@ def run_gpu(mech, y, method, dt=DT, t0=0.0, q=None, **kw):
         step.close()
 
 
-def check_q(oracle, mech, yg, st, q_g, q_in):
-    """Q of integrated cells = QR at the cell's final state; Q of the other cells untouched.  The
-    kernel takes QR before the renormalisation, which moves Y by a few ulp: where forward and
-    reverse rates nearly cancel that moves QR by up to ~5e-8 relative (the oracle against itself
-    on the 53-species cells), hence 1e-6."""
+def check_q(oracle, mech, yg, st, q_g, q_in, rtol=RTOL, atol=ATOL):
+    """Q of integrated cells = QR at the cell's final state; Q of the other cells untouched.
+
+    Q is the QR of the integrator's last right-hand-side evaluation (HeatRelease,
+    KineticsMap_CHEMKIN.hpp:1961-1964, read at chemistry_DI.H:168), which is taken at the last
+    Newton iterate, not at the accepted state: the two differ by the last Newton correction, which
+    the convergence test keeps within the weights rtol |y_i| + atol.  So the bound is 1e-9 |QR| plus
+    the change of QR over one weight in every component, sum_i |QR(y + w_i e_i) - QR(y)|: a Q of
+    another cell, of the initial state, or from a wrong formula is far outside it."""
     for c in range(yg.shape[1]):
         if st[c] > 0:
-            _, qr = oracle.rhs(mech, yg[:, c])
-            assert abs(q_g[c] - qr) <= 1e-6 * abs(qr) + 1e-300, (c, q_g[c], qr)
+            y = yg[:, c]
+            _, qr = oracle.rhs(mech, y)
+            w = rtol * np.abs(y) + atol
+            tol = 0.0
+            for i in range(y.size):
+                yp = y.copy()
+                yp[i] += w[i]
+                tol += abs(oracle.rhs(mech, yp)[1] - qr)
+            assert abs(q_g[c] - qr) <= 1e-9 * abs(qr) + tol, (c, q_g[c], qr, tol)
         else:
             assert q_g[c] == q_in[c], (c, q_g[c], q_in[c])
 
@@ -68,8 +80,11 @@ def check_q(oracle, mech, yg, st, q_g, q_in):
 def parity_at(oracle, mech, y, method, t0, what):
     n = y.shape[1]
     q_in = np.full(n, -12345.0)
-    yo, st_o, _, c_o = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=8, t0=t0)
-    env = oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, yo, samples=5, t0=t0)
+    yo, st_o, q_o, c_o = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=8, t0=t0, q=q_in)
+    check_q(oracle, mech, yo, st_o, q_o, q_in)                          # the criterion holds for the oracle
+    env = oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, yo, samples=8, t0=t0)
+    if newton_matrix_changed(method):
+        env = np.maximum(env, oracle_jacobian_envelope(oracle, mech, y, DT, RTOL, ATOL, yo, t0=t0))
     yg, st_g, q_g, c_g = run_gpu(mech, y, method, t0=t0, q=q_in)
     assert np.array_equal(st_g, st_o), (what, st_g, st_o)
     cell, frac = assert_parity(yg, yo, RTOL, ATOL, env, what=what)
@@ -129,6 +144,7 @@ def test_gate_skip_and_illegal_constraints(hip_device, oracle, method):
     print(f"gate/-9 method={method} cells={n}", flush=True)
     yo, st_o, q_o, _ = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=8, min_temperature=1000.0,
                                              q=q_in)
+    check_q(oracle, mech, yo, st_o, q_o, q_in)
     yg, st_g, q_g, c_g = run_gpu(mech, y, method, q=q_in, min_temperature=1000.0)
     assert (st_o[:8] == 0).all() and (st_o[8:12] == -9).all() and (st_o[12:16] == 2).all(), st_o
     assert np.array_equal(st_g, st_o), (st_g, st_o)
