@@ -1085,7 +1085,7 @@ struct TeamCtx {
     // registers (lane i holds row i, the pivot row is broadcast with v_readlane)
     // and kept there for the solves.  Rows are not moved: lane i records the
     // step ps at which it was chosen; the largest |G(i,k)| among rows not yet
-    // chosen wins (ties and the 6 lowest mantissa bits go to the lower lane).
+    // chosen wins (the lowest lane on a tie).
     __device__ __forceinline__ void factor_impl(double hr0) {
         // Opaque copies of lane and ne: without them the compiler hoists the NMAX
         // identity entries and the NMAX "j < ne" masks out of the persistent cell
@@ -1344,7 +1344,7 @@ struct WideCtx {
     unsigned long long *pkey; // LDS [2]    pivot keys (parity of the elimination step)
     float *fsc;               // LDS [1536] factorisation scratch
     float *dsc;               // LDS [256]  1 / pivot of step k (U rows are stored scaled by it)
-    float *A;                 // LDS [ne + 1][lda] LU factors (spare last row); dR/dc in an analytic Jacobian
+    float *A;                 // LDS [ne + 1][lda] LU factors (spare last row); FP64 blocks of dR/dc in an analytic Jacobian
     double *J;                // HBM [ne][ne] column-major, J[j*ne + i] = d f_i / d y_j
     double *G64;              // HBM [ne][ne] column-major: FP64 LU of G when refinement does not converge
     int *p64;                 // LDS [256]  row interchanges of the FP64 LU
@@ -1525,25 +1525,31 @@ struct WideCtx {
         }
     }
 
+    // Accumulator of the analytic Jacobian's reaction loop for one block of rows: dR_k/dc_i for
+    // i in [i0, i1) in FP64 (D64, row stride ldd), R_k and dR_k/dT only in the first block.
     struct WideJacAcc {
         double *sRw, *sDTw;
-        float *Dm;
-        int lda;
+        double *D64;
+        int ldd, i0, i1, ns;
+        bool first;
         __device__ __forceinline__ void R(unsigned k, double r, double drdT) const {
+            if (!first) return;
             atomicAdd(&sRw[k], r);
             atomicAdd(&sDTw[k], drdT);
         }
         __device__ __forceinline__ void D(unsigned i, unsigned k, double v) const {
-            atomicAdd(&Dm[i * lda + k], (float)v);
+            if ((int)i >= i0 && (int)i < i1 && (int)k < ns) atomicAdd(&D64[(i - i0) * ldd + k], v);
         }
-        int ns;
         __device__ __forceinline__ void U(unsigned k, double v) const {
-            for (int i = 0; i < ns; i++) D(i, k, v);
+            if ((int)k < ns)
+                for (int i = i0; i < i1; i++) atomicAdd(&D64[(i - i0) * ldd + k], v);
         }
     };
 
-    // Analytic Jacobian (same closed forms as the team kernel).  dR/dc is gathered
-    // in FP32 in the LU area of LDS: it only feeds the FP32 Newton matrix.
+    // Analytic Jacobian (same closed forms as the team kernel).  dR/dc is gathered in FP64 in
+    // the LU area of LDS, which holds a block of its rows: the reaction loop runs once per block
+    // (two blocks for the mechanisms here), each block's rows go to J as they are, and a last
+    // sweep over J completes the entries that need all of dR/dc (S_k = sum_i dR_k/dc_i c_i).
     __device__ __forceinline__ void jacobian_analytic() {
         const int k = tid;
         const bool isp = k < ns;
@@ -1567,31 +1573,44 @@ struct WideCtx {
         if (k == (int)dummy) { scg[k] = make_double2(1., 0.); sH[k] = 0.; }
 #pragma unroll
         for (int w = 0; w < 4; w++) { sR[w * 256 + k] = 0.; sDT[w * 256 + k] = 0.; }
-        for (int idx = tid; idx < ns * lda; idx += WIDE_THREADS) A[idx] = 0.f;
         double cTot = ck, cpx = isp ? cp * xk : 0.;
         wg_sum2(cTot, cpx);
         const double logPRT = log(101325. / RGAS / T);
-        const WideJacAcc acc{sR + 256 * wave, sDT + 256 * wave, A, lda, ns};
-        // The waves scatter into the shared dR/dc one after another: FP32 sums are
-        // order-dependent, and concurrent waves would make the Jacobian (and so the
-        // step sequence) vary from run to run.  Within a wave the LDS atomics are
-        // applied in a fixed order.
-        for (int w = 0; w < WIDE_THREADS / 64; w++) {
-            if (wave == w)
-                for (int j = tid; j < nr; j += WIDE_THREADS)
-                    rxn_jac<XK>(L, scg, sH, ns, dummy, j, T, logT, uT, logPRT, K.pressure, cTot, acc);
+        double *D64 = (double *)A;
+        const int ldd = ns | 1;
+        const int rows = (int)(((size_t)4 * (ne + 1) * lda) / ((size_t)8 * ldd));
+        double S = 0., Acol = 0.;
+        for (int i0 = 0; i0 < ns; i0 += rows) {
+            const int i1 = i0 + rows < ns ? i0 + rows : ns;
+            for (int idx = tid; idx < (i1 - i0) * ldd; idx += WIDE_THREADS) D64[idx] = 0.;
             __syncthreads();
+            const WideJacAcc acc{sR + 256 * wave, sDT + 256 * wave, D64, ldd, i0, i1, ns, i0 == 0};
+            // The waves scatter into the shared block one after another: FP64 sums are
+            // order-dependent, and concurrent waves would make the Jacobian (and so the step
+            // sequence) vary from run to run.  Within a wave the LDS atomics are applied in a
+            // fixed order.
+            for (int w = 0; w < WIDE_THREADS / 64; w++) {
+                if (wave == w)
+                    for (int j = tid; j < nr; j += WIDE_THREADS)
+                        rxn_jac<XK>(L, scg, sH, ns, dummy, j, T, logT, uT, logPRT, K.pressure, cTot, acc);
+                __syncthreads();
+            }
+            if (isp) {
+                for (int i = i0; i < i1; i++) S += D64[(i - i0) * ldd + k] * scg[i].x;
+                if (k >= i0 && k < i1)
+                    for (int i = 0; i < ns; i++) Acol += sH[i] * D64[(k - i0) * ldd + i];
+                for (int j = i0; j < i1; j++) J[(size_t)j * ne + k] = D64[(j - i0) * ldd + k];
+            }
+            __syncthreads();   // the block is overwritten by the next one
         }
         const double Rk = isp ? ((sR[k] + sR[256 + k]) + (sR[512 + k] + sR[768 + k])) : 0.;
         const double dRdT = isp ? ((sDT[k] + sDT[256 + k]) + (sDT[512 + k] + sDT[768 + k])) : 0.;
-        double S = 0.;
-        if (isp)
-            for (int i = 0; i < ns; i++) S += (double)A[i * lda + k] * scg[i].x;
         const double fk = isp ? (mwk * Rk) / rho : 0.;
         if (isp) {
             for (int j = 0; j < ns; j++) {
                 const double mwj = L.sp[15 * P + j];
-                J[(size_t)j * ne + k] = (mwk / (rho * mwj)) * (rho * (double)A[j * lda + k] - MW * S) + fk * MW / mwj;
+                const double Djk = J[(size_t)j * ne + k];
+                J[(size_t)j * ne + k] = (mwk / (rho * mwj)) * (rho * Djk - MW * S) + fk * MW / mwj;
             }
             J[(size_t)ns * ne + k] = (mwk / rho) * (dRdT - S * uT) + fk * uT;
         }
@@ -1601,9 +1620,6 @@ struct WideCtx {
         wg_sum2(hs, hS);
         const double QR = -hs * RGAS * T;
         const double fT = K.energy ? QR / Phi : 0.;
-        double Acol = 0.;
-        if (isp)
-            for (int i = 0; i < ns; i++) Acol += sH[i] * (double)A[k * lda + i];
         double dQ = isp ? -(cp * RGAS * Rk + h * RGAS * T * (dRdT - S * uT)) : 0.;
         double dP = isp ? Yk * dcp * RGAS / mwk : 0.;
         wg_sum2(dQ, dP);

@@ -42,6 +42,7 @@ typedef struct {
     int *fptr, *frx;     /* [ns+1], entries                                  */
     double *fnu;         /* signed coefficient (negative for reactants)      */
     int *fcls;           /* 1,2,3,4(=0.5),5(other)                           */
+    int rev_sum;         /* test option: formation-rate sums in reverse order */
     /* per-reaction lists sorted by species (reactant entry first on ties):
      * rx_* for dH/dS (signed nu), fw_* reactants, bw_* products               */
     int *rx_ptr, *rx_sp, *fw_ptr, *fw_sp, *bw_ptr, *bw_sp;
@@ -82,7 +83,7 @@ static void sort_pairs(int n, int *sp, double *nu) {
 
 static void mech_init(mech_t *M, const lsm_mechanism *m) {
     const int ns = m->ns, nr = m->nr;
-    M->m = m; M->ns = ns; M->nr = nr;
+    M->m = m; M->ns = ns; M->nr = nr; M->rev_sum = 0;
     M->cpHT = malloc(sizeof(double) * 5 * ns); M->cpLT = malloc(sizeof(double) * 5 * ns);
     M->dhHT = malloc(sizeof(double) * 6 * ns); M->dhLT = malloc(sizeof(double) * 6 * ns);
     M->dsHT = malloc(sizeof(double) * 6 * ns); M->dsLT = malloc(sizeof(double) * 6 * ns);
@@ -425,7 +426,9 @@ static void kinetics_eval(const mech_t *M, kin_ws_t *w, double T, double P, cons
     /* FormationRatesFromReactionRates StoichiometricMap.hpp:656-730 */
     for (int i = 0; i < ns; i++) {
         double rate = 0.;
-        for (int e = M->fptr[i]; e < M->fptr[i + 1]; e++) {
+        const int e0 = M->fptr[i], e1 = M->fptr[i + 1];
+        for (int q = e0; q < e1; q++) {
+            const int e = M->rev_sum ? e0 + e1 - 1 - q : q;
             const double nu = M->fnu[e];
             if (nu < 0) rate -= cls_val(M->fcls[e], -nu, w->r[M->frx[e]]);
             else rate += cls_val(M->fcls[e], nu, w->r[M->frx[e]]);
@@ -1291,6 +1294,7 @@ static int integrate_cell(cell_ctx_t *c, const lsm_oracle_options *o, double dt,
         c->s.rel_tol = o->rel_tol; c->s.abs_tol = o->abs_tol; c->s.full_pivoting = o->full_pivoting;
         if (o->max_steps > 0) c->s.max_steps = (unsigned)o->max_steps;
         if (o->jac_increment_scale > 0.) c->s.jac_scale = o->jac_increment_scale;
+        c->M.rev_sum = o->reverse_sums ? 1 : 0;
         c->s.min_c = c->s.max_c = 1; c->s.ymin = c->ymin; c->s.ymax = c->ymax;
         c->R.nrhs = 0;
         set_initial_conditions(&c->s, o->t0, c->y0);            /* :138 */

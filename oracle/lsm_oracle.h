@@ -54,6 +54,12 @@ typedef struct lsm_oracle_options {
                                   an equally valid Jacobian approximation, which
                                   measures how the reference's result depends on
                                   its Newton matrix.                          */
+    int32_t reverse_sums;      /* test option: 1 sums each species' formation
+                                  rate over its reactions in reverse order (0:
+                                  the reference's order).  An equally valid
+                                  rounding of the same sums, which measures how
+                                  the reference's result depends on the order of
+                                  its floating-point additions.               */
 } lsm_oracle_options;
 
 typedef void (*lsm_oracle_rhs_fn)(void *ctx, const double *y, double *f);

@@ -28,7 +28,7 @@ class OracleOptions(ctypes.Structure):
         ("pressure", ctypes.c_double), ("min_temperature", ctypes.c_double),
         ("energy_equation", ctypes.c_int32), ("mass_fractions_tol", ctypes.c_double),
         ("t0", ctypes.c_double), ("constant_volume", ctypes.c_int32), ("strang_compact", ctypes.c_int32),
-        ("rho", _d), ("jac_increment_scale", ctypes.c_double),
+        ("rho", _d), ("jac_increment_scale", ctypes.c_double), ("reverse_sums", ctypes.c_int32),
     ]
 
 
@@ -70,12 +70,13 @@ def _p(a, t=ctypes.c_double):
 
 def options(rtol=1e-7, atol=1e-12, full_pivoting=False, max_steps=0, pressure=101325.0,
             min_temperature=0.0, energy=True, mass_fractions_tol=1e-4, t0=0.0, constant_volume=False,
-            compact=False, rho=None, jac_increment_scale=1.0) -> OracleOptions:
+            compact=False, rho=None, jac_increment_scale=1.0, reverse_sums=False) -> OracleOptions:
     """rho: per-cell densities of the constant-volume reactor (kept alive by the caller);
-    jac_increment_scale: factor on the FD-Jacobian increments (1: the reference's)."""
+    jac_increment_scale: factor on the FD-Jacobian increments (1: the reference's);
+    reverse_sums: formation rates summed over reactions in reverse order (False: the reference's)."""
     return OracleOptions(rtol, atol, int(full_pivoting), int(max_steps), pressure, min_temperature,
                          int(energy), mass_fractions_tol, t0, int(constant_volume), int(compact), _p(rho),
-                         float(jac_increment_scale))
+                         float(jac_increment_scale), int(bool(reverse_sums)))
 
 
 def species_thermo(mech, T):

