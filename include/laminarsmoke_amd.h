@@ -241,6 +241,26 @@ int lsm_integrate_batch_device_rho(lsm_handle *h, int64_t n_cells, double t0, do
                                    double *d_y_out, int32_t *d_status, double *d_q, int32_t *d_counters,
                                    const int32_t *d_order, void *stream);
 
+/* lsm_integrate_batch_device_ordered (d_order may be NULL) that allocates no memory:
+ * no counterpart in the reference; for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The scratch
+ * its kernels need besides inputs and outputs (the work-queue counter; the wide
+ * kernel's per-workgroup Jacobian and FP64 LU; the plain kernel's per-cell
+ * workspaces) is a static device array of LSM_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends; the wide and plain kernels run on as many workgroups / cell
+ * slots as it holds (same results, possibly fewer in flight), and a mechanism whose
+ * single workspace exceeds it is rejected (LSM_E_INVALID).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads, other handles) race and must be serialised by the caller.  The queue
+ * counter is reset by a kernel on `stream` at every call.  FP64 refactorisations of
+ * the wide kernel are not added to lsm_fp64_fallbacks.  The current device must be
+ * the handle's; constant-volume handles are rejected (they need per-cell densities). */
+enum { LSM_NOALLOC_SCRATCH_BYTES = 134217728 };   /* 128 MiB */
+int lsm_integrate_batch_device_noalloc(lsm_handle *h, int64_t n_cells, double t0, double pressure,
+                                       const double *d_dt, const double *d_y_in, double *d_y_out,
+                                       int32_t *d_status, double *d_q, int32_t *d_counters,
+                                       const int32_t *d_order, void *stream);
+
 /* Visiting order from the per-cell counters of a previous call on the same cells
  * (the cpuChemistry field of chemistry_DI.H:170 plays this role in the reference):
  * cells sorted by BDF steps, most first (device radix sort).  Asynchronous on

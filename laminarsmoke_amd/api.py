@@ -70,6 +70,7 @@ def lib():
         L.lsm_transport_properties.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp]
         L.lsm_transport_properties_device.argtypes = [vp, i64, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_integrate_batch_device_rho.argtypes = [vp, i64, f64, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.lsm_integrate_batch_device_noalloc.argtypes = [vp, i64, f64, f64, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_cost_order.argtypes = [vp, i64, vp, vp, vp]
         L.lsm_autotune.argtypes = [vp, i64, f64, f64, vp, vp, i64, vp, vp]
         L.lsm_get_launch.argtypes = [vp, vp]
@@ -290,6 +291,14 @@ class ChemistryStep:
         else:
             _check(lib().lsm_integrate_batch_device(self.h, int(n), t0, P, dt_ptr, y_in_ptr, y_out_ptr, status_ptr,
                                                     q_ptr, counters_ptr, stream))
+
+    def integrate_device_noalloc(self, n, dt_ptr, y_in_ptr, y_out_ptr, status_ptr=None, q_ptr=None,
+                                 counters_ptr=None, stream=None, order_ptr=None, t0=0.0, pressure=None):
+        """integrate_device without any device allocation: the kernels' scratch is the library's
+        static per-GPU array (lsm_integrate_batch_device_noalloc); calls on one GPU must not overlap."""
+        _check(lib().lsm_integrate_batch_device_noalloc(self.h, int(n), float(t0), self._p(pressure), dt_ptr,
+                                                        y_in_ptr, y_out_ptr, status_ptr, q_ptr, counters_ptr,
+                                                        order_ptr, stream))
 
     def cost_order(self, n, counters_ptr, order_ptr, stream=None):
         _check(lib().lsm_cost_order(self.h, int(n), counters_ptr, order_ptr, stream))

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1024,50 +1025,151 @@ static int check_step_args(double t0, double pressure) {
     return LSM_OK;
 }
 
-static int integrate_device(lsm_handle *h, int64_t n, double t0, double pressure, const double *d_dt,
-                            const double *d_rho, const double *d_y_in, double *d_y_out, int32_t *d_status,
-                            double *d_q, int32_t *d_counters, const int32_t *d_order, void *stream) {
+// Device memory the kernels of one chemistry-step call use besides its inputs and outputs.
+struct StepScratch {
+    unsigned *queue = nullptr;                 // work-queue counter (team, wide)
+    double *wide_ws = nullptr;                 // wide: [wide_grid][2][ne][ne] Jacobian, FP64 LU
+    int wide_grid = 0;
+    unsigned long long *fallbacks = nullptr;   // wide: FP64 refactorisation count, or null
+    double *plain_ws = nullptr;                // plain: [plain_slots] per-cell workspaces
+    int64_t plain_slots = 0;
+};
+
+// Checks of a chemistry-step call and its kernel parameters; *skip when there is nothing to do.
+static int prepare_step(lsm_handle *h, int64_t n, double t0, double pressure, const double *d_dt,
+                        const double *d_rho, const double *d_y_in, double *d_y_out, KParams &kp, int &method,
+                        bool &skip) {
+    skip = false;
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (n < 0) return fail(LSM_E_INVALID, "negative cell count");
     if (int rc = check_step_args(t0, pressure)) return rc;
-    if (n == 0) return LSM_OK;
+    if (n == 0) { skip = true; return LSM_OK; }
     if (!d_dt || !d_y_in || !d_y_out) return fail(LSM_E_INVALID, "null dt / y pointer");
     if (n > (int64_t)0x7fffffff) return fail(LSM_E_INVALID, "more than 2^31-1 cells in one call");
-    const int method = pick_method(h);
-    KParams kp = h->kp;                          // per call: start time, pressure, constant-volume densities
+    method = pick_method(h);
+    kp = h->kp;                                  // per call: start time, pressure, constant-volume densities
     kp.bk.t0 = t0;
     kp.pressure = pressure;
     kp.rho = h->kp.const_volume ? d_rho : nullptr;
+    return LSM_OK;
+}
+
+// The kernels of one call, enqueued on `stream` with the given scratch.
+static int launch_step(lsm_handle *h, int method, const KParams &kp_in, int64_t n, const double *d_dt,
+                       const double *d_y_in, double *d_y_out, int32_t *d_status, double *d_q, int32_t *d_counters,
+                       const int32_t *d_order, const StepScratch &sc, void *stream) {
+    KParams kp = kp_in;
     if (method == LSM_METHOD_TEAM) {
-        if (launch_reset_queue(h->d_queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
-        int rc = launch_team(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue, d_order,
+        if (launch_reset_queue(sc.queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
+        int rc = launch_team(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, sc.queue, d_order,
                              h->team_grid, h->team_wpb, stream);
         if (rc) return fail(LSM_E_HIP, std::string("team kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
         return LSM_OK;
     }
     if (method == LSM_METHOD_WIDE) {
-        if (launch_reset_queue(h->d_queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
-        int rc = launch_wide(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_queue, d_order,
-                             h->wide_grid, h->d_wjws, stream);
+        kp.fp64_fallbacks = sc.fallbacks;
+        if (launch_reset_queue(sc.queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
+        int rc = launch_wide(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, sc.queue, d_order,
+                             sc.wide_grid, sc.wide_ws, stream);
         if (rc) return fail(LSM_E_HIP, std::string("wide kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
         return LSM_OK;
     }
-    // plain: one thread per cell slot, workspace in HBM
-    size_t free_b = 0, total_b = 0;
-    HIP_OK(hipMemGetInfo(&free_b, &total_b));
-    const size_t per = plain_work_doubles(h->dm, 1) * sizeof(double);
-    int64_t slots = std::min<int64_t>(n, (int64_t)h->num_cu * 1024);
-    const size_t budget = std::min<size_t>((size_t)32 << 30, free_b / 4 + (h->d_work ? (size_t)h->work_slots * per : 0));
-    slots = std::max<int64_t>(1, std::min<int64_t>(slots, (int64_t)(budget / per)));
-    if (slots > h->work_slots) {
-        if (h->d_work) { HIP_OK(hipStreamSynchronize((hipStream_t)stream)); (void)hipFree(h->d_work); h->d_work = nullptr; }
-        HIP_OK(hipMalloc(&h->d_work, (size_t)slots * per));
-        h->work_slots = slots;
-    }
-    int rc = launch_plain(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, h->d_work, h->work_slots,
+    int rc = launch_plain(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, sc.plain_ws, sc.plain_slots,
                           stream);
     if (rc) return fail(LSM_E_HIP, std::string("plain kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
     return LSM_OK;
+}
+
+static int integrate_device(lsm_handle *h, int64_t n, double t0, double pressure, const double *d_dt,
+                            const double *d_rho, const double *d_y_in, double *d_y_out, int32_t *d_status,
+                            double *d_q, int32_t *d_counters, const int32_t *d_order, void *stream) {
+    KParams kp;
+    int method = 0;
+    bool skip = false;
+    if (int rc = prepare_step(h, n, t0, pressure, d_dt, d_rho, d_y_in, d_y_out, kp, method, skip)) return rc;
+    if (skip) return LSM_OK;
+    StepScratch sc;
+    sc.queue = h->d_queue;
+    sc.wide_ws = h->d_wjws;
+    sc.wide_grid = h->wide_grid;
+    sc.fallbacks = h->d_fallbacks;
+    if (method == LSM_METHOD_PLAIN) {
+        // one thread per cell slot, workspace in HBM, grown on demand
+        size_t free_b = 0, total_b = 0;
+        HIP_OK(hipMemGetInfo(&free_b, &total_b));
+        const size_t per = plain_work_doubles(h->dm, 1) * sizeof(double);
+        int64_t slots = std::min<int64_t>(n, (int64_t)h->num_cu * 1024);
+        const size_t budget = std::min<size_t>((size_t)32 << 30, free_b / 4 + (h->d_work ? (size_t)h->work_slots * per : 0));
+        slots = std::max<int64_t>(1, std::min<int64_t>(slots, (int64_t)(budget / per)));
+        if (slots > h->work_slots) {
+            if (h->d_work) { HIP_OK(hipStreamSynchronize((hipStream_t)stream)); (void)hipFree(h->d_work); h->d_work = nullptr; }
+            HIP_OK(hipMalloc(&h->d_work, (size_t)slots * per));
+            h->work_slots = slots;
+        }
+        sc.plain_ws = h->d_work;
+        sc.plain_slots = h->work_slots;
+    }
+    return launch_step(h, method, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, d_order, sc, stream);
+}
+
+// ---- Chemistry step that allocates nothing ----------------------------------------------------
+// The scratch of lsm_integrate_batch_device_noalloc, one copy per GPU, held until the process
+// ends: the work-queue counter in its first 256 bytes, then the wide kernel's per-workgroup
+// Jacobian / FP64 LU workspaces or the plain kernel's per-cell-slot workspaces.  Host code
+// never passes the array's name to a kernel (on the host it is a stand-in, not a device
+// address): the address of each device's copy comes from hipGetSymbolAddress, once per
+// device, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[LSM_NOALLOC_SCRATCH_BYTES];
+
+enum { NOALLOC_MAX_DEVICES = 64, NOALLOC_QUEUE_BYTES = 256 };
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `dev` (the current device); nullptr when it cannot be had.
+static unsigned char *scratch_on_device(int dev) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+        g_scratch_ptr[dev] = (unsigned char *)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+int lsm_integrate_batch_device_noalloc(lsm_handle *h, int64_t n, double t0, double pressure, const double *d_dt,
+                                       const double *d_y_in, double *d_y_out, int32_t *d_status, double *d_q,
+                                       int32_t *d_counters, const int32_t *d_order, void *stream) {
+    KParams kp;
+    int method = 0;
+    bool skip = false;
+    if (int rc = prepare_step(h, n, t0, pressure, d_dt, nullptr, d_y_in, d_y_out, kp, method, skip)) return rc;
+    if (skip) return LSM_OK;
+    if (h->kp.const_volume) return fail(LSM_E_UNSUPPORTED, "constant-volume handles need per-cell densities: "
+                                                           "use lsm_integrate_batch_device_rho");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(LSM_E_HIP, "hipGetDevice failed");
+    if (dev != h->device) return fail(LSM_E_INVALID, "the current device is not the handle's device");
+    unsigned char *base = scratch_on_device(dev);
+    if (!base) return fail(LSM_E_HIP, "hipGetSymbolAddress of the no-allocation scratch failed");
+    const size_t room = (size_t)LSM_NOALLOC_SCRATCH_BYTES - NOALLOC_QUEUE_BYTES;
+    StepScratch sc;
+    sc.queue = (unsigned *)base;
+    if (method == LSM_METHOD_WIDE) {
+        const size_t per = sizeof(double) * wide_jac_doubles(h->dm);
+        const int64_t fit = (int64_t)(room / per);
+        if (fit < 1) return fail(LSM_E_INVALID, "one wide workgroup's workspace exceeds the no-allocation scratch");
+        sc.wide_ws = (double *)(base + NOALLOC_QUEUE_BYTES);
+        sc.wide_grid = (int)std::min<int64_t>(h->wide_grid, fit);
+        sc.fallbacks = nullptr;                  // not counted: lsm_fp64_fallbacks reports the other entry points
+    } else if (method == LSM_METHOD_PLAIN) {
+        const size_t per = plain_work_doubles(h->dm, 1) * sizeof(double);
+        const int64_t fit = (int64_t)(room / per);
+        if (fit < 1) return fail(LSM_E_INVALID, "one plain-kernel workspace exceeds the no-allocation scratch");
+        sc.plain_ws = (double *)(base + NOALLOC_QUEUE_BYTES);
+        sc.plain_slots = std::min<int64_t>(std::min<int64_t>(n, (int64_t)h->num_cu * 1024), fit);
+    }
+    return launch_step(h, method, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, d_order, sc, stream);
 }
 
 int lsm_integrate_batch(lsm_handle *h, int64_t n, double t0, double pressure, const double *dt, const double *y_in,
