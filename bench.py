@@ -177,6 +177,9 @@ def main():
     ap.add_argument("--step-cap", type=int, default=0,
                     help="@MaximumNumberOfSteps for every cell (equal-work kernel comparisons; cells stop with "
                          "status -1 at the cap)")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="run each step as consecutive launches over blocks of this many cells, with a progress "
+                         "line after each (steps of minutes; implies --no-order)")
     ap.add_argument("--cpu-sample", type=int, default=None, help="cells of the 1-thread CPU sample (the 16-thread "
                                                                  "sample is 16 times larger)")
     ap.add_argument("--cpu-threads", type=int, default=1)
@@ -241,9 +244,28 @@ def main():
 
     use_order = False
 
+    blocks = [(a, min(a + args.chunk, n)) for a in range(0, n, args.chunk)] if args.chunk > 0 else [(0, n)]
+    if len(blocks) > 1:
+        if args.graph:
+            print("bench.py: --graph and --chunk cannot be combined", file=sys.stderr)
+            sys.exit(2)
+        args.no_order = True
+        # each block is its own SoA batch [ns+1][cells of the block]
+        parts = [dict(y_in=y_in[:, a:b].contiguous(), y_out=torch.empty((y.shape[0], b - a), dtype=torch.float64,
+                                                                         device=dev),
+                      dt=dts[a:b].contiguous(), st=st[a:b].clone(), q=q[a:b].clone(), cnt=cnt[a:b].clone())
+                 for a, b in blocks]
+
     def run():
-        step.integrate_device(n, dts.data_ptr(), y_in.data_ptr(), y_out.data_ptr(), st.data_ptr(), q.data_ptr(),
-                              cnt.data_ptr(), stream.cuda_stream, order.data_ptr() if use_order else None)
+        if len(blocks) == 1:
+            step.integrate_device(n, dts.data_ptr(), y_in.data_ptr(), y_out.data_ptr(), st.data_ptr(), q.data_ptr(),
+                                  cnt.data_ptr(), stream.cuda_stream, order.data_ptr() if use_order else None)
+            return
+        for i, ((a, b), p) in enumerate(zip(blocks, parts)):
+            step.integrate_device(b - a, p["dt"].data_ptr(), p["y_in"].data_ptr(), p["y_out"].data_ptr(),
+                                  p["st"].data_ptr(), p["q"].data_ptr(), p["cnt"].data_ptr(), stream.cuda_stream)
+            torch.cuda.synchronize()
+            progress(f"block {i + 1}/{len(blocks)} done")
 
     progress(f"{n} cells of workload {w['name']}; warm-up steps: {args.warmup}")
     for _ in range(args.warmup):
@@ -282,6 +304,10 @@ def main():
     progress(f"{args.steps} timed steps: {total:.2f} s")
     lat_ms = np.array([a.elapsed_time(b) for a, b in evs])
 
+    if len(blocks) > 1:                          # gather the blocks' outputs
+        for (a, b), p in zip(blocks, parts):
+            y_out[:, a:b] = p["y_out"]
+            st[a:b], q[a:b], cnt[a:b] = p["st"], p["q"], p["cnt"]
     status = st.cpu().numpy()
     counters = cnt.cpu().numpy()
     work = workmodel.flops_from_counters(mech, counters, args.jacobian)
@@ -304,7 +330,8 @@ def main():
                                 "rows (fullPivoting requested)") if full_pivoting else "rows",
                    "waves_per_workgroup": launch["waves_per_block"], "grid": launch["grid"],
                    "visit_order": "cost (previous step's counters)" if use_order else "cell index",
-                   "hip_graph": graph is not None, "step_cap": args.step_cap or None},
+                   "hip_graph": graph is not None, "step_cap": args.step_cap or None,
+                   "launch_blocks": len(blocks)},
         "p50_batch_ms": float(np.percentile(lat_ms, 50)), "p99_batch_ms": float(np.percentile(lat_ms, 99)),
         "roofline": {"bound": "valu-fp64", "pipe": "fp64 vector (no MFMA in this kernel)", "achieved": achieved_tflops,
                      "peak": PEAK_FP64_TFLOPS,
