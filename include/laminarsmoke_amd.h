@@ -281,6 +281,12 @@ int lsm_autotune(lsm_handle *h, int64_t n_cells, double t0, double pressure, con
  * refinement sweeps}. */
 int lsm_get_launch(lsm_handle *h, int32_t *out5);
 
+/* Device bytes the handle holds now: mechanism tables, kernel workspaces, and the
+ * buffers it grows on demand (host-call staging, plain-kernel workspace, cost-order
+ * sort, transport tables).  No counterpart in the reference; for callers that
+ * budget HBM (include/lsm_serve.h counts it). */
+int lsm_device_bytes(lsm_handle *h, uint64_t *bytes);
+
 /* Right-hand side and analytic Jacobian of the constant-pressure batch reactor at
  * n_cells states: BatchReactorHomogeneousConstantPressure_ODE_OpenSMOKE::GetEquations
  * (solvers/laminarSMOKE/unsteady/BatchReactorHomogeneousConstantPressure_ODE_Interface.H:96)

@@ -5,11 +5,12 @@
  * cases (each with its own mechanism handle and mesh) submit chemistry steps to one
  * GPU:
  *   - each case's state [ns+1][n_cells] lives in HBM while the case is resident;
- *   - admission is by an HBM budget over the case states the server allocates (the
- *     handles' tables and workspaces, allocated by lsm_create, are reported but not
- *     counted): a step whose case is not resident waits until its state fits, idle
- *     resident cases being moved to pinned host memory (least recently used first)
- *     to make room;
+ *   - admission is by an HBM budget over everything the cases pin on the device:
+ *     every case's handle (its tables and kernel workspaces, allocated by lsm_create,
+ *     and the buffers the handle grows, re-read after each step) always, and the
+ *     state of each resident case (allocated by the server).  A step whose case is
+ *     not resident waits until its state fits, idle resident cases being moved to
+ *     pinned host memory (least recently used first) to make room;
  *   - steps of one case run in submission order on that case's stream; steps of
  *     different cases overlap on the device (up to max_inflight at once): as the
  *     workgroups of one step drain, those of the next admitted step fill the CUs;
@@ -32,7 +33,7 @@ extern "C" {
 
 typedef struct lsm_server lsm_server;
 
-/* hbm_budget_bytes: device bytes the server may hold for case states;
+/* hbm_budget_bytes: device bytes the cases' handles and resident states may hold;
  * max_inflight: steps on the device at once (1-64). */
 int lsm_server_create(size_t hbm_budget_bytes, int32_t max_inflight, lsm_server **out);
 int lsm_server_destroy(lsm_server *s);
@@ -40,7 +41,7 @@ int lsm_server_destroy(lsm_server *s);
 /* Registers a case: its handle (mechanism + ODE parameters, owned by the caller,
  * used by this case only, must outlive the server), its cell count and initial
  * state y0 (host, SoA [ns+1][n_cells], copied).  Fails with LSM_E_NOMEM if the
- * case's state alone exceeds the budget. */
+ * handles of all cases and this case's state exceed the budget. */
 int lsm_server_add_case(lsm_server *s, lsm_handle *h, int64_t n_cells, const double *y0, int32_t *case_id);
 
 /* Queues one chemistry step of every cell of a case from t0 (the case's time at the
@@ -53,9 +54,12 @@ int lsm_server_submit(lsm_server *s, int32_t case_id, double t0, double dt, cons
 
 /* Non-blocking progress: retires finished steps, admits queued ones.  Writes up to
  * `capacity` finished tickets with their latency (submission to observed
- * completion, host clock) and device time (s); *n_done = number written. */
-int lsm_server_poll(lsm_server *s, uint64_t *tickets, double *latency_s, double *device_s, int32_t capacity,
-                    int32_t *n_done);
+ * completion, host clock), device time, and the device timestamps of the step's
+ * start and end (s after an event recorded when the server was created; steps of
+ * different cases whose [start, end] intervals intersect were on the device
+ * together); *n_done = number written.  Any output array may be NULL. */
+int lsm_server_poll(lsm_server *s, uint64_t *tickets, double *latency_s, double *device_s, double *start_s,
+                    double *end_s, int32_t capacity, int32_t *n_done);
 
 /* Blocks until every queued step has finished (results stay available to poll). */
 int lsm_server_drain(lsm_server *s);
@@ -64,8 +68,8 @@ int lsm_server_drain(lsm_server *s);
  * case's running step. */
 int lsm_server_read_state(lsm_server *s, int32_t case_id, double *y);
 
-/* {state bytes in use, peak state bytes, evictions, steps admitted, steps finished,
- * cases, bytes held by the cases' handles}. */
+/* {resident state bytes, peak of handle + resident state bytes, evictions, steps
+ * admitted, steps finished, cases, bytes held by the cases' handles, budget}. */
 int lsm_server_stats(lsm_server *s, int64_t *out7);
 
 #ifdef __cplusplus

@@ -72,13 +72,14 @@ def lib():
         L.lsm_integrate_batch_device_rho.argtypes = [vp, i64, f64, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_integrate_batch_device_noalloc.argtypes = [vp, i64, f64, f64, vp, vp, vp, vp, vp, vp, vp, vp]
         L.lsm_cost_order.argtypes = [vp, i64, vp, vp, vp]
+        L.lsm_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
         L.lsm_autotune.argtypes = [vp, i64, f64, f64, vp, vp, i64, vp, vp]
         L.lsm_get_launch.argtypes = [vp, vp]
         L.lsm_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int32, ctypes.POINTER(vp)]
         L.lsm_server_destroy.argtypes = [vp]
         L.lsm_server_add_case.argtypes = [vp, vp, i64, vp, ctypes.POINTER(ctypes.c_int32)]
         L.lsm_server_submit.argtypes = [vp, ctypes.c_int32, f64, f64, vp, ctypes.POINTER(ctypes.c_uint64)]
-        L.lsm_server_poll.argtypes = [vp, vp, vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.lsm_server_poll.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
         L.lsm_server_drain.argtypes = [vp]
         L.lsm_server_read_state.argtypes = [vp, ctypes.c_int32, vp]
         L.lsm_server_stats.argtypes = [vp, vp]
@@ -300,6 +301,12 @@ class ChemistryStep:
                                                         y_in_ptr, y_out_ptr, status_ptr, q_ptr, counters_ptr,
                                                         order_ptr, stream))
 
+    def device_bytes(self):
+        """Device bytes the handle holds now (tables, workspaces, buffers grown on demand)."""
+        b = ctypes.c_uint64()
+        _check(lib().lsm_device_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
     def cost_order(self, n, counters_ptr, order_ptr, stream=None):
         _check(lib().lsm_cost_order(self.h, int(n), counters_ptr, order_ptr, stream))
 
@@ -353,14 +360,20 @@ class Server:
                                        ctypes.byref(t)))
         return t.value
 
-    def poll(self, capacity=256):
+    def poll(self, capacity=256, timeline=False):
+        """Finished steps: (ticket, latency s, device s), with timeline=True also the step's device
+        start and end (s after the server's epoch event)."""
         tk = np.zeros(capacity, dtype=np.uint64)
         lat = np.zeros(capacity)
         dev = np.zeros(capacity)
+        t0 = np.zeros(capacity)
+        t1 = np.zeros(capacity)
         nd = ctypes.c_int32()
-        _check(lib().lsm_server_poll(self.s, tk.ctypes.data, lat.ctypes.data, dev.ctypes.data, capacity,
-                                     ctypes.byref(nd)))
+        _check(lib().lsm_server_poll(self.s, tk.ctypes.data, lat.ctypes.data, dev.ctypes.data, t0.ctypes.data,
+                                     t1.ctypes.data, capacity, ctypes.byref(nd)))
         k = nd.value
+        if timeline:
+            return [(int(tk[i]), float(lat[i]), float(dev[i]), float(t0[i]), float(t1[i])) for i in range(k)]
         return [(int(tk[i]), float(lat[i]), float(dev[i])) for i in range(k)]
 
     def drain(self):
@@ -372,10 +385,10 @@ class Server:
         return y
 
     def stats(self):
-        out = np.zeros(7, dtype=np.int64)
+        out = np.zeros(8, dtype=np.int64)
         _check(lib().lsm_server_stats(self.s, out.ctypes.data))
-        return dict(bytes_in_use=int(out[0]), peak_bytes=int(out[1]), evictions=int(out[2]), admitted=int(out[3]),
-                    finished=int(out[4]), cases=int(out[5]), handle_bytes=int(out[6]))
+        return dict(state_bytes=int(out[0]), peak_bytes=int(out[1]), evictions=int(out[2]), admitted=int(out[3]),
+                    finished=int(out[4]), cases=int(out[5]), handle_bytes=int(out[6]), budget=int(out[7]))
 
     def close(self):
         if self.s:

@@ -242,6 +242,7 @@ struct lsm_handle {
     DevTransport tr{};
     void *d_transport = nullptr;
     void *d_transport_opts = nullptr;        // Lewis numbers and soot corrections (lsm_set_transport_options)
+    size_t transport_bytes = 0, transport_opts_bytes = 0;
 };
 
 int lsm_fail(int code, const char *msg) { return fail(code, msg); }
@@ -264,10 +265,15 @@ static bool team_shape_default(lsm_handle *h) {
     return true;
 }
 
+// Device bytes the handle holds now: tables, queue, kernel workspaces, and the buffers grown on
+// demand (plain-kernel workspace, host-call staging, cost-order sort, transport tables).
 size_t lsm_handle_device_bytes(const lsm_handle *h) {
     size_t b = h->tables_bytes + 256;
-    b += sizeof(double) * wide_jac_doubles(h->dm) * (size_t)h->wide_grid;
+    b += sizeof(double) * wide_jac_doubles(h->dm) * (size_t)h->wide_grid + (h->d_fallbacks ? 8 : 0);
     b += h->d_work ? (size_t)h->work_slots * plain_work_doubles(h->dm, 1) * sizeof(double) : 0;
+    b += (size_t)h->cap * (sizeof(double) * (h->dm.ne + 2) + sizeof(int) * (1 + LSM_NCOUNTERS));
+    b += (size_t)h->sort_cap * (2 * sizeof(unsigned) + sizeof(int)) + h->sort_tmp_bytes;
+    b += h->transport_bytes + h->transport_opts_bytes;
     return b;
 }
 
@@ -1334,6 +1340,7 @@ int lsm_set_transport(lsm_handle *h, const lsm_transport *t) {
     }
     if (h->d_transport) (void)hipFree(h->d_transport);
     h->d_transport = d;
+    h->transport_bytes = b.bytes.size();
     char *base = (char *)d;
     DevTransport &T = h->tr;
     T.ns = ns; T.nteta = t->nteta; T.model = t->viscosity_model;
@@ -1351,6 +1358,7 @@ int lsm_set_transport_options(lsm_handle *h, const lsm_transport_options *o) {
     if (!o) {
         if (h->d_transport_opts) (void)hipFree(h->d_transport_opts);
         h->d_transport_opts = nullptr;
+        h->transport_opts_bytes = 0;
         T.dmodel = LSM_DIFFUSIVITY_MULTICOMPONENT; T.simplified = 0; T.soret = 0; T.soot_ref = -1;
         T.lewis = nullptr; T.soot_slot = nullptr; T.soot_corr = nullptr;
         return LSM_OK;
@@ -1395,6 +1403,7 @@ int lsm_set_transport_options(lsm_handle *h, const lsm_transport_options *o) {
     }
     if (h->d_transport_opts) (void)hipFree(h->d_transport_opts);
     h->d_transport_opts = d;
+    h->transport_opts_bytes = d ? b.bytes.size() : 0;
     char *base = (char *)d;
     T.dmodel = o->diffusivity_model;
     T.simplified = o->simplified ? 1 : 0;
@@ -1597,6 +1606,12 @@ int lsm_fp64_fallbacks(lsm_handle *h, uint64_t *count) {
     HIP_OK(hipDeviceSynchronize());
     HIP_OK(hipMemcpy(&v, h->d_fallbacks, sizeof(v), hipMemcpyDeviceToHost));
     *count = v;
+    return LSM_OK;
+}
+
+int lsm_device_bytes(lsm_handle *h, uint64_t *bytes) {
+    if (!h || !bytes) return fail(LSM_E_INVALID, "null argument");
+    *bytes = (uint64_t)lsm_handle_device_bytes(h);
     return LSM_OK;
 }
 

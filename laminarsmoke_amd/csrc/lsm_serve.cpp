@@ -28,7 +28,7 @@ struct Case {
     int64_t n = 0;
     int ne = 0;
     size_t state_bytes = 0;      // device bytes while resident (state + per-call arrays)
-    size_t handle_bytes = 0;     // the handle's own tables and workspaces (outside the budget)
+    size_t handle_bytes = 0;     // the handle's own tables and workspaces (always held)
     double *h_y = nullptr;       // pinned host copy while not resident
     double *d_y = nullptr, *d_dt = nullptr;
     int32_t *d_status = nullptr, *d_cnt = nullptr, *d_order = nullptr;
@@ -43,13 +43,17 @@ struct Case {
 struct Done {
     uint64_t ticket;
     double latency, device;
+    double start, end;           // device timestamps of the step, s after the server's epoch event
 };
 
 }  // namespace
 
 struct lsm_server {
-    size_t budget = 0, used = 0, peak = 0;   // case states (server allocations)
-    size_t handle_bytes = 0;                  // the cases' handles (allocated by lsm_create)
+    size_t budget = 0;                        // device bytes for the cases' handles and resident states
+    size_t used = 0;                          // resident case states (server allocations)
+    size_t handle_bytes = 0;                  // the cases' handles (allocated by lsm_create), always held
+    size_t peak = 0;                          // peak of handle_bytes + used
+    hipEvent_t epoch = nullptr;               // time origin of the step timestamps
     int max_inflight = 1, inflight = 0;
     uint64_t next_ticket = 1;
     int64_t evictions = 0, admitted = 0, finished = 0;
@@ -83,10 +87,10 @@ int evict(lsm_server *s, Case &c) {
 }
 
 // Makes room for `need` bytes by evicting idle resident cases, least recently used
-// first.  Returns false when the running cases alone leave too little room.
+// first.  Returns false when the handles and the running cases alone leave too little room.
 bool make_room(lsm_server *s, size_t need, int *rc) {
     *rc = LSM_OK;
-    while (s->used + need > s->budget) {
+    while (s->handle_bytes + s->used + need > s->budget) {
         int victim = -1;
         for (size_t i = 0; i < s->cases.size(); i++) {
             const Case &c = s->cases[i];
@@ -120,7 +124,7 @@ int make_resident(lsm_server *s, Case &c) {
         return lsm_fail(LSM_E_HIP, "case upload failed");
     c.resident = true;
     s->used += c.state_bytes;
-    s->peak = std::max(s->peak, s->used);
+    s->peak = std::max(s->peak, s->handle_bytes + s->used);
     return LSM_OK;
 }
 
@@ -184,10 +188,17 @@ int retire(lsm_server *s) {
         const hipError_t q = hipEventQuery(c.ev1);
         if (q == hipErrorNotReady) continue;
         if (q != hipSuccess) return lsm_fail(LSM_E_HIP, "chemistry step failed");
-        float ms = 0.f;
+        float ms = 0.f, m0 = 0.f, m1 = 0.f;
         (void)hipEventElapsedTime(&ms, c.ev0, c.ev1);
+        (void)hipEventElapsedTime(&m0, s->epoch, c.ev0);
+        (void)hipEventElapsedTime(&m1, s->epoch, c.ev1);
         const double t = now_s();
-        s->done.push_back(Done{c.running.ticket, t - c.running.t_submit, ms * 1e-3});
+        s->done.push_back(Done{c.running.ticket, t - c.running.t_submit, ms * 1e-3, m0 * 1e-3, m1 * 1e-3});
+        // the handle may have grown buffers during the step (plain-kernel workspace)
+        const size_t hb = lsm_handle_device_bytes(c.h);
+        s->handle_bytes = s->handle_bytes - c.handle_bytes + hb;
+        c.handle_bytes = hb;
+        s->peak = std::max(s->peak, s->handle_bytes + s->used);
         c.in_flight = false;
         c.last_used = t;
         s->inflight--;
@@ -207,6 +218,12 @@ int lsm_server_create(size_t hbm_budget_bytes, int32_t max_inflight, lsm_server 
     lsm_server *s = new lsm_server();
     s->budget = hbm_budget_bytes;
     s->max_inflight = max_inflight;
+    if (hipEventCreate(&s->epoch) != hipSuccess || hipEventRecord(s->epoch, nullptr) != hipSuccess ||
+        hipEventSynchronize(s->epoch) != hipSuccess) {
+        if (s->epoch) (void)hipEventDestroy(s->epoch);
+        delete s;
+        return lsm_fail(LSM_E_HIP, "server epoch event");
+    }
     *out = s;
     return LSM_OK;
 }
@@ -221,6 +238,7 @@ int lsm_server_destroy(lsm_server *s) {
         if (c.ev1) (void)hipEventDestroy(c.ev1);
         if (c.stream) (void)hipStreamDestroy(c.stream);
     }
+    if (s->epoch) (void)hipEventDestroy(s->epoch);
     delete s;
     return LSM_OK;
 }
@@ -236,7 +254,8 @@ int lsm_server_add_case(lsm_server *s, lsm_handle *h, int64_t n_cells, const dou
     c.ne = lsm_handle_neq(h);
     c.state_bytes = (size_t)n_cells * (sizeof(double) * (c.ne + 1) + sizeof(int32_t) * (LSM_NCOUNTERS + 2));
     c.handle_bytes = lsm_handle_device_bytes(h);
-    if (c.state_bytes > s->budget) return lsm_fail(LSM_E_NOMEM, "case state exceeds the HBM budget");
+    if (s->handle_bytes + c.handle_bytes + c.state_bytes > s->budget)
+        return lsm_fail(LSM_E_NOMEM, "the cases' handles and this case's state exceed the HBM budget");
     if (hipHostMalloc((void **)&c.h_y, sizeof(double) * c.ne * n_cells, hipHostMallocDefault) != hipSuccess)
         return lsm_fail(LSM_E_NOMEM, "pinned host state");
     memcpy(c.h_y, y0, sizeof(double) * c.ne * n_cells);
@@ -246,6 +265,7 @@ int lsm_server_add_case(lsm_server *s, lsm_handle *h, int64_t n_cells, const dou
         return lsm_fail(LSM_E_HIP, "stream / event creation failed");
     }
     s->handle_bytes += c.handle_bytes;
+    s->peak = std::max(s->peak, s->handle_bytes + s->used);
     *case_id = (int32_t)s->cases.size();
     s->cases.push_back(c);
     return LSM_OK;
@@ -264,8 +284,8 @@ int lsm_server_submit(lsm_server *s, int32_t case_id, double t0, double dt, cons
     return admit(s);
 }
 
-int lsm_server_poll(lsm_server *s, uint64_t *tickets, double *latency_s, double *device_s, int32_t capacity,
-                    int32_t *n_done) {
+int lsm_server_poll(lsm_server *s, uint64_t *tickets, double *latency_s, double *device_s, double *start_s,
+                    double *end_s, int32_t capacity, int32_t *n_done) {
     if (!s) return lsm_fail(LSM_E_INVALID, "null server");
     int rc = retire(s);
     if (rc) return rc;
@@ -278,6 +298,8 @@ int lsm_server_poll(lsm_server *s, uint64_t *tickets, double *latency_s, double 
         if (tickets) tickets[k] = d.ticket;
         if (latency_s) latency_s[k] = d.latency;
         if (device_s) device_s[k] = d.device;
+        if (start_s) start_s[k] = d.start;
+        if (end_s) end_s[k] = d.end;
         k++;
     }
     if (n_done) *n_done = k;
@@ -319,7 +341,7 @@ int lsm_server_read_state(lsm_server *s, int32_t case_id, double *y) {
     return LSM_OK;
 }
 
-int lsm_server_stats(lsm_server *s, int64_t *out) {   // out[7]
+int lsm_server_stats(lsm_server *s, int64_t *out) {   // out[8]
     if (!s || !out) return lsm_fail(LSM_E_INVALID, "null argument");
     out[0] = (int64_t)s->used;
     out[1] = (int64_t)s->peak;
@@ -328,6 +350,7 @@ int lsm_server_stats(lsm_server *s, int64_t *out) {   // out[7]
     out[4] = s->finished;
     out[5] = (int64_t)s->cases.size();
     out[6] = (int64_t)s->handle_bytes;
+    out[7] = (int64_t)s->budget;
     return LSM_OK;
 }
 

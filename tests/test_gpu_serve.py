@@ -1,85 +1,203 @@
 """GPU: serving chemistry steps from several CFD cases (include/lsm_serve.h).
 
-Three cases with different mechanisms (14, 53 and 160 species), mesh sizes and
-time steps each run two chemistry steps alone, then all together on one server
-whose HBM budget cannot hold every case at once (cases are evicted to host memory
-and brought back), with their steps overlapping on the device.  Each case's
-final state must be the same alone and in the mix: bit for bit (the FP64 team and
-the wide kernel with the numerical Jacobian integrate every cell independently of
-what else runs), which is stricter than the stated tolerance.  The states also
-equal two plain ChemistryStep calls.
+* Three cases with different mechanisms (14, 53 and 160 species), mesh sizes and time
+  steps each run two chemistry steps alone, then all together on one server whose HBM
+  budget (the cases' handles plus their resident states) cannot hold every state at
+  once, so cases are evicted to host memory and brought back.
+* The mix of config 5: eight cases (three 14-species, three 53-species, two
+  160-species), mesh sizes log-uniform in [1e3, 5e5] cells capped at 20,000 /
+  6,000 / 192 cells so that the test runs in seconds (the benchmark in
+  laminarsmoke_amd/serve_bench.py uses larger caps), time steps log-uniform in
+  [1e-6, 1e-4] s, requests arriving as a Poisson process (seeded), each continuing
+  its case from its current state at the case's own time t0.
+
+Each case's final state must be the same in the mix as from direct chemistry-step
+calls in sequence, bit for bit (the FP64 team and the wide kernel with the numerical
+Jacobian integrate every cell independently of what else runs), which is stricter
+than the stated tolerance.  The server's peak device bytes stay within its budget;
+and in the mix of eight, steps of different cases overlap on the device (their
+device start/end timestamps intersect).
 """
+import time
+
 import numpy as np
 import pytest
 
 from laminarsmoke_amd import api
-from laminarsmoke_amd.fields import workload_cells
+from laminarsmoke_amd.fields import coflow_field, dirichlet_cells, hot_cells, workload_cells
+from laminarsmoke_amd.mechanism import workload_mechanism
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(1, 2000, (1e-5, 3e-6)), (2, 1000, (1e-5, 2e-5)), (4, 48, (1e-6, 1e-6))]
+RTOL = {14: 1e-7, 53: 1e-7, 160: 1e-6}
 
 
-def make_step(cfg):
+def make_step(mech):
     ode = api.ODE_Parameters()
-    ode.SetRelativeTolerance(1e-6 if cfg == 4 else 1e-7)
+    ode.SetRelativeTolerance(RTOL[mech.ns])
     ode.SetAbsoluteTolerance(1e-12)
-    return api.ChemistryStep(workload_cells(cfg, n=8)[0], ode, method="auto")
+    return api.ChemistryStep(mech, ode, method="auto")
 
 
-def case_inputs():
-    return [(cfg, workload_cells(cfg, n=n)[1], dts) for cfg, n, dts in CASES]
+def warm_handle(step, n):
+    """Allocates the buffers a served step grows in the handle (the cost-order sort), so that
+    device_bytes() is what the handle will hold while it serves n cells."""
+    import torch
+    cnt = torch.zeros((n, 12), dtype=torch.int32, device="cuda")
+    order = torch.zeros(n, dtype=torch.int32, device="cuda")
+    step.cost_order(n, cnt.data_ptr(), order.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
 
 
-def run_server(inputs, budget, max_inflight):
+def state_bytes(y):
+    return y.shape[1] * (8 * (y.shape[0] + 1) + 4 * 14)     # as lsm_server_add_case counts it
+
+
+def direct(step, y, requests, what=""):
+    """The case's requests [(t0, dt)] as plain chemistry-step calls in sequence."""
+    for t0, dt in requests:
+        y, *_ = step.integrate(y, dt, t0=t0)
+    print(f"  direct calls done: {what} ({y.shape[1]} cells)", flush=True)
+    return y
+
+
+def serve(steps, fields, requests, budget, max_inflight, arrivals=None):
+    """requests[c]: the (t0, dt) of case c in order; arrivals: [(time offset s, case)] or None (all
+    at once, in case-interleaved order).  Returns final states, done list (with timeline), stats."""
     srv = api.Server(budget, max_inflight)
-    steps, ids = [], []
     try:
-        for cfg, y, dts in inputs:
-            st = make_step(cfg)
-            steps.append(st)
-            ids.append(srv.add_case(st, y))
-        for k in range(2):                               # interleaved submissions
-            for cid, (cfg, y, dts) in zip(ids, inputs):
-                srv.submit(cid, dts[k])
-        srv.drain()
-        done = srv.poll()
+        ids = [srv.add_case(st, y) for st, y in zip(steps, fields)]
+        if arrivals is None:
+            order = [c for k in range(max(len(r) for r in requests)) for c in range(len(ids)) if k < len(requests[c])]
+            arrivals = [(0.0, c) for c in order]
+        nxt = [0] * len(ids)
+        owner, done, k = {}, [], 0
+        t_start = time.perf_counter()
+        while k < len(arrivals) or len(done) < len(arrivals):
+            now = time.perf_counter() - t_start
+            while k < len(arrivals) and arrivals[k][0] <= now:
+                c = arrivals[k][1]
+                t0, dt = requests[c][nxt[c]]
+                nxt[c] += 1
+                owner[srv.submit(ids[c], dt, t0=t0)] = c
+                k += 1
+            for d in srv.poll(timeline=True):
+                done.append((owner[d[0]],) + d)
+                print(f"  served {len(done)}/{len(arrivals)}: case {owner[d[0]]}, latency {d[1]:.3f} s", flush=True)
+            if k < len(arrivals) or len(done) < len(arrivals):
+                time.sleep(0.001)
         out = [srv.state(cid) for cid in ids]
         stats = srv.stats()
     finally:
         srv.close()
-        for st in steps:
-            st.close()
     return out, done, stats
 
 
+def overlapping_pairs(done):
+    """Pairs of finished steps of different cases whose device intervals intersect."""
+    n = 0
+    for i in range(len(done)):
+        for j in range(i + 1, len(done)):
+            a, b = done[i], done[j]
+            if a[0] != b[0] and a[4] < b[5] and b[4] < a[5]:
+                n += 1
+    return n
+
+
 def test_same_alone_and_in_the_mix(hip_device):
-    inputs = case_inputs()
-    print("serving: alone", flush=True)
-    alone = []
-    for inp in inputs:
-        out, done, stats = run_server([inp], 8 << 30, 1)
-        assert len(done) == 2 and stats["evictions"] == 0
-        alone.append(out[0])
-    # direct calls give the same states
-    for (cfg, y, dts), ya in zip(inputs, alone):
-        st = make_step(cfg)
-        try:
-            y1, s1, _, _ = st.integrate(y, dts[0])
-            y2, s2, _, _ = st.integrate(y1, dts[1])
-        finally:
+    cases = [(1, 2000, (1e-5, 3e-6)), (2, 1000, (1e-5, 2e-5)), (4, 48, (1e-6, 1e-6))]
+    fields, steps, requests = [], [], []
+    for cfg, n, dts in cases:
+        mech, y = workload_cells(cfg, n=n)
+        fields.append(y)
+        steps.append(make_step(mech))
+        requests.append([(0.0, dts[0]), (dts[0], dts[1])])
+    try:
+        print("serving: alone", flush=True)
+        ref = [direct(st, y, r, f"case {c}") for c, (st, y, r) in enumerate(zip(steps, fields, requests))]
+        for c in range(3):
+            out, done, stats = serve([steps[c]], [fields[c]], [requests[c]], 8 << 30, 1)
+            assert len(done) == 2 and stats["evictions"] == 0
+            assert np.array_equal(out[0], ref[c]), f"case {c}: server differs from direct calls"
+        # the mix: the handles (which the runs above have grown to their serving size) plus the
+        # two largest states, not all three
+        hb = sum(st.device_bytes() for st in steps)
+        sizes = sorted(state_bytes(y) for y in fields)
+        budget = hb + int(1.05 * (sizes[-1] + sizes[-2]))
+        print(f"serving: mix, handles {hb} B, states {sizes} B, budget {budget} B", flush=True)
+        mixed, done, stats = serve(steps, fields, requests, budget, 3)
+    finally:
+        for st in steps:
             st.close()
-        assert np.array_equal(y2, ya), f"cfg{cfg}: server differs from direct calls"
-    # the mix, with a budget that holds the two largest footprints but not all three
-    sizes = [y.shape[1] * (8 * (y.shape[0] + 1) + 4 * 14) for cfg, y, dts in inputs]   # state bytes per case
-    budget = int(sum(sorted(sizes)[-2:]) * 1.05)
-    print(f"serving: mix, footprints {sizes}, budget {budget}", flush=True)
-    mixed, done, stats = run_server(inputs, budget, 3)
     assert len(done) == 6
     assert stats["evictions"] > 0, stats
     assert stats["peak_bytes"] <= budget, stats
-    for (cfg, y, dts), ya, ym in zip(inputs, alone, mixed):
-        assert np.array_equal(ym, ya), f"cfg{cfg}: result in the mix differs from alone"
-    lat = sorted(d[1] for d in done)
-    print(f"serving: identical alone and in the mix; stats {stats}; latencies {['%.3f' % v for v in lat]} s",
-          flush=True)
+    assert stats["handle_bytes"] >= hb, stats
+    for c, (ym, yr) in enumerate(zip(mixed, ref)):
+        assert np.array_equal(ym, yr), f"case {c}: result in the mix differs from direct calls"
+    print(f"serving: identical alone and in the mix; stats {stats}", flush=True)
+
+
+def config5_cases(seed=5):
+    rng = np.random.default_rng(seed)
+    cap = {14: 20000, 53: 6000, 160: 192}
+    out = []
+    for k, ns in enumerate([14, 14, 14, 53, 53, 53, 160, 160]):
+        n = min(int(np.exp(rng.uniform(np.log(1e3), np.log(5e5)))), cap[ns])
+        mech = workload_mechanism(ns)
+        if ns == 14:
+            y = dirichlet_cells(mech, n, 900.0, 2400.0, 100 + k)
+        elif ns == 53:
+            nx = int(np.sqrt(n))
+            y, _ = coflow_field(mech, nx, max(1, n // nx), 100 + k)
+        else:
+            y = hot_cells(mech, n, 100 + k)
+        out.append((mech, np.ascontiguousarray(y)))
+    return out
+
+
+def test_config5_poisson_mix(hip_device):
+    rng = np.random.default_rng(11)
+    cases = config5_cases()
+    steps = [make_step(m) for m, _ in cases]
+    fields = [y for _, y in cases]
+    n_req = 3
+    requests = []
+    for _ in cases:
+        t, r = 0.0, []
+        for _ in range(n_req):
+            dt = float(np.exp(rng.uniform(np.log(1e-6), np.log(1e-4))))
+            r.append((t, dt))
+            t += dt
+        requests.append(r)
+    # Poisson arrivals, mean gap 30 ms, the case of each drawn at random among those with requests left
+    left = [n_req] * len(cases)
+    arrivals, t = [], 0.0
+    for _ in range(n_req * len(cases)):
+        t += float(rng.exponential(0.03))
+        c = int(rng.choice([i for i in range(len(cases)) if left[i] > 0]))
+        left[c] -= 1
+        arrivals.append((t, c))
+    try:
+        print(f"config 5 mix: cells {[y.shape[1] for y in fields]}", flush=True)
+        ref = [direct(st, y, r, f"case {c}") for c, (st, y, r) in enumerate(zip(steps, fields, requests))]
+        for st, y in zip(steps, fields):
+            warm_handle(st, y.shape[1])
+        hb = sum(st.device_bytes() for st in steps)
+        sz = [state_bytes(y) for y in fields]
+        budget = hb + max(int(1.05 * max(sz)), int(0.6 * sum(sz)))
+        print(f"config 5 mix: handles {hb} B, states {sum(sz)} B, budget {budget} B", flush=True)
+        mixed, done, stats = serve(steps, fields, requests, budget, 4, arrivals)
+    finally:
+        for st in steps:
+            st.close()
+    assert len(done) == len(arrivals)
+    assert stats["evictions"] > 0, stats
+    assert stats["peak_bytes"] <= budget, stats
+    for c, (ym, yr) in enumerate(zip(mixed, ref)):
+        assert np.array_equal(ym, yr), f"case {c}: result in the mix differs from direct calls"
+    ov = overlapping_pairs(done)
+    lat = np.array([d[2] for d in done])
+    print(f"config 5 mix: identical to direct calls; {ov} overlapping step pairs of different cases; "
+          f"latency p50 {np.percentile(lat, 50):.3f} s max {lat.max():.3f} s; stats {stats}", flush=True)
+    assert ov > 0
