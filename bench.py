@@ -8,7 +8,9 @@ integrated as a constant-pressure adiabatic reactor over dt = 1e-5 s at rtol 1e-
 atol 1e-12.  A "step" is one chemistry step over all cells, i.e. the batched forAll
 loop of chemistry_DI.H.  Every step integrates the same input field, which stays
 resident in HBM.  Other workloads:
-  --workload 2: the same mechanism on the 512x512 coflow-like field (262144 cells;
+  --workload 1: the 14-species / 34-reaction mechanism on its 4096 cells (T 900-2400 K,
+                Y Dirichlet), dt = 1e-5 s, rtol 1e-7;
+  --workload 2: the 53-species mechanism on the 512x512 coflow-like field (262144 cells;
                 60% cold 300-600 K, 40% flame 1200-2200 K);
   --workload 4: the 160-species / 1000-reaction mechanism (wide kernel, FP32
                 Newton LU with FP64 refinement) on a subsample of its
@@ -22,7 +24,8 @@ resident in HBM.  Other workloads:
                 dt = 1e-5 s.
 
 Before the timed steps:
-  * the launch shape is autotuned on a 16384-cell sample;
+  * the launch shape (cells per wavefront, waves per workgroup, workgroups per CU) is
+    autotuned on a 16384-cell sample;
   * one warm-up step runs, and its per-cell step counts give the visiting order
     of the timed steps (most expensive cells first), as the previous CFD step's
     counters would.
@@ -166,12 +169,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="3", choices=["2", "3", "4", "ch4"])
+    ap.add_argument("--workload", default="3", choices=["1", "2", "3", "4", "ch4"])
     ap.add_argument("--cells", type=int, default=None, help="subsample of the field (default: all of it; "
                                                                "workload 4: 4096)")
     ap.add_argument("--method", default="auto", choices=["team", "plain", "wide", "auto"])
     ap.add_argument("--jacobian", default="analytic", choices=["numerical", "analytic"])
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--cells-per-wave", type=int, default=0,
+                    help="team kernel: 1, 2 (ns+1 <= 32) or 4 (ns+1 <= 16) cells per wavefront (default: the "
+                         "autotuner's choice, 1 without it)")
     ap.add_argument("--no-order", action="store_true")
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--step-cap", type=int, default=0,
@@ -241,6 +247,8 @@ def main():
         tune = step.autotune(n, dts.data_ptr(), y_in.data_ptr(), 16384, stream.cuda_stream)
         tune["seconds"] = time.perf_counter() - t0
         progress(f"autotuned launch shape {tune}")
+    if args.cells_per_wave > 0:
+        step.set_cells_per_wave(args.cells_per_wave)
 
     use_order = False
 
@@ -329,6 +337,7 @@ def main():
                    "pivoting": ("full (plain kernel)" if launch["method"] == "plain" else
                                 "rows (fullPivoting requested)") if full_pivoting else "rows",
                    "waves_per_workgroup": launch["waves_per_block"], "grid": launch["grid"],
+                   "cells_per_wave": launch["cells_per_wave"],
                    "visit_order": "cost (previous step's counters)" if use_order else "cell index",
                    "hip_graph": graph is not None, "step_cap": args.step_cap or None,
                    "launch_blocks": len(blocks)},

@@ -166,6 +166,16 @@ int lsm_create(const lsm_mechanism *mech, const lsm_ode_parameters *ode,
 int lsm_set_method(lsm_handle *h, int32_t method);
 int lsm_destroy(lsm_handle *h);
 
+/* Cells per wavefront of the team kernel (no counterpart in the reference, whose cell loop
+ * integrates one cell at a time, chemistry_DI.H:92-328): 1 (default) gives each cell a
+ * whole 64-lane wavefront; 2 (ns+1 <= 32) and 4 (ns+1 <= 16) split the wavefront into
+ * segments of 32 or 16 lanes, one cell each, so that small mechanisms leave fewer lanes
+ * idle.  The arithmetic per cell is the same in every packing; a wavefront's segments
+ * branch apart where their cells' step histories differ.  lsm_autotune searches it.
+ * LSM_E_UNSUPPORTED when the mechanism is too large for the packing. */
+int lsm_set_cells_per_wave(lsm_handle *h, int32_t cells);
+int lsm_get_cells_per_wave(lsm_handle *h, int32_t *cells);
+
 /* Jacobian used by the Newton iteration of the team kernel.  NUMERICAL (default)
  * is the forward-difference Jacobian of KernelDense::NumericalJacobian
  * (math/multivalue-ode-solvers/KernelDense.hpp:160-183), ne extra RHS evaluations
@@ -268,10 +278,10 @@ int lsm_integrate_batch_device_noalloc(lsm_handle *h, int64_t n_cells, double t0
 int lsm_cost_order(lsm_handle *h, int64_t n_cells, const int32_t *d_counters, int32_t *d_order,
                    void *stream);
 
-/* Launch-shape autotuner for the team kernel: times waves per workgroup (the
- * workgroup shares one LDS copy of the mechanism) x workgroups per CU on a strided
- * sample of `sample` cells (0: 65536) of the given device batch, keeps the fastest
- * in the handle.  t0 and pressure: those of the steps to come (lsm_integrate_batch).
+/* Launch-shape autotuner for the team kernel: times cells per wavefront (1, 2, 4 as the
+ * mechanism allows, lsm_set_cells_per_wave) x waves per workgroup (the workgroup shares
+ * one LDS copy of the mechanism) x workgroups per CU on a strided sample of `sample`
+ * cells (0: 65536) of the given device batch, keeps the fastest in the handle.  t0 and pressure: those of the steps to come (lsm_integrate_batch).
  * result (may be NULL): {waves per workgroup, grid, best time in microseconds}.
  * Synchronous; allocates temporaries.  Other methods: no-op. */
 int lsm_autotune(lsm_handle *h, int64_t n_cells, double t0, double pressure, const double *d_dt,

@@ -52,6 +52,8 @@ def lib():
         L.lsm_reactor_options_default.argtypes = [vp]
         L.lsm_create.argtypes = [vp, vp, vp, ctypes.POINTER(vp)]
         L.lsm_set_method.argtypes = [vp, ctypes.c_int32]
+        L.lsm_set_cells_per_wave.argtypes = [vp, ctypes.c_int32]
+        L.lsm_get_cells_per_wave.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
         L.lsm_set_jacobian.argtypes = [vp, ctypes.c_int32]
         L.lsm_set_refinement.argtypes = [vp, ctypes.c_int32]
         L.lsm_fp64_fallbacks.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
@@ -152,7 +154,8 @@ def reactor_options(pressure=101325.0, min_temperature=0.0, energy=True, mass_fr
 class ChemistryStep:
     """Batched chemistry step of one mechanism on the current HIP device."""
 
-    def __init__(self, mech, ode: ODE_Parameters | None = None, method="auto", jacobian="numerical", **reactor_kw):
+    def __init__(self, mech, ode: ODE_Parameters | None = None, method="auto", jacobian="numerical",
+                 cells_per_wave=1, **reactor_kw):
         self.mech = mech
         self.ode = ode or ODE_Parameters()
         self.reactor = reactor_options(**reactor_kw)
@@ -163,12 +166,23 @@ class ChemistryStep:
         try:
             self.set_method(method)
             self.set_jacobian(jacobian)
+            if cells_per_wave != 1:
+                self.set_cells_per_wave(cells_per_wave)
         except LsmError:
             self.close()
             raise
 
     def set_method(self, method):
         _check(lib().lsm_set_method(self.h, METHODS[method] if isinstance(method, str) else int(method)))
+
+    def set_cells_per_wave(self, cells):
+        """Cells per wavefront of the team kernel: 1, 2 (ns+1 <= 32) or 4 (ns+1 <= 16)."""
+        _check(lib().lsm_set_cells_per_wave(self.h, int(cells)))
+
+    def cells_per_wave(self):
+        v = ctypes.c_int32()
+        _check(lib().lsm_get_cells_per_wave(self.h, ctypes.byref(v)))
+        return v.value
 
     def set_refinement(self, sweeps):
         _check(lib().lsm_set_refinement(self.h, int(sweeps)))
@@ -314,14 +328,15 @@ class ChemistryStep:
         out = (ctypes.c_int32 * 3)()
         _check(lib().lsm_autotune(self.h, int(n), float(t0), self._p(pressure), dt_ptr, y_ptr, int(sample), stream,
                                   out))
-        return dict(waves_per_block=out[0], grid=out[1], best_us=out[2])
+        return dict(waves_per_block=out[0], grid=out[1], best_us=out[2], cells_per_wave=self.cells_per_wave())
 
     def launch(self):
         out = (ctypes.c_int32 * 5)()
         _check(lib().lsm_get_launch(self.h, out))
         names = {v: k for k, v in METHODS.items()}
         return dict(method=names.get(out[0], out[0]), waves_per_block=out[1], grid=out[2],
-                    jacobian="analytic" if out[3] == 1 else "numerical", refine=out[4])
+                    jacobian="analytic" if out[3] == 1 else "numerical", refine=out[4],
+                    cells_per_wave=self.cells_per_wave())
 
 
 def counters_dict(cnt: np.ndarray) -> dict:

@@ -15,9 +15,10 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "liblsm_hip.so")
 ARCH = os.environ.get("LSM_OFFLOAD_ARCH", "gfx950")
 
-# (source, extra defines): lsm_kernels.hip is compiled as six units (LSM_PART 1-6), the team
-# kernel's register-array sizes apart from the rest, so that they build in parallel.
-SOURCES = [("lsm_kernels.hip", [f"-DLSM_PART={p}"]) for p in range(1, 7)] + [("lsm_api.cpp", []), ("lsm_serve.cpp", [])]
+# (source, extra defines): lsm_kernels.hip is compiled as eight units (LSM_PART 1-8): the team
+# kernel's register-array sizes (1-5), the rest (6) and the packed kernels' segment widths
+# (7, 8) apart, so that they build in parallel.
+SOURCES = [("lsm_kernels.hip", [f"-DLSM_PART={p}"]) for p in range(1, 9)] + [("lsm_api.cpp", []), ("lsm_serve.cpp", [])]
 HEADERS = ["lsm_bdf.hpp", "lsm_internal.hpp"]
 
 # Packed-FP32 instructions are switched off for every HIP translation unit.

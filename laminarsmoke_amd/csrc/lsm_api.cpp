@@ -226,6 +226,8 @@ struct lsm_handle {
     int64_t cap = 0;
     int team_grid = 0;
     int team_wpb = 0;
+    int team_cpw = 1;           // cells per wavefront of the team kernel (1, 2 or 4)
+    int pack_grid = 0, pack_wpb = 0;   // launch shape of the packed kernel (team_cpw > 1)
     size_t lds_cu = 0;          // LDS bytes per CU
     int wide_grid = 0;
     double *d_wjws = nullptr;   // wide kernel workspaces, [wide_grid][2][ne][ne]: Jacobian, FP64 LU
@@ -262,6 +264,22 @@ static bool team_shape_default(lsm_handle *h) {
     h->team_wpb = best_wpb;
     h->team_grid = best_bpc * h->num_cu;
     h->team_max_waves = cap * h->num_cu;
+    return true;
+}
+
+// Default launch shape of the packed team kernel with cpw cells per wavefront (as above);
+// false when the mechanism does not allow it.
+static bool pack_shape_default(lsm_handle *h, int cpw) {
+    const int cap = team_waves_per_cu_cpw(h->dm, cpw);
+    int best_w = 0, best_wpb = 0, best_bpc = 0;
+    for (int wpb = 4; wpb >= 1; wpb--)
+        for (int bpc = 1; wpb * bpc <= cap; bpc++) {
+            if (!team_shape_fits_cpw(h->dm, cpw, wpb, bpc, h->lds_cu)) continue;
+            if (wpb * bpc > best_w) { best_w = wpb * bpc; best_wpb = wpb; best_bpc = bpc; }
+        }
+    if (best_w == 0) return false;
+    h->pack_wpb = best_wpb;
+    h->pack_grid = best_bpc * h->num_cu;
     return true;
 }
 
@@ -952,6 +970,26 @@ int lsm_set_method(lsm_handle *h, int32_t method) {
     return LSM_OK;
 }
 
+int lsm_set_cells_per_wave(lsm_handle *h, int32_t cells) {
+    if (!h) return fail(LSM_E_INVALID, "null handle");
+    if (cells != 1 && cells != 2 && cells != 4) return fail(LSM_E_INVALID, "cells per wavefront must be 1, 2 or 4");
+    if (h->team_wpb < 1)
+        return fail(LSM_E_UNSUPPORTED, "team kernel needs ns+1 <= 64 and a mechanism image that fits in LDS");
+    if (cells > team_max_cells_per_wave(h->dm))
+        return fail(LSM_E_UNSUPPORTED, cells == 4 ? "4 cells per wavefront need ns+1 <= 16"
+                                                  : "2 cells per wavefront need ns+1 <= 32");
+    if (cells > 1 && !pack_shape_default(h, cells))
+        return fail(LSM_E_UNSUPPORTED, "packed team kernel: the per-cell LDS does not fit");
+    h->team_cpw = cells;
+    return LSM_OK;
+}
+
+int lsm_get_cells_per_wave(lsm_handle *h, int32_t *cells) {
+    if (!h || !cells) return fail(LSM_E_INVALID, "null argument");
+    *cells = h->team_cpw;
+    return LSM_OK;
+}
+
 int lsm_set_jacobian(lsm_handle *h, int32_t type) {
     if (!h) return fail(LSM_E_INVALID, "null handle");
     if (type != LSM_JACOBIAN_NUMERICAL && type != LSM_JACOBIAN_ANALYTIC) return fail(LSM_E_INVALID, "unknown Jacobian type");
@@ -1067,8 +1105,10 @@ static int launch_step(lsm_handle *h, int method, const KParams &kp_in, int64_t 
     KParams kp = kp_in;
     if (method == LSM_METHOD_TEAM) {
         if (launch_reset_queue(sc.queue, stream)) return fail(LSM_E_HIP, "queue reset launch failed");
-        int rc = launch_team(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, sc.queue, d_order,
-                             h->team_grid, h->team_wpb, stream);
+        const bool packed = h->team_cpw > 1;
+        int rc = launch_team_cpw(h->dm, kp, n, d_dt, d_y_in, d_y_out, d_status, d_q, d_counters, sc.queue, d_order,
+                                 packed ? h->pack_grid : h->team_grid, packed ? h->pack_wpb : h->team_wpb,
+                                 h->team_cpw, stream);
         if (rc) return fail(LSM_E_HIP, std::string("team kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
         return LSM_OK;
     }
@@ -1569,31 +1609,40 @@ int lsm_autotune(lsm_handle *h, int64_t n, double t0, double pressure, const dou
         return fail(LSM_E_NOMEM, "autotune buffers");
     }
     int rc = launch_gather_cells(d_y, d_dt, n, ne, m, stride, ys, dts, stream);
-    const int cap = team_waves_per_cu(h->dm);
-    int best_wpb = h->team_wpb, best_grid = h->team_grid;
+    int best_wpb = h->team_wpb, best_grid = h->team_grid, best_cpw = 1;
     float best_ms = 3.4e38f;
     bool warmed = false;
-    for (int wpb = 1; wpb <= 4 && !rc; wpb++) {
-        for (int bpc = 1; wpb * bpc <= cap; bpc++) {
-            const int grid = bpc * h->num_cu;
-            if (!team_shape_fits(h->dm, wpb, bpc, h->lds_cu)) continue;
-            for (int rep = warmed ? 0 : -1; rep < 1 && !rc; rep++) {   // the first candidate runs twice (warm-up)
-                if (launch_reset_queue(h->d_queue, stream)) { rc = -1; break; }
-                if (hipEventRecord(e0, s) != hipSuccess) { rc = -1; break; }
-                if (launch_team(h->dm, kp, m, dts, ys, yo, nullptr, nullptr, nullptr, h->d_queue, nullptr, grid, wpb,
-                                stream)) { rc = -1; break; }
-                if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = -1; break; }
-                float ms = 0.f;
-                if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = -1; break; }
-                warmed = true;
-                if (rep == 0 && ms < best_ms) { best_ms = ms; best_wpb = wpb; best_grid = grid; }
+    // cells per wavefront x waves per workgroup x workgroups per CU
+    for (int cpw = 1; cpw <= team_max_cells_per_wave(h->dm) && !rc; cpw *= 2) {
+        const int cap = team_waves_per_cu_cpw(h->dm, cpw);
+        for (int wpb = 1; wpb <= 4 && !rc; wpb++) {
+            for (int bpc = 1; wpb * bpc <= cap; bpc++) {
+                const int grid = bpc * h->num_cu;
+                if (!team_shape_fits_cpw(h->dm, cpw, wpb, bpc, h->lds_cu)) continue;
+                for (int rep = warmed ? 0 : -1; rep < 1 && !rc; rep++) {   // the first candidate runs twice (warm-up)
+                    if (launch_reset_queue(h->d_queue, stream)) { rc = -1; break; }
+                    if (hipEventRecord(e0, s) != hipSuccess) { rc = -1; break; }
+                    if (launch_team_cpw(h->dm, kp, m, dts, ys, yo, nullptr, nullptr, nullptr, h->d_queue, nullptr,
+                                        grid, wpb, cpw, stream)) { rc = -1; break; }
+                    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = -1; break; }
+                    float ms = 0.f;
+                    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = -1; break; }
+                    warmed = true;
+                    if (rep == 0 && ms < best_ms) { best_ms = ms; best_wpb = wpb; best_grid = grid; best_cpw = cpw; }
+                }
             }
         }
     }
     cleanup();
     if (rc) return fail(LSM_E_HIP, "autotune launch failed");
-    h->team_wpb = best_wpb;
-    h->team_grid = best_grid;
+    h->team_cpw = best_cpw;
+    if (best_cpw > 1) {
+        h->pack_wpb = best_wpb;
+        h->pack_grid = best_grid;
+    } else {
+        h->team_wpb = best_wpb;
+        h->team_grid = best_grid;
+    }
     if (result) { result[0] = best_wpb; result[1] = best_grid; result[2] = (int32_t)(best_ms * 1000.f); }
     return LSM_OK;
 }
@@ -1619,8 +1668,10 @@ int lsm_get_launch(lsm_handle *h, int32_t *out) {
     if (!h || !out) return fail(LSM_E_INVALID, "null argument");
     const int method = pick_method(h);
     out[0] = method;
-    out[1] = method == LSM_METHOD_TEAM ? h->team_wpb : (method == LSM_METHOD_WIDE ? 4 : 4);
-    out[2] = method == LSM_METHOD_TEAM ? h->team_grid : (method == LSM_METHOD_WIDE ? h->wide_grid : 0);
+    const bool packed = method == LSM_METHOD_TEAM && h->team_cpw > 1;
+    out[1] = method == LSM_METHOD_TEAM ? (packed ? h->pack_wpb : h->team_wpb) : 4;
+    out[2] = method == LSM_METHOD_TEAM ? (packed ? h->pack_grid : h->team_grid)
+                                       : (method == LSM_METHOD_WIDE ? h->wide_grid : 0);
     out[3] = h->kp.jac_mode;
     out[4] = h->kp.refine;
     return LSM_OK;

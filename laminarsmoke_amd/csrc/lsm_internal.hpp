@@ -158,6 +158,14 @@ bool team_shape_fits(const DevMech &m, int wpb, int blocks_per_cu, size_t lds_cu
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                 double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order,
                 int grid, int waves_per_block, void *stream);
+// Cells per wavefront of the team kernel: 1 (one cell per wave), 2 (ns + 1 <= 32) or 4
+// (ns + 1 <= 16); the packed variants run 2 waves per SIMD.
+int team_max_cells_per_wave(const DevMech &m);
+int team_waves_per_cu_cpw(const DevMech &m, int cpw);
+bool team_shape_fits_cpw(const DevMech &m, int cpw, int wpb, int blocks_per_cu, size_t lds_cu);
+int launch_team_cpw(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
+                    double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order, int grid,
+                    int waves_per_block, int cells_per_wave, void *stream);
 int launch_plain(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
                  double *yout, int *status, double *q, int *counters, double *work, int64_t slots,
                  void *stream);

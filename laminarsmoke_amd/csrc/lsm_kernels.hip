@@ -14,6 +14,8 @@
 
 #include <math.h>
 
+#include <utility>
+
 #include "../../include/lsm_mechanism.h"
 #include "../../include/lsm_transport.h"
 #include "lsm_internal.hpp"
@@ -1314,6 +1316,636 @@ team_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt, cons
 }
 
 // ---------------------------------------------------------------------------
+// Packed context: CPW = 64 / SEG cells per wavefront, SEG = 16 (ns + 1 <= 16) or 32
+// (ns + 1 <= 32).  The wave is split into segments of SEG lanes; segment s integrates
+// its own cell, element e of every vector on lane s*SEG + e.  The arithmetic per cell
+// is the team context's, operation for operation; what changes is how values move
+// between the lanes of one cell:
+//   * sums: the team's row scan (row_shr 1, 2, 4, 8; for SEG = 32 row_bcast:15 into the
+//     odd row), so a sum over a cell adds in the team's order, then the segment's last
+//     lane is broadcast (DPP row_newbcast, and v_permlane16_swap across the two rows of
+//     a 32-lane segment);
+//   * element j of a cell to all its lanes, j known at compile time (solves, J*v): DPP
+//     row_newbcast:j (+ permlane16_swap), no LDS and no scalar unit;
+//   * element j at run time (T, FD columns, the pivot-order permutation): ds_bpermute;
+//   * the pivot row of an elimination step: written by the pivot lane to a per-cell LDS
+//     row and read back by the segment.
+// After each factorisation the rows are permuted into pivot order (lane k holds the row
+// chosen at step k), so that every broadcast of the triangular solves is a DPP move from a
+// compile-time lane.  The BDF driver runs per segment: its scalars are per-lane values,
+// uniform within a segment, and segments of one wave branch apart under the exec mask.
+// Per-cell LDS: (c, g) pairs and h [64] (slot 63 is the dummy species of fixed-format
+// records), R, dR/dT, collider terms and the pivot row [SEG] each, then the Jacobian, ne
+// columns of SEG + 1 doubles (lane k's row at offset k).
+// ---------------------------------------------------------------------------
+template <int SEG>
+__device__ __forceinline__ long long seg_bcast_last_i64(long long v) {
+    // the last lane of each segment to the whole segment
+    int lo = __builtin_amdgcn_update_dpp(0, (int)(v & 0xffffffffll), 0x15F, 0xf, 0xf, false);   // row_newbcast:15
+    int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), 0x15F, 0xf, 0xf, false);
+    if (SEG == 32) {   // odd rows hold the segment's value: copy them onto the even rows
+        lo = (int)__builtin_amdgcn_permlane16_swap((unsigned)lo, (unsigned)lo, false, false)[1];
+        hi = (int)__builtin_amdgcn_permlane16_swap((unsigned)hi, (unsigned)hi, false, false)[1];
+    }
+    return ((long long)hi << 32) | (unsigned)lo;
+}
+template <int SEG>
+__device__ __forceinline__ double seg_bcast_last(double x) {
+    return __longlong_as_double(seg_bcast_last_i64<SEG>(__double_as_longlong(x)));
+}
+
+// Element J (compile time) of each segment to the whole segment.
+template <int SEG, int J>
+__device__ __forceinline__ double seg_bcast(double x) {
+    static_assert(J >= 0 && J < SEG, "lane outside the segment");
+    const long long v = __double_as_longlong(x);
+    int lo = __builtin_amdgcn_update_dpp(0, (int)(v & 0xffffffffll), 0x150 + (J & 15), 0xf, 0xf, false);
+    int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), 0x150 + (J & 15), 0xf, 0xf, false);
+    if (SEG == 32) {   // [0]: even rows copied onto odd rows, [1]: odd rows onto even rows
+        lo = (int)__builtin_amdgcn_permlane16_swap((unsigned)lo, (unsigned)lo, false, false)[J < 16 ? 0 : 1];
+        hi = (int)__builtin_amdgcn_permlane16_swap((unsigned)hi, (unsigned)hi, false, false)[J < 16 ? 0 : 1];
+    }
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Value of lane `src` (a wave lane index, any at run time) on every lane.
+__device__ __forceinline__ double lane_shfl_d(double x, int src) {
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(v & 0xffffffffll));
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(v >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over each segment, in the order of wave_sum restricted to the segment's lanes.
+template <int SEG>
+__device__ __forceinline__ double seg_sum(double x) {
+    x += dpp0_d<0x111, 0xf>(x);
+    x += dpp0_d<0x112, 0xf>(x);
+    x += dpp0_d<0x114, 0xf>(x);
+    x += dpp0_d<0x118, 0xf>(x);
+    if (SEG == 32) x += dpp0_d<0x142, 0xa>(x);
+    return seg_bcast_last<SEG>(x);
+}
+template <int SEG>
+__device__ __forceinline__ void seg_sum2(double &a, double &b) {
+    a += dpp0_d<0x111, 0xf>(a); b += dpp0_d<0x111, 0xf>(b);
+    a += dpp0_d<0x112, 0xf>(a); b += dpp0_d<0x112, 0xf>(b);
+    a += dpp0_d<0x114, 0xf>(a); b += dpp0_d<0x114, 0xf>(b);
+    a += dpp0_d<0x118, 0xf>(a); b += dpp0_d<0x118, 0xf>(b);
+    if (SEG == 32) { a += dpp0_d<0x142, 0xa>(a); b += dpp0_d<0x142, 0xa>(b); }
+    a = seg_bcast_last<SEG>(a);
+    b = seg_bcast_last<SEG>(b);
+}
+// Maximum over each segment of non-negative doubles.
+template <int SEG>
+__device__ __forceinline__ double seg_max_nonneg(double x) {
+    x = fmax(x, dpp0_d<0x111, 0xf>(x));
+    x = fmax(x, dpp0_d<0x112, 0xf>(x));
+    x = fmax(x, dpp0_d<0x114, 0xf>(x));
+    x = fmax(x, dpp0_d<0x118, 0xf>(x));
+    if (SEG == 32) x = fmax(x, dpp0_d<0x142, 0xa>(x));
+    return seg_bcast_last<SEG>(x);
+}
+
+// Per-cell LDS of the packed context, in doubles: 128 (c, g), 64 h, 4 x SEG (R, dR/dT,
+// collider terms, pivot row), the Jacobian ne x (SEG + 1); even, so that every cell's pairs
+// stay 16-byte aligned.
+__host__ __device__ inline size_t pack_cell_doubles(int seg, int ne) {
+    return ((size_t)192 + 4 * (size_t)seg + (size_t)ne * (seg + 1) + 1) / 2 * 2;
+}
+__host__ __device__ inline size_t pack_wave_bytes(int seg, int ne) {
+    return (size_t)(64 / seg) * pack_cell_doubles(seg, ne) * 8;
+}
+
+template <int SEG, bool XK>
+struct SegCtx {
+    static constexpr int JS = SEG + 1;              // Jacobian column stride
+    static constexpr unsigned long long SEGMASK = SEG == 64 ? ~0ull : ((1ull << SEG) - 1ull);
+    double V[NVEC][1];
+    double G[SEG];   // this lane's row of the LU factors; after a factorisation, the row chosen at
+                     // elimination step el (multipliers in columns < el, U from el on)
+    int ps;          // elimination step at which this lane's row became the pivot (during factor)
+    int plv;         // row chosen as pivot at step el
+    double dinv;     // 1 / pivot of this lane's row
+    double *Jl;      // per-cell LDS: Jacobian [ne][SEG + 1], lane el holds row el
+    double2 *scg;    // per-cell LDS: (c_k, g_k)
+    double *sR, *sDT, *sH, *sU, *prow;
+    LdsMech L;
+    const KParams &K;
+    int el, sbase, ne, ns, nr;
+    int s0;
+    double qr;
+    double rho0;
+#ifdef LSM_PHASE_TIMERS
+    __device__ __forceinline__ void add_time(int, unsigned long long) {}
+#endif
+
+    __device__ SegCtx(const DevMech &m, const KParams &k, const unsigned char *img, double *cell_lds)
+        : L(img, m.img, m.ns_pad), K(k), el(threadIdx.x & (SEG - 1)), sbase(threadIdx.x & 63 & ~(SEG - 1)),
+          ne(m.ne), ns(m.ns), nr(m.nr), s0(m.img.simple0), qr(0.), rho0(0.) {
+        scg = (double2 *)cell_lds;
+        sH = cell_lds + 128;
+        sR = sH + 64;
+        sDT = sR + SEG;
+        sU = sDT + SEG;
+        prow = sU + SEG;
+        Jl = prow + SEG;
+    }
+
+    template <class F>
+    __device__ __forceinline__ void each(F f) { f(0); }
+    __device__ __forceinline__ double &x(int vid, int e) { return V[vid][e]; }
+    __device__ __forceinline__ double ymin(int) const { return el < ns ? 0. : (el == ns ? 200. : NEG_INF); }
+    __device__ __forceinline__ double ymax(int) const { return el < ns ? 1. : (el == ns ? 6000. : POS_INF); }
+    __device__ __forceinline__ double reduce(double v) { return seg_sum<SEG>(v); }
+    __device__ __forceinline__ int count(bool b) {
+        return __popcll((__ballot(b) >> sbase) & SEGMASK);
+    }
+    // element j (run time) of this cell
+    __device__ __forceinline__ double elem(double v, int j) const { return lane_shfl_d(v, sbase + j); }
+
+    __device__ __forceinline__ double *racc(unsigned k) const {
+        return k == (unsigned)LSM_DUMMY_SP ? sU + el : sR + k;
+    }
+    __device__ __forceinline__ void reaction(int j, double T, double logT, double uT, double logPRT, double Pk,
+                                            double cTot) {
+        rxn_rate<XK>(L, scg, j, T, logT, uT, logPRT, Pk, cTot, [&](unsigned k, double v) { atomicAdd(racc(k), v); });
+    }
+    // Two fixed-format records without collider, reactions j and j + SEG (the team context's
+    // simple_pair with the segment's stride).
+    __device__ __forceinline__ void simple_pair(int j, double logT, double uT, double logPRT) {
+        const int j2 = j + SEG;
+        const bool has2 = j2 < nr;
+        const int jb = has2 ? j2 : j;
+        const uint4 w1 = L.W[j], w2 = L.W[jb];
+        const double2 ab1 = ((const double2 *)L.P)[2 * j], ed1 = ((const double2 *)L.P)[2 * j + 1];
+        const double2 ab2 = ((const double2 *)L.P)[2 * jb], ed2 = ((const double2 *)L.P)[2 * jb + 1];
+        const unsigned r10 = w1.y & 0xffu, r11 = (w1.y >> 8) & 0xffu, r12 = (w1.y >> 16) & 0xffu;
+        const unsigned p10 = w1.z & 0xffu, p11 = (w1.z >> 8) & 0xffu, p12 = (w1.z >> 16) & 0xffu;
+        const unsigned r20 = w2.y & 0xffu, r21 = (w2.y >> 8) & 0xffu, r22 = (w2.y >> 16) & 0xffu;
+        const unsigned p20 = w2.z & 0xffu, p21 = (w2.z >> 8) & 0xffu, p22 = (w2.z >> 16) & 0xffu;
+        const double2 a10 = scg[r10], a11 = scg[r11], a12 = scg[r12], b10 = scg[p10], b11 = scg[p11], b12 = scg[p12];
+        const double2 a20 = scg[r20], a21 = scg[r21], a22 = scg[r22], b20 = scg[p20], b21 = scg[p21], b22 = scg[p22];
+        double kf1 = exp(ab1.x + ab1.y * logT - ed1.x * uT);
+        double kf2 = exp(ab2.x + ab2.y * logT - ed2.x * uT);
+        kf1 = (w1.x & 2u) ? -kf1 : kf1;
+        kf2 = (w2.x & 2u) ? -kf2 : kf2;
+        const double dg1 = (b10.y + b11.y + b12.y) - (a10.y + a11.y + a12.y);
+        const double dg2 = (b20.y + b21.y + b22.y) - (a20.y + a21.y + a22.y);
+        const double uk1 = exp(dg1 - logPRT * ed1.y), uk2 = exp(dg2 - logPRT * ed2.y);
+        double net1 = (a10.x * a11.x) * a12.x, net2 = (a20.x * a21.x) * a22.x;
+        const double rb1 = ((b10.x * b11.x) * b12.x) * uk1, rb2 = ((b20.x * b21.x) * b22.x) * uk2;
+        net1 = (w1.x & 1u) ? net1 - rb1 : net1;
+        net2 = (w2.x & 1u) ? net2 - rb2 : net2;
+        const double r1 = net1 * kf1;
+        const double r2 = has2 ? net2 * kf2 : 0.;
+        atomicAdd(racc(r10), -r1); atomicAdd(racc(r11), -r1); atomicAdd(racc(r12), -r1);
+        atomicAdd(racc(p10), r1);  atomicAdd(racc(p11), r1);  atomicAdd(racc(p12), r1);
+        atomicAdd(racc(r20), -r2); atomicAdd(racc(r21), -r2); atomicAdd(racc(r22), -r2);
+        atomicAdd(racc(p20), r2);  atomicAdd(racc(p21), r2);  atomicAdd(racc(p22), r2);
+    }
+
+    // BatchReactorHomogeneousConstantPressure::Equations (:175-247) / ConstantVolume::Equations
+    // (BatchReactorHomogeneousConstantVolume.H:150-199), one cell per segment.
+    __device__ __forceinline__ void rhs(int src, int dst) {
+        const double T = elem(V[src][0], ns);
+        const int k = el;
+        const bool isp = k < ns;
+        const double Yk = isp ? V[src][0] : 0.;
+        const int P = L.SP;
+        const double mwk = isp ? L.sp[15 * P + k] : 1.;
+        const double MW = 1. / seg_sum<SEG>(Yk / mwk);
+        const double xk = (MW * Yk) / mwk;
+        const double cTotP = K.const_volume ? rho0 / MW : K.pressure / RGAS / T;
+        const double ck = cTotP * xk;
+        const double rho = K.const_volume ? rho0 : cTotP * MW;
+        const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, uT = 1. / T, logT = log(T);
+        double cp = 0., h = 0.;
+        if (isp) {
+            const double *a = L.sp + ((T > L.sp[14 * P + k]) ? 0 : 7 * P) + k;
+            const double a0 = a[0], a1 = a[P], a2 = a[2 * P], a3 = a[3 * P], a4 = a[4 * P];
+            cp = a0 + T * a1 + T2 * a2 + T3 * a3 + T4 * a4;
+            h = a0 + T * (0.50 * a1) + T2 * ((1. / 3.) * a2) + T3 * (0.25 * a3) + T4 * (0.20 * a4) + uT * a[5 * P];
+            const double s = a0 * logT + T * a1 + T2 * (0.50 * a2) + T3 * ((1. / 3.) * a3) + T4 * (0.25 * a4) +
+                             a[6 * P];
+            scg[k] = make_double2(ck, h - s);
+        }
+        if (k == SEG - 1) scg[LSM_DUMMY_SP] = make_double2(1., 0.);
+        sR[k] = 0.;
+        double cTot = ck, cpx = isp ? cp * xk : 0.;
+        seg_sum2<SEG>(cTot, cpx);
+        wave_sync();
+        const double logPRT = log(101325. / RGAS / T);
+        const double Pk = K.const_volume ? cTotP * RGAS * T : K.pressure;
+        for (int j = el; j < s0; j += SEG) reaction(j, T, logT, uT, logPRT, Pk, cTot);
+        for (int j = s0 + el; j < nr; j += 2 * SEG) simple_pair(j, logT, uT, logPRT);
+        wave_sync();
+        const double Rk = isp ? sR[k] : 0.;
+        double dy = isp ? (mwk * Rk) / rho : 0.;
+        if (K.energy && K.const_volume) {
+            double hs = isp ? Rk * h : 0., rs = Rk;
+            seg_sum2<SEG>(hs, rs);
+            qr = -hs * RGAS * T;
+            const double CvMixMass = (cpx * RGAS - RGAS) / MW;
+            if (k == ns) dy = (qr + RGAS * T * rs) / (rho * CvMixMass);
+        } else if (K.energy) {
+            const double hs = seg_sum<SEG>(isp ? Rk * h : 0.);
+            qr = -hs * RGAS * T;
+            const double CpMixMass = (cpx * RGAS) / MW;
+            if (k == ns) dy = qr / (rho * CpMixMass);
+        }
+        wave_sync();
+        V[dst][0] = dy;
+    }
+
+    __device__ __forceinline__ void set_density(double y0, double T0, const double *rho_cells, int64_t cell) {
+        const double mwk = el < ns ? L.sp[15 * L.SP + el] : 1.;
+        const double MW0 = 1. / seg_sum<SEG>(el < ns ? y0 / mwk : 0.);
+        rho0 = rho_cells ? rho_cells[cell] : K.pressure * MW0 / (RGAS * T0);
+    }
+
+    __device__ __forceinline__ double compact_sources(double y, double yin, double Tin, double dtc) {
+        const bool isp = el < ns;
+        const double Tf = elem(y, ns);
+        const int P = L.SP;
+        const double mwk = isp ? L.sp[15 * P + el] : 1.;
+        double cp = 0.;
+        if (isp) {
+            const double *a = L.sp + ((Tf > L.sp[14 * P + el]) ? 0 : 7 * P) + el;
+            cp = a[0] + Tf * (a[P] + Tf * (a[2 * P] + Tf * (a[3 * P] + Tf * a[4 * P])));
+        }
+        double u = isp ? y / mwk : 0., cy = isp ? cp * (y / mwk) : 0.;
+        seg_sum2<SEG>(u, cy);
+        const double rho = K.pressure * (1. / u) / (RGAS * Tf);
+        const bool ok = dtc > 1.e-14;
+        if (isp) return ok ? rho * (y - yin) / dtc : 0.;
+        return (ok && K.energy) ? rho * (cy * RGAS) * (Tf - Tin) / dtc : 0.;
+    }
+
+    struct JacAcc {
+        double *sR, *sDT, *Dm, *sU;
+        __device__ __forceinline__ void R(unsigned k, double r, double drdT) const {
+            if (k != (unsigned)LSM_DUMMY_SP) {
+                atomicAdd(&sR[k], r);
+                atomicAdd(&sDT[k], drdT);
+            }
+        }
+        __device__ __forceinline__ void D(unsigned i, unsigned k, double v) const { atomicAdd(&Dm[i * JS + k], v); }
+        __device__ __forceinline__ void U(unsigned k, double v) const { atomicAdd(&sU[k], v); }
+    };
+
+    // The team context's analytic Jacobian (same formulas, same order), one cell per segment.
+    __device__ __forceinline__ void jacobian_analytic_impl() {
+        const double T = elem(V[VB_][0], ns);
+        const int k = el;
+        const bool isp = k < ns;
+        const double Yk = isp ? V[VB_][0] : 0.;
+        const int P = L.SP;
+        const double mwk = isp ? L.sp[15 * P + k] : 1.;
+        const double MW = 1. / seg_sum<SEG>(Yk / mwk);
+        const double xk = (MW * Yk) / mwk;
+        const double cTotP = K.pressure / RGAS / T;
+        const double ck = cTotP * xk;
+        const double rho = cTotP * MW;
+        const double T2 = T * T, T3 = T2 * T, T4 = T3 * T, uT = 1. / T, logT = log(T);
+        double cp = 0., h = 0., dcp = 0.;
+        if (isp) {
+            const double *a = L.sp + ((T > L.sp[14 * P + k]) ? 0 : 7 * P) + k;
+            const double a0 = a[0], a1 = a[P], a2 = a[2 * P], a3 = a[3 * P], a4 = a[4 * P];
+            cp = a0 + T * a1 + T2 * a2 + T3 * a3 + T4 * a4;
+            dcp = a1 + T * (2. * a2) + T2 * (3. * a3) + T3 * (4. * a4);
+            h = a0 + T * (0.50 * a1) + T2 * ((1. / 3.) * a2) + T3 * (0.25 * a3) + T4 * (0.20 * a4) + uT * a[5 * P];
+            const double s = a0 * logT + T * a1 + T2 * (0.50 * a2) + T3 * ((1. / 3.) * a3) + T4 * (0.25 * a4) +
+                             a[6 * P];
+            scg[k] = make_double2(ck, h - s);
+            sH[k] = h;
+        }
+        if (k == SEG - 1) { scg[LSM_DUMMY_SP] = make_double2(1., 0.); sH[LSM_DUMMY_SP] = 0.; }
+        sR[k] = 0.;
+        sDT[k] = 0.;
+        sU[k] = 0.;
+        for (int i = 0; i < ne; i++) Jl[i * JS + k] = 0.;
+        double cTot = ck, cpx = isp ? cp * xk : 0.;
+        seg_sum2<SEG>(cTot, cpx);
+        wave_sync();
+        const double logPRT = log(101325. / RGAS / T);
+        const JacAcc acc{sR, sDT, Jl, sU};
+        for (int j = el; j < nr; j += SEG)
+            rxn_jac<XK>(L, scg, sH, ns, LSM_DUMMY_SP, j, T, logT, uT, logPRT, K.pressure, cTot, acc);
+        wave_sync();
+        const double Rk = isp ? sR[k] : 0.;
+        const double dRdT = isp ? sDT[k] : 0.;
+        const double Uk = isp ? sU[k] : 0.;
+        double S = 0., A = 0.;
+        if (isp) {
+            for (int i = 0; i < ns; i++) S += (Jl[i * JS + k] + Uk) * scg[i].x;
+            for (int i = 0; i < ns; i++) A += sH[i] * (Jl[k * JS + i] + sU[i]);
+        }
+        wave_sync();
+        const double fk = isp ? (mwk * Rk) / rho : 0.;
+        if (isp) {
+            const double g = fk - mwk * S / rho;
+            for (int j = 0; j < ns; j++) {
+                const double mwj = L.sp[15 * P + j];
+                Jl[j * JS + k] = (mwk / mwj) * (Jl[j * JS + k] + Uk) + (MW / mwj) * g;
+            }
+            Jl[ns * JS + k] = (mwk / rho) * (dRdT - S * uT) + fk * uT;
+        } else if (k > ns) {
+            for (int j = 0; j < ne; j++) Jl[j * JS + k] = 0.;
+        }
+        const double cpm = (cpx * RGAS) / MW;
+        const double Phi = rho * cpm;
+        double hs = isp ? h * Rk : 0., hS = isp ? h * S : 0.;
+        seg_sum2<SEG>(hs, hS);
+        const double QR = -hs * RGAS * T;
+        const double fT = K.energy ? QR / Phi : 0.;
+        double dQ = isp ? -(cp * RGAS * Rk + h * RGAS * T * (dRdT - S * uT)) : 0.;
+        double dP = isp ? Yk * dcp * RGAS / mwk : 0.;
+        seg_sum2<SEG>(dQ, dP);
+        if (K.energy) {
+            if (isp) {
+                const double dQdY = -(RGAS * T / mwk) * (rho * A - MW * hS);
+                const double dPdY = (rho / mwk) * (cp * RGAS - MW * cpm);
+                Jl[k * JS + ns] = (dQdY - fT * dPdY) / Phi;
+            }
+            if (k == ns) Jl[ns * JS + ns] = (dQ - fT * (-Phi * uT + rho * dP)) / Phi;
+        } else {
+            if (isp) Jl[k * JS + ns] = 0.;
+            if (k == ns) Jl[ns * JS + ns] = 0.;
+        }
+        wave_sync();
+    }
+
+    // KernelDense::NumericalJacobian, max-constraint branch (KernelDense.hpp:160-183)
+    __device__ __forceinline__ void jacobian_impl(double h) {
+        const double ZERO_DER = sqrt(TINY_F), ETA2 = sqrt(EPS_D), BETA = 1.e+3 * EPS_D;
+        double fw = V[F_][0] * V[OOE_][0];
+        double hf = BETA * fabs(h) * sqrt(seg_sum<SEG>(fw * fw) / (double)ne) * (double)ne;
+        if (hf < 1.e-10) hf = 1.;
+        for (int j = 0; j < ne; j++) {
+            const double yh = elem(V[VB_][0], j);
+            const double ej = elem(V[OOE_][0], j);
+            const double a = fabs(yh), b = fabs(1. / ej);
+            double hJ = ETA2 * (a > b ? a : b);
+            hJ = fmax(hJ, hf / ej);
+            hJ = fmax(hJ, ZERO_DER);
+            const double ymx = j < ns ? 1. : 6000.;
+            if (yh + hJ > ymx) hJ = -hJ;
+            if (el == j) V[VB_][0] = yh + hJ;
+            rhs(VB_, AUX_);
+            if (el == j) V[VB_][0] = yh;
+            const double hInv = 1. / hJ;
+            Jl[j * JS + el] = hInv * (V[AUX_][0] - V[F_][0]);
+        }
+    }
+
+    // Element of the largest |g| among the eligible elements of this cell (the lowest on a tie;
+    // the first eligible one when every eligible |g| is 0).
+    __device__ __forceinline__ int pivot_el(double g, bool elig) {
+        const double a = elig ? fabs(g) : 0.;
+        const double m = seg_max_nonneg<SEG>(a);
+        const unsigned long long hit = (__ballot(elig && a == m) >> sbase) & SEGMASK;
+        return __ffsll(hit) - 1;
+    }
+
+    // G = I - hr0 J and row-pivoted LU (KernelDense.hpp:196-221), the team context's
+    // elimination with the pivot row staged in LDS; the rows are then permuted into pivot order.
+    __device__ __forceinline__ void factor_impl(double hr0) {
+        int el = this->el, ne = this->ne;
+        asm volatile("" : "+v"(el));
+        asm volatile("" : "+s"(ne));
+#pragma unroll
+        for (int j = 0; j < SEG; j++) {
+            const double jij = (j < ne) ? Jl[j * JS + el] : 0.;
+            G[j] = jij * -hr0 + ((el == j) ? 1. : 0.);
+        }
+        ps = (el < ne) ? -1 : 1 << 20;
+        dinv = 1.;
+        int p = pivot_el(G[0], ps < 0);
+#pragma unroll
+        for (int k = 0; k < SEG; k++) {
+            if (k < ne) {
+                if (el == p) {
+#pragma unroll
+                    for (int j = k; j < SEG; j++) prow[j] = G[j];
+                }
+                wave_sync();
+                const double piv = prow[k];
+                const double inv = 1. / piv;
+                if (el == p) { ps = k; dinv = inv; }
+                double l = 0.;
+                if (ps < 0 && piv != 0.) {
+                    l = G[k] * inv;
+                    G[k] = l;
+                }
+                int pn = 0;
+                if (k + 1 < SEG) {
+                    G[k + 1] = fma(-l, prow[k + 1], G[k + 1]);
+                    pn = pivot_el(G[k + 1], ps < 0);
+                }
+#pragma unroll
+                for (int j = k + 2; j < SEG; j++) G[j] = fma(-l, prow[j], G[j]);
+                wave_sync();   // the pivot row is read before the next step overwrites it
+                p = pn;
+            }
+        }
+        // pivot order: element k learns which row was chosen at step k, then takes that row
+        // (a column with no eligible finite entry leaves ps at -1: its slot is skipped, the masks
+        // keep every index inside the segment, and the NaNs fail the step as in the team context)
+        int *pl = (int *)prow;
+        if (el < ne && ps >= 0) pl[ps] = el;
+        wave_sync();
+        plv = (el < ne) ? (pl[el] & (SEG - 1)) : el;
+        wave_sync();
+        const int src = sbase + plv;
+#pragma unroll
+        for (int j = 0; j < SEG; j++) G[j] = lane_shfl_d(G[j], src);
+        dinv = lane_shfl_d(dinv, src);
+    }
+
+    // Forward / backward substitution with the rows in pivot order: the broadcasts of the
+    // dependent chain come from compile-time lanes (DPP).  Same operations and order as the
+    // team context's solve, in groups of 8 steps.
+    template <int K0>
+    __device__ __forceinline__ void fwd_group(double &b, int ne, int el_l) const {
+        if (K0 < ne) {
+            double lk[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) lk[u] = G[K0 + u] * ((el_l > K0 + u) ? 1. : 0.);
+            b = fma(-lk[0], seg_bcast<SEG, K0 + 0>(b), b);
+            b = fma(-lk[1], seg_bcast<SEG, K0 + 1>(b), b);
+            b = fma(-lk[2], seg_bcast<SEG, K0 + 2>(b), b);
+            b = fma(-lk[3], seg_bcast<SEG, K0 + 3>(b), b);
+            b = fma(-lk[4], seg_bcast<SEG, K0 + 4>(b), b);
+            b = fma(-lk[5], seg_bcast<SEG, K0 + 5>(b), b);
+            b = fma(-lk[6], seg_bcast<SEG, K0 + 6>(b), b);
+            b = fma(-lk[7], seg_bcast<SEG, K0 + 7>(b), b);
+        }
+    }
+    template <int K0>
+    __device__ __forceinline__ void bwd_group(double &b, double &xr, double dinv_l, int ne, int el_l) const {
+        if (K0 < ne) {
+            double uk[8], dk[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) uk[u] = G[K0 + u] * ((el_l < K0 + u) ? 1. : 0.);
+            dk[7] = seg_bcast<SEG, K0 + 7>(dinv_l);
+            dk[6] = seg_bcast<SEG, K0 + 6>(dinv_l);
+            dk[5] = seg_bcast<SEG, K0 + 5>(dinv_l);
+            dk[4] = seg_bcast<SEG, K0 + 4>(dinv_l);
+            dk[3] = seg_bcast<SEG, K0 + 3>(dinv_l);
+            dk[2] = seg_bcast<SEG, K0 + 2>(dinv_l);
+            dk[1] = seg_bcast<SEG, K0 + 1>(dinv_l);
+            dk[0] = seg_bcast<SEG, K0 + 0>(dinv_l);
+            double xk;
+            xk = seg_bcast<SEG, K0 + 7>(b) * dk[7]; b = fma(-uk[7], xk, b); if (el_l == K0 + 7) xr = xk;
+            xk = seg_bcast<SEG, K0 + 6>(b) * dk[6]; b = fma(-uk[6], xk, b); if (el_l == K0 + 6) xr = xk;
+            xk = seg_bcast<SEG, K0 + 5>(b) * dk[5]; b = fma(-uk[5], xk, b); if (el_l == K0 + 5) xr = xk;
+            xk = seg_bcast<SEG, K0 + 4>(b) * dk[4]; b = fma(-uk[4], xk, b); if (el_l == K0 + 4) xr = xk;
+            xk = seg_bcast<SEG, K0 + 3>(b) * dk[3]; b = fma(-uk[3], xk, b); if (el_l == K0 + 3) xr = xk;
+            xk = seg_bcast<SEG, K0 + 2>(b) * dk[2]; b = fma(-uk[2], xk, b); if (el_l == K0 + 2) xr = xk;
+            xk = seg_bcast<SEG, K0 + 1>(b) * dk[1]; b = fma(-uk[1], xk, b); if (el_l == K0 + 1) xr = xk;
+            xk = seg_bcast<SEG, K0 + 0>(b) * dk[0]; b = fma(-uk[0], xk, b); if (el_l == K0 + 0) xr = xk;
+        }
+    }
+    template <int... I>
+    __device__ __forceinline__ void fwd_all(double &b, int ne, int el_l, std::integer_sequence<int, I...>) const {
+        (fwd_group<8 * I>(b, ne, el_l), ...);                       // groups 0, 8, ...
+    }
+    template <int... I>
+    __device__ __forceinline__ void bwd_all(double &b, double &xr, double dinv_l, int ne, int el_l,
+                                            std::integer_sequence<int, I...>) const {
+        (bwd_group<SEG - 8 - 8 * I>(b, xr, dinv_l, ne, el_l), ...);  // groups SEG - 8, ..., 0
+    }
+    __device__ __forceinline__ void solve_impl(int vid) {
+        int ne = this->ne, el_l = el, src = sbase + plv;
+        double dinv_l = dinv;
+        asm volatile("" : "+s"(ne));
+        asm volatile("" : "+v"(el_l), "+v"(src), "+v"(dinv_l));
+        double b = lane_shfl_d(V[vid][0], src);   // b of the row chosen at step el
+        fwd_all(b, ne, el_l, std::make_integer_sequence<int, SEG / 8>{});
+        double xr = 0.;
+        bwd_all(b, xr, dinv_l, ne, el_l, std::make_integer_sequence<int, SEG / 8>{});
+        V[vid][0] = (el_l < ne) ? xr : 0.;
+    }
+
+    // J v, the team context's four partial sums (element j's column to a[j % 4] for j below the
+    // largest multiple of 4 not above ne, the rest to a[0]), with v_j broadcast by DPP.
+    template <int J>
+    __device__ __forceinline__ void jv_term(double v, int nq, double &a0, double &a1, double &a2, double &a3) const {
+        if (J < ne) {
+            const double c = Jl[J * JS + el];
+            const double vj = seg_bcast<SEG, J>(v);
+            if (J < nq) {
+                if ((J & 3) == 0) a0 = fma(c, vj, a0);
+                else if ((J & 3) == 1) a1 = fma(c, vj, a1);
+                else if ((J & 3) == 2) a2 = fma(c, vj, a2);
+                else a3 = fma(c, vj, a3);
+            } else {
+                a0 = fma(c, vj, a0);
+            }
+        }
+    }
+    template <int... J>
+    __device__ __forceinline__ void jv_all(double v, int nq, double &a0, double &a1, double &a2, double &a3,
+                                           std::integer_sequence<int, J...>) const {
+        (jv_term<J>(v, nq, a0, a1, a2, a3), ...);
+    }
+    __device__ __forceinline__ void jac_times_impl(int src, int dst) {
+        const double v = V[src][0];
+        double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
+        jv_all(v, ne & ~3, a0, a1, a2, a3, std::make_integer_sequence<int, SEG>{});
+        V[dst][0] = (a0 + a1) + (a2 + a3);
+    }
+
+    __device__ __forceinline__ int jacobian(double h) {
+        if (K.jac_mode == 1) { jacobian_analytic_impl(); return 0; }
+        jacobian_impl(h);
+        return ne;
+    }
+    __device__ __forceinline__ void factor(double hr0) { factor_impl(hr0); }
+    __device__ __forceinline__ void solve(int vid) { solve_impl(vid); }
+    __device__ __forceinline__ void jac_times(int src, int dst) { jac_times_impl(src, dst); }
+};
+
+// Persistent packed kernel: each wave claims CPW consecutive tickets of the queue, segment s
+// integrates the cell of ticket base + s.  The per-cell harness is the team kernel's
+// (chemistry_DI.H:97-282).
+#define PACK_WAVES_PER_SIMD 1
+template <int SEG, bool XK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PACK_WAVES_PER_SIMD)))
+pack_kernel(DevMech M, KParams K, int64_t n, const double *__restrict__ dt, const double *yin, double *yout,
+            int *status, double *q, int *counters, unsigned *queue, const int *order) {
+    constexpr int CPW = 64 / SEG;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    {
+        const int4 *src = (const int4 *)M.img.src;
+        int4 *dst = (int4 *)lds_raw;
+        const int nv = M.img.bytes / 16;
+        for (int i = threadIdx.x; i < nv; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+    const int wave = threadIdx.x >> 6;
+    const int sg = (threadIdx.x & 63) / SEG;
+    double *cell_lds = (double *)(lds_raw + M.img.bytes + (size_t)wave * pack_wave_bytes(SEG, M.ne)) +
+                       (size_t)sg * pack_cell_doubles(SEG, M.ne);
+    SegCtx<SEG, XK> c(M, K, lds_raw, cell_lds);
+    const int ns = M.ns, ne = M.ne;
+    for (;;) {
+        unsigned base_u = 0;
+        if ((threadIdx.x & 63) == 0) base_u = atomicAdd(queue, (unsigned)CPW);
+        base_u = __shfl(base_u, 0, 64);
+        const int64_t base = (int64_t)__builtin_amdgcn_readfirstlane((int)base_u);
+        if (base >= n) break;
+        const int64_t ticket = base + sg;
+        int64_t cell = -1;
+        if (ticket < n) cell = order ? (int64_t)order[ticket] : ticket;
+        if (cell >= 0 && cell < n) {
+            {
+                int l = threadIdx.x & (SEG - 1), a = M.ne, b = M.ns, r = M.nr, s0 = M.img.simple0;
+                asm volatile("" : "+v"(l));
+                asm volatile("" : "+s"(a), "+s"(b), "+s"(r), "+s"(s0));
+                c.el = l; c.ne = a; c.ns = b; c.nr = r; c.s0 = s0;
+            }
+            const int el = c.el;
+            const double yi = el < ne ? yin[(int64_t)el * n + cell] : 0.;
+            const double T0 = c.elem(yi, ns);
+            double y;
+            int st = 0;
+            BdfCounters cnt = BdfCounters{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (T0 > K.min_temperature) {                                  // chemistry_DI.H:97
+                double y0 = yi;                                            // :101-128
+                if (el < ns && y0 < 0.) y0 = 0.;
+                const double sum = seg_sum<SEG>(el < ns ? y0 : 0.);
+                if (el < ns) y0 /= sum;
+                c.V[Y0_][0] = el < ne ? y0 : 0.;
+                if (K.const_volume) c.set_density(y0, T0, K.rho, cell);     // :173
+                st = bdf_integrate(c, K.bk, K.bk.t0 + dt[cell], cnt);       // :138-157
+                y = st > 0 ? c.V[Y_][0] : c.V[Y0_][0];
+                if (st > 0 && q != nullptr && el == 0) q[cell] = c.qr;      // :168
+            } else {
+                y = yi;
+            }
+            const double sum = seg_sum<SEG>(el < ns ? y : 0.);             // :219
+            if (el < ns) y /= sum;
+            if (!K.energy && el == ns) y = yi;
+            if (K.compact) y = c.compact_sources(y, yi, T0, dt[cell]);     // :236-282
+            if (el < ne) yout[(int64_t)el * n + cell] = y;
+            if (el == 0 && status) status[cell] = st;
+            if (counters != nullptr && el < LSM_NCOUNTERS_DEV) {
+                const unsigned vals[12] = {cnt.steps, cnt.fcalls, cnt.fcalls_jac, cnt.jacobians, cnt.factorizations,
+                                           cnt.solves, cnt.conv_failures, cnt.err_failures, cnt.max_order,
+                                           cnt.rhs_total, cnt.clamps, cnt.newton};
+                unsigned v = 0;
+#pragma unroll
+                for (int i = 0; i < 12; i++)
+                    if (el == i) v = vals[i];
+                counters[cell * LSM_NCOUNTERS_DEV + el] = (int)v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Wide context: one cell per 256-thread workgroup (ne <= 256).  Thread e owns
 // element e of every vector (registers); sums are wave DPP reductions combined
 // across the four waves through LDS (one barrier, fixed order, identical result
@@ -2436,7 +3068,73 @@ LSM_TEAM_LAUNCHER_BODY(56)
 LSM_TEAM_LAUNCHER_BODY(64)
 #endif
 
+template <int SEG, bool XK>
+static int launch_pack_n(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
+                         double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order,
+                         int grid, int wpb, hipStream_t s) {
+    const size_t lds = (size_t)m.img.bytes + (size_t)wpb * pack_wave_bytes(SEG, m.ne);
+    static size_t lds_set = 65536;   // largest dynamic LDS size granted so far (not re-set under graph capture)
+    if (lds > lds_set) {
+        if (hipFuncSetAttribute((const void *)pack_kernel<SEG, XK>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds) != hipSuccess)
+            return -1;
+        lds_set = lds;
+    }
+    hipLaunchKernelGGL((pack_kernel<SEG, XK>), dim3(grid), dim3(64 * wpb), lds, s, m, k, n, dt, yin, yout, status, q,
+                       counters, queue, order);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Packed kernels (2 or 4 cells per wavefront), one translation unit per segment width.
+#define LSM_PACK_LAUNCHER(SG)                                                                                       \
+    int launch_pack_##SG(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,        \
+                         double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order,    \
+                         int grid, int wpb, hipStream_t s)
+#define LSM_PACK_LAUNCHER_BODY(SG)                                                                                  \
+    LSM_PACK_LAUNCHER(SG) {                                                                                         \
+        return m.xk ? launch_pack_n<SG, true>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, \
+                                              s)                                                                    \
+                    : launch_pack_n<SG, false>(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid,    \
+                                               wpb, s);                                                             \
+    }
+LSM_PACK_LAUNCHER(16);
+LSM_PACK_LAUNCHER(32);
+#if LSM_IN_PART(7)
+LSM_PACK_LAUNCHER_BODY(16)
+#endif
+#if LSM_IN_PART(8)
+LSM_PACK_LAUNCHER_BODY(32)
+#endif
+
 #if LSM_IN_PART(6)
+int team_max_cells_per_wave(const DevMech &m) {
+    if (!m.img.team_ok) return 1;
+    return m.ne <= 16 ? 4 : (m.ne <= 32 ? 2 : 1);
+}
+
+int team_waves_per_cu_cpw(const DevMech &m, int cpw) {
+    return cpw > 1 ? 4 * PACK_WAVES_PER_SIMD : team_waves_per_cu(m);
+}
+
+bool team_shape_fits_cpw(const DevMech &m, int cpw, int wpb, int blocks_per_cu, size_t lds_cu) {
+    if (cpw == 1) return team_shape_fits(m, wpb, blocks_per_cu, lds_cu);
+    if ((cpw != 2 && cpw != 4) || cpw > team_max_cells_per_wave(m) || wpb < 1 || wpb > 4 || blocks_per_cu < 1)
+        return false;
+    const size_t wave = pack_wave_bytes(64 / cpw, m.ne);
+    return (size_t)blocks_per_cu * ((size_t)m.img.bytes + (size_t)wpb * wave) <= lds_cu;
+}
+
+int launch_team_cpw(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin,
+                    double *yout, int *status, double *q, int *counters, unsigned *queue, const int *order, int grid,
+                    int wpb, int cpw, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (cpw == 1) return launch_team(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, stream);
+    if (wpb < 1 || wpb > 4 || cpw > team_max_cells_per_wave(m)) return -2;
+    if (cpw == 4) return launch_pack_16(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, s);
+    if (cpw == 2) return launch_pack_32(m, k, n, dt, yin, yout, status, q, counters, queue, order, grid, wpb, s);
+    return -2;
+}
+
 int launch_team(const DevMech &m, const KParams &k, int64_t n, const double *dt, const double *yin, double *yout,
                 int *status, double *q, int *counters, unsigned *queue, const int *order, int grid, int wpb,
                 void *stream) {

@@ -80,6 +80,19 @@ def oracle_jacobian_envelope(oracle, mech, y, dt, rtol, atol, y_ref, scales=(0.5
     return env
 
 
+def split_method(method):
+    """Method names of the tests: "team", "wide", "plain", "pack2" / "pack4" (the team kernel with 2
+    or 4 cells per wavefront), optionally with "-analytic" for the analytic Jacobian
+    ("pack4-analytic").  Returns the keyword arguments of api.ChemistryStep."""
+    jacobian = "numerical"
+    if method.endswith("-analytic"):
+        method, jacobian = method[:-len("-analytic")], "analytic"
+    cpw = 1
+    if method.startswith("pack"):
+        method, cpw = "team", int(method[4:])
+    return dict(method=method, jacobian=jacobian, cells_per_wave=cpw)
+
+
 def newton_matrix_changed(method):
     """Variants whose Newton matrix differs from the reference's on purpose: the analytic
     Jacobian, and the wide integrator's FP32 factorisation (its refined solves carry errors up to

@@ -16,7 +16,8 @@
 * the heat release Q (HeatRelease, KineticsMap_CHEMKIN.hpp:1961-1964) of every
   integrated cell equals the reactor's QR at that cell's own final state.
 
-Each case runs on the team, plain and wide kernels against the oracle on the same cells;
+Each case runs on the team (one cell per wavefront, and 4 or 2 cells per wavefront: "pack4",
+"pack2"), plain and wide kernels against the oracle on the same cells;
 statuses must be identical and final states within tests/parity.py's bound.
 """
 import numpy as np
@@ -26,7 +27,7 @@ from laminarsmoke_amd import api
 from laminarsmoke_amd.fields import workload_cells, coflow_field
 from laminarsmoke_amd.mechanism import workload_mechanism
 from parity import (assert_parity, assert_parity_adaptive, assert_element_conservation, oracle_envelope, oracle_jacobian_envelope,
-                    newton_matrix_changed, reconcile_statuses)
+                    newton_matrix_changed, reconcile_statuses, split_method)
 
 pytestmark = pytest.mark.gpu
 
@@ -39,10 +40,7 @@ def make_step(mech, method, rtol=RTOL, atol=ATOL, max_steps=None, **kw):
     ode.SetAbsoluteTolerance(atol)
     if max_steps:
         ode.SetMaximumNumberOfSteps(max_steps)
-    jac = "numerical"
-    if method.endswith("-analytic"):
-        method, jac = method[:-len("-analytic")], "analytic"
-    return api.ChemistryStep(mech, ode, method=method, jacobian=jac, **kw)
+    return api.ChemistryStep(mech, ode, **split_method(method), **kw)
 
 
 def run_gpu(mech, y, method, dt=DT, t0=0.0, q=None, **kw):
@@ -101,7 +99,7 @@ def parity_at(oracle, mech, y, method, t0, what):
           flush=True)
 
 
-METHODS = ["team", "plain", "wide", "team-analytic", "wide-analytic"]
+METHODS = ["team", "plain", "wide", "team-analytic", "wide-analytic", "pack4", "pack4-analytic"]
 
 
 @pytest.mark.parametrize("method", METHODS)
@@ -138,7 +136,7 @@ def gate_cells():
     return mech, y
 
 
-@pytest.mark.parametrize("method", ["team", "plain", "wide"])
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "pack4"])
 def test_gate_skip_and_illegal_constraints(hip_device, oracle, method):
     mech, y = gate_cells()
     n = y.shape[1]
@@ -159,7 +157,7 @@ def test_gate_skip_and_illegal_constraints(hip_device, oracle, method):
     assert (yg[:mech.ns] >= 0).all()
 
 
-@pytest.mark.parametrize("method", ["team", "plain", "wide"])
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "pack4"])
 def test_step_limit_status(hip_device, oracle, method):
     """@MaximumNumberOfSteps 5: cells that need more steps stop with -1 and return their input."""
     mech, y = workload_cells(1, n=32)
@@ -173,7 +171,7 @@ def test_step_limit_status(hip_device, oracle, method):
     assert (c_g[f, 0] == 5).all() and (c_o[f, 0] == 5).all()
 
 
-@pytest.mark.parametrize("method", ["team", "plain", "wide"])
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "pack2"])
 def test_too_strict_tolerance_status(hip_device, oracle, method):
     """rtol 1e-17: EPS * ||y||_wrms > 1 at the first step, status -2, input returned."""
     mech, y = workload_cells(1, n=16)

@@ -11,7 +11,7 @@ from laminarsmoke_amd import api
 from laminarsmoke_amd.fields import workload_cells, coflow_field
 from laminarsmoke_amd.mechanism import workload_mechanism
 from parity import (assert_parity, assert_parity_adaptive, assert_element_conservation, oracle_envelope, oracle_jacobian_envelope,
-                    newton_matrix_changed, reconcile_statuses, ORACLE_THREADS)
+                    newton_matrix_changed, reconcile_statuses, split_method, ORACLE_THREADS)
 
 pytestmark = pytest.mark.gpu
 
@@ -22,10 +22,7 @@ def gpu_step(mech, y, dt, method, rtol=RTOL, atol=ATOL, **kw):
     ode = api.ODE_Parameters()
     ode.SetRelativeTolerance(rtol)
     ode.SetAbsoluteTolerance(atol)
-    jacobian = "numerical"
-    if method.endswith("-analytic"):
-        method, jacobian = method[:-len("-analytic")], "analytic"
-    step = api.ChemistryStep(mech, ode, method=method, jacobian=jacobian, **kw)
+    step = api.ChemistryStep(mech, ode, **split_method(method), **kw)
     try:
         return step.integrate(y, dt)
     finally:
@@ -53,7 +50,8 @@ def _check(oracle, mech, y, method, what, ref=None, env=None):
     return yg, yo, q_g, q_o
 
 
-@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic", "wide-analytic"])
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "team-analytic", "wide-analytic", "pack4", "pack2",
+                                    "pack4-analytic", "pack2-analytic"])
 def test_cfg1_parity(hip_device, oracle, method):
     mech, y = workload_cells(1, n=64)
     print(f"cfg1 parity method={method} cells={y.shape[1]}", flush=True)

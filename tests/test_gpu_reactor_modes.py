@@ -1,6 +1,6 @@
 """GPU: the constant-volume reactor and the Strang "compact" output.
 
-* constant-volume chemistry steps (team, wide, plain kernels; the reference's
+* constant-volume chemistry steps (team with 1 or 4 cells per wavefront, wide, plain; the reference's
   forward-difference Jacobian) against the oracle: identical OdeStatus and the
   envelope criterion, on the 14-species workload cells and on the reference's H2/CO
   mechanism (dt = 1e-4 s, most cells ignite);
@@ -19,7 +19,7 @@ import pytest
 from laminarsmoke_amd import api
 from laminarsmoke_amd.chemkin import load_chemkin
 from laminarsmoke_amd.fields import premixed_cells, workload_cells
-from parity import assert_parity, assert_element_conservation, oracle_envelope
+from parity import assert_parity, assert_element_conservation, oracle_envelope, split_method
 
 pytestmark = pytest.mark.gpu
 
@@ -41,10 +41,10 @@ def step(mech, method, **kw):
     ode = api.ODE_Parameters()
     ode.SetRelativeTolerance(RTOL)
     ode.SetAbsoluteTolerance(ATOL)
-    return api.ChemistryStep(mech, ode, method=method, **kw)
+    return api.ChemistryStep(mech, ode, **split_method(method), **kw)
 
 
-@pytest.mark.parametrize("method", ["team", "wide", "plain"])
+@pytest.mark.parametrize("method", ["team", "wide", "plain", "pack4"])
 @pytest.mark.parametrize("which", ["cfg1", "h2co"])
 def test_constant_volume_parity(hip_device, oracle, which, method):
     mech, y, dt = cells(which)
@@ -83,7 +83,7 @@ def test_cell_densities(hip_device, oracle):
 
 
 @pytest.mark.parametrize("cv", [False, True])
-@pytest.mark.parametrize("method", ["team", "wide", "plain"])
+@pytest.mark.parametrize("method", ["team", "wide", "plain", "pack4"])
 def test_compact_output(hip_device, oracle, method, cv):
     mech, y, _ = cells("h2co")
     n = y.shape[1]
