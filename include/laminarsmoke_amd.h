@@ -109,9 +109,10 @@ enum lsm_method {
     LSM_METHOD_PLAIN = 1,       /* one cell per thread, FP64, state in HBM                    */
     LSM_METHOD_TEAM = 2,        /* one cell per wavefront, persistent work queue, FP64        */
     LSM_METHOD_WIDE = 3         /* one cell per 256-thread workgroup, persistent work queue;
-                                   FP64 rates, residuals and error control, Newton-matrix LU
-                                   in FP32 (registers, then LDS for the solves) with iterative
-                                   refinement (ns+1 <= 176)                                    */
+                                   FP64 rates, residuals and error control; the Newton matrix
+                                   inverted in FP32 in LDS (blocked Gauss-Jordan with partial
+                                   pivoting, MFMA rank-16 updates), each solve an FP32
+                                   mat-vec with FP64 iterative refinement (ns+1 <= 176)        */
 };
 
 /* OdeStatus values (math/native-ode-solvers/OdeSolverUtilities.h:83-99); a cell
@@ -191,7 +192,8 @@ typedef enum lsm_jacobian_type {
 int lsm_set_jacobian(lsm_handle *h, int32_t type);
 
 /* Most iterative-refinement sweeps after each FP32 Newton solve of the wide kernel
- * (0-8, default 4): x += LU^-1 (b - G x) with the residual in FP64 from the FP64
+ * (0-8, default 4): x += G32^-1 (b - G x), G32^-1 the FP32 inverse of the Newton
+ * matrix, with the residual in FP64 from the FP64
  * Jacobian, until the correction's WRMS norm in the integration weights is below 1e-5
  * (of the tolerance).  A factorisation whose solves do not get there (a sweep that
  * does not halve the correction, or no sweeps left) is redone in FP64 for this and the

@@ -40,7 +40,8 @@ constexpr double POS_INF = __builtin_huge_val();
 // Optional phase timers (-DLSM_PHASE_TIMERS): cycles spent in the RHS, the
 // Jacobian (including its RHS calls), the factorisation, the solves and J*v,
 // summed over waves into KParams::phase_cycles[0..4]; [5] whole cells, [6] cells;
-// team RHS parts: [7] species and sums, [8] reactions, [9] assembly.
+// team RHS parts: [7] species and sums, [8] reactions, [9] assembly; wide solves: [7] FP32
+// triangular solves, [8] FP64 residual mat-vecs, [9] refinement sweeps (a count).
 #ifdef LSM_PHASE_TIMERS
 #define LSM_TIC(v) const unsigned long long v = clock64()
 #define LSM_TOC(i, v) (tp[i] += clock64() - (v))
@@ -2295,35 +2296,136 @@ struct WideCtx {
         return ((unsigned long long)__float_as_uint(fabsf(a)) << 32) | (unsigned)(0x7fffffff - row);
     }
 
-    // G = I - hr0 J and its LU with partial pivoting (KernelDense.hpp:196-221), in
-    // FP32 in LDS.  Rows stay in place ("virtual" pivoting): row i records the
-    // step ps(i) at which it became the pivot row, plv[k] the row chosen at step k.
-    // Lane l of every wave keeps rows l, l+64, l+128 (rows >= ne map to a spare
-    // row); wave w updates the columns k+1+w, k+5+w, ... of step k, so no entry is
-    // written by two waves, and one barrier per step suffices:
-    //  * every wave reads column k of its rows and forms the same multipliers;
-    //  * the multipliers of column k are written one step later (nobody reads
-    //    column k then);
-    //  * wave 0 owns column k+1 and proposes its pivot (largest |entry| among rows
-    //    not yet chosen, ties to the lowest row) before the barrier.
-    // Loads are masked by multiplication, not by condition: a select on a load
-    // result lets the compiler sink the load into a branch with its own wait.
-    __device__ __forceinline__ int wave_pivot(const int rr[3], const int psr[3], int col) const {
-        unsigned long long key = 0ull;
-#pragma unroll
-        for (int m = 0; m < 3; m++) {
-            const float v = A[rr[m] * lda + col];
-            const unsigned long long kk = piv_key(v, rr[m]);
-            key = umax64(key, psr[m] < 0 ? kk : 0ull);
-        }
-        key = wave_max_u64(key);
-        return 0x7fffffff - (int)(unsigned)(key & 0xffffffffull);
+    // G = I - hr0 J and its inverse (the Newton matrix LU of KernelDense.hpp:196-221, done as a
+    // Gauss-Jordan inversion with the same partial pivoting), in FP32 in LDS, 16 columns per panel:
+    //  1. the panel's remaining rows, one per thread, are factorised in registers to choose the
+    //     pivots (largest |entry| among rows not yet chosen, the lowest row on a tie): one
+    //     barrier per column, each wave's candidate writing its row to a slot of its own before
+    //     it, so that every thread reads the winner's row after it;
+    //  2. wave 0 inverts the pivot block, T = A[pivot rows][panel]^-1, by Gauss-Jordan in
+    //     registers (lane i: row i; its pivots are the ones chosen above, in order);
+    //  3. every thread swaps the rows of its own column (the pivots as interchanges);
+    //  4. X = T A[pivot rows][other columns]                  (MFMA 16x16x4, one tile per wave)
+    //  5. A[other rows][other columns] -= A[other rows][panel] X     (MFMA, the rank-16 update)
+    //  6. A[other rows][panel] = -A[other rows][panel] T, A[pivot rows][panel] = T.
+    // After the last panel A holds (P G)^-1, P the row interchanges (perm[i]: the row of G at
+    // position i), so G^-1 b = A (P b): each FP32 solve is one mat-vec.  MFMA operands are loaded
+    // from clamped addresses unconditionally and masked afterwards (a masked load would become a
+    // branch with a wait of its own).
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    __device__ __forceinline__ static f32x4 mfma16(float a, float b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    // Position of the x-th row (column) outside the panel [k0, k0 + w).
+    __device__ __forceinline__ static int skip_panel(int x, int k0, int w) { return x < k0 ? x : x + w; }
+
+    // Step 1: the panel's pivot rows (positions before the interchanges) in prow[0..w); thread j
+    // < w also keeps its own, prow[j], in mine.
+    __device__ __forceinline__ void panel_pivots(int k0, int w, int (&prow)[16], int &mine) {
+        const int n = ne;
+        float *slot = fsc + 800;                  // [2][4][16] candidate rows by parity and wave
+        unsigned long long *kslot = (unsigned long long *)(fsc + 928);   // [2][4] wave keys
+        const int row = k0 + tid;
+        bool elig = row < n;
+        float pr[16];
+        {
+            const float *Ar = A + (elig ? row : 0) * lda + k0;
+#pragma unroll
+            for (int j = 0; j < 16; j++) pr[j] = Ar[j < w ? j : 0];
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int j = 0; j < 16; j++) pr[j] = (elig && j < w) ? pr[j] : 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            prow[j] = 0;
+            if (j < w) {
+                const int par = j & 1;
+                const unsigned long long key = elig ? piv_key(pr[j], row) : 0ull;
+                const unsigned long long wkey = wave_max_u64(key);
+                if (key == wkey && key != 0ull) {
+                    float4 *dst = (float4 *)(slot + (par * 4 + wave) * 16);
+                    dst[0] = make_float4(pr[0], pr[1], pr[2], pr[3]);
+                    dst[1] = make_float4(pr[4], pr[5], pr[6], pr[7]);
+                    dst[2] = make_float4(pr[8], pr[9], pr[10], pr[11]);
+                    dst[3] = make_float4(pr[12], pr[13], pr[14], pr[15]);
+                }
+                if (lane == 0) kslot[par * 4 + wave] = wkey;
+                __syncthreads();
+                unsigned long long best = kslot[par * 4];
+                int ww = 0;
+#pragma unroll
+                for (int v = 1; v < 4; v++) {
+                    const unsigned long long kv = kslot[par * 4 + v];
+                    if (kv > best) { best = kv; ww = v; }
+                }
+                // uniform across the workgroup (every thread read the same keys)
+                const int p = __builtin_amdgcn_readfirstlane(0x7fffffff - (int)(unsigned)(best & 0xffffffffull));
+                const int pw = __builtin_amdgcn_readfirstlane(ww);
+                prow[j] = p;
+                if (tid == j) mine = p;
+                const float4 *src = (const float4 *)(slot + (par * 4 + pw) * 16);
+                float u[16];
+                {
+                    const float4 u0 = src[0], u1 = src[1], u2 = src[2], u3 = src[3];
+                    u[0] = u0.x; u[1] = u0.y; u[2] = u0.z; u[3] = u0.w;
+                    u[4] = u1.x; u[5] = u1.y; u[6] = u1.z; u[7] = u1.w;
+                    u[8] = u2.x; u[9] = u2.y; u[10] = u2.z; u[11] = u2.w;
+                    u[12] = u3.x; u[13] = u3.y; u[14] = u3.z; u[15] = u3.w;
+                }
+                if (row == p) elig = false;
+                const float rpiv = u[j] != 0.f ? 1.f / u[j] : 0.f;
+                const float l = elig ? pr[j] * rpiv : 0.f;
+#pragma unroll
+                for (int jj = j + 1; jj < 16; jj++) pr[jj] = fmaf(-l, u[jj], pr[jj]);
+            }
+        }
+    }
+
+    // Step 2 (wave 0): T = A11^-1 by Gauss-Jordan without further pivoting, lane i < w holding
+    // row i of A11 = A[prow][panel] (pi: prow[i]); rows and columns past w stay the identity.
+    __device__ __forceinline__ void pivot_block_inverse(int k0, int w, int pi, float *Ts) {
+        float t[16];
+        const bool own = lane < w;
+        const float *Ar = A + (own ? pi : 0) * lda + k0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) t[j] = Ar[j < w ? j : 0];
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int j = 0; j < 16; j++) t[j] = (own && j < w) ? t[j] : (lane == j ? 1.f : 0.f);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            if (j < w) {
+                const float piv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t[j]), j));
+                const float rp = piv != 0.f ? 1.f / piv : 0.f;
+                const float f = t[j];
+                if (lane == j) {
+#pragma unroll
+                    for (int jj = 0; jj < 16; jj++) t[jj] = jj == j ? rp : t[jj] * rp;
+                }
+                float rj[16];
+#pragma unroll
+                for (int jj = 0; jj < 16; jj++) rj[jj] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t[jj]), j));
+                if (lane != j) {
+#pragma unroll
+                    for (int jj = 0; jj < 16; jj++) t[jj] = jj == j ? -f * rp : fmaf(-f, rj[jj], t[jj]);
+                }
+            }
+        }
+        if (lane < 16) {
+            float4 *dst = (float4 *)(Ts + lane * 16);
+            dst[0] = make_float4(t[0], t[1], t[2], t[3]);
+            dst[1] = make_float4(t[4], t[5], t[6], t[7]);
+            dst[2] = make_float4(t[8], t[9], t[10], t[11]);
+            dst[3] = make_float4(t[12], t[13], t[14], t[15]);
+        }
+    }
+
     __device__ __forceinline__ void factor_impl(double hr) {
         hr0 = hr;
         fp64 = false;
         const int n = ne;
-        int *pslot = (int *)pkey;                // [2] pivot of the next step (parity)
+        float *Ts = fsc + 512;                    // [16][16] inverse of the pivot block
         if (tid < n) {
             const size_t t = tid;
             int j = 0;
@@ -2336,189 +2438,192 @@ struct WideCtx {
                 A[tid * lda + j + 3] = (float)((tid == j + 3 ? 1. : 0.) - hr * j3);
             }
             for (; j < n; j++) A[tid * lda + j] = (float)((tid == j ? 1. : 0.) - hr * J[(size_t)j * n + t]);
-        } else {
-            for (int j = tid - n; j < lda; j += WIDE_THREADS - n) A[n * lda + j] = 0.f;   // spare row: finite
-        }
-        int rr[3], psr[3];
-#pragma unroll
-        for (int m = 0; m < 3; m++) {
-            const int r = lane + 64 * m;
-            rr[m] = r < n ? r : n;                   // row n: spare
-            psr[m] = r < n ? -1 : 0x7fffffff;
+            perm[tid] = tid;
         }
         __syncthreads();
-        int p = wave_pivot(rr, psr, 0);
-        float lprev[3] = {0.f, 0.f, 0.f};
-        bool wprev[3] = {false, false, false};
-        for (int k = 0; k < n; k++) {
-            if (k > 0) p = pslot[k & 1];
-            const float piv = A[p * lda + k];
-            float c[3];
+        const int il = lane & 15, kl = lane >> 4;     // MFMA operand coordinates of this lane
+        for (int k0 = 0; k0 < n; k0 += 16) {
+            const int w = n - k0 < 16 ? n - k0 : 16;
+            int prow[16], mine = 0;
+            panel_pivots(k0, w, prow, mine);
+            if (wave == 0) pivot_block_inverse(k0, w, mine, Ts);
+            // the pivots as interchanges: step j swaps positions k0 + j and ipiv[j]
+            int ipiv[16];
 #pragma unroll
-            for (int m = 0; m < 3; m++) c[m] = A[rr[m] * lda + k];
-            if (k > 0 && wave == ((k - 1) & 3)) {
+            for (int j = 0; j < 16; j++) {
+                ipiv[j] = prow[j];
 #pragma unroll
-                for (int m = 0; m < 3; m++)
-                    if (wprev[m]) A[rr[m] * lda + k - 1] = lprev[m];
+                for (int jj = j + 1; jj < 16; jj++)
+                    if (prow[jj] == k0 + j) prow[jj] = prow[j];   // the row at k0 + j moves to prow[j]
             }
-            const float rpiv = piv != 0.f ? 1.f / piv : 0.f;
-            float l[3];
+            // 3. row interchanges, one column per thread (the permutation by the last thread);
+            // the panel columns are read by the inverse above, so this waits for it
+            __syncthreads();
+            if (tid < n) {
 #pragma unroll
-            for (int m = 0; m < 3; m++) {
-                if (rr[m] == p) psr[m] = k;
-                wprev[m] = psr[m] < 0;
-                l[m] = wprev[m] ? c[m] * rpiv : 0.f;
-                lprev[m] = l[m];
+                for (int j = 0; j < 16; j++) {
+                    const int a = k0 + j, b = ipiv[j];
+                    if (j < w && a != b) {
+                        const float va = A[a * lda + tid], vb = A[b * lda + tid];
+                        A[a * lda + tid] = vb;
+                        A[b * lda + tid] = va;
+                    }
+                }
+            } else if (tid == WIDE_THREADS - 1) {
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const int a = k0 + j, b = ipiv[j];
+                    if (j < w) {
+                        const int pa = perm[a];
+                        perm[a] = perm[b];
+                        perm[b] = pa;
+                    }
+                }
             }
-            if (tid == 0) {
-                perm[k] = p;
-                dinv[p] = piv != 0.f ? 1. / (double)piv : 0.;
+            __syncthreads();
+            const int no = n - w;                     // rows (columns) outside the panel
+            const int nt = (no + 15) >> 4;
+            // 4. X = T A[pivot rows][other columns], in place
+            for (int tc = wave; tc < nt; tc += 4) {
+                const int xc = 16 * tc + il;
+                const bool okc = xc < no;
+                const int col = skip_panel(okc ? xc : 0, k0, w);
+                float av[4], bv[4];
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const int kk = 4 * s + kl;
+                    av[s] = Ts[il * 16 + kk];
+                    bv[s] = A[(k0 + (kk < w ? kk : 0)) * lda + col];
+                }
+                asm volatile("" ::: "memory");
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int s = 0; s < 4; s++) acc = mfma16(av[s], (okc && 4 * s + kl < w) ? bv[s] : 0.f, acc);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = 4 * kl + r;
+                    if (okc && i < w) A[(k0 + i) * lda + col] = acc[r];
+                }
             }
-            const float *prow = A + p * lda;
-            int j = k + 1 + wave;
-            for (; j + 12 < n; j += 16) {            // four columns per pass: loads, then FMAs and stores
-                float u[4], a[4][3];
+            __syncthreads();
+            // 5. rank-16 update of the rest: A[Ro][Co] -= A[Ro][panel] X
+            for (int t = wave; t < nt * nt; t += 4) {
+                const int tr = t / nt, tc = t - tr * nt;
+                const int xa = 16 * tr + il, xc = 16 * tc + il;
+                const bool oka = xa < no, okc = xc < no;
+                const int arow = skip_panel(oka ? xa : 0, k0, w);
+                const int col = skip_panel(okc ? xc : 0, k0, w);
+                float av[4], bv[4], cv[4];
+                int crow[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    u[q] = prow[j + 4 * q];
-#pragma unroll
-                    for (int m = 0; m < 3; m++) a[q][m] = A[rr[m] * lda + j + 4 * q];
+                for (int r = 0; r < 4; r++) {
+                    const int xr = 16 * tr + 4 * kl + r;
+                    crow[r] = skip_panel(xr < no ? xr : 0, k0, w);
+                    cv[r] = A[crow[r] * lda + col];
                 }
 #pragma unroll
-                for (int q = 0; q < 4; q++)
+                for (int s = 0; s < 4; s++) {
+                    const int kk = 4 * s + kl;
+                    av[s] = A[arow * lda + k0 + (kk < w ? kk : 0)];
+                    bv[s] = A[(k0 + (kk < w ? kk : 0)) * lda + col];
+                }
+                asm volatile("" ::: "memory");
+                f32x4 acc;
 #pragma unroll
-                    for (int m = 0; m < 3; m++) A[rr[m] * lda + j + 4 * q] = fmaf(-l[m], u[q], a[q][m]);
+                for (int r = 0; r < 4; r++) acc[r] = cv[r];
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const bool okk = 4 * s + kl < w;
+                    acc = mfma16((oka && okk) ? -av[s] : 0.f, (okc && okk) ? bv[s] : 0.f, acc);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (16 * tr + 4 * kl + r < no && okc) A[crow[r] * lda + col] = acc[r];
             }
-            for (; j < n; j += 4) {
-                const float u = prow[j];
-                float a[3];
+            __syncthreads();
+            // 6. A[Ro][panel] = -A[Ro][panel] T; A[pivot rows][panel] = T
+            for (int tr = wave; tr < nt; tr += 4) {
+                const int xa = 16 * tr + il;
+                const bool oka = xa < no;
+                const int arow = skip_panel(oka ? xa : 0, k0, w);
+                float av[4], bv[4];
 #pragma unroll
-                for (int m = 0; m < 3; m++) a[m] = A[rr[m] * lda + j];
+                for (int s = 0; s < 4; s++) {
+                    const int kk = 4 * s + kl;
+                    av[s] = A[arow * lda + k0 + (kk < w ? kk : 0)];
+                    bv[s] = Ts[kk * 16 + il];
+                }
+                asm volatile("" ::: "memory");
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int m = 0; m < 3; m++) A[rr[m] * lda + j] = fmaf(-l[m], u, a[m]);
+                for (int s = 0; s < 4; s++) acc = mfma16((oka && 4 * s + kl < w) ? av[s] : 0.f, bv[s], acc);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int xr = 16 * tr + 4 * kl + r;
+                    if (xr < no && il < w) A[skip_panel(xr, k0, w) * lda + k0 + il] = -acc[r];
+                }
             }
-            if (wave == 0 && k + 1 < n) {
-                const int pn = wave_pivot(rr, psr, k + 1);
-                if (lane == 0) pslot[(k + 1) & 1] = pn;
+            {
+                const int i = tid >> 4, j = tid & 15;
+                if (i < w && j < w) A[(k0 + i) * lda + k0 + j] = Ts[i * 16 + j];
             }
             __syncthreads();
         }
-        // Rows to pivot order (row k <- the row chosen at step k), U rows scaled by
-        // 1/pivot (unit diagonal), 32 columns at a time through registers.
-        for (int c0 = 0; c0 < n; c0 += 32) {
-            float v[22];                          // 22 x 256 >= 176 rows x 32 columns
-#pragma unroll
-            for (int q = 0; q < 22; q++) {
-                const int e = tid + 256 * q;
-                const int i = e >> 5, j = c0 + (e & 31);
-                const int ic = i < n ? i : n - 1, jc = j < n ? j : n - 1;
-                const int src = perm[ic];
-                const float a = A[src * lda + jc];
-                v[q] = (j > i) ? a * (float)dinv[src] : a;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < 22; q++) {
-                const int e = tid + 256 * q;
-                const int i = e >> 5, j = c0 + (e & 31);
-                if (i < n && j < n) A[i * lda + j] = v[q];
-            }
-            __syncthreads();
-        }
-        if (tid < n) dsc[tid] = (float)dinv[perm[tid]];
-        __syncthreads();
     }
     __device__ __forceinline__ void factor(double hr) { LSM_TIC(t0); factor_impl(hr); LSM_TOC(2, t0); }
 
-    // Forward/backward substitution in FP32 on the factors in pivot order (L unit
-    // lower, U scaled to a unit diagonal, scales in dsc), by wave 0; the vector in
-    // bs (FP64, natural order) is overwritten with the solution.  Lane l holds
-    // rows l, l+64, l+128 (three registers, one per 64-row group, so the pivot
-    // element of step k is lane k & 63 of group k >> 6); the factors of 8 steps
-    // are fetched ahead of the dependent b -> readlane -> fma chain.  Accuracy
-    // comes from the FP64 residuals of iterative refinement.
-    template <int MK>
-    __device__ __forceinline__ void fwd_group(float (&b)[3], const int (&rr)[3], int k0, int k1) const {
-        for (int kb = k0; kb < k1; kb += 8) {
-            float lv[8][3];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int k = kb + u < k1 ? kb + u : k1 - 1;
-#pragma unroll
-                for (int m = MK; m < 3; m++) {
-                    const float v = A[rr[m] * lda + k];
-                    lv[u][m] = v * (((m > MK || lane > (k & 63)) && kb + u < k1) ? 1.f : 0.f);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                if (kb + u >= k1) break;            // past the group: lane k & 63 holds no row of it
-                const float xk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b[MK]), (kb + u) & 63));
-#pragma unroll
-                for (int m = MK; m < 3; m++) b[m] = fmaf(-lv[u][m], xk, b[m]);
-            }
-        }
-    }
-    template <int MK>
-    __device__ __forceinline__ void bwd_group(float (&b)[3], const int (&rr)[3], int k1, int k0) {
-        for (int kb = k1 - 1; kb >= k0; kb -= 8) {
-            float uv[8][3];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int k = kb - u >= k0 ? kb - u : k0;
-#pragma unroll
-                for (int m = 0; m <= MK; m++) {
-                    const float v = A[rr[m] * lda + k];
-                    uv[u][m] = v * (((m < MK || lane < (k & 63)) && kb - u >= k0) ? 1.f : 0.f);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                if (kb - u < k0) break;
-                const int k = kb - u;
-                const float xk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b[MK]), k & 63));
-#pragma unroll
-                for (int m = 0; m <= MK; m++) b[m] = fmaf(-uv[u][m], xk, b[m]);
-                if (lane == 0) bs[k] = (double)xk;
-            }
-        }
-    }
+    // bs <- G^-1 bs (FP64 in, FP64 out) in FP32: x = A (P b), A = (P G)^-1 from factor_impl, one
+    // row per thread, four partial sums.  Accuracy comes from the FP64 residuals of iterative
+    // refinement.
     __device__ __forceinline__ void lu_solve_bs() {
-        if (wave == 0) {
-            const int n = ne;
-            float b[3];
-            int rr[3];
+        const int n = ne;
+        float *ys = fsc;                              // FP32 (P b) [256]
+        if (tid < n) ys[tid] = (float)bs[perm[tid]];
+        __syncthreads();
+        const int row = tid < n ? tid : n - 1;
+        const float *Ar = A + row * lda;
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+        int j = 0;
+        for (; j + 16 <= n; j += 16) {
+            float v[16], x[16];
 #pragma unroll
-            for (int m = 0; m < 3; m++) {
-                const int r = lane + 64 * m;
-                const int rc = r < n ? r : n - 1;
-                rr[m] = r < n ? r : n;
-                const double v = bs[perm[rc]];
-                b[m] = r < n ? (float)v : 0.f;
-            }
-            const int e0 = n < 64 ? n : 64, e1 = n < 128 ? n : 128;
-            fwd_group<0>(b, rr, 0, e0);
-            if (n > 64) fwd_group<1>(b, rr, 64, e1);
-            if (n > 128) fwd_group<2>(b, rr, 128, n);
+            for (int u = 0; u < 16; u++) { v[u] = Ar[j + u]; x[u] = ys[j + u]; }
 #pragma unroll
-            for (int m = 0; m < 3; m++) {
-                const int r = lane + 64 * m;
-                const float d = dsc[r < n ? r : n - 1];
-                b[m] *= d;
+            for (int u = 0; u < 16; u += 4) {
+                a0 = fmaf(v[u], x[u], a0);
+                a1 = fmaf(v[u + 1], x[u + 1], a1);
+                a2 = fmaf(v[u + 2], x[u + 2], a2);
+                a3 = fmaf(v[u + 3], x[u + 3], a3);
             }
-            if (n > 128) bwd_group<2>(b, rr, n, 128);
-            if (n > 64) bwd_group<1>(b, rr, e1, 64);
-            bwd_group<0>(b, rr, e0, 0);
         }
+        for (; j < n; j++) a0 = fmaf(Ar[j], ys[j], a0);
+        const float x = (a0 + a1) + (a2 + a3);
+        if (tid < n) bs[tid] = (double)x;
         __syncthreads();
     }
 
-    // Row `tid` of J times the vector in bs (loads unconditional, 4 partial sums).
+    // Row `tid` of J times the vector in bs (loads unconditional, 4 partial sums: column j goes to
+    // a[j % 4] below the largest multiple of 4 not above n, the rest to a0).  The Jacobian sits in
+    // the workgroup's HBM workspace: its loads are issued 32 at a time, so that the latency of one
+    // batch is paid once, not per column.
     __device__ __forceinline__ double jrow_dot() const {
         const int n = ne;
         const int row = tid < n ? tid : n - 1;
+        const double *Jr = J + row;
         double a0 = 0., a1 = 0., a2 = 0., a3 = 0.;
         int j = 0;
+        for (; j + 32 <= n; j += 32) {
+            double v[32];
+#pragma unroll
+            for (int u = 0; u < 32; u++) v[u] = Jr[(size_t)(j + u) * n];
+#pragma unroll
+            for (int u = 0; u < 32; u += 4) {
+                a0 = fma(v[u], bs[j + u], a0);
+                a1 = fma(v[u + 1], bs[j + u + 1], a1);
+                a2 = fma(v[u + 2], bs[j + u + 2], a2);
+                a3 = fma(v[u + 3], bs[j + u + 3], a3);
+            }
+        }
         for (; j + 4 <= n; j += 4) {
             a0 = fma(J[(size_t)j * n + row], bs[j], a0);
             a1 = fma(J[(size_t)(j + 1) * n + row], bs[j + 1], a1);
@@ -2611,17 +2716,26 @@ struct WideCtx {
             __syncthreads();
             return;
         }
+        LSM_TIC(ts0);
         lu_solve_bs();
+        LSM_TOC(7, ts0);
         double x = tid < ne ? bs[tid] : 0.;
         bool ok = K.refine == 0;
         double prev = BIG_D;
         for (int it = 0; it < K.refine; it++) {
+            LSM_TIC(tr0);
             const double jx = jrow_dot();
+            LSM_TOC(8, tr0);
+#ifdef LSM_PHASE_TIMERS
+            tp[9] += 1;
+#endif
             const double r = b - (x - hr0 * jx);
             __syncthreads();
             if (tid < ne) bs[tid] = r;
             __syncthreads();
+            LSM_TIC(ts1);
             lu_solve_bs();
+            LSM_TOC(7, ts1);
             const double d = tid < ne ? bs[tid] : 0.;
             x += d;
             const double dw = d * V[OOE_][0];

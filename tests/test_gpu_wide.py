@@ -8,9 +8,14 @@ Criterion: identical status, and per cell max_k |y_gpu - y_oracle| / (rtol |y| +
 atol) <= max(10, 3 x envelope), the envelope being the larger of the oracle's
 self-envelope and its Jacobian envelope on that cell, both stored with the fixture
 (the rule of the other parity tests for variants whose Newton matrix differs on
-purpose: the FP32 factorisation perturbs the Newton iterates by up to its refined
+purpose: the FP32 inverse perturbs the Newton iterates by up to its refined
 accuracy, far less than an integration tolerance, but on ill-conditioned cells any
-perturbation moves the step sequence).  The errors of both against the tight
+perturbation moves the step sequence); as in every other parity test, a cell beyond
+that bound gets 256 more oracle draws on 1e-15-perturbed copies of its input before
+it counts as out of bound (the stored envelope has five).  These cells are chaotic at
+rtol 1e-8: over 256 draws the oracle's own deviation from its unperturbed result
+reaches 1.22e4 tolerance units on cell 0 (90th percentile 3.1e3) and 1.15e4 on cell 4,
+where five draws see 106 and 254.  The errors of both against the tight
 solution and the largest |dT| and |dY_k| against the oracle are printed (the
 precision study's figures).
 """
@@ -22,7 +27,7 @@ import pytest
 from laminarsmoke_amd import api
 from laminarsmoke_amd.fields import workload_cells
 from laminarsmoke_amd.mechanism import workload_mechanism
-from parity import ratio, assert_parity, check_against_truth, oracle_envelope
+from parity import ratio, assert_parity, assert_parity_adaptive, check_against_truth, oracle_envelope
 
 pytestmark = pytest.mark.gpu
 
@@ -46,7 +51,7 @@ def wide_step(mech, y, dt, rtol, atol, jacobian="numerical", refine=4, fallbacks
 
 @pytest.mark.parametrize("tag,rtol,atol", [("r6", 1e-6, 1e-12), ("r8", 1e-8, 1e-14)])
 @pytest.mark.parametrize("jacobian", ["numerical", "analytic"])
-def test_cfg4_against_oracle(hip_device, tag, rtol, atol, jacobian):
+def test_cfg4_against_oracle(hip_device, oracle, tag, rtol, atol, jacobian):
     d = np.load(GOLDEN)
     mech = workload_mechanism(160)
     y = d["y_in"]
@@ -55,7 +60,8 @@ def test_cfg4_against_oracle(hip_device, tag, rtol, atol, jacobian):
     yo, st_o, tight = d[f"y_{tag}"], d[f"status_{tag}"], d["y_tight"]
     assert np.array_equal(st, st_o), (st, st_o)
     env = np.maximum(d[f"env_{tag}"], d[f"jenv_{tag}"])
-    cell, _ = assert_parity(yg, yo, rtol, atol, env, what=f"cfg4/{jacobian}/{tag}")
+    (cell, _), env = assert_parity_adaptive(oracle, mech, y, float(d["dt"]), rtol, atol, yo, yg, env,
+                                            what=f"cfg4/{jacobian}/{tag}")
     e_ref = ratio(yo, tight, rtol, atol).max(axis=0)
     e_gpu = ratio(yg, tight, rtol, atol).max(axis=0)
     np.testing.assert_allclose(yg[:mech.ns].sum(0), 1.0, atol=1e-14)
