@@ -2290,12 +2290,6 @@ struct WideCtx {
         return nrhs;
     }
 
-    // 64-bit pivot key: |a| (FP32 bits) high, inverted row low, so that the largest
-    // magnitude wins and ties go to the lowest row (Eigen PartialPivLU's choice).
-    __device__ __forceinline__ static unsigned long long piv_key(float a, int row) {
-        return ((unsigned long long)__float_as_uint(fabsf(a)) << 32) | (unsigned)(0x7fffffff - row);
-    }
-
     // G = I - hr0 J and its inverse (the Newton matrix LU of KernelDense.hpp:196-221, done as a
     // Gauss-Jordan inversion with the same partial pivoting), in FP32 in LDS, 16 columns per panel:
     //  1. the panel's remaining rows, one per thread, are factorised in registers to choose the
@@ -2307,7 +2301,8 @@ struct WideCtx {
     //  3. every thread swaps the rows of its own column (the pivots as interchanges);
     //  4. X = T A[pivot rows][other columns]                  (MFMA 16x16x4, one tile per wave)
     //  5. A[other rows][other columns] -= A[other rows][panel] X     (MFMA, the rank-16 update)
-    //  6. A[other rows][panel] = -A[other rows][panel] T, A[pivot rows][panel] = T.
+    //  6. A[other rows][panel] = -A[other rows][panel] T, A[pivot rows][panel] = T;
+    //     5 and 6 by 16-row blocks, each owned by one wave.
     // After the last panel A holds (P G)^-1, P the row interchanges (perm[i]: the row of G at
     // position i), so G^-1 b = A (P b): each FP32 solve is one mat-vec.  MFMA operands are loaded
     // from clamped addresses unconditionally and masked afterwards (a masked load would become a
@@ -2319,12 +2314,28 @@ struct WideCtx {
     // Position of the x-th row (column) outside the panel [k0, k0 + w).
     __device__ __forceinline__ static int skip_panel(int x, int k0, int w) { return x < k0 ? x : x + w; }
 
+    // Wavefront maximum of floats >= -1 (lanes without a DPP source read 0, so a wave of -1
+    // reports 0; callers check eligibility with a ballot).
+    __device__ __forceinline__ static float wave_max_f(float x) {
+        x = fmaxf(x, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xf, 0xf, false)));
+        x = fmaxf(x, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xf, 0xf, false)));
+        x = fmaxf(x, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xf, 0xf, false)));
+        x = fmaxf(x, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xf, 0xf, false)));
+        x = fmaxf(x, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x142, 0xa, 0xf, false)));
+        x = fmaxf(x, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x143, 0xc, 0xf, false)));
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+    }
+
     // Step 1: the panel's pivot rows (positions before the interchanges) in prow[0..w); thread j
-    // < w also keeps its own, prow[j], in mine.
+    // < w also keeps its own, prow[j], in mine.  Per column: each wave's largest |entry| among
+    // its eligible rows (NaN counts as +inf, so some eligible row always wins) and the lowest such
+    // row (a ballot), its row written to the wave's slot; after the barrier every thread takes
+    // the largest over the waves, the lowest wave on a tie (rows grow with the wave index).
     __device__ __forceinline__ void panel_pivots(int k0, int w, int (&prow)[16], int &mine) {
         const int n = ne;
         float *slot = fsc + 800;                  // [2][4][16] candidate rows by parity and wave
-        unsigned long long *kslot = (unsigned long long *)(fsc + 928);   // [2][4] wave keys
+        float *mslot = fsc + 928;                 // [2][4] wave maxima (-1: no eligible row)
+        int *rslot = (int *)(fsc + 936);          // [2][4] their rows
         const int row = k0 + tid;
         bool elig = row < n;
         float pr[16];
@@ -2341,26 +2352,32 @@ struct WideCtx {
             prow[j] = 0;
             if (j < w) {
                 const int par = j & 1;
-                const unsigned long long key = elig ? piv_key(pr[j], row) : 0ull;
-                const unsigned long long wkey = wave_max_u64(key);
-                if (key == wkey && key != 0ull) {
+                const float av = fabsf(pr[j]);
+                const float a = elig ? (av == av ? av : __builtin_huge_valf()) : -1.f;
+                const float m = wave_max_f(a);
+                const unsigned long long hit = __ballot(elig && a == m);
+                const int wl = hit ? __ffsll(hit) - 1 : -1;          // the wave's lowest such lane
+                if (lane == wl) {
                     float4 *dst = (float4 *)(slot + (par * 4 + wave) * 16);
                     dst[0] = make_float4(pr[0], pr[1], pr[2], pr[3]);
                     dst[1] = make_float4(pr[4], pr[5], pr[6], pr[7]);
                     dst[2] = make_float4(pr[8], pr[9], pr[10], pr[11]);
                     dst[3] = make_float4(pr[12], pr[13], pr[14], pr[15]);
                 }
-                if (lane == 0) kslot[par * 4 + wave] = wkey;
-                __syncthreads();
-                unsigned long long best = kslot[par * 4];
-                int ww = 0;
-#pragma unroll
-                for (int v = 1; v < 4; v++) {
-                    const unsigned long long kv = kslot[par * 4 + v];
-                    if (kv > best) { best = kv; ww = v; }
+                if (lane == 0) {
+                    mslot[par * 4 + wave] = hit ? m : -1.f;
+                    rslot[par * 4 + wave] = k0 + 64 * wave + wl;
                 }
-                // uniform across the workgroup (every thread read the same keys)
-                const int p = __builtin_amdgcn_readfirstlane(0x7fffffff - (int)(unsigned)(best & 0xffffffffull));
+                __syncthreads();
+                const float4 mv = *(const float4 *)(mslot + par * 4);
+                const int4 rv = *(const int4 *)(rslot + par * 4);
+                float best = mv.x;
+                int ww = 0, p = rv.x;
+                if (mv.y > best) { best = mv.y; ww = 1; p = rv.y; }
+                if (mv.z > best) { best = mv.z; ww = 2; p = rv.z; }
+                if (mv.w > best) { best = mv.w; ww = 3; p = rv.w; }
+                // uniform across the workgroup (every thread read the same slots)
+                p = __builtin_amdgcn_readfirstlane(p);
                 const int pw = __builtin_amdgcn_readfirstlane(ww);
                 prow[j] = p;
                 if (tid == j) mine = p;
@@ -2506,62 +2523,69 @@ struct WideCtx {
                 }
             }
             __syncthreads();
-            // 5. rank-16 update of the rest: A[Ro][Co] -= A[Ro][panel] X
-            for (int t = wave; t < nt * nt; t += 4) {
-                const int tr = t / nt, tc = t - tr * nt;
-                const int xa = 16 * tr + il, xc = 16 * tc + il;
-                const bool oka = xa < no, okc = xc < no;
-                const int arow = skip_panel(oka ? xa : 0, k0, w);
-                const int col = skip_panel(okc ? xc : 0, k0, w);
-                float av[4], bv[4], cv[4];
-                int crow[4];
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int xr = 16 * tr + 4 * kl + r;
-                    crow[r] = skip_panel(xr < no ? xr : 0, k0, w);
-                    cv[r] = A[crow[r] * lda + col];
-                }
-#pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    const int kk = 4 * s + kl;
-                    av[s] = A[arow * lda + k0 + (kk < w ? kk : 0)];
-                    bv[s] = A[(k0 + (kk < w ? kk : 0)) * lda + col];
-                }
-                asm volatile("" ::: "memory");
-                f32x4 acc;
-#pragma unroll
-                for (int r = 0; r < 4; r++) acc[r] = cv[r];
-#pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    const bool okk = 4 * s + kl < w;
-                    acc = mfma16((oka && okk) ? -av[s] : 0.f, (okc && okk) ? bv[s] : 0.f, acc);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-                    if (16 * tr + 4 * kl + r < no && okc) A[crow[r] * lda + col] = acc[r];
-            }
-            __syncthreads();
-            // 6. A[Ro][panel] = -A[Ro][panel] T; A[pivot rows][panel] = T
+            // 5. and 6., by 16-row blocks of the other rows, each block owned by one wave:
+            //    A[rows][Co] -= A[rows][panel] X, two column tiles at a time (their MFMA chains
+            //    interleave and share the panel operand), then A[rows][panel] = -A[rows][panel] T
+            //    (only this wave reads those entries, so no barrier between the two)
             for (int tr = wave; tr < nt; tr += 4) {
                 const int xa = 16 * tr + il;
                 const bool oka = xa < no;
                 const int arow = skip_panel(oka ? xa : 0, k0, w);
-                float av[4], bv[4];
+                float av[4];
+                int crow[4];
+                bool okr[4];
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
                     const int kk = 4 * s + kl;
                     av[s] = A[arow * lda + k0 + (kk < w ? kk : 0)];
-                    bv[s] = Ts[kk * 16 + il];
                 }
-                asm volatile("" ::: "memory");
-                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int s = 0; s < 4; s++) acc = mfma16((oka && 4 * s + kl < w) ? av[s] : 0.f, bv[s], acc);
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int xr = 16 * tr + 4 * kl + r;
-                    if (xr < no && il < w) A[skip_panel(xr, k0, w) * lda + k0 + il] = -acc[r];
+                    okr[r] = xr < no;
+                    crow[r] = skip_panel(okr[r] ? xr : 0, k0, w);
                 }
+                asm volatile("" ::: "memory");
+                float na[4];
+#pragma unroll
+                for (int s = 0; s < 4; s++) na[s] = (oka && 4 * s + kl < w) ? -av[s] : 0.f;
+                for (int tc = 0; tc < nt; tc += 2) {
+                    const int xc0 = 16 * tc + il, xc1 = xc0 + 16;
+                    const bool ok0 = xc0 < no, ok1 = xc1 < no;
+                    const int col0 = skip_panel(ok0 ? xc0 : 0, k0, w), col1 = skip_panel(ok1 ? xc1 : 0, k0, w);
+                    float b0[4], b1[4];
+                    f32x4 c0, c1;
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        const int kk = 4 * s + kl;
+                        const int prw = (k0 + (kk < w ? kk : 0)) * lda;
+                        b0[s] = A[prw + col0];
+                        b1[s] = A[prw + col1];
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        c0[r] = A[crow[r] * lda + col0];
+                        c1[r] = A[crow[r] * lda + col1];
+                    }
+                    asm volatile("" ::: "memory");
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        const bool okk = 4 * s + kl < w;
+                        c0 = mfma16(na[s], (ok0 && okk) ? b0[s] : 0.f, c0);
+                        c1 = mfma16(na[s], (ok1 && okk) ? b1[s] : 0.f, c1);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        if (okr[r] && ok0) A[crow[r] * lda + col0] = c0[r];
+                        if (okr[r] && ok1) A[crow[r] * lda + col1] = c1[r];
+                    }
+                }
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int s = 0; s < 4; s++) acc = mfma16((oka && 4 * s + kl < w) ? av[s] : 0.f, Ts[(4 * s + kl) * 16 + il], acc);
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (okr[r] && il < w) A[crow[r] * lda + k0 + il] = -acc[r];
             }
             {
                 const int i = tid >> 4, j = tid & 15;
