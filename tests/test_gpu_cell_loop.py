@@ -40,7 +40,7 @@ def make_step(mech, method, rtol=RTOL, atol=ATOL, max_steps=None, **kw):
     ode.SetAbsoluteTolerance(atol)
     if max_steps:
         ode.SetMaximumNumberOfSteps(max_steps)
-    return api.ChemistryStep(mech, ode, **split_method(method), **kw)
+    return api.ChemistryStep(mech, ode, **{**split_method(method), **kw})
 
 
 def run_gpu(mech, y, method, dt=DT, t0=0.0, q=None, **kw):

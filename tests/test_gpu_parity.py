@@ -22,7 +22,7 @@ def gpu_step(mech, y, dt, method, rtol=RTOL, atol=ATOL, **kw):
     ode = api.ODE_Parameters()
     ode.SetRelativeTolerance(rtol)
     ode.SetAbsoluteTolerance(atol)
-    step = api.ChemistryStep(mech, ode, **split_method(method), **kw)
+    step = api.ChemistryStep(mech, ode, **{**split_method(method), **kw})
     try:
         return step.integrate(y, dt)
     finally:

@@ -41,7 +41,7 @@ def step(mech, method, **kw):
     ode = api.ODE_Parameters()
     ode.SetRelativeTolerance(RTOL)
     ode.SetAbsoluteTolerance(ATOL)
-    return api.ChemistryStep(mech, ode, **split_method(method), **kw)
+    return api.ChemistryStep(mech, ode, **{**split_method(method), **kw})
 
 
 @pytest.mark.parametrize("method", ["team", "wide", "plain", "pack4"])
