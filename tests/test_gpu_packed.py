@@ -3,11 +3,10 @@
 With 2 (ns+1 <= 32) or 4 (ns+1 <= 16) cells per 64-lane wavefront each cell runs on its
 own 32- or 16-lane segment.  Its arithmetic is the one-cell-per-wavefront kernel's; only
 the order of the formation-rate scatter changes (reactions are strided over the segment's
-lanes), so results are compared with the oracle, under the criteria of tests/parity.py:
+lanes), so results are compared with the oracle, under the criteria of tests/parity.py
+(every cell of the 4,096-cell 14-species field, 4 and 2 cells per wavefront, is in
+test_gpu_parity.py::test_cfg1_whole_field).  Here:
 
-* every cell of the 4,096-cell 14-species field, 4 and 2 cells per wavefront, with the
-  reference's forward-difference Jacobian: identical OdeStatus, envelope bound, element
-  mass, sum Y = 1;
 * a cell's result does not depend on the segment or wavefront it lands in: bit-identical
   under any visiting order, for batches that leave segments of the last wavefront empty,
   and for one cell alone;
@@ -65,29 +64,6 @@ class Batch:
         torch.cuda.synchronize()
         return (self.out.cpu().numpy().copy(), self.st.cpu().numpy().copy(), self.cnt.cpu().numpy().copy(),
                 self.q.cpu().numpy().copy())
-
-
-@pytest.mark.parametrize("cpw", [4, 2])
-def test_whole_field_parity(hip_device, oracle, cpw):
-    """All 4,096 cells of the 14-species field against the oracle."""
-    mech, y = workload_cells(1)
-    n = y.shape[1]
-    print(f"packed whole field cpw={cpw} cells={n}", flush=True)
-    yo, st_o, q_o, c_o = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=ORACLE_THREADS)
-    step = make(mech, cpw)
-    try:
-        assert step.cells_per_wave() == cpw and step.launch()["cells_per_wave"] == cpw
-        yg, st_g, q_g, c_g = step.integrate(y, DT)
-    finally:
-        step.close()
-    assert np.array_equal(st_g, st_o), (np.where(st_g != st_o)[0], st_g[st_g != st_o], st_o[st_g != st_o])
-    env = oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, yo)
-    (cell, frac), env = assert_parity_adaptive(oracle, mech, y, DT, RTOL, ATOL, yo, yg, env, what=f"cfg1/@{cpw}")
-    assert_element_conservation(mech, y, yg, c_g[:, 10], what=f"cfg1/@{cpw}")
-    np.testing.assert_allclose(yg[:mech.ns].sum(0), 1.0, atol=1e-14)
-    print(f"cfg1/@{cpw}: {n} cells, max ratio {cell.max():.3g} (envelope max {env.max():.3g}), components "
-          f"within 10x {100 * frac:.2f}%, steps gpu/oracle median {np.median(c_g[:, 0])}/{np.median(c_o[:, 0])}",
-          flush=True)
 
 
 @pytest.mark.parametrize("cpw,jac", [(4, "numerical"), (4, "analytic"), (2, "analytic")])

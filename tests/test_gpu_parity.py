@@ -61,6 +61,28 @@ def test_cfg1_parity(hip_device, oracle, method):
         assert abs(q_g[c] - oracle.rhs(mech, yg[:, c])[1]) <= 1e-6 * abs(q_g[c])
 
 
+_CFG1 = {}
+
+
+def cfg1_field(oracle):
+    """All 4,096 cells of the 14-species field with the oracle's result and 5-draw self-envelope
+    (computed once per session)."""
+    if not _CFG1:
+        mech, y = workload_cells(1)
+        ref = oracle.chemistry_step(mech, y, DT, rtol=RTOL, atol=ATOL, threads=ORACLE_THREADS)
+        _CFG1.update(mech=mech, y=y, ref=ref, env=oracle_envelope(oracle, mech, y, DT, RTOL, ATOL, ref[0]))
+    return _CFG1
+
+
+@pytest.mark.parametrize("method", ["team", "plain", "wide", "pack4", "pack2", "team-analytic"])
+def test_cfg1_whole_field(hip_device, oracle, method):
+    """Every cell of the 14-species field, every kernel (the wide kernel's FP32 inverse and the
+    analytic Jacobian under the widened envelope of variants whose Newton matrix differs)."""
+    d = cfg1_field(oracle)
+    print(f"cfg1 whole field method={method} cells={d['y'].shape[1]}", flush=True)
+    _check(oracle, d["mech"], d["y"], method, f"cfg1-field/{method}", ref=d["ref"], env=d["env"])
+
+
 _CFG2 = {}
 
 

@@ -27,7 +27,8 @@ import pytest
 from laminarsmoke_amd import api
 from laminarsmoke_amd.fields import workload_cells
 from laminarsmoke_amd.mechanism import workload_mechanism
-from parity import ratio, assert_parity, assert_parity_adaptive, check_against_truth, oracle_envelope
+from parity import (ratio, assert_parity, assert_parity_adaptive, assert_element_conservation, check_against_truth,
+                    oracle_envelope, oracle_jacobian_envelope, reconcile_statuses, ORACLE_THREADS)
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +72,31 @@ def test_cfg4_against_oracle(hip_device, oracle, tag, rtol, atol, jacobian):
           f"error vs tight max {e_gpu.max():.3g} median {np.median(e_gpu):.3g} "
           f"(oracle {e_ref.max():.3g} / {np.median(e_ref):.3g}); vs oracle max|dT| {dT:.3g} K, max|dY| {dY:.3g}; "
           f"steps {cnt[:, 0]} (oracle {d[f'steps_{tag}']})", flush=True)
+
+
+@pytest.mark.parametrize("jacobian", ["numerical", "analytic"])
+def test_cfg4_random_cells(hip_device, oracle, jacobian):
+    """16 cells drawn at random (seed 9) from the whole 131,072-cell 160-species field, against the
+    oracle run here at rtol 1e-6 (its 5-draw self-envelope and Jacobian envelope, widened by 256
+    more draws for a cell beyond the bound); identical statuses (or the status the oracle gives
+    with its finite-difference increments rescaled), element mass, sum Y = 1."""
+    mech, yfull = workload_cells(4)
+    idx = np.sort(np.random.default_rng(9).choice(yfull.shape[1], 16, replace=False))
+    y = np.ascontiguousarray(yfull[:, idx])
+    rtol, atol, dt = 1e-6, 1e-12, 1e-5
+    print(f"cfg4 random cells {idx.tolist()} wide {jacobian}", flush=True)
+    yo, st_o, _, c_o = oracle.chemistry_step(mech, y, dt, rtol=rtol, atol=atol, threads=ORACLE_THREADS)
+    env = np.maximum(oracle_envelope(oracle, mech, y, dt, rtol, atol, yo),
+                     oracle_jacobian_envelope(oracle, mech, y, dt, rtol, atol, yo))
+    yg, st, _, cnt = wide_step(mech, y, dt, rtol, atol, jacobian)
+    yo, st_o = reconcile_statuses(oracle, mech, y, dt, rtol, atol, st, st_o, yo, what=f"cfg4-random/{jacobian}")
+    assert np.array_equal(st, st_o), (st, st_o)
+    (cell, frac), env = assert_parity_adaptive(oracle, mech, y, dt, rtol, atol, yo, yg, env,
+                                               what=f"cfg4-random/{jacobian}")
+    assert_element_conservation(mech, y, yg, cnt[:, 10], what=f"cfg4-random/{jacobian}")
+    np.testing.assert_allclose(yg[:mech.ns].sum(0), 1.0, atol=1e-14)
+    print(f"cfg4-random/{jacobian}: max ratio {cell.max():.3g} (envelope max {env.max():.3g}), components within "
+          f"10x {100 * frac:.2f}%, steps gpu/oracle median {np.median(cnt[:, 0])}/{np.median(c_o[:, 0])}", flush=True)
 
 
 @pytest.mark.parametrize("refine", [0, 1, 4])
