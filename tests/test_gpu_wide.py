@@ -74,24 +74,38 @@ def test_cfg4_against_oracle(hip_device, oracle, tag, rtol, atol, jacobian):
           f"steps {cnt[:, 0]} (oracle {d[f'steps_{tag}']})", flush=True)
 
 
+_CFG4 = {}
+
+
+def cfg4_random(oracle):
+    """16 cells drawn at random (seed 9) from the whole 131,072-cell 160-species field, with the
+    oracle's rtol 1e-6 result, its 5-draw self-envelope and its Jacobian envelope (once per
+    session)."""
+    if not _CFG4:
+        mech, yfull = workload_cells(4)
+        idx = np.sort(np.random.default_rng(9).choice(yfull.shape[1], 16, replace=False))
+        y = np.ascontiguousarray(yfull[:, idx])
+        rtol, atol, dt = 1e-6, 1e-12, 1e-5
+        ref = oracle.chemistry_step(mech, y, dt, rtol=rtol, atol=atol, threads=ORACLE_THREADS)
+        env = np.maximum(oracle_envelope(oracle, mech, y, dt, rtol, atol, ref[0]),
+                         oracle_jacobian_envelope(oracle, mech, y, dt, rtol, atol, ref[0]))
+        _CFG4.update(mech=mech, idx=idx, y=y, ref=ref, env=env, rtol=rtol, atol=atol, dt=dt)
+    return _CFG4
+
+
 @pytest.mark.parametrize("jacobian", ["numerical", "analytic"])
 def test_cfg4_random_cells(hip_device, oracle, jacobian):
-    """16 cells drawn at random (seed 9) from the whole 131,072-cell 160-species field, against the
-    oracle run here at rtol 1e-6 (its 5-draw self-envelope and Jacobian envelope, widened by 256
-    more draws for a cell beyond the bound); identical statuses (or the status the oracle gives
-    with its finite-difference increments rescaled), element mass, sum Y = 1."""
-    mech, yfull = workload_cells(4)
-    idx = np.sort(np.random.default_rng(9).choice(yfull.shape[1], 16, replace=False))
-    y = np.ascontiguousarray(yfull[:, idx])
-    rtol, atol, dt = 1e-6, 1e-12, 1e-5
-    print(f"cfg4 random cells {idx.tolist()} wide {jacobian}", flush=True)
-    yo, st_o, _, c_o = oracle.chemistry_step(mech, y, dt, rtol=rtol, atol=atol, threads=ORACLE_THREADS)
-    env = np.maximum(oracle_envelope(oracle, mech, y, dt, rtol, atol, yo),
-                     oracle_jacobian_envelope(oracle, mech, y, dt, rtol, atol, yo))
+    """The 16 random cells against the oracle run here (its envelope widened by 256 more draws for
+    a cell beyond the bound); identical statuses (or the status the oracle gives with its
+    finite-difference increments rescaled), element mass, sum Y = 1."""
+    d = cfg4_random(oracle)
+    mech, y, rtol, atol, dt = d["mech"], d["y"], d["rtol"], d["atol"], d["dt"]
+    yo, st_o, _, c_o = d["ref"]
+    print(f"cfg4 random cells {d['idx'].tolist()} wide {jacobian}", flush=True)
     yg, st, _, cnt = wide_step(mech, y, dt, rtol, atol, jacobian)
     yo, st_o = reconcile_statuses(oracle, mech, y, dt, rtol, atol, st, st_o, yo, what=f"cfg4-random/{jacobian}")
     assert np.array_equal(st, st_o), (st, st_o)
-    (cell, frac), env = assert_parity_adaptive(oracle, mech, y, dt, rtol, atol, yo, yg, env,
+    (cell, frac), env = assert_parity_adaptive(oracle, mech, y, dt, rtol, atol, yo, yg, d["env"],
                                                what=f"cfg4-random/{jacobian}")
     assert_element_conservation(mech, y, yg, cnt[:, 10], what=f"cfg4-random/{jacobian}")
     np.testing.assert_allclose(yg[:mech.ns].sum(0), 1.0, atol=1e-14)
@@ -123,21 +137,30 @@ def test_refinement_sweeps_cfg1(hip_device, oracle, refine):
           f"{np.median(cnt[:, 0])}, factorisations {cnt[:, 4].mean():.1f}, FP64 fallbacks {fb[0]}", flush=True)
 
 
-def test_fp64_fallback_on_ill_conditioned_newton_matrices(hip_device):
+def test_fp64_fallback_on_ill_conditioned_newton_matrices(hip_device, oracle):
     """rtol 1e-10 on the 160-species cells drives h l0 J far from the identity: the Newton
     matrices become ill-conditioned, FP32 refinement stops contracting on some of them and those
-    factorisations are redone in FP64.  The result must still match the stored FP64 oracle run at
-    rtol 1e-10 (the tight solution of the fixture) within 10x its tolerance or the oracle's own
-    deviation from the rtol 1e-8 run, whichever is larger."""
+    factorisations are redone in FP64.  The result must match the stored oracle run at rtol 1e-10
+    by the parity rule of the other tests: per cell within max(10, 3 x envelope), the envelope
+    being the larger of the oracle's self-envelope and its Jacobian envelope over increments scaled
+    by 0.1 to 100 at rtol 1e-10 (stored with the run), widened by 64 more oracle draws for a cell
+    beyond the bound.  At this tolerance a change of Newton matrix moves these cells between a few
+    step sequences (11k to 25k steps) and the oracle itself by up to 3.4e3 tolerance units; its
+    rtol 1e-8 run lies up to 3.1e4 units from it."""
     d = np.load(GOLDEN)
     mech = workload_mechanism(160)
     y = d["y_in"]
+    dt, rtol, atol = float(d["dt"]), 1e-10, 1e-16
     print(f"cfg4 wide rtol 1e-10 cells={y.shape[1]}", flush=True)
     fb = []
-    yg, st, _, cnt = wide_step(mech, y, float(d["dt"]), 1e-10, 1e-16, refine=2, fallbacks=fb)
+    yg, st, _, cnt = wide_step(mech, y, dt, rtol, atol, refine=2, fallbacks=fb)
     assert (st == 2).all(), st
-    e = ratio(yg, d["y_tight"], 1e-10, 1e-16).max(axis=0)
-    e_ref = ratio(d["y_r8"], d["y_tight"], 1e-10, 1e-16).max(axis=0)
+    assert fb[0] > 0, "no factorisation fell back to FP64"
+    env = np.maximum(d["env_tight"], d["jenv_tight"])
+    e = ratio(yg, d["y_tight"], rtol, atol).max(axis=0)
+    e_ref = ratio(d["y_r8"], d["y_tight"], rtol, atol).max(axis=0)
     print(f"cfg4/wide/rtol1e-10: FP64 fallbacks {fb[0]} of {cnt[:, 4].sum()} factorisations; error vs oracle "
-          f"max {e.max():.3g} (oracle rtol 1e-8 run: {e_ref.max():.3g})", flush=True)
-    assert (e <= np.maximum(10.0, e_ref)).all(), (e, e_ref)
+          f"{np.array2string(e, precision=3)} (envelope {np.array2string(env, precision=3)}; oracle rtol 1e-8 run "
+          f"{np.array2string(e_ref, precision=3)})", flush=True)
+    assert_parity_adaptive(oracle, mech, y, dt, rtol, atol, d["y_tight"], yg, env, what="cfg4/wide/rtol1e-10",
+                           draws=64)
