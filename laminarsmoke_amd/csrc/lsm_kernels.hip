@@ -1124,10 +1124,19 @@ struct TeamCtx {
                     G[k + 1] = fma(-l, readlane_d(G[k + 1], p), G[k + 1]);
                     pn = pivot_lane(G[k + 1], ps < 0);
                 }
+                // four columns at a time, their pivot-row elements read first: the barrier keeps
+                // the compiler from interleaving read and update column by column through a
+                // single scalar register pair (groups of 8 need more scalar registers than are free)
 #pragma unroll
-                for (int j = k + 2; j < NMAX; j++) {
-                    const double u = readlane_d(G[j], p);
-                    G[j] = fma(-l, u, G[j]);
+                for (int j = k + 2; j < NMAX; j += 4) {
+                    double u[4];
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+                        if (j + t < NMAX) u[t] = readlane_d(G[j + t], p);
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+                        if (j + t < NMAX) G[j + t] = fma(-l, u[t], G[j + t]);
                 }
                 p = pn;
             }
