@@ -1984,8 +1984,9 @@ struct WideCtx {
     double *dinv;             // LDS [256]  1 / pivot, by row
     int *perm;                // LDS [256]  row chosen as pivot at each elimination step
     unsigned long long *pkey; // LDS [2]    pivot keys (parity of the elimination step)
-    float *fsc;               // LDS [1536] factorisation scratch
-    float *dsc;               // LDS [256]  1 / pivot of step k (U rows are stored scaled by it)
+    float *fsc;               // LDS [1536] factorisation scratch: [0,256) the FP32 solve vector,
+                              // [256,384) per-lane targets of the MFMA phases' masked-off stores,
+                              // [512,768) the pivot block's inverse, [800,944) pivot-search slots
     float *A;                 // LDS [ne + 1][lda] LU factors (spare last row); FP64 blocks of dR/dc in an analytic Jacobian
     double *J;                // HBM [ne][ne] column-major, J[j*ne + i] = d f_i / d y_j
     double *G64;              // HBM [ne][ne] column-major: FP64 LU of G when refinement does not converge
@@ -2011,8 +2012,7 @@ struct WideCtx {
         dinv = d + 3096;
         pkey = (unsigned long long *)(d + 3352);
         perm = (int *)(d + 3354);
-        fsc = (float *)(d + 3482);  // [1536]: [256,512) 1/pivot by step
-        dsc = fsc + 256;
+        fsc = (float *)(d + 3482);  // [1536]
         p64 = (int *)(d + 4250);
         A = (float *)(d + 4378);
     }
@@ -2315,7 +2315,10 @@ struct WideCtx {
     // After the last panel A holds (P G)^-1, P the row interchanges (perm[i]: the row of G at
     // position i), so G^-1 b = A (P b): each FP32 solve is one mat-vec.  MFMA operands are loaded
     // from clamped addresses unconditionally and masked afterwards (a masked load would become a
-    // branch with a wait of its own).
+    // branch with a wait of its own), all of a tile's loads issued before its first MFMA (a
+    // scheduling barrier: otherwise the compiler moves MFMAs up between the loads and waits on each
+    // load on its own); masked-off results are stored to a per-lane dummy slot instead of under a
+    // branch.
     typedef float f32x4 __attribute__((ext_vector_type(4)));
     __device__ __forceinline__ static f32x4 mfma16(float a, float b, f32x4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -2522,13 +2525,15 @@ struct WideCtx {
                     bv[s] = A[(k0 + (kk < w ? kk : 0)) * lda + col];
                 }
                 asm volatile("" ::: "memory");
+                __builtin_amdgcn_sched_barrier(0);     // every load issued before the first MFMA
                 f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int s = 0; s < 4; s++) acc = mfma16(av[s], (okc && 4 * s + kl < w) ? bv[s] : 0.f, acc);
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int i = 4 * kl + r;
-                    if (okc && i < w) A[(k0 + i) * lda + col] = acc[r];
+                    float *dst = (okc && i < w) ? A + (k0 + i) * lda + col : fsc + 256 + lane;
+                    *dst = acc[r];
                 }
             }
             __syncthreads();
@@ -2577,6 +2582,7 @@ struct WideCtx {
                         c1[r] = A[crow[r] * lda + col1];
                     }
                     asm volatile("" ::: "memory");
+                    __builtin_amdgcn_sched_barrier(0); // all 16 loads in flight before the MFMAs
 #pragma unroll
                     for (int s = 0; s < 4; s++) {
                         const bool okk = 4 * s + kl < w;
@@ -2585,16 +2591,24 @@ struct WideCtx {
                     }
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
-                        if (okr[r] && ok0) A[crow[r] * lda + col0] = c0[r];
-                        if (okr[r] && ok1) A[crow[r] * lda + col1] = c1[r];
+                        float *d0 = (okr[r] && ok0) ? A + crow[r] * lda + col0 : fsc + 256 + lane;
+                        float *d1 = (okr[r] && ok1) ? A + crow[r] * lda + col1 : fsc + 320 + lane;
+                        *d0 = c0[r];
+                        *d1 = c1[r];
                     }
                 }
+                float tv[4];
+#pragma unroll
+                for (int s = 0; s < 4; s++) tv[s] = Ts[(4 * s + kl) * 16 + il];
+                __builtin_amdgcn_sched_barrier(0);
                 f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int s = 0; s < 4; s++) acc = mfma16((oka && 4 * s + kl < w) ? av[s] : 0.f, Ts[(4 * s + kl) * 16 + il], acc);
+                for (int s = 0; s < 4; s++) acc = mfma16((oka && 4 * s + kl < w) ? av[s] : 0.f, tv[s], acc);
 #pragma unroll
-                for (int r = 0; r < 4; r++)
-                    if (okr[r] && il < w) A[crow[r] * lda + k0 + il] = -acc[r];
+                for (int r = 0; r < 4; r++) {
+                    float *dst = (okr[r] && il < w) ? A + crow[r] * lda + k0 + il : fsc + 256 + lane;
+                    *dst = -acc[r];
+                }
             }
             {
                 const int i = tid >> 4, j = tid & 15;
