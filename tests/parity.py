@@ -169,16 +169,28 @@ def element_mass(mech, y):
     return mech.element_mass_matrix() @ y[:mech.ns]
 
 
-def assert_element_conservation(mech, y_in, y_out, clamps, what="", atol=1e-12):
-    """Element mass fractions conserved to 1e-11 + 10 atol per yMin/yMax clamp event (a clamp
-    adds or removes mass at the atol scale)."""
+def assert_element_conservation(mech, y_in, y_out, clamps, what="", atol=1e-12, y_ref=None):
+    """Element mass fractions conserved to 1e-12 + 10 atol per yMin/yMax clamp event (a clamp
+    adds or removes mass at the atol scale; the final renormalisation changes nothing when the
+    state before it conserves elements, since its mass fractions then sum to 1), plus twice the
+    oracle's own deviation on the cell when its result y_ref is given.  The reference algorithm
+    itself misses 1e-12: on the 14-species field 3 of its 4,084 cells without clamps reach
+    1.1e-12 to 4.0e-12 (roundoff over about 500 steps), and on 512 cells of the 1M-cell field,
+    with up to 3,550 clamps per cell, it moves element mass by up to 1.9e-9, the same deviations
+    as the GPU's to three digits.  Without y_ref the base is 1e-11."""
     yin = y_in.copy()
     yin[:mech.ns] = np.clip(yin[:mech.ns], 0.0, None)
     yin[:mech.ns] /= yin[:mech.ns].sum(axis=0)
-    dz = np.abs(element_mass(mech, y_out) - element_mass(mech, yin)).max(axis=0)
-    bound = 1e-11 + 10.0 * atol * clamps
+    z_in = element_mass(mech, yin)
+    dz = np.abs(element_mass(mech, y_out) - z_in).max(axis=0)
+    if y_ref is None:
+        bound = 1e-11 + 10.0 * atol * clamps
+    else:
+        dz_ref = np.abs(element_mass(mech, y_ref) - z_in).max(axis=0)
+        bound = 1e-12 + 10.0 * atol * clamps + 2.0 * dz_ref
     bad = np.where(dz > bound)[0]
-    assert bad.size == 0, f"{what}: element mass not conserved in {bad.size} cells (worst {dz.max():.3g})"
+    assert bad.size == 0, (f"{what}: element mass not conserved in {bad.size} cells (worst {dz.max():.3g}; "
+                           f"cells {bad[:8].tolist()}, deviation {dz[bad[:8]]}, bound {bound[bad[:8]]})")
     return dz
 
 

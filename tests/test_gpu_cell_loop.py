@@ -88,7 +88,7 @@ def parity_at(oracle, mech, y, method, t0, what):
         yo, st_o = reconcile_statuses(oracle, mech, y, DT, RTOL, ATOL, st_g, st_o, yo, what=what, t0=t0)
     assert np.array_equal(st_g, st_o), (what, st_g, st_o)
     (cell, frac), env = assert_parity_adaptive(oracle, mech, y, DT, RTOL, ATOL, yo, yg, env, what=what, t0=t0)
-    assert_element_conservation(mech, y, yg, c_g[:, 10], what=what)
+    assert_element_conservation(mech, y, yg, c_g[:, 10], what=what, y_ref=yo)
     np.testing.assert_allclose(yg[:mech.ns].sum(0), 1.0, atol=1e-14)
     check_q(oracle, mech, yg, st_g, q_g, q_in)
     fail = st_o <= 0

@@ -123,7 +123,7 @@ def test_sample_against_oracle(full_field, oracle, jacobian):
         yo, st_o = reconcile_statuses(oracle, mech, y, DT, RTOL, ATOL, st_g, st_o, yo, what=f"cfg3/{jacobian}")
     assert np.array_equal(st_g, st_o), (st_g, st_o)
     (cell, frac), env = assert_parity_adaptive(oracle, mech, y, DT, RTOL, ATOL, yo, yg, env, what=f"cfg3/{jacobian}")
-    assert_element_conservation(mech, y, yg, cg[:, 10], what=f"cfg3/{jacobian}")
+    assert_element_conservation(mech, y, yg, cg[:, 10], what=f"cfg3/{jacobian}", y_ref=yo)
     print(f"cfg3/{jacobian}: max ratio {cell.max():.3g} (envelope max {env.max():.3g}), components within 10x "
           f"{100 * frac:.2f}%; steps gpu/oracle median {np.median(cg[:, 0])}/{np.median(c_o[:, 0])}, max "
           f"{cg[:, 0].max()}/{c_o[:, 0].max()}", flush=True)

@@ -42,7 +42,7 @@ def _check(oracle, mech, y, method, what, ref=None, env=None):
         yo, st_o = reconcile_statuses(oracle, mech, y, DT, RTOL, ATOL, st_g, st_o, yo, what=what)
     assert np.array_equal(st_g, st_o), (st_g, st_o)
     (cell, frac), env = assert_parity_adaptive(oracle, mech, y, DT, RTOL, ATOL, yo, yg, env, what=what)
-    assert_element_conservation(mech, y, yg, c_g[:, 10], what=what)
+    assert_element_conservation(mech, y, yg, c_g[:, 10], what=what, y_ref=yo)
     np.testing.assert_allclose(yg[:mech.ns].sum(0), 1.0, atol=1e-14)
     print(f"{what}: max ratio {cell.max():.3g} (envelope max {env.max():.3g}), "
           f"components within 10x: {100 * frac:.2f}%, steps gpu/oracle median {np.median(c_g[:, 0])}/"

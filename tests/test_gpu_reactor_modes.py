@@ -58,7 +58,7 @@ def test_constant_volume_parity(hip_device, oracle, which, method):
         s.close()
     assert np.array_equal(st_g, st_o), (st_g, st_o)
     cell, _ = assert_parity(yg, yo, RTOL, ATOL, env, what=f"cv/{which}/{method}")
-    assert_element_conservation(mech, y, yg, c_g[:, 10], what=f"cv/{which}/{method}")
+    assert_element_conservation(mech, y, yg, c_g[:, 10], what=f"cv/{which}/{method}", y_ref=yo)
     np.testing.assert_allclose(q_g, q_o, rtol=1e-3, atol=1e-6 * np.abs(q_o).max())
     print(f"cv/{which}/{method}: max ratio {cell.max():.3g} (envelope max {env.max():.3g}), T rise up to "
           f"{(yg[-1] - y[-1]).max():.0f} K", flush=True)

@@ -107,7 +107,7 @@ def test_cfg4_random_cells(hip_device, oracle, jacobian):
     assert np.array_equal(st, st_o), (st, st_o)
     (cell, frac), env = assert_parity_adaptive(oracle, mech, y, dt, rtol, atol, yo, yg, d["env"],
                                                what=f"cfg4-random/{jacobian}")
-    assert_element_conservation(mech, y, yg, cnt[:, 10], what=f"cfg4-random/{jacobian}")
+    assert_element_conservation(mech, y, yg, cnt[:, 10], what=f"cfg4-random/{jacobian}", y_ref=yo)
     np.testing.assert_allclose(yg[:mech.ns].sum(0), 1.0, atol=1e-14)
     print(f"cfg4-random/{jacobian}: max ratio {cell.max():.3g} (envelope max {env.max():.3g}), components within "
           f"10x {100 * frac:.2f}%, steps gpu/oracle median {np.median(cnt[:, 0])}/{np.median(c_o[:, 0])}", flush=True)
